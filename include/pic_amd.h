@@ -1,0 +1,237 @@
+/*
+ * pic_amd.h -- C ABI of the MI355X (gfx950) 2D electrostatic PIC time step.
+ *
+ * Drop-in boundary for the step of dimad285/PIC_CUDA.  Every entry point names
+ * the reference interface it replaces.  Conventions:
+ *   - every function returns an int status: 0 on success, negative on error
+ *     (PIC_ERR_*); pic_last_error() gives the message of the last failure on
+ *     the calling thread.  Nothing exits the process (the reference calls exit()
+ *     on allocation failure, particles.cu:135-136).
+ *   - particle positions cross the boundary in CELL units (x/dx, y/dy: the
+ *     domain is [0, m-1] x [0, n-1]), velocities in m/s.  The C++ and Python
+ *     mirrors of Particles2D convert from domain units.
+ *   - grid fields are FP64, row-major, m (x, fast) by n (y) nodes.
+ *   - one HIP stream per object; calls on one object are not thread-safe.
+ */
+#ifndef PIC_AMD_H
+#define PIC_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PIC_OK 0
+#define PIC_ERR_ARG -1
+#define PIC_ERR_HIP -2
+#define PIC_ERR_STATE -3
+#define PIC_ERR_CAPACITY -4
+#define PIC_ERR_IO -5
+
+#define PIC_PREC_FP32 4
+#define PIC_PREC_FP64 8
+
+/* collision process types (mcc.hpp:6-16 lists the channels; mcc.cu implements elastic) */
+#define PIC_PROC_ELASTIC 0
+#define PIC_PROC_EXCITATION 1
+#define PIC_PROC_IONIZATION 2
+#define PIC_PROC_CHARGE_EXCHANGE 3
+
+/* per-species event counters, index into the 16-entry array of pic_sim_counters */
+#define PIC_CNT_ABSORBED 0
+#define PIC_CNT_NULL_CANDIDATES 1
+#define PIC_CNT_PROCESS0 2          /* 2..5: real collisions of process 0..3 */
+#define PIC_CNT_BORN_SELF 6
+#define PIC_CNT_BORN_OTHER 7
+#define PIC_CNT_BORN_ABSORBED 8
+#define PIC_CNT_NUMAX_EXCEEDED 9
+#define PIC_CNT_REJECT_FALLBACK 10
+#define PIC_CNT_SPILLED 11
+#define PIC_CNT_LOST 12
+#define PIC_CNT_GLOBAL_DEPOSIT 13
+
+#define PIC_FIELD_RHO 0
+#define PIC_FIELD_PHI 1
+#define PIC_FIELD_EX 2
+#define PIC_FIELD_EY 3
+#define PIC_FIELD_RHS 4
+
+#define PIC_MG_FP64 0
+#define PIC_MG_MIXED 1
+#define PIC_MG_PRESET_REFERENCE 0   /* Multigrid.cu defaults: SOR 1.25, 3/3/20, 1e-5, 50 cycles, FP64 */
+#define PIC_MG_PRESET_STEP 1        /* 2+2 Gauss-Seidel to 1e-8, direct 3x3 coarse, FP64 refinement */
+#define PIC_MG_PRESET_POISSON 2     /* 2+2 Gauss-Seidel to 1e-10, FP64 refinement */
+
+typedef struct pic_particles pic_particles;
+typedef struct pic_grid pic_grid;
+typedef struct pic_walls pic_walls;
+typedef struct pic_mg pic_mg;
+typedef struct pic_xs pic_xs;
+typedef struct pic_sim pic_sim;
+
+/* ---- library ---------------------------------------------------------- */
+const char* pic_last_error(void);
+int pic_version(int* major, int* minor);
+int pic_device_synchronize(void);
+
+/* ---- Particles2D: particles.cuh:3-39, particles.cu:112-261 --------------- */
+/* Particles2D(int num_particles) particles.cu:112-150.  Species 0 exists with
+   m = 1, q = 0.1 like the reference constructor (particles.cu:146-149).
+   max_particles is the capacity PER SPECIES.  precision: PIC_PREC_FP32/64. */
+int pic_particles_create(int64_t max_particles, int precision, pic_particles** out);
+/* ~Particles2D() particles.cu:153-171 */
+int pic_particles_destroy(pic_particles* p);
+/* add_species(double m, double q) particles.cu:174-190 (argument order m, q) */
+int pic_particles_add_species(pic_particles* p, double m, double q, int* index);
+/* change_species(int s, double m, double q) particles.cu:193-197 */
+int pic_particles_change_species(pic_particles* p, int species, double m, double q);
+/* macro-particle weight (real particles per macro-particle per metre of depth) */
+int pic_particles_set_weight(pic_particles* p, int species, double weight);
+int pic_particles_num_species(const pic_particles* p, int* n);
+/* public member last_alive (particles.cuh:20); species -1 sums all species */
+int pic_particles_last_alive(pic_particles* p, int species, int64_t* n);
+/* host -> device append (replaces the cudaMemcpy of main.cu:232-238); id may be NULL */
+int pic_particles_upload(pic_particles* p, int species, int64_t n, const double* x, const double* y,
+                         const double* vx, const double* vy, const uint32_t* id);
+/* device -> host copy of a species (order unspecified; sort by id to compare) */
+int pic_particles_download(pic_particles* p, int species, int64_t cap, double* x, double* y, double* vx,
+                           double* vy, uint32_t* id, int64_t* n);
+/* seeded initial state on the device (replaces initializeParticles, main.cu:17-32):
+   uniform in the open box [0,nx-1]x[0,ny-1], Maxwellian sigma = vth per component */
+int pic_particles_init_maxwellian(pic_particles* p, int species, int64_t n, int nx, int ny, double vth,
+                                  uint32_t seed, uint32_t case_id, uint32_t id0);
+/* removeCollidedParticles() particles.cu:217-261: compaction is fused into
+   pic_sim_step; kept for the interface and returns immediately */
+int pic_particles_remove_collided(pic_particles* p);
+
+/* ---- Grid2D: grid.cuh:7-24, grid.cu:10-53 ------------------------------- */
+/* Grid2D(int M, int N, double X, double Y): dx = X/(M-1).  Fields are zeroed
+   (the reference leaves them uninitialised). */
+int pic_grid_create(int m, int n, double X, double Y, pic_grid** out);
+int pic_grid_destroy(pic_grid* g);
+int pic_grid_info(const pic_grid* g, int* m, int* n, double* dx, double* dy);
+/* public members rho, phi, E (device pointers; E is interleaved Ex,Ey) */
+int pic_grid_device_ptrs(pic_grid* g, double** rho, double** phi, double** E);
+int pic_grid_download(pic_grid* g, int field, double* out);
+int pic_grid_upload(pic_grid* g, int field, const double* in);
+/* computeElectricField kernels.cu:134-191 into the E member */
+int pic_grid_compute_E(pic_grid* g);
+
+/* ---- Boundaries2D: boundaries.h:7-23, boundaries.cu:3-25 ----------------- */
+int pic_walls_create(double width, double height, int nx, int ny, pic_walls** out);
+/* add_wall(x1,y1,x2,y2) in domain units */
+int pic_walls_add(pic_walls* w, double x1, double y1, double x2, double y2);
+/* the four domain edges as absorbing walls (evaluated by a box test) */
+int pic_walls_add_box(pic_walls* w);
+/* compile_walls() */
+int pic_walls_compile(pic_walls* w);
+int pic_walls_count(const pic_walls* w, int* n);
+int pic_walls_destroy(pic_walls* w);
+
+/* ---- multigrid: Multigrid.hpp:3-54, Multigrid.cu:30-642 ------------------ */
+typedef struct {
+    int levels;          /* 0: coarsen to a 3x3 grid */
+    int pre, post;       /* sweeps per level, 0..3 */
+    int coarse_sweeps;   /* sweeps on the coarsest level */
+    double omega;        /* SOR factor */
+    double tol;          /* relative residual ||r||/||rhs|| */
+    int max_cycles;
+    int mode;            /* PIC_MG_FP64 or PIC_MG_MIXED */
+} pic_mg_params;
+
+typedef struct {
+    int cycles;
+    double rel_residual;
+    double rhs_norm;
+    double ms;           /* wall time of the solve */
+    int launches;        /* kernel launches of the solve */
+} pic_mg_stats;
+
+int pic_mg_default_params(int preset, pic_mg_params* out);
+/* explicit solver context (replaces the process-global pool, Multigrid.cu:30-187) */
+int pic_mg_create(int n, const pic_mg_params* params, pic_mg** out);
+/* solve lap(phi) = rhs; phi_d in/out (warm start, boundary kept), device pointers */
+int pic_mg_solve(pic_mg* mg, double* phi_d, const double* rhs_d, double h, pic_mg_stats* st);
+int pic_mg_destroy(pic_mg* mg);
+/* solve_multigrid(phi, rho, nx, ny, levels, h, omega) Multigrid.hpp:3-11 with the
+   reference's other settings (3/3/20 sweeps, tol 1e-5, 50 cycles, FP64) */
+int pic_solve_multigrid(double* phi_d, double* rho_d, int nx, int ny, int levels, double h, double omega);
+
+/* ---- cross sections and MCC: mcc.hpp:6-21, mcc.cu:17-150 ----------------- */
+/* read_cross_section(filename) mcc.cu:17-41: "E sigma" pairs, '#' comments.
+   energies/sigmas may be NULL to query the count. */
+int pic_read_cross_section(const char* path, int cap, double* energies, double* sigmas, int* n);
+/* background gas: density (m^-3) and temperature (eV) */
+int pic_xs_create(double gas_density, double gas_temperature_eV, pic_xs** out);
+/* tabulated process (energies eV ascending, sigma m^2); product_species receives
+   the ion of an ionization (ignored otherwise) */
+int pic_xs_add_process(pic_xs* xs, int species, int type, double threshold_eV, int n, const double* energy_eV,
+                       const double* sigma_m2, int product_species);
+/* analytic process: constant sigma (elastic, charge exchange) or the Lotz-like
+   shape sigma = peak * e * ln(E/thr)/(E/thr) above thr (excitation, ionization) */
+int pic_xs_add_analytic(pic_xs* xs, int species, int type, double threshold_eV, double peak_m2,
+                        int product_species);
+/* uniform energy table the processes are sampled on (default 2048 points, 200 eV) */
+int pic_xs_set_table(pic_xs* xs, int points, double emax_eV);
+int pic_xs_destroy(pic_xs* xs);
+/* elasticCollisions(particles, density, dt, xs) mcc.cu:114-150: one null-collision
+   elastic pass over a species with a tabulated sigma, keyed on (id, step) */
+int pic_elastic_collisions(pic_particles* p, int species, double density, double dt, int n,
+                           const double* energy_eV, const double* sigma_m2, uint32_t seed, uint32_t step);
+
+/* ---- step driver: updateSimulationCUDA main.cu:37-140 -------------------- */
+typedef struct {
+    double dt;           /* s */
+    double eps0;         /* F/m */
+    uint32_t seed;
+    uint32_t case_id;
+    int bin_w, bin_h;    /* particle bin tile in cells */
+    int halo;            /* deposit window halo in nodes */
+    int mcc;             /* run MCC when a pic_xs is attached */
+    int rebin_every;     /* full re-binning period in steps (0: only on spill) */
+    double wall_eps;     /* parallel cut of the segment wall test (cell units^2) */
+    pic_mg_params mg;
+} pic_sim_params;
+
+typedef struct {
+    int64_t steps;
+    int64_t alive;
+    int last_mg_cycles;
+    double last_mg_rel;
+    int64_t total_mg_cycles;
+    int rebins;
+    double ms_push, ms_field, ms_total;
+} pic_sim_report;
+
+/* null-collision constants of a species (inputs of the oracle's MCC) */
+typedef struct {
+    int n_proc;
+    int K;
+    int type[4];
+    double threshold[4];
+    double inv_de;
+    double nu_max;
+    double p_null;
+    double efac;
+    double gas_vth;
+    int product_species;
+} pic_mcc_info;
+
+int pic_sim_default_params(pic_sim_params* out);
+int pic_sim_create(pic_particles* p, pic_grid* g, pic_walls* w, pic_xs* xs, const pic_sim_params* params,
+                   pic_sim** out);
+/* updateSimulationCUDA(particles, grid, boundaries, step, dt, steps, timing) */
+int pic_sim_step(pic_sim* s, int n_steps);
+/* complete the owed half of the last step (kick and collisions) */
+int pic_sim_sync(pic_sim* s);
+int pic_sim_counters(pic_sim* s, int species, uint64_t out[16]);
+int pic_sim_stats(pic_sim* s, pic_sim_report* out);
+int pic_sim_mcc_info(pic_sim* s, int species, pic_mcc_info* out, double* table, int cap);
+/* re-bin now with a new tile (0 keeps the current one) */
+int pic_sim_rebin(pic_sim* s, int bin_w, int bin_h);
+int pic_sim_destroy(pic_sim* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

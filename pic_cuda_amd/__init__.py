@@ -1,0 +1,499 @@
+"""pic_cuda_amd -- MI355X (gfx950) 2D electrostatic PIC time step.
+
+Python mirror of the reference's host interfaces, over the C ABI of
+``libpic_amd.so`` (declared in include/pic_amd.h):
+
+=======================  =====================================================
+this module              reference (dimad285/PIC_CUDA)
+=======================  =====================================================
+``Particles2D``          particles.cuh:3-39 / particles.cu:112-261
+``Grid2D``               grid.cuh:7-24 / grid.cu:10-53
+``Boundaries2D``         boundaries.h:7-23 / boundaries.cu:3-25
+``solve_multigrid``      Multigrid.hpp:3-11 / Multigrid.cu:573-642
+``Multigrid``            the solver context (Multigrid.cu:30-187 pool)
+``read_cross_section``   mcc.cu:17-41
+``CrossSections``        cross_sections, mcc.hpp:6-16
+``elasticCollisions``    mcc.cu:114-150
+``Simulation.step``      updateSimulationCUDA, main.cu:37-140
+=======================  =====================================================
+
+Positions cross the ABI in cell units; the Python classes accept domain units
+(metres) when ``units="domain"`` and convert with the grid spacing.
+
+The library is required: importing works without it, but every call raises
+``RuntimeError`` when ``libpic_amd.so`` is missing.  There is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpic_amd.so")
+_lib = None
+
+# constants mirrored from include/pic_amd.h
+PREC_FP32, PREC_FP64 = 4, 8
+ELASTIC, EXCITATION, IONIZATION, CHARGE_EXCHANGE = 0, 1, 2, 3
+FIELD_RHO, FIELD_PHI, FIELD_EX, FIELD_EY, FIELD_RHS = 0, 1, 2, 3, 4
+MG_FP64, MG_MIXED = 0, 1
+PRESET_REFERENCE, PRESET_STEP, PRESET_POISSON = 0, 1, 2
+CNT = dict(absorbed=0, null_candidates=1, process0=2, process1=3, process2=4, process3=5, born_self=6,
+           born_other=7, born_absorbed=8, numax_exceeded=9, reject_fallback=10, spilled=11, lost=12,
+           global_deposit=13)
+
+ELEMENTARY_CHARGE = 1.602176634e-19
+ELECTRON_MASS = 9.1093837015e-31
+AMU = 1.66053906660e-27
+EPS0 = 8.8541878128e-12
+
+
+class MGParams(ctypes.Structure):
+    _fields_ = [("levels", ctypes.c_int), ("pre", ctypes.c_int), ("post", ctypes.c_int),
+                ("coarse_sweeps", ctypes.c_int), ("omega", ctypes.c_double), ("tol", ctypes.c_double),
+                ("max_cycles", ctypes.c_int), ("mode", ctypes.c_int)]
+
+
+class MGStats(ctypes.Structure):
+    _fields_ = [("cycles", ctypes.c_int), ("rel_residual", ctypes.c_double), ("rhs_norm", ctypes.c_double),
+                ("ms", ctypes.c_double), ("launches", ctypes.c_int)]
+
+
+class SimParams(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("eps0", ctypes.c_double), ("seed", ctypes.c_uint32),
+                ("case_id", ctypes.c_uint32), ("bin_w", ctypes.c_int), ("bin_h", ctypes.c_int),
+                ("halo", ctypes.c_int), ("mcc", ctypes.c_int), ("rebin_every", ctypes.c_int),
+                ("wall_eps", ctypes.c_double), ("mg", MGParams)]
+
+
+class SimReport(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int64), ("alive", ctypes.c_int64), ("last_mg_cycles", ctypes.c_int),
+                ("last_mg_rel", ctypes.c_double), ("total_mg_cycles", ctypes.c_int64), ("rebins", ctypes.c_int),
+                ("ms_push", ctypes.c_double), ("ms_field", ctypes.c_double), ("ms_total", ctypes.c_double)]
+
+
+class MccInfo(ctypes.Structure):
+    _fields_ = [("n_proc", ctypes.c_int), ("K", ctypes.c_int), ("type", ctypes.c_int * 4),
+                ("threshold", ctypes.c_double * 4), ("inv_de", ctypes.c_double), ("nu_max", ctypes.c_double),
+                ("p_null", ctypes.c_double), ("efac", ctypes.c_double), ("gas_vth", ctypes.c_double),
+                ("product_species", ctypes.c_int)]
+
+
+# every symbol declared in include/pic_amd.h, with its ctypes signature
+_vp, _i, _d, _u32, _i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_uint32, ctypes.c_int64
+_P = ctypes.POINTER
+SIGNATURES = {
+    "pic_last_error": (ctypes.c_char_p, []),
+    "pic_version": (_i, [_P(_i), _P(_i)]),
+    "pic_device_synchronize": (_i, []),
+    "pic_particles_create": (_i, [_i64, _i, _P(_vp)]),
+    "pic_particles_destroy": (_i, [_vp]),
+    "pic_particles_add_species": (_i, [_vp, _d, _d, _P(_i)]),
+    "pic_particles_change_species": (_i, [_vp, _i, _d, _d]),
+    "pic_particles_set_weight": (_i, [_vp, _i, _d]),
+    "pic_particles_num_species": (_i, [_vp, _P(_i)]),
+    "pic_particles_last_alive": (_i, [_vp, _i, _P(_i64)]),
+    "pic_particles_upload": (_i, [_vp, _i, _i64, _vp, _vp, _vp, _vp, _vp]),
+    "pic_particles_download": (_i, [_vp, _i, _i64, _vp, _vp, _vp, _vp, _vp, _P(_i64)]),
+    "pic_particles_init_maxwellian": (_i, [_vp, _i, _i64, _i, _i, _d, _u32, _u32, _u32]),
+    "pic_particles_remove_collided": (_i, [_vp]),
+    "pic_grid_create": (_i, [_i, _i, _d, _d, _P(_vp)]),
+    "pic_grid_destroy": (_i, [_vp]),
+    "pic_grid_info": (_i, [_vp, _P(_i), _P(_i), _P(_d), _P(_d)]),
+    "pic_grid_device_ptrs": (_i, [_vp, _P(_vp), _P(_vp), _P(_vp)]),
+    "pic_grid_download": (_i, [_vp, _i, _vp]),
+    "pic_grid_upload": (_i, [_vp, _i, _vp]),
+    "pic_grid_compute_E": (_i, [_vp]),
+    "pic_walls_create": (_i, [_d, _d, _i, _i, _P(_vp)]),
+    "pic_walls_add": (_i, [_vp, _d, _d, _d, _d]),
+    "pic_walls_add_box": (_i, [_vp]),
+    "pic_walls_compile": (_i, [_vp]),
+    "pic_walls_count": (_i, [_vp, _P(_i)]),
+    "pic_walls_destroy": (_i, [_vp]),
+    "pic_mg_default_params": (_i, [_i, _P(MGParams)]),
+    "pic_mg_create": (_i, [_i, _P(MGParams), _P(_vp)]),
+    "pic_mg_solve": (_i, [_vp, _vp, _vp, _d, _P(MGStats)]),
+    "pic_mg_destroy": (_i, [_vp]),
+    "pic_solve_multigrid": (_i, [_vp, _vp, _i, _i, _i, _d, _d]),
+    "pic_read_cross_section": (_i, [ctypes.c_char_p, _i, _vp, _vp, _P(_i)]),
+    "pic_xs_create": (_i, [_d, _d, _P(_vp)]),
+    "pic_xs_add_process": (_i, [_vp, _i, _i, _d, _i, _vp, _vp, _i]),
+    "pic_xs_add_analytic": (_i, [_vp, _i, _i, _d, _d, _i]),
+    "pic_xs_set_table": (_i, [_vp, _i, _d]),
+    "pic_xs_destroy": (_i, [_vp]),
+    "pic_elastic_collisions": (_i, [_vp, _i, _d, _d, _i, _vp, _vp, _u32, _u32]),
+    "pic_sim_default_params": (_i, [_P(SimParams)]),
+    "pic_sim_create": (_i, [_vp, _vp, _vp, _vp, _P(SimParams), _P(_vp)]),
+    "pic_sim_step": (_i, [_vp, _i]),
+    "pic_sim_sync": (_i, [_vp]),
+    "pic_sim_counters": (_i, [_vp, _i, _vp]),
+    "pic_sim_stats": (_i, [_vp, _P(SimReport)]),
+    "pic_sim_mcc_info": (_i, [_vp, _i, _P(MccInfo), _vp, _i]),
+    "pic_sim_rebin": (_i, [_vp, _i, _i]),
+    "pic_sim_destroy": (_i, [_vp]),
+}
+
+
+def load(path=None):
+    """Load libpic_amd.so (raises if it is missing: there is no fallback)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError("libpic_amd.so not found at %s: run __graft_entry__.build() "
+                           "(make -C pic_cuda_amd/csrc)" % p)
+    lib = ctypes.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+class PicError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc != 0:
+        msg = load().pic_last_error()
+        raise PicError("pic error %d: %s" % (rc, msg.decode() if msg else ""))
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _devptr(t):
+    """Device pointer of a torch tensor, an int address, or None."""
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return ctypes.c_void_p(t)
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def synchronize():
+    _check(load().pic_device_synchronize())
+
+
+# ---------------------------------------------------------------------------
+class Particles2D:
+    """Particles2D(max_particles) -- particles.cu:112-150.
+
+    Species 0 exists with m = 1, q = 0.1 like the reference constructor.
+    ``max_particles`` is the capacity of each species."""
+
+    def __init__(self, max_particles, precision=PREC_FP32):
+        self._h = ctypes.c_void_p()
+        _check(load().pic_particles_create(int(max_particles), int(precision), ctypes.byref(self._h)))
+        self.precision = precision
+        self.max_particles = int(max_particles)
+        self.grid = None
+
+    def close(self):
+        if self._h:
+            _check(load().pic_particles_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            if self._h and _lib is not None:
+                _lib.pic_particles_destroy(self._h)
+        except Exception:
+            pass
+
+    def add_species(self, m, q):
+        """add_species(double m, double q) -- argument order (m, q), particles.cu:174."""
+        idx = ctypes.c_int()
+        _check(load().pic_particles_add_species(self._h, float(m), float(q), ctypes.byref(idx)))
+        return idx.value
+
+    def change_species(self, species, m, q):
+        _check(load().pic_particles_change_species(self._h, int(species), float(m), float(q)))
+
+    def set_weight(self, species, w):
+        _check(load().pic_particles_set_weight(self._h, int(species), float(w)))
+
+    @property
+    def species_num(self):
+        n = ctypes.c_int()
+        _check(load().pic_particles_num_species(self._h, ctypes.byref(n)))
+        return n.value
+
+    def count(self, species=-1):
+        n = ctypes.c_int64()
+        _check(load().pic_particles_last_alive(self._h, int(species), ctypes.byref(n)))
+        return n.value
+
+    @property
+    def last_alive(self):
+        return self.count(-1)
+
+    def _scale(self, units):
+        if units == "cell":
+            return 1.0, 1.0
+        if self.grid is None:
+            raise ValueError("domain units need the grid: set particles.grid")
+        return self.grid.dx, self.grid.dy
+
+    def upload(self, species, x, y, vx, vy, ids=None, units="cell"):
+        sx, sy = self._scale(units)
+        x = np.ascontiguousarray(np.asarray(x, dtype=np.float64) / sx)
+        y = np.ascontiguousarray(np.asarray(y, dtype=np.float64) / sy)
+        vx = np.ascontiguousarray(vx, dtype=np.float64)
+        vy = np.ascontiguousarray(vy, dtype=np.float64)
+        idp = None
+        if ids is not None:
+            ids = np.ascontiguousarray(ids, dtype=np.uint32)
+            idp = _ptr(ids)
+        _check(load().pic_particles_upload(self._h, int(species), len(x), _ptr(x), _ptr(y), _ptr(vx), _ptr(vy),
+                                           idp))
+
+    def download(self, species, units="cell"):
+        n = ctypes.c_int64()
+        _check(load().pic_particles_download(self._h, int(species), 0, None, None, None, None, None,
+                                             ctypes.byref(n)))
+        cap = n.value + 1024
+        x, y, vx, vy = (np.zeros(cap) for _ in range(4))
+        ids = np.zeros(cap, dtype=np.uint32)
+        _check(load().pic_particles_download(self._h, int(species), cap, _ptr(x), _ptr(y), _ptr(vx), _ptr(vy),
+                                             _ptr(ids), ctypes.byref(n)))
+        k = n.value
+        sx, sy = self._scale(units)
+        return x[:k] * sx, y[:k] * sy, vx[:k], vy[:k], ids[:k]
+
+    def init_maxwellian(self, species, n, nx, ny, vth, seed=1, case_id=0, id0=0):
+        _check(load().pic_particles_init_maxwellian(self._h, int(species), int(n), int(nx), int(ny), float(vth),
+                                                    int(seed), int(case_id), int(id0)))
+
+    def removeCollidedParticles(self):
+        _check(load().pic_particles_remove_collided(self._h))
+
+
+class Grid2D:
+    """Grid2D(M, N, X, Y) -- grid.cu:10-20 (dx = X/(M-1)); fields start at zero."""
+
+    def __init__(self, M, N, X, Y):
+        self._h = ctypes.c_void_p()
+        _check(load().pic_grid_create(int(M), int(N), float(X), float(Y), ctypes.byref(self._h)))
+        m, n, dx, dy = ctypes.c_int(), ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        _check(load().pic_grid_info(self._h, ctypes.byref(m), ctypes.byref(n), ctypes.byref(dx), ctypes.byref(dy)))
+        self.m, self.n, self.dx, self.dy = m.value, n.value, dx.value, dy.value
+        self.x, self.y = float(X), float(Y)
+
+    def __del__(self):
+        try:
+            if self._h and _lib is not None:
+                _lib.pic_grid_destroy(self._h)
+        except Exception:
+            pass
+
+    def device_ptrs(self):
+        r, p, e = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        _check(load().pic_grid_device_ptrs(self._h, ctypes.byref(r), ctypes.byref(p), ctypes.byref(e)))
+        return r.value, p.value, e.value
+
+    def download(self, field):
+        out = np.zeros((self.n, self.m))
+        _check(load().pic_grid_download(self._h, int(field), _ptr(out)))
+        return out
+
+    def upload(self, field, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(self.n, self.m)
+        _check(load().pic_grid_upload(self._h, int(field), _ptr(a)))
+
+    @property
+    def rho(self):
+        return self.download(FIELD_RHO)
+
+    @property
+    def phi(self):
+        return self.download(FIELD_PHI)
+
+    def computeElectricField(self):
+        """computeElectricField (kernels.cu:134-191) into the E member."""
+        _check(load().pic_grid_compute_E(self._h))
+        return self.download(FIELD_EX), self.download(FIELD_EY)
+
+
+class Boundaries2D:
+    """Boundaries2D(width, height, nx, ny) -- boundaries.h:7-23."""
+
+    def __init__(self, width, height, nx, ny):
+        self._h = ctypes.c_void_p()
+        _check(load().pic_walls_create(float(width), float(height), int(nx), int(ny), ctypes.byref(self._h)))
+
+    def __del__(self):
+        try:
+            if self._h and _lib is not None:
+                _lib.pic_walls_destroy(self._h)
+        except Exception:
+            pass
+
+    def add_wall(self, x1, y1, x2, y2):
+        _check(load().pic_walls_add(self._h, float(x1), float(y1), float(x2), float(y2)))
+
+    def add_box(self):
+        _check(load().pic_walls_add_box(self._h))
+
+    def compile_walls(self):
+        _check(load().pic_walls_compile(self._h))
+
+    @property
+    def wall_count(self):
+        n = ctypes.c_int()
+        _check(load().pic_walls_count(self._h, ctypes.byref(n)))
+        return n.value
+
+
+def mg_params(preset=PRESET_STEP, **over):
+    p = MGParams()
+    _check(load().pic_mg_default_params(int(preset), ctypes.byref(p)))
+    for k, v in over.items():
+        setattr(p, k, v)
+    return p
+
+
+class Multigrid:
+    """Multigrid solver context for an n x n grid (replaces the global pool)."""
+
+    def __init__(self, n, params=None, **over):
+        self._h = ctypes.c_void_p()
+        self.params = params if params is not None else mg_params(PRESET_STEP, **over)
+        _check(load().pic_mg_create(int(n), ctypes.byref(self.params), ctypes.byref(self._h)))
+        self.n = n
+
+    def __del__(self):
+        try:
+            if self._h and _lib is not None:
+                _lib.pic_mg_destroy(self._h)
+        except Exception:
+            pass
+
+    def solve(self, phi_dev, rhs_dev, h):
+        st = MGStats()
+        _check(load().pic_mg_solve(self._h, _devptr(phi_dev), _devptr(rhs_dev), float(h), ctypes.byref(st)))
+        return st
+
+
+def solve_multigrid(phi_d, rho_d, nx, ny, levels, h, omega=1.25):
+    """solve_multigrid(phi, rho, nx, ny, levels, h, omega) -- Multigrid.hpp:3-11 (device tensors)."""
+    _check(load().pic_solve_multigrid(_devptr(phi_d), _devptr(rho_d), int(nx), int(ny), int(levels), float(h),
+                                      float(omega)))
+
+
+def read_cross_section(path):
+    """read_cross_section(filename) -- mcc.cu:17-41; returns (energies, sigmas)."""
+    n = ctypes.c_int()
+    _check(load().pic_read_cross_section(path.encode(), 0, None, None, ctypes.byref(n)))
+    e = np.zeros(n.value)
+    s = np.zeros(n.value)
+    _check(load().pic_read_cross_section(path.encode(), n.value, _ptr(e), _ptr(s), ctypes.byref(n)))
+    return e, s
+
+
+class CrossSections:
+    """Background gas and collision processes (cross_sections, mcc.hpp:6-16)."""
+
+    def __init__(self, gas_density, gas_temperature_eV, points=2048, emax_eV=200.0):
+        self._h = ctypes.c_void_p()
+        _check(load().pic_xs_create(float(gas_density), float(gas_temperature_eV), ctypes.byref(self._h)))
+        _check(load().pic_xs_set_table(self._h, int(points), float(emax_eV)))
+
+    def __del__(self):
+        try:
+            if self._h and _lib is not None:
+                _lib.pic_xs_destroy(self._h)
+        except Exception:
+            pass
+
+    def add_process(self, species, ptype, threshold_eV, energies, sigmas, product_species=-1):
+        e = np.ascontiguousarray(energies, dtype=np.float64)
+        s = np.ascontiguousarray(sigmas, dtype=np.float64)
+        _check(load().pic_xs_add_process(self._h, int(species), int(ptype), float(threshold_eV), len(e), _ptr(e),
+                                         _ptr(s), int(product_species)))
+
+    def add_analytic(self, species, ptype, threshold_eV, peak_m2, product_species=-1):
+        _check(load().pic_xs_add_analytic(self._h, int(species), int(ptype), float(threshold_eV), float(peak_m2),
+                                          int(product_species)))
+
+
+def elasticCollisions(particles, species, density, dt, energies, sigmas, seed=1, step=0):
+    """elasticCollisions(particles, density, dt, xs) -- mcc.cu:114-150 (null-collision, Philox)."""
+    e = np.ascontiguousarray(energies, dtype=np.float64)
+    s = np.ascontiguousarray(sigmas, dtype=np.float64)
+    _check(load().pic_elastic_collisions(particles._h, int(species), float(density), float(dt), len(e), _ptr(e),
+                                         _ptr(s), int(seed), int(step)))
+
+
+def sim_params(**over):
+    p = SimParams()
+    _check(load().pic_sim_default_params(ctypes.byref(p)))
+    for k, v in over.items():
+        if k == "mg":
+            p.mg = v
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class Simulation:
+    """The step driver (updateSimulationCUDA, main.cu:37-140) bound to its objects."""
+
+    def __init__(self, particles, grid, walls=None, xs=None, params=None, **over):
+        self.params = params if params is not None else sim_params(**over)
+        self._h = ctypes.c_void_p()
+        self.particles, self.grid, self.walls, self.xs = particles, grid, walls, xs
+        particles.grid = grid
+        _check(load().pic_sim_create(particles._h, grid._h, walls._h if walls else None, xs._h if xs else None,
+                                     ctypes.byref(self.params), ctypes.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            _check(load().pic_sim_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            if self._h and _lib is not None:
+                _lib.pic_sim_destroy(self._h)
+        except Exception:
+            pass
+
+    def step(self, n=1):
+        _check(load().pic_sim_step(self._h, int(n)))
+
+    def sync(self):
+        _check(load().pic_sim_sync(self._h))
+
+    def counters(self, species):
+        out = np.zeros(16, dtype=np.uint64)
+        _check(load().pic_sim_counters(self._h, int(species), _ptr(out)))
+        return out
+
+    def stats(self):
+        r = SimReport()
+        _check(load().pic_sim_stats(self._h, ctypes.byref(r)))
+        return r
+
+    def mcc_info(self, species):
+        info = MccInfo()
+        _check(load().pic_sim_mcc_info(self._h, int(species), ctypes.byref(info), None, 0))
+        tab = np.zeros(max(1, info.n_proc * info.K))
+        _check(load().pic_sim_mcc_info(self._h, int(species), ctypes.byref(info), _ptr(tab), tab.size))
+        return dict(n_proc=info.n_proc, K=info.K, type=list(info.type)[:info.n_proc],
+                    threshold=list(info.threshold)[:info.n_proc], inv_de=info.inv_de, nu_max=info.nu_max,
+                    p_null=info.p_null, efac=info.efac, gas_vth=info.gas_vth,
+                    product_species=info.product_species, table=tab[:info.n_proc * info.K])
+
+    def rebin(self, bin_w=0, bin_h=0):
+        _check(load().pic_sim_rebin(self._h, int(bin_w), int(bin_h)))
+
+
+def updateSimulation(sim, steps):
+    """updateSimulationCUDA(particles, grid, boundaries, step, dt, steps, timing)."""
+    sim.step(steps)
+    return sim.stats()

@@ -1,0 +1,1320 @@
+// capi.cpp -- host side of the C ABI declared in include/pic_amd.h.
+//
+// Objects mirror the reference's host interfaces (Particles2D, Grid2D,
+// Boundaries2D, solve_multigrid, cross_sections/elasticCollisions and the step
+// driver updateSimulationCUDA) and own their device memory.  All work runs in
+// order on one HIP stream of the library.  Errors are returned as status codes.
+#include "objects.hpp"
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <tuple>
+
+namespace pic {
+
+static thread_local std::string t_err;
+void set_error(const std::string& msg) { t_err = msg; }
+const char* last_error() { return t_err.c_str(); }
+
+hipStream_t lib_stream() {
+    static hipStream_t s = nullptr;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        if (hipStreamCreate(&s) != hipSuccess) s = nullptr;
+    });
+    return s;
+}
+
+template <typename T> CaseDev<T>& hcase(pic_particles* p);
+template <> CaseDev<float>& hcase<float>(pic_particles* p) { return p->hf; }
+template <> CaseDev<double>& hcase<double>(pic_particles* p) { return p->hd; }
+
+static int upload_case(pic_particles* p) {
+    hipStream_t s = lib_stream();
+    if (p->prec == 4) PIC_HIP(hipMemcpyAsync(p->d_case, &p->hf, sizeof(p->hf), hipMemcpyHostToDevice, s));
+    else PIC_HIP(hipMemcpyAsync(p->d_case, &p->hd, sizeof(p->hd), hipMemcpyHostToDevice, s));
+    PIC_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+template <typename T>
+static void fill_species_ptrs(pic_particles* p, int s) {
+    SpeciesDev<T>& S = hcase<T>(p).sp[s];
+    SpeciesStore& st = p->sp[s];
+    for (int par = 0; par < 2; ++par) {
+        S.x[par] = (T*)st.x[par];
+        S.y[par] = (T*)st.y[par];
+        S.vx[par] = (T*)st.vx[par];
+        S.vy[par] = (T*)st.vy[par];
+        S.id[par] = st.id[par];
+        S.start[par] = st.d_start[par];
+        S.cap[par] = st.d_cap[par];
+        S.count[par] = st.d_count[par];
+        S.nb[par] = st.nb;
+    }
+    S.counters = st.d_counters;
+    S.key = (uint32_t)s;
+    S.other = -1;
+}
+
+static int ensure_scratch(pic_particles* p, int n) {
+    if (p->scratch_alloc >= n) return 0;
+    if (p->d_scratch) PIC_HIP(hipFree(p->d_scratch));
+    p->d_scratch = nullptr;
+    PIC_HIP(hipMalloc(&p->d_scratch, (size_t)n * sizeof(int32_t)));
+    p->scratch_alloc = n;
+    return 0;
+}
+
+static int upload_items(WorkItem** dptr, int* alloc, const std::vector<WorkItem>& items) {
+    if ((int)items.size() > *alloc) {
+        if (*dptr) PIC_HIP(hipFree(*dptr));
+        *dptr = nullptr;
+        const int n = std::max<int>((int)items.size(), 64);
+        PIC_HIP(hipMalloc(dptr, (size_t)n * sizeof(WorkItem)));
+        *alloc = n;
+    }
+    if (!items.empty()) {
+        PIC_HIP(hipMemcpyAsync(*dptr, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice,
+                               lib_stream()));
+        PIC_HIP(hipStreamSynchronize(lib_stream()));
+    }
+    return 0;
+}
+
+static void items_for_species(const pic_particles* p, int s, int chunk, std::vector<WorkItem>& out) {
+    const SpeciesStore& st = p->sp[s];
+    for (int b = 0; b < st.nb; ++b) out.push_back(WorkItem{0, s, b, 0});
+    const int spill_cap = st.h_cap[st.nb];
+    const int chunks = std::max(1, (spill_cap + chunk - 1) / chunk);
+    for (int k = 0; k < chunks; ++k) out.push_back(WorkItem{0, s, st.nb, k});
+}
+
+static int alloc_layout_parity(SpeciesStore& st, int par, int entries) {
+    if (st.d_start[par]) PIC_HIP(hipFree(st.d_start[par]));
+    if (st.d_cap[par]) PIC_HIP(hipFree(st.d_cap[par]));
+    if (st.d_count[par]) PIC_HIP(hipFree(st.d_count[par]));
+    st.d_start[par] = st.d_cap[par] = st.d_count[par] = nullptr;
+    PIC_HIP(hipMalloc(&st.d_start[par], (size_t)entries * sizeof(int32_t)));
+    PIC_HIP(hipMalloc(&st.d_cap[par], (size_t)entries * sizeof(int32_t)));
+    PIC_HIP(hipMalloc(&st.d_count[par], (size_t)entries * sizeof(int32_t)));
+    PIC_HIP(hipMemset(st.d_count[par], 0, (size_t)entries * sizeof(int32_t)));
+    return 0;
+}
+
+static int create_species(pic_particles* p, double m, double q) {
+    PIC_REQUIRE((int)p->sp.size() < PIC_MAX_SPECIES, "at most 4 species");
+    p->sp.emplace_back();
+    SpeciesStore& st = p->sp.back();
+    st.m = m;
+    st.q = q;
+    st.w = 1.0;
+    const int64_t cap = p->max_per_species + p->max_per_species / 4 + 65536;
+    PIC_REQUIRE(cap < (int64_t)INT32_MAX, "capacity per species must stay below 2^31 slots");
+    st.capacity = cap;
+    const size_t eb = (size_t)p->prec;
+    for (int par = 0; par < 2; ++par) {
+        PIC_HIP(hipMalloc(&st.x[par], cap * eb));
+        PIC_HIP(hipMalloc(&st.y[par], cap * eb));
+        PIC_HIP(hipMalloc(&st.vx[par], cap * eb));
+        PIC_HIP(hipMalloc(&st.vy[par], cap * eb));
+        PIC_HIP(hipMalloc(&st.id[par], cap * sizeof(uint32_t)));
+        if (int rc = alloc_layout_parity(st, par, 1)) return rc;
+    }
+    st.layout_alloc = 1;
+    st.nb = 0;
+    st.h_start.assign(1, 0);
+    st.h_cap.assign(1, (int32_t)cap);
+    for (int par = 0; par < 2; ++par) {
+        PIC_HIP(hipMemcpy(st.d_start[par], st.h_start.data(), sizeof(int32_t), hipMemcpyHostToDevice));
+        PIC_HIP(hipMemcpy(st.d_cap[par], st.h_cap.data(), sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    PIC_HIP(hipMalloc(&st.d_counters, PIC_NCOUNT * sizeof(unsigned long long)));
+    PIC_HIP(hipMemset(st.d_counters, 0, PIC_NCOUNT * sizeof(unsigned long long)));
+    const int s = (int)p->sp.size() - 1;
+    if (p->prec == 4) {
+        fill_species_ptrs<float>(p, s);
+        p->hf.ns = (int)p->sp.size();
+    } else {
+        fill_species_ptrs<double>(p, s);
+        p->hd.ns = (int)p->sp.size();
+    }
+    return upload_case(p);
+}
+
+static int read_count(pic_particles* p, int s, int par, int b, int32_t* out) {
+    PIC_HIP(hipMemcpyAsync(out, p->sp[s].d_count[par] + b, sizeof(int32_t), hipMemcpyDeviceToHost, lib_stream()));
+    PIC_HIP(hipStreamSynchronize(lib_stream()));
+    return 0;
+}
+
+}  // namespace pic
+
+using namespace pic;
+
+// forward declarations of the sim helpers used by the particle API
+static int sim_sync_impl(pic_sim* s);
+
+// ============================================================================
+// library
+// ============================================================================
+extern "C" const char* pic_last_error(void) { return pic::last_error(); }
+
+extern "C" int pic_version(int* major, int* minor) {
+    if (major) *major = 1;
+    if (minor) *minor = 0;
+    return 0;
+}
+
+extern "C" int pic_device_synchronize(void) {
+    PIC_HIP(hipDeviceSynchronize());
+    return 0;
+}
+
+// ============================================================================
+// Particles2D
+// ============================================================================
+extern "C" int pic_particles_create(int64_t max_particles, int precision, pic_particles** out) {
+    PIC_REQUIRE(out, "out is NULL");
+    PIC_REQUIRE(max_particles > 0, "max_particles must be positive");
+    PIC_REQUIRE(precision == 4 || precision == 8, "precision must be PIC_PREC_FP32 or PIC_PREC_FP64");
+    PIC_REQUIRE(lib_stream() != nullptr, "could not create a HIP stream");
+    auto* p = new pic_particles();
+    p->prec = precision;
+    p->max_per_species = max_particles;
+    if (hipMalloc(&p->d_case, precision == 4 ? sizeof(CaseDev<float>) : sizeof(CaseDev<double>)) != hipSuccess ||
+        hipMalloc(&p->d_state, sizeof(CaseState)) != hipSuccess || hipMalloc(&p->d_ll, sizeof(long long)) != hipSuccess) {
+        set_error("pic_particles_create: device allocation failed");
+        delete p;
+        return PIC_ERR_HIP;
+    }
+    CaseState cs{0u, 0, 0, 0};
+    if (hipMemcpy(p->d_state, &cs, sizeof(cs), hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("pic_particles_create: state upload failed");
+        return PIC_ERR_HIP;
+    }
+    // reference constructor: species 0 with q = 0.1, m = 1 (particles.cu:146-149)
+    if (int rc = create_species(p, 1.0, 0.1)) {
+        pic_particles_destroy(p);
+        return rc;
+    }
+    *out = p;
+    return 0;
+}
+
+extern "C" int pic_particles_destroy(pic_particles* p) {
+    if (!p) return 0;
+    PIC_REQUIRE(!p->attached, "destroy the pic_sim before its particles");
+    (void)hipStreamSynchronize(lib_stream());
+    for (auto& st : p->sp) {
+        for (int par = 0; par < 2; ++par) {
+            (void)hipFree(st.x[par]);
+            (void)hipFree(st.y[par]);
+            (void)hipFree(st.vx[par]);
+            (void)hipFree(st.vy[par]);
+            (void)hipFree(st.id[par]);
+            (void)hipFree(st.d_start[par]);
+            (void)hipFree(st.d_cap[par]);
+            (void)hipFree(st.d_count[par]);
+        }
+        (void)hipFree(st.d_counters);
+    }
+    (void)hipFree(p->d_case);
+    (void)hipFree(p->d_state);
+    (void)hipFree(p->d_items);
+    (void)hipFree(p->d_scratch);
+    (void)hipFree(p->d_ll);
+    delete p;
+    return 0;
+}
+
+extern "C" int pic_particles_add_species(pic_particles* p, double m, double q, int* index) {
+    PIC_REQUIRE(p, "particles is NULL");
+    PIC_REQUIRE(!p->attached, "species cannot be added while a pic_sim uses the particles");
+    PIC_REQUIRE(m > 0, "mass must be positive");
+    if (int rc = create_species(p, m, q)) return rc;
+    if (index) *index = (int)p->sp.size() - 1;
+    return 0;
+}
+
+extern "C" int pic_particles_change_species(pic_particles* p, int species, double m, double q) {
+    PIC_REQUIRE(p && species >= 0 && species < (int)p->sp.size(), "bad species");
+    PIC_REQUIRE(!p->attached, "species cannot change while a pic_sim uses the particles");
+    PIC_REQUIRE(m > 0, "mass must be positive");
+    p->sp[species].m = m;
+    p->sp[species].q = q;
+    return 0;
+}
+
+extern "C" int pic_particles_set_weight(pic_particles* p, int species, double weight) {
+    PIC_REQUIRE(p && species >= 0 && species < (int)p->sp.size(), "bad species");
+    PIC_REQUIRE(!p->attached, "weights cannot change while a pic_sim uses the particles");
+    p->sp[species].w = weight;
+    return 0;
+}
+
+extern "C" int pic_particles_num_species(const pic_particles* p, int* n) {
+    PIC_REQUIRE(p && n, "NULL argument");
+    *n = (int)p->sp.size();
+    return 0;
+}
+
+extern "C" int pic_particles_last_alive(pic_particles* p, int species, int64_t* n) {
+    PIC_REQUIRE(p && n, "NULL argument");
+    PIC_REQUIRE(species >= -1 && species < (int)p->sp.size(), "bad species");
+    int64_t tot = 0;
+    for (int s = 0; s < (int)p->sp.size(); ++s) {
+        if (species >= 0 && s != species) continue;
+        int rc = p->prec == 4 ? launch_counts<float>((CaseDev<float>*)p->d_case, p->d_state, 0, s, p->d_ll, lib_stream())
+                              : launch_counts<double>((CaseDev<double>*)p->d_case, p->d_state, 0, s, p->d_ll, lib_stream());
+        if (rc) return rc;
+        long long v = 0;
+        PIC_HIP(hipMemcpyAsync(&v, p->d_ll, sizeof(v), hipMemcpyDeviceToHost, lib_stream()));
+        PIC_HIP(hipStreamSynchronize(lib_stream()));
+        tot += v;
+    }
+    *n = tot;
+    return 0;
+}
+
+template <typename T>
+static int upload_t(pic_particles* p, int s, int64_t n, const double* x, const double* y, const double* vx,
+                    const double* vy, const uint32_t* id) {
+    SpeciesStore& st = p->sp[s];
+    const int par = p->h_cur, b = st.nb;
+    int32_t cnt = 0;
+    if (int rc = read_count(p, s, par, b, &cnt)) return rc;
+    cnt = std::min(cnt, st.h_cap[b]);
+    if ((int64_t)cnt + n > st.h_cap[b]) {
+        set_error("pic_particles_upload: not enough free slots (capacity per species exceeded)");
+        return PIC_ERR_CAPACITY;
+    }
+    std::vector<T> bx(n), by(n), bvx(n), bvy(n);
+    std::vector<uint32_t> bid(n);
+    // ids continue after the particles already held when none are given
+    for (int64_t i = 0; i < n; ++i) {
+        bx[i] = (T)x[i];
+        by[i] = (T)y[i];
+        bvx[i] = (T)vx[i];
+        bvy[i] = (T)vy[i];
+        bid[i] = id ? id[i] : (uint32_t)(cnt + i);
+    }
+    const size_t off = (size_t)st.h_start[b] + cnt;
+    hipStream_t sm = lib_stream();
+    PIC_HIP(hipMemcpyAsync((T*)st.x[par] + off, bx.data(), n * sizeof(T), hipMemcpyHostToDevice, sm));
+    PIC_HIP(hipMemcpyAsync((T*)st.y[par] + off, by.data(), n * sizeof(T), hipMemcpyHostToDevice, sm));
+    PIC_HIP(hipMemcpyAsync((T*)st.vx[par] + off, bvx.data(), n * sizeof(T), hipMemcpyHostToDevice, sm));
+    PIC_HIP(hipMemcpyAsync((T*)st.vy[par] + off, bvy.data(), n * sizeof(T), hipMemcpyHostToDevice, sm));
+    PIC_HIP(hipMemcpyAsync(st.id[par] + off, bid.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, sm));
+    const int32_t nc = cnt + (int32_t)n;
+    PIC_HIP(hipMemcpyAsync(st.d_count[par] + b, &nc, sizeof(nc), hipMemcpyHostToDevice, sm));
+    PIC_HIP(hipStreamSynchronize(sm));
+    return 0;
+}
+
+extern "C" int pic_particles_upload(pic_particles* p, int species, int64_t n, const double* x, const double* y,
+                                    const double* vx, const double* vy, const uint32_t* id) {
+    PIC_REQUIRE(p && species >= 0 && species < (int)p->sp.size(), "bad species");
+    PIC_REQUIRE(n >= 0, "n must be non-negative");
+    if (n == 0) return 0;
+    PIC_REQUIRE(x && y && vx && vy, "NULL array");
+    if (p->attached) {
+        if (int rc = sim_sync_impl(p->attached)) return rc;
+    }
+    return p->prec == 4 ? upload_t<float>(p, species, n, x, y, vx, vy, id)
+                        : upload_t<double>(p, species, n, x, y, vx, vy, id);
+}
+
+template <typename T>
+static int download_t(pic_particles* p, int s, int64_t cap, double* x, double* y, double* vx, double* vy,
+                      uint32_t* id, int64_t* nout) {
+    SpeciesStore& st = p->sp[s];
+    std::vector<WorkItem> items;
+    items_for_species(p, s, 4096, items);
+    if (int rc = upload_items(&p->d_items, &p->items_alloc, items)) return rc;
+    if (int rc = ensure_scratch(p, 1)) return rc;
+    hipStream_t sm = lib_stream();
+    PIC_HIP(hipMemsetAsync(p->d_scratch, 0, sizeof(int32_t), sm));
+    if (int rc = launch_gather<T>((CaseDev<T>*)p->d_case, p->d_state, p->d_items, (int)items.size(), 4096,
+                                  p->d_scratch, sm))
+        return rc;
+    int32_t n = 0;
+    PIC_HIP(hipMemcpyAsync(&n, p->d_scratch, sizeof(n), hipMemcpyDeviceToHost, sm));
+    PIC_HIP(hipStreamSynchronize(sm));
+    if (nout) *nout = n;
+    if (!x) return 0;   // count query
+    if (n > cap) {
+        set_error("pic_particles_download: output capacity too small");
+        return PIC_ERR_CAPACITY;
+    }
+    const int par = p->h_cur ^ 1;
+    std::vector<T> b(n);
+    void* src[4] = {st.x[par], st.y[par], st.vx[par], st.vy[par]};
+    double* dst[4] = {x, y, vx, vy};
+    for (int k = 0; k < 4; ++k) {
+        if (!dst[k]) continue;
+        PIC_HIP(hipMemcpy(b.data(), src[k], (size_t)n * sizeof(T), hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i) dst[k][i] = (double)b[i];
+    }
+    if (id) PIC_HIP(hipMemcpy(id, st.id[par], (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int pic_particles_download(pic_particles* p, int species, int64_t cap, double* x, double* y, double* vx,
+                                      double* vy, uint32_t* id, int64_t* n) {
+    PIC_REQUIRE(p && species >= 0 && species < (int)p->sp.size(), "bad species");
+    if (p->attached) {
+        if (int rc = sim_sync_impl(p->attached)) return rc;
+    }
+    return p->prec == 4 ? download_t<float>(p, species, cap, x, y, vx, vy, id, n)
+                        : download_t<double>(p, species, cap, x, y, vx, vy, id, n);
+}
+
+template <typename T>
+static int init_t(pic_particles* p, int s, int64_t n, int nx, int ny, double vth, uint32_t seed, uint32_t case_id,
+                  uint32_t id0) {
+    SpeciesStore& st = p->sp[s];
+    const int par = p->h_cur, b = st.nb;
+    int32_t cnt = 0;
+    if (int rc = read_count(p, s, par, b, &cnt)) return rc;
+    // the kernel appends at the raw device count, so that count must fit as well
+    if (cnt < 0 || (int64_t)cnt + n > st.h_cap[b]) {
+        set_error("pic_particles_init_maxwellian: not enough free slots (capacity per species exceeded)");
+        return PIC_ERR_CAPACITY;
+    }
+    const size_t off = (size_t)st.h_start[b];
+    return launch_init<T>((T*)st.x[par] + off, (T*)st.y[par] + off, (T*)st.vx[par] + off, (T*)st.vy[par] + off,
+                          st.id[par] + off, st.d_count[par] + b, (int)n, nx, ny, vth, seed, case_id, (uint32_t)s,
+                          id0, lib_stream());
+}
+
+extern "C" int pic_particles_init_maxwellian(pic_particles* p, int species, int64_t n, int nx, int ny, double vth,
+                                             uint32_t seed, uint32_t case_id, uint32_t id0) {
+    PIC_REQUIRE(p && species >= 0 && species < (int)p->sp.size(), "bad species");
+    PIC_REQUIRE(n >= 0 && n < INT32_MAX, "bad particle count");
+    PIC_REQUIRE(nx >= 2 && ny >= 2, "grid must have at least 2 nodes per axis");
+    if (n == 0) return 0;
+    if (p->attached) {
+        if (int rc = sim_sync_impl(p->attached)) return rc;
+    }
+    int rc = p->prec == 4 ? init_t<float>(p, species, n, nx, ny, vth, seed, case_id, id0)
+                          : init_t<double>(p, species, n, nx, ny, vth, seed, case_id, id0);
+    if (rc) return rc;
+    PIC_HIP(hipStreamSynchronize(lib_stream()));
+    return 0;
+}
+
+extern "C" int pic_particles_remove_collided(pic_particles* p) {
+    PIC_REQUIRE(p, "particles is NULL");
+    return 0;
+}
+
+// ============================================================================
+// Grid2D
+// ============================================================================
+extern "C" int pic_grid_create(int m, int n, double X, double Y, pic_grid** out) {
+    PIC_REQUIRE(out, "out is NULL");
+    PIC_REQUIRE(m >= 3 && n >= 3, "grid needs at least 3 nodes per axis");
+    PIC_REQUIRE(X > 0 && Y > 0, "domain size must be positive");
+    auto* g = new pic_grid();
+    g->m = m;
+    g->n = n;
+    g->X = X;
+    g->Y = Y;
+    g->dx = X / (m - 1);
+    g->dy = Y / (n - 1);
+    const size_t nn = (size_t)m * n;
+    if (hipMalloc(&g->rho, nn * 8) != hipSuccess || hipMalloc(&g->phi, nn * 8) != hipSuccess ||
+        hipMalloc(&g->E, nn * 16) != hipSuccess || hipMalloc(&g->rhs, nn * 8) != hipSuccess) {
+        set_error("pic_grid_create: device allocation failed");
+        pic_grid_destroy(g);
+        return PIC_ERR_HIP;
+    }
+    PIC_HIP(hipMemset(g->rho, 0, nn * 8));
+    PIC_HIP(hipMemset(g->phi, 0, nn * 8));
+    PIC_HIP(hipMemset(g->E, 0, nn * 16));
+    PIC_HIP(hipMemset(g->rhs, 0, nn * 8));
+    *out = g;
+    return 0;
+}
+
+extern "C" int pic_grid_destroy(pic_grid* g) {
+    if (!g) return 0;
+    (void)hipStreamSynchronize(lib_stream());
+    (void)hipFree(g->rho);
+    (void)hipFree(g->phi);
+    (void)hipFree(g->E);
+    (void)hipFree(g->rhs);
+    delete g;
+    return 0;
+}
+
+extern "C" int pic_grid_info(const pic_grid* g, int* m, int* n, double* dx, double* dy) {
+    PIC_REQUIRE(g, "grid is NULL");
+    if (m) *m = g->m;
+    if (n) *n = g->n;
+    if (dx) *dx = g->dx;
+    if (dy) *dy = g->dy;
+    return 0;
+}
+
+extern "C" int pic_grid_device_ptrs(pic_grid* g, double** rho, double** phi, double** E) {
+    PIC_REQUIRE(g, "grid is NULL");
+    if (rho) *rho = g->rho;
+    if (phi) *phi = g->phi;
+    if (E) *E = g->E;
+    return 0;
+}
+
+extern "C" int pic_grid_download(pic_grid* g, int field, double* out) {
+    PIC_REQUIRE(g && out, "NULL argument");
+    const size_t nn = (size_t)g->m * g->n;
+    PIC_HIP(hipStreamSynchronize(lib_stream()));
+    switch (field) {
+        case PIC_FIELD_RHO: PIC_HIP(hipMemcpy(out, g->rho, nn * 8, hipMemcpyDeviceToHost)); break;
+        case PIC_FIELD_PHI: PIC_HIP(hipMemcpy(out, g->phi, nn * 8, hipMemcpyDeviceToHost)); break;
+        case PIC_FIELD_RHS: PIC_HIP(hipMemcpy(out, g->rhs, nn * 8, hipMemcpyDeviceToHost)); break;
+        case PIC_FIELD_EX:
+        case PIC_FIELD_EY: {
+            std::vector<double> e(2 * nn);
+            PIC_HIP(hipMemcpy(e.data(), g->E, nn * 16, hipMemcpyDeviceToHost));
+            const int o = field == PIC_FIELD_EX ? 0 : 1;
+            for (size_t i = 0; i < nn; ++i) out[i] = e[2 * i + o];
+            break;
+        }
+        default: set_error("bad field"); return PIC_ERR_ARG;
+    }
+    return 0;
+}
+
+extern "C" int pic_grid_upload(pic_grid* g, int field, const double* in) {
+    PIC_REQUIRE(g && in, "NULL argument");
+    const size_t nn = (size_t)g->m * g->n;
+    PIC_HIP(hipStreamSynchronize(lib_stream()));
+    switch (field) {
+        case PIC_FIELD_RHO: PIC_HIP(hipMemcpy(g->rho, in, nn * 8, hipMemcpyHostToDevice)); break;
+        case PIC_FIELD_PHI: PIC_HIP(hipMemcpy(g->phi, in, nn * 8, hipMemcpyHostToDevice)); break;
+        case PIC_FIELD_RHS: PIC_HIP(hipMemcpy(g->rhs, in, nn * 8, hipMemcpyHostToDevice)); break;
+        default: set_error("field not uploadable"); return PIC_ERR_ARG;
+    }
+    return 0;
+}
+
+extern "C" int pic_grid_compute_E(pic_grid* g) {
+    PIC_REQUIRE(g, "grid is NULL");
+    if (int rc = launch_efield(g->phi, (double2*)g->E, g->m, g->n, g->dx, g->dy, lib_stream())) return rc;
+    PIC_HIP(hipStreamSynchronize(lib_stream()));
+    return 0;
+}
+
+// ============================================================================
+// Boundaries2D
+// ============================================================================
+extern "C" int pic_walls_create(double width, double height, int nx, int ny, pic_walls** out) {
+    PIC_REQUIRE(out, "out is NULL");
+    auto* w = new pic_walls();
+    w->width = width;
+    w->height = height;
+    w->nx = nx;
+    w->ny = ny;
+    *out = w;
+    return 0;
+}
+
+extern "C" int pic_walls_add(pic_walls* w, double x1, double y1, double x2, double y2) {
+    PIC_REQUIRE(w, "walls is NULL");
+    w->walls.insert(w->walls.end(), {x1, y1, x2, y2});
+    w->compiled = false;
+    return 0;
+}
+
+extern "C" int pic_walls_add_box(pic_walls* w) {
+    PIC_REQUIRE(w, "walls is NULL");
+    w->box = true;
+    return 0;
+}
+
+extern "C" int pic_walls_compile(pic_walls* w) {
+    PIC_REQUIRE(w, "walls is NULL");
+    if (w->d_walls) PIC_HIP(hipFree(w->d_walls));
+    w->d_walls = nullptr;
+    if (!w->walls.empty()) {
+        PIC_HIP(hipMalloc(&w->d_walls, w->walls.size() * sizeof(double)));
+        PIC_HIP(hipMemcpy(w->d_walls, w->walls.data(), w->walls.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    w->compiled = true;
+    return 0;
+}
+
+extern "C" int pic_walls_count(const pic_walls* w, int* n) {
+    PIC_REQUIRE(w && n, "NULL argument");
+    *n = (int)(w->walls.size() / 4) + (w->box ? 4 : 0);
+    return 0;
+}
+
+extern "C" int pic_walls_destroy(pic_walls* w) {
+    if (!w) return 0;
+    (void)hipFree(w->d_walls);
+    delete w;
+    return 0;
+}
+
+// ============================================================================
+// multigrid
+// ============================================================================
+extern "C" int pic_mg_default_params(int preset, pic_mg_params* out) {
+    PIC_REQUIRE(out, "out is NULL");
+    switch (preset) {
+        case PIC_MG_PRESET_REFERENCE: *out = pic_mg_params{0, 3, 3, 20, 1.25, 1e-5, 50, PIC_MG_FP64}; break;
+        case PIC_MG_PRESET_STEP: *out = pic_mg_params{0, 2, 2, 1, 1.0, 1e-8, 100, PIC_MG_MIXED}; break;
+        case PIC_MG_PRESET_POISSON: *out = pic_mg_params{0, 2, 2, 1, 1.0, 1e-10, 100, PIC_MG_MIXED}; break;
+        default: set_error("unknown multigrid preset"); return PIC_ERR_ARG;
+    }
+    return 0;
+}
+
+static MGParams to_mg(const pic_mg_params& p) {
+    return MGParams{p.levels, p.pre, p.post, p.coarse_sweeps, p.omega, p.tol, p.max_cycles, p.mode};
+}
+
+extern "C" int pic_mg_create(int n, const pic_mg_params* params, pic_mg** out) {
+    PIC_REQUIRE(out && params, "NULL argument");
+    PIC_REQUIRE(params->mode == PIC_MG_FP64 || params->mode == PIC_MG_MIXED, "bad multigrid mode");
+    PIC_REQUIRE(lib_stream() != nullptr, "could not create a HIP stream");
+    auto* m = new pic_mg();
+    m->solver.reset(new MGSolver(n, to_mg(*params), lib_stream()));
+    if (int rc = m->solver->init()) {
+        delete m;
+        return rc;
+    }
+    *out = m;
+    return 0;
+}
+
+extern "C" int pic_mg_solve(pic_mg* mg, double* phi_d, const double* rhs_d, double h, pic_mg_stats* st) {
+    PIC_REQUIRE(mg && phi_d && rhs_d, "NULL argument");
+    PIC_REQUIRE(h > 0, "h must be positive");
+    PIC_HIP(hipStreamSynchronize(lib_stream()));
+    auto t0 = std::chrono::steady_clock::now();
+    MGStats s{};
+    if (int rc = mg->solver->solve(phi_d, rhs_d, h, &s)) return rc;
+    auto t1 = std::chrono::steady_clock::now();
+    if (st) {
+        st->cycles = s.cycles;
+        st->rel_residual = s.rel_residual;
+        st->rhs_norm = s.rhs_norm;
+        st->ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        st->launches = mg->solver->last_launches();
+    }
+    return 0;
+}
+
+extern "C" int pic_mg_destroy(pic_mg* mg) {
+    if (!mg) return 0;
+    (void)hipStreamSynchronize(lib_stream());
+    delete mg;
+    return 0;
+}
+
+extern "C" int pic_solve_multigrid(double* phi_d, double* rho_d, int nx, int ny, int levels, double h, double omega) {
+    PIC_REQUIRE(phi_d && rho_d, "NULL argument");
+    PIC_REQUIRE(nx == ny, "solve_multigrid needs a square grid (Multigrid.cu:84-88)");
+    // process-wide solver contexts, like the reference's static pool
+    // (Multigrid.cu:187); kept until the process ends
+    static std::mutex mu;
+    static auto* cache = new std::map<std::tuple<int, int, double>, MGSolver*>();
+    std::lock_guard<std::mutex> lk(mu);
+    auto key = std::make_tuple(nx, levels, omega);
+    auto it = cache->find(key);
+    if (it == cache->end()) {
+        MGParams p{levels, 3, 3, 20, omega, 1e-5, 50, PIC_MG_FP64};
+        auto* s = new MGSolver(nx, p, lib_stream());
+        if (int rc = s->init()) {
+            delete s;
+            return rc;
+        }
+        it = cache->emplace(key, s).first;
+    }
+    MGStats st{};
+    return it->second->solve(phi_d, rho_d, h, &st);
+}
+
+// ============================================================================
+// cross sections and MCC
+// ============================================================================
+extern "C" int pic_read_cross_section(const char* path, int cap, double* energies, double* sigmas, int* n) {
+    PIC_REQUIRE(path && n, "NULL argument");
+    std::ifstream f(path);
+    if (!f.is_open()) {
+        set_error(std::string("cannot open cross-section file: ") + path);
+        return PIC_ERR_IO;
+    }
+    std::string line;
+    int k = 0;
+    while (std::getline(f, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::istringstream iss(line);
+        double e, s;
+        if (iss >> e >> s) {
+            if (energies && sigmas) {
+                if (k >= cap) {
+                    set_error("pic_read_cross_section: output capacity too small");
+                    return PIC_ERR_CAPACITY;
+                }
+                energies[k] = e;
+                sigmas[k] = s;
+            }
+            ++k;
+        }
+    }
+    *n = k;
+    return 0;
+}
+
+extern "C" int pic_xs_create(double gas_density, double gas_temperature_eV, pic_xs** out) {
+    PIC_REQUIRE(out, "out is NULL");
+    PIC_REQUIRE(gas_density >= 0 && gas_temperature_eV >= 0, "bad gas parameters");
+    auto* x = new pic_xs();
+    x->density = gas_density;
+    x->t_gas = gas_temperature_eV;
+    *out = x;
+    return 0;
+}
+
+extern "C" int pic_xs_add_process(pic_xs* xs, int species, int type, double threshold_eV, int n,
+                                  const double* energy_eV, const double* sigma_m2, int product_species) {
+    PIC_REQUIRE(xs && energy_eV && sigma_m2 && n >= 2, "need a table of at least 2 points");
+    PIC_REQUIRE(type >= 0 && type <= 3, "bad process type");
+    for (int i = 1; i < n; ++i) PIC_REQUIRE(energy_eV[i] > energy_eV[i - 1], "energies must increase");
+    pic_xs::Proc p{species, type, product_species, threshold_eV, 0.0, {}, {}};
+    p.e.assign(energy_eV, energy_eV + n);
+    p.s.assign(sigma_m2, sigma_m2 + n);
+    xs->procs.push_back(p);
+    return 0;
+}
+
+extern "C" int pic_xs_add_analytic(pic_xs* xs, int species, int type, double threshold_eV, double peak_m2,
+                                   int product_species) {
+    PIC_REQUIRE(xs, "xs is NULL");
+    PIC_REQUIRE(type >= 0 && type <= 3, "bad process type");
+    PIC_REQUIRE(!((type == PIC_PROC_EXCITATION || type == PIC_PROC_IONIZATION) && threshold_eV <= 0),
+                "inelastic processes need a positive threshold");
+    xs->procs.push_back(pic_xs::Proc{species, type, product_species, threshold_eV, peak_m2, {}, {}});
+    return 0;
+}
+
+extern "C" int pic_xs_set_table(pic_xs* xs, int points, double emax_eV) {
+    PIC_REQUIRE(xs && points >= 2 && emax_eV > 0, "bad table");
+    xs->points = points;
+    xs->emax = emax_eV;
+    return 0;
+}
+
+extern "C" int pic_xs_destroy(pic_xs* xs) {
+    delete xs;
+    return 0;
+}
+
+// sigma of a process at energy E (eV): the analytic shape, or the clamped
+// linear interpolation of interpolate_cross_sections (kernels.cu:218-263)
+static double proc_sigma(const pic_xs::Proc& p, double E) {
+    if (p.e.empty()) {
+        if (p.type == PIC_PROC_ELASTIC || p.type == PIC_PROC_CHARGE_EXCHANGE) return p.peak;
+        const double u = E / p.thr;
+        if (u <= 1.0) return 0.0;
+        return p.peak * 2.718281828459045 * std::log(u) / u;
+    }
+    const auto& e = p.e;
+    const auto& s = p.s;
+    if (E <= e.front()) return s.front();
+    if (E >= e.back()) return s.back();
+    int lo = 0, hi = (int)e.size() - 2;
+    while (lo <= hi) {
+        const int mid = (lo + hi) / 2;
+        if (E < e[mid]) hi = mid - 1;
+        else if (E >= e[mid + 1]) lo = mid + 1;
+        else {
+            const double t = (E - e[mid]) / (e[mid + 1] - e[mid]);
+            return s[mid] + t * (s[mid + 1] - s[mid]);
+        }
+    }
+    return s.back();
+}
+
+struct MccBuild {
+    pic_mcc_info info{};
+    std::vector<double> tab;   // n_proc * K of n_gas * sigma
+};
+
+// null-collision constants of one species: uniform table, nu_max over the
+// table (16 samples per interval of the interpolant), p_null = 1-exp(-nu_max dt)
+static int build_mcc(const pic_xs* xs, const std::vector<SpeciesStore>& sp, int s, double dt, MccBuild& out) {
+    std::vector<const pic_xs::Proc*> ps;
+    for (auto& p : xs->procs)
+        if (p.species == s) ps.push_back(&p);
+    PIC_REQUIRE(ps.size() <= PIC_MAX_PROC, "at most 4 processes per species");
+    const int K = xs->points;
+    const double de = xs->emax / (K - 1);
+    pic_mcc_info& I = out.info;
+    I = pic_mcc_info{};
+    I.n_proc = (int)ps.size();
+    I.K = K;
+    I.inv_de = 1.0 / de;
+    I.efac = 0.5 * sp[s].m / kElementaryCharge;
+    I.product_species = -1;
+    double partner_mass = sp[s].m;
+    for (int q = 0; q < I.n_proc; ++q) {
+        I.type[q] = ps[q]->type;
+        I.threshold[q] = (ps[q]->type == PIC_PROC_EXCITATION || ps[q]->type == PIC_PROC_IONIZATION) ? ps[q]->thr : 0.0;
+        if (ps[q]->type == PIC_PROC_IONIZATION) {
+            PIC_REQUIRE(ps[q]->product >= 0 && ps[q]->product < (int)sp.size() && ps[q]->product != s,
+                        "ionization needs a product species different from the projectile");
+            I.product_species = ps[q]->product;
+            partner_mass = sp[ps[q]->product].m;
+        }
+    }
+    I.gas_vth = std::sqrt(xs->t_gas * kElementaryCharge / partner_mass);
+    out.tab.assign((size_t)I.n_proc * K, 0.0);
+    for (int q = 0; q < I.n_proc; ++q)
+        for (int j = 0; j < K; ++j) out.tab[(size_t)q * K + j] = xs->density * proc_sigma(*ps[q], j * de);
+    double numax = 0.0;
+    const double m = sp[s].m;
+    for (int j = 0; j < K - 1; ++j)
+        for (int k = 0; k <= 16; ++k) {
+            const double t = k / 16.0;
+            const double E = (j + t) * de;
+            const double v = std::sqrt(2.0 * E * kElementaryCharge / m);
+            double nu = 0.0;
+            for (int q = 0; q < I.n_proc; ++q) {
+                if (E < I.threshold[q]) continue;
+                const double a = out.tab[(size_t)q * K + j], b = out.tab[(size_t)q * K + j + 1];
+                nu += (a + t * (b - a)) * v;
+            }
+            numax = std::max(numax, nu);
+        }
+    I.nu_max = numax * 1.001;
+    I.p_null = -std::expm1(-I.nu_max * dt);
+    return 0;
+}
+
+// ============================================================================
+// step driver
+// ============================================================================
+static size_t cur_lds(pic_sim* S) {
+    return S->p->prec == 4 ? push_lds_bytes<float>(S->bin_w, S->bin_h, S->halo)
+                           : push_lds_bytes<double>(S->bin_w, S->bin_h, S->halo);
+}
+
+static int rebuild_push_items(pic_sim* S) {
+    std::vector<WorkItem> items;
+    for (int s = 0; s < (int)S->p->sp.size(); ++s) items_for_species(S->p, s, S->spill_chunk, items);
+    static_assert(sizeof(WorkItem) == 16, "work item layout");
+    int alloc = S->n_items > 0 ? S->n_items : 0;
+    // the sim keeps its own table sized exactly
+    if (S->d_items) PIC_HIP(hipFree(S->d_items));
+    S->d_items = nullptr;
+    alloc = 0;
+    if (int rc = upload_items(&S->d_items, &alloc, items)) return rc;
+    S->n_items = (int)items.size();
+    return 0;
+}
+
+template <typename T>
+static int rebin_t(pic_sim* S, int bw, int bh) {
+    pic_particles* p = S->p;
+    CaseDev<T>& C = hcase<T>(p);
+    hipStream_t sm = lib_stream();
+    const int in = p->h_cur, out = in ^ 1;
+    const int nbx = div_up(C.nx - 1, bw), nby = div_up(C.ny - 1, bh);
+    const int nb_new = nbx * nby;
+    const int ns = (int)p->sp.size();
+    std::vector<WorkItem> items;
+    std::vector<int> off(ns + 1, 0);
+    for (int s = 0; s < ns; ++s) {
+        off[s] = (int)items.size();
+        items_for_species(p, s, S->spill_chunk, items);
+    }
+    off[ns] = (int)items.size();
+    if (int rc = upload_items(&p->d_items, &p->items_alloc, items)) return rc;
+    if (int rc = ensure_scratch(p, nb_new + 1)) return rc;
+    std::vector<std::vector<int32_t>> new_start(ns), new_cap(ns);
+    for (int s = 0; s < ns; ++s) {
+        SpeciesStore& st = p->sp[s];
+        PIC_HIP(hipMemsetAsync(p->d_scratch, 0, (size_t)(nb_new + 1) * sizeof(int32_t), sm));
+        if (int rc = launch_hist<T>((CaseDev<T>*)p->d_case, p->d_state, p->d_items + off[s], off[s + 1] - off[s],
+                                    S->spill_chunk, bw, bh, nbx, nb_new, p->d_scratch, sm))
+            return rc;
+        std::vector<int32_t> hist(nb_new + 1);
+        PIC_HIP(hipMemcpyAsync(hist.data(), p->d_scratch, (size_t)(nb_new + 1) * sizeof(int32_t),
+                               hipMemcpyDeviceToHost, sm));
+        PIC_HIP(hipStreamSynchronize(sm));
+        int64_t total = 0;
+        for (int b = 0; b < nb_new; ++b) total += hist[b];
+        const int64_t C_ = st.capacity;
+        int64_t spill_min = std::min<int64_t>(std::max<int64_t>(4096, total / 64), C_ / 8);
+        if (total + spill_min > C_) {
+            set_error("re-binning: particle count exceeds the species capacity");
+            return PIC_ERR_CAPACITY;
+        }
+        const int64_t avail = C_ - total - spill_min;
+        int64_t want = 0;
+        for (int b = 0; b < nb_new; ++b) want += hist[b] / 4 + 32 + 15;
+        const bool full = want <= avail;
+        const double scale = full ? 1.0 : (double)avail / (double)std::max<int64_t>(want, 1);
+        auto& ns_ = new_start[s];
+        auto& nc_ = new_cap[s];
+        ns_.assign(nb_new + 1, 0);
+        nc_.assign(nb_new + 1, 0);
+        int64_t pos = 0;
+        for (int b = 0; b < nb_new; ++b) {
+            int64_t c;
+            if (full) c = (hist[b] + hist[b] / 4 + 32 + 15) / 16 * 16;
+            else c = hist[b] + (int64_t)std::floor((hist[b] / 4 + 32) * scale);
+            ns_[b] = (int32_t)pos;
+            nc_[b] = (int32_t)c;
+            pos += c;
+        }
+        ns_[nb_new] = (int32_t)pos;
+        nc_[nb_new] = (int32_t)(C_ - pos);
+        // parity `out` gets fresh arrays of the new size (parity `in` is still in use)
+        if (int rc = alloc_layout_parity(st, out, nb_new + 1)) return rc;
+        PIC_HIP(hipMemcpyAsync(st.d_start[out], ns_.data(), (nb_new + 1) * sizeof(int32_t), hipMemcpyHostToDevice, sm));
+        PIC_HIP(hipMemcpyAsync(st.d_cap[out], nc_.data(), (nb_new + 1) * sizeof(int32_t), hipMemcpyHostToDevice, sm));
+        PIC_HIP(hipMemsetAsync(st.d_count[out], 0, (nb_new + 1) * sizeof(int32_t), sm));
+        SpeciesDev<T>& D = C.sp[s];
+        D.start[out] = st.d_start[out];
+        D.cap[out] = st.d_cap[out];
+        D.count[out] = st.d_count[out];
+        D.nb[out] = nb_new;
+    }
+    if (int rc = upload_case(p)) return rc;
+    PIC_HIP(hipMemsetAsync(p->d_ll, 0, sizeof(long long), sm));
+    if (int rc = launch_scatter<T>((CaseDev<T>*)p->d_case, p->d_state, p->d_items, (int)items.size(), S->spill_chunk,
+                                   bw, bh, nbx, (unsigned long long*)p->d_ll, sm))
+        return rc;
+    if (int rc = launch_end_step<T>((CaseDev<T>*)p->d_case, p->d_state, 1, 0, sm)) return rc;
+    long long lost = 0;
+    PIC_HIP(hipMemcpyAsync(&lost, p->d_ll, sizeof(lost), hipMemcpyDeviceToHost, sm));
+    PIC_HIP(hipStreamSynchronize(sm));
+    p->h_cur = out;
+    for (int s = 0; s < ns; ++s) {
+        SpeciesStore& st = p->sp[s];
+        st.nb = nb_new;
+        st.h_start = new_start[s];
+        st.h_cap = new_cap[s];
+        st.layout_alloc = nb_new + 1;
+        if (int rc = alloc_layout_parity(st, in, nb_new + 1)) return rc;
+        PIC_HIP(hipMemcpyAsync(st.d_start[in], st.d_start[out], (nb_new + 1) * sizeof(int32_t), hipMemcpyDeviceToDevice, sm));
+        PIC_HIP(hipMemcpyAsync(st.d_cap[in], st.d_cap[out], (nb_new + 1) * sizeof(int32_t), hipMemcpyDeviceToDevice, sm));
+        SpeciesDev<T>& D = C.sp[s];
+        D.start[in] = st.d_start[in];
+        D.cap[in] = st.d_cap[in];
+        D.count[in] = st.d_count[in];
+        D.nb[in] = nb_new;
+    }
+    C.bw = bw;
+    C.bh = bh;
+    C.nbx = nbx;
+    C.nby = nby;
+    S->bin_w = bw;
+    S->bin_h = bh;
+    if (int rc = upload_case(p)) return rc;
+    if (int rc = rebuild_push_items(S)) return rc;
+    S->rebins++;
+    S->need_rebin = false;
+    if (lost) {
+        set_error("re-binning lost particles (spill region full)");
+        return PIC_ERR_CAPACITY;
+    }
+    return 0;
+}
+
+static int sim_rebin(pic_sim* S, int bw, int bh) {
+    return S->p->prec == 4 ? rebin_t<float>(S, bw, bh) : rebin_t<double>(S, bw, bh);
+}
+
+template <typename T>
+static int setup_t(pic_sim* S) {
+    pic_particles* p = S->p;
+    pic_grid* g = S->g;
+    CaseDev<T>& C = hcase<T>(p);
+    const int ns = (int)p->sp.size();
+    C.nx = g->m;
+    C.ny = g->n;
+    C.bw = S->bin_w;
+    C.bh = S->bin_h;
+    C.nbx = div_up(g->m - 1, S->bin_w);
+    C.nby = div_up(g->n - 1, S->bin_h);
+    C.halo = S->halo;
+    C.box = (S->w && S->w->box) ? 1 : 0;
+    C.ns = ns;
+    C.dx = g->dx;
+    C.dy = g->dy;
+    C.wall_eps = S->wall_eps;
+    C.dtdx = (T)(S->dt / g->dx);
+    C.dtdy = (T)(S->dt / g->dy);
+    C.phi = g->phi;
+    C.n_walls = 0;
+    C.walls = nullptr;
+    if (S->w && !S->w->walls.empty()) {
+        std::vector<double> wc(S->w->walls.size());
+        for (size_t i = 0; i < wc.size(); i += 4) {
+            wc[i] = S->w->walls[i] / g->dx;
+            wc[i + 1] = S->w->walls[i + 1] / g->dy;
+            wc[i + 2] = S->w->walls[i + 2] / g->dx;
+            wc[i + 3] = S->w->walls[i + 3] / g->dy;
+        }
+        PIC_HIP(hipMalloc(&S->d_walls_cell, wc.size() * sizeof(double)));
+        PIC_HIP(hipMemcpy(S->d_walls_cell, wc.data(), wc.size() * sizeof(double), hipMemcpyHostToDevice));
+        C.walls = S->d_walls_cell;
+        C.n_walls = (int)(wc.size() / 4);
+    }
+    const size_t nn = (size_t)g->m * g->n;
+    S->acc.assign(ns, nullptr);
+    S->tabs.assign(ns, nullptr);
+    S->tab_host.assign(ns, {});
+    S->info.assign(ns, pic_mcc_info{});
+    S->charge = ChargeArgs{};
+    S->charge.ns = ns;
+    S->charge.zero = 1;
+    S->charge.rho = g->rho;
+    S->charge.rhs = g->rhs;
+    const double qscale = 1.0 / ((double)(1ull << PIC_QBITS) * (double)(1ull << PIC_QBITS));
+    for (int s = 0; s < ns; ++s) {
+        fill_species_ptrs<T>(p, s);
+        SpeciesDev<T>& D = C.sp[s];
+        const SpeciesStore& st = p->sp[s];
+        D.kq = (T)(st.q * S->dt / st.m);
+        D.efac = (T)(0.5 * st.m / kElementaryCharge);
+        PIC_HIP(hipMalloc(&S->acc[s], nn * sizeof(unsigned long long)));
+        PIC_HIP(hipMemset(S->acc[s], 0, nn * sizeof(unsigned long long)));
+        D.acc = S->acc[s];
+        D.n_proc = 0;
+        D.tab = nullptr;
+        D.K = 2;
+        if (S->xs && S->mcc) {
+            MccBuild mb;
+            if (int rc = build_mcc(S->xs, p->sp, s, S->dt, mb)) return rc;
+            S->info[s] = mb.info;
+            S->tab_host[s] = mb.tab;
+            if (mb.info.n_proc > 0) {
+                std::vector<T> tt(mb.tab.size());
+                for (size_t i = 0; i < tt.size(); ++i) tt[i] = (T)mb.tab[i];
+                PIC_HIP(hipMalloc(&S->tabs[s], tt.size() * sizeof(T)));
+                PIC_HIP(hipMemcpy(S->tabs[s], tt.data(), tt.size() * sizeof(T), hipMemcpyHostToDevice));
+                D.n_proc = mb.info.n_proc;
+                D.K = mb.info.K;
+                for (int q = 0; q < PIC_MAX_PROC; ++q) {
+                    D.ptype[q] = q < mb.info.n_proc ? mb.info.type[q] : 0;
+                    D.thr[q] = (T)(q < mb.info.n_proc ? mb.info.threshold[q] : 0.0);
+                }
+                D.tab = (const T*)S->tabs[s];
+                D.inv_de = (T)mb.info.inv_de;
+                D.nu_max = (T)mb.info.nu_max;
+                D.p_null = (T)mb.info.p_null;
+                D.gas_vth = mb.info.gas_vth;
+                D.other = mb.info.product_species;
+            }
+        }
+        const double qf = st.q * st.w * qscale / (g->dx * g->dy);
+        S->charge.acc[s] = S->acc[s];
+        S->charge.qfac[s] = qf;
+        S->charge.cfac[s] = -qf / S->eps0;
+    }
+    if (int rc = upload_case(p)) return rc;
+    return 0;
+}
+
+extern "C" int pic_sim_default_params(pic_sim_params* out) {
+    PIC_REQUIRE(out, "out is NULL");
+    *out = pic_sim_params{};
+    out->dt = 1e-10;
+    out->eps0 = 8.8541878128e-12;
+    out->seed = 1;
+    out->case_id = 0;
+    out->bin_w = 16;
+    out->bin_h = 16;
+    out->halo = 2;
+    out->mcc = 1;
+    out->rebin_every = 0;
+    out->wall_eps = 1e-10;
+    return pic_mg_default_params(PIC_MG_PRESET_STEP, &out->mg);
+}
+
+extern "C" int pic_sim_create(pic_particles* p, pic_grid* g, pic_walls* w, pic_xs* xs, const pic_sim_params* prm,
+                              pic_sim** out) {
+    PIC_REQUIRE(p && g && prm && out, "NULL argument");
+    PIC_REQUIRE(!p->attached, "particles already belong to a pic_sim");
+    PIC_REQUIRE(g->m == g->n, "the multigrid solver needs a square grid (Multigrid.cu:84-88)");
+    PIC_REQUIRE(std::fabs(g->dx - g->dy) <= 1e-12 * g->dx, "the multigrid solver needs dx == dy");
+    PIC_REQUIRE(prm->dt > 0 && prm->eps0 != 0, "bad dt or eps0");
+    PIC_REQUIRE(prm->bin_w >= 2 && prm->bin_w <= 64 && prm->bin_h >= 2 && prm->bin_h <= 64, "bin tile must be 2..64 cells");
+    PIC_REQUIRE(prm->halo >= 0 && prm->halo <= 8, "halo must be 0..8");
+    auto* S = new pic_sim();
+    S->p = p;
+    S->g = g;
+    S->w = w;
+    S->xs = xs;
+    S->dt = prm->dt;
+    S->eps0 = prm->eps0;
+    S->bin_w = prm->bin_w;
+    S->bin_h = prm->bin_h;
+    S->halo = prm->halo;
+    S->mcc = prm->mcc;
+    S->rebin_every = prm->rebin_every;
+    S->wall_eps = prm->wall_eps;
+    S->mgp = to_mg(prm->mg);
+    CaseDev<float>& cf = p->hf;
+    CaseDev<double>& cd = p->hd;
+    cf.seed = cd.seed = prm->seed;
+    cf.case_id = cd.case_id = prm->case_id;
+    auto fail = [&](int rc) {
+        pic_sim_destroy(S);
+        return rc;
+    };
+    int rc = p->prec == 4 ? setup_t<float>(S) : setup_t<double>(S);
+    if (rc) return fail(rc);
+    S->mg.reset(new MGSolver(g->m, S->mgp, lib_stream()));
+    if ((rc = S->mg->init())) return fail(rc);
+    if (hipMalloc(&S->d_status, PIC_MAX_SPECIES * sizeof(int)) != hipSuccess ||
+        hipHostMalloc(&S->h_status, PIC_MAX_SPECIES * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        set_error("status buffers");
+        return fail(PIC_ERR_HIP);
+    }
+    std::memset(S->h_status, 0, PIC_MAX_SPECIES * sizeof(int));
+    for (int k = 0; k < 4; ++k)
+        if (hipEventCreate(&S->ev[k]) != hipSuccess) return fail(PIC_ERR_HIP);
+    p->attached = S;
+    if ((rc = sim_rebin(S, S->bin_w, S->bin_h))) {
+        p->attached = nullptr;
+        return fail(rc);
+    }
+    S->rebins = 0;
+    *out = S;
+    return 0;
+}
+
+static int mcc_active(pic_sim* S) {
+    if (!S->xs || !S->mcc) return 0;
+    for (auto& i : S->info)
+        if (i.n_proc > 0) return 1;
+    return 0;
+}
+
+template <typename T>
+static int step_t(pic_sim* S, int n) {
+    pic_particles* p = S->p;
+    hipStream_t sm = lib_stream();
+    const size_t nn = (size_t)S->g->m * S->g->n;
+    const size_t lds = cur_lds(S);
+    const int mcc = mcc_active(S);
+    for (int i = 0; i < n; ++i) {
+        if (S->need_rebin || (S->rebin_every > 0 && S->steps > 0 && S->steps % S->rebin_every == 0)) {
+            if (int rc = sim_rebin(S, S->bin_w, S->bin_h)) return rc;
+        }
+        const int flags = PIC_F_DRIFT | PIC_F_DEPOSIT | (S->pending ? (PIC_F_KICK | (mcc ? PIC_F_MCC : 0)) : 0);
+        PIC_HIP(hipEventRecord(S->ev[0], sm));
+        if (int rc = launch_push<T>((CaseDev<T>*)p->d_case, p->d_state, S->d_items, S->n_items, S->spill_chunk, flags,
+                                    -1, lds, sm))
+            return rc;
+        PIC_HIP(hipEventRecord(S->ev[1], sm));
+        if (int rc = launch_end_step<T>((CaseDev<T>*)p->d_case, p->d_state, 1, 1, sm)) return rc;
+        p->h_cur ^= 1;
+        if (int rc = launch_status<T>((CaseDev<T>*)p->d_case, p->d_state, 1, S->d_status, sm)) return rc;
+        PIC_HIP(hipMemcpyAsync(S->h_status, S->d_status, PIC_MAX_SPECIES * sizeof(int), hipMemcpyDeviceToHost, sm));
+        if (int rc = launch_charge(S->charge, nn, sm)) return rc;
+        MGStats st{};
+        if (int rc = S->mg->solve(S->g->phi, S->g->rhs, S->g->dx, &st)) return rc;
+        PIC_HIP(hipEventRecord(S->ev[2], sm));
+        PIC_HIP(hipEventSynchronize(S->ev[2]));
+        float a = 0, b = 0;
+        (void)hipEventElapsedTime(&a, S->ev[0], S->ev[1]);
+        (void)hipEventElapsedTime(&b, S->ev[1], S->ev[2]);
+        S->ms_push += a;
+        S->ms_field += b;
+        S->pending = true;
+        S->steps++;
+        S->last_cycles = st.cycles;
+        S->last_rel = st.rel_residual;
+        S->total_cycles += st.cycles;
+        for (int s = 0; s < (int)p->sp.size(); ++s) {
+            const SpeciesStore& ss = p->sp[s];
+            if (S->h_status[s] > ss.h_cap[ss.nb] / 2) S->need_rebin = true;
+        }
+    }
+    return 0;
+}
+
+extern "C" int pic_sim_step(pic_sim* S, int n_steps) {
+    PIC_REQUIRE(S, "sim is NULL");
+    PIC_REQUIRE(n_steps >= 0, "n_steps must be non-negative");
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = S->p->prec == 4 ? step_t<float>(S, n_steps) : step_t<double>(S, n_steps);
+    S->ms_total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}
+
+template <typename T>
+static int sync_t(pic_sim* S) {
+    pic_particles* p = S->p;
+    hipStream_t sm = lib_stream();
+    const int flags = PIC_F_KICK | (mcc_active(S) ? PIC_F_MCC : 0);
+    if (int rc = launch_push<T>((CaseDev<T>*)p->d_case, p->d_state, S->d_items, S->n_items, S->spill_chunk, flags, -1,
+                                cur_lds(S), sm))
+        return rc;
+    if (int rc = launch_end_step<T>((CaseDev<T>*)p->d_case, p->d_state, 1, 0, sm)) return rc;
+    p->h_cur ^= 1;
+    PIC_HIP(hipStreamSynchronize(sm));
+    S->pending = false;
+    return 0;
+}
+
+static int sim_sync_impl(pic_sim* S) {
+    if (!S->pending) return 0;
+    return S->p->prec == 4 ? sync_t<float>(S) : sync_t<double>(S);
+}
+
+extern "C" int pic_sim_sync(pic_sim* S) {
+    PIC_REQUIRE(S, "sim is NULL");
+    return sim_sync_impl(S);
+}
+
+extern "C" int pic_sim_counters(pic_sim* S, int species, uint64_t out[16]) {
+    PIC_REQUIRE(S && out, "NULL argument");
+    PIC_REQUIRE(species >= 0 && species < (int)S->p->sp.size(), "bad species");
+    PIC_HIP(hipStreamSynchronize(lib_stream()));
+    PIC_HIP(hipMemcpy(out, S->p->sp[species].d_counters, PIC_NCOUNT * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int pic_sim_stats(pic_sim* S, pic_sim_report* out) {
+    PIC_REQUIRE(S && out, "NULL argument");
+    *out = pic_sim_report{};
+    out->steps = S->steps;
+    int64_t n = 0;
+    if (int rc = pic_particles_last_alive(S->p, -1, &n)) return rc;
+    out->alive = n;
+    out->last_mg_cycles = S->last_cycles;
+    out->last_mg_rel = S->last_rel;
+    out->total_mg_cycles = S->total_cycles;
+    out->rebins = S->rebins;
+    out->ms_push = S->ms_push;
+    out->ms_field = S->ms_field;
+    out->ms_total = S->ms_total;
+    return 0;
+}
+
+extern "C" int pic_sim_mcc_info(pic_sim* S, int species, pic_mcc_info* out, double* table, int cap) {
+    PIC_REQUIRE(S && out, "NULL argument");
+    PIC_REQUIRE(species >= 0 && species < (int)S->p->sp.size(), "bad species");
+    *out = S->info[species];
+    if (table) {
+        const auto& t = S->tab_host[species];
+        PIC_REQUIRE((int)t.size() <= cap, "table capacity too small");
+        std::copy(t.begin(), t.end(), table);
+    }
+    return 0;
+}
+
+extern "C" int pic_sim_rebin(pic_sim* S, int bin_w, int bin_h) {
+    PIC_REQUIRE(S, "sim is NULL");
+    const int bw = bin_w > 0 ? bin_w : S->bin_w, bh = bin_h > 0 ? bin_h : S->bin_h;
+    PIC_REQUIRE(bw >= 2 && bw <= 64 && bh >= 2 && bh <= 64, "bin tile must be 2..64 cells");
+    return sim_rebin(S, bw, bh);
+}
+
+extern "C" int pic_sim_destroy(pic_sim* S) {
+    if (!S) return 0;
+    (void)hipStreamSynchronize(lib_stream());
+    if (S->p && S->p->attached == S) {
+        (void)sim_sync_impl(S);
+        S->p->attached = nullptr;
+    }
+    for (auto* a : S->acc) (void)hipFree(a);
+    for (auto* t : S->tabs) (void)hipFree(t);
+    (void)hipFree(S->d_walls_cell);
+    (void)hipFree(S->d_items);
+    (void)hipFree(S->d_status);
+    if (S->h_status) (void)hipHostFree(S->h_status);
+    for (auto& e : S->ev)
+        if (e) (void)hipEventDestroy(e);
+    // the particles keep their data; clear the grid-dependent device pointers
+    if (S->p) {
+        for (int s = 0; s < (int)S->p->sp.size(); ++s) {
+            S->p->hf.sp[s].acc = nullptr;
+            S->p->hd.sp[s].acc = nullptr;
+            S->p->hf.sp[s].n_proc = 0;
+            S->p->hd.sp[s].n_proc = 0;
+        }
+        (void)upload_case(S->p);
+    }
+    delete S;
+    return 0;
+}
+
+// ============================================================================
+// elasticCollisions (mcc.cu:114-150)
+// ============================================================================
+template <typename T>
+static int elastic_t(pic_particles* p, int species, double density, double dt, int n, const double* E,
+                     const double* sig, uint32_t seed, uint32_t step) {
+    pic_xs xs;
+    xs.density = density;
+    xs.t_gas = 0.0;
+    xs.points = 2048;
+    xs.emax = E[n - 1];
+    xs.procs.push_back(pic_xs::Proc{species, PIC_PROC_ELASTIC, -1, 0.0, 0.0, std::vector<double>(E, E + n),
+                                    std::vector<double>(sig, sig + n)});
+    MccBuild mb;
+    if (int rc = build_mcc(&xs, p->sp, species, dt, mb)) return rc;
+    CaseDev<T>& C = hcase<T>(p);
+    const CaseDev<T> saved = C;
+    for (int s = 0; s < (int)p->sp.size(); ++s) C.sp[s].n_proc = 0;
+    std::vector<T> tt(mb.tab.size());
+    for (size_t i = 0; i < tt.size(); ++i) tt[i] = (T)mb.tab[i];
+    T* dtab = nullptr;
+    PIC_HIP(hipMalloc(&dtab, tt.size() * sizeof(T)));
+    PIC_HIP(hipMemcpy(dtab, tt.data(), tt.size() * sizeof(T), hipMemcpyHostToDevice));
+    SpeciesDev<T>& D = C.sp[species];
+    D.n_proc = 1;
+    D.K = mb.info.K;
+    D.ptype[0] = PIC_PROC_ELASTIC;
+    D.thr[0] = (T)0;
+    D.tab = dtab;
+    D.inv_de = (T)mb.info.inv_de;
+    D.nu_max = (T)mb.info.nu_max;
+    D.p_null = (T)mb.info.p_null;
+    D.efac = (T)mb.info.efac;
+    D.other = -1;
+    C.seed = seed;
+    int rc = upload_case(p);
+    std::vector<WorkItem> items;
+    for (int s = 0; s < (int)p->sp.size() && !rc; ++s) items_for_species(p, s, 4096, items);
+    if (!rc) rc = upload_items(&p->d_items, &p->items_alloc, items);
+    const size_t lds = push_lds_bytes<T>(C.bw > 0 ? C.bw : 2, C.bh > 0 ? C.bh : 2, C.halo);
+    if (!rc)
+        rc = launch_push<T>((CaseDev<T>*)p->d_case, p->d_state, p->d_items, (int)items.size(), 4096, PIC_F_MCC,
+                            (long long)step, lds, lib_stream());
+    if (!rc) rc = launch_end_step<T>((CaseDev<T>*)p->d_case, p->d_state, 1, 0, lib_stream());
+    if (!rc) p->h_cur ^= 1;
+    (void)hipStreamSynchronize(lib_stream());
+    C = saved;
+    int rc2 = upload_case(p);
+    (void)hipFree(dtab);
+    return rc ? rc : rc2;
+}
+
+extern "C" int pic_elastic_collisions(pic_particles* p, int species, double density, double dt, int n,
+                                      const double* energy_eV, const double* sigma_m2, uint32_t seed, uint32_t step) {
+    PIC_REQUIRE(p && energy_eV && sigma_m2 && n >= 2, "NULL argument or table shorter than 2");
+    PIC_REQUIRE(species >= 0 && species < (int)p->sp.size(), "bad species");
+    PIC_REQUIRE(!p->attached, "use the pic_sim MCC for particles that belong to a simulation");
+    return p->prec == 4 ? elastic_t<float>(p, species, density, dt, n, energy_eV, sigma_m2, seed, step)
+                        : elastic_t<double>(p, species, density, dt, n, energy_eV, sigma_m2, seed, step);
+}

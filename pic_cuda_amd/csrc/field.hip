@@ -1,0 +1,90 @@
+// field.hip -- grid kernels of the step outside the Poisson solve.
+//
+//  charge_kernel   Q20 accumulators -> charge density rho and Poisson rhs
+//                  (-rho/eps0); clears the accumulators for the next step.
+//                  Replaces cudaMemset(rho) + updateDensity's FP64 result
+//                  (main.cu:93-102, kernels.cu:26-69).
+//  efield_kernel   E = -grad(phi) into an interleaved double2 grid,
+//                  computeElectricField (kernels.cu:134-191).
+//  deposit_kernel  stand-alone Q20 CIC deposit of a flat particle array
+//                  (updateDensity without the step around it).
+#include "field.hpp"
+
+namespace pic {
+
+__global__ __launch_bounds__(256) void charge_kernel(ChargeArgs a, size_t nn) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    double r = 0.0, q = 0.0;
+    for (int s = 0; s < a.ns; ++s) {
+        const double v = (double)a.acc[s][i];
+        r = fma(a.cfac[s], v, r);
+        q = fma(a.qfac[s], v, q);
+        if (a.zero) a.acc[s][i] = 0ull;
+    }
+    if (a.rho) a.rho[i] = q;
+    a.rhs[i] = r;
+}
+
+__global__ __launch_bounds__(256) void efield_kernel(const double* __restrict__ phi, double2* __restrict__ E,
+                                                     int nx, int ny, double dx, double dy) {
+    const int i = blockIdx.x * 16 + threadIdx.x % 16;
+    const int j = blockIdx.y * 16 + threadIdx.x / 16;
+    if (i >= nx || j >= ny) return;
+    const double* r = phi + (size_t)j * nx;
+    double ex, ey;
+    if (i == 0) ex = -(r[1] - r[0]) / dx;
+    else if (i == nx - 1) ex = -(r[i] - r[i - 1]) / dx;
+    else ex = -(r[i + 1] - r[i - 1]) / (2.0 * dx);
+    if (j == 0) ey = -(phi[(size_t)nx + i] - phi[i]) / dy;
+    else if (j == ny - 1) ey = -(phi[(size_t)j * nx + i] - phi[(size_t)(j - 1) * nx + i]) / dy;
+    else ey = -(phi[(size_t)(j + 1) * nx + i] - phi[(size_t)(j - 1) * nx + i]) / (2.0 * dy);
+    E[(size_t)j * nx + i] = make_double2(ex, ey);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void deposit_kernel(const T* __restrict__ x, const T* __restrict__ y, int n,
+                                                      int nx, int ny, unsigned long long* acc) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const T xp = x[p], yp = y[p];
+    const int ix = (int)floor(xp), iy = (int)floor(yp);
+    if (ix < 0 || ix >= nx - 1 || iy < 0 || iy >= ny - 1) return;
+    const T fx = xp - (T)ix, fy = yp - (T)iy;
+    const unsigned long long W = 1ull << PIC_QBITS;
+    const unsigned long long qx = (unsigned)(fx * (T)W), qy = (unsigned)(fy * (T)W);
+    unsigned long long* b = acc + (size_t)iy * nx + ix;
+    atomicAdd(b, (W - qx) * (W - qy));
+    atomicAdd(b + 1, qx * (W - qy));
+    atomicAdd(b + nx, (W - qx) * qy);
+    atomicAdd(b + nx + 1, qx * qy);
+}
+
+int launch_charge(const ChargeArgs& a, size_t nn, hipStream_t s) {
+    if (nn == 0) return 0;
+    hipLaunchKernelGGL(charge_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, a, nn);
+    PIC_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_efield(const double* phi, double2* E, int nx, int ny, double dx, double dy, hipStream_t s) {
+    hipLaunchKernelGGL(efield_kernel, dim3(div_up(nx, 16), div_up(ny, 16)), dim3(256), 0, s, phi, E, nx, ny, dx,
+                       dy);
+    PIC_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_deposit(int prec, const void* x, const void* y, int n, int nx, int ny, unsigned long long* acc,
+                   hipStream_t s) {
+    if (n <= 0) return 0;
+    if (prec == 4)
+        hipLaunchKernelGGL(deposit_kernel<float>, dim3(div_up(n, 256)), dim3(256), 0, s, (const float*)x,
+                           (const float*)y, n, nx, ny, acc);
+    else
+        hipLaunchKernelGGL(deposit_kernel<double>, dim3(div_up(n, 256)), dim3(256), 0, s, (const double*)x,
+                           (const double*)y, n, nx, ny, acc);
+    PIC_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace pic

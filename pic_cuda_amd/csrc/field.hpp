@@ -1,0 +1,22 @@
+// field.hpp -- launchers of the grid kernels in field.hip.
+#pragma once
+#include "pic_common.hpp"
+
+namespace pic {
+
+struct ChargeArgs {
+    unsigned long long* acc[PIC_MAX_SPECIES];
+    double cfac[PIC_MAX_SPECIES];   // rhs factor: -q w / (dx dy eps0 2^(2Q))
+    double qfac[PIC_MAX_SPECIES];   // rho factor:  q w / (dx dy 2^(2Q))
+    int ns;
+    int zero;                       // clear the accumulators after reading
+    double* rho;                    // may be null
+    double* rhs;
+};
+
+int launch_charge(const ChargeArgs& a, size_t nn, hipStream_t s);
+int launch_efield(const double* phi, double2* E, int nx, int ny, double dx, double dy, hipStream_t s);
+int launch_deposit(int prec, const void* x, const void* y, int n, int nx, int ny, unsigned long long* acc,
+                   hipStream_t s);
+
+}  // namespace pic

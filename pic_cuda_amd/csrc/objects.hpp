@@ -1,0 +1,120 @@
+// objects.hpp -- host objects behind the C ABI (pic_amd.h).
+#pragma once
+#include "pic_device.hpp"
+#include "mg.hpp"
+#include "field.hpp"
+#include "push.hpp"
+#include "../../include/pic_amd.h"
+#include <vector>
+#include <string>
+#include <memory>
+
+struct pic_sim;
+
+namespace pic {
+
+hipStream_t lib_stream();          // the library's stream (default priority)
+constexpr double kElementaryCharge = 1.602176634e-19;
+
+// one species store: two SoA buffers + per-parity bin layouts
+struct SpeciesStore {
+    double m = 1.0, q = 0.1, w = 1.0;
+    int64_t capacity = 0;          // slots per buffer
+    void* x[2] = {nullptr, nullptr};
+    void* y[2] = {nullptr, nullptr};
+    void* vx[2] = {nullptr, nullptr};
+    void* vy[2] = {nullptr, nullptr};
+    uint32_t* id[2] = {nullptr, nullptr};
+    int32_t* d_start[2] = {nullptr, nullptr};
+    int32_t* d_cap[2] = {nullptr, nullptr};
+    int32_t* d_count[2] = {nullptr, nullptr};
+    int layout_alloc = 0;          // entries allocated in the layout arrays
+    int nb = 0;                    // bins of the current layout (spill is index nb)
+    std::vector<int32_t> h_start, h_cap;
+    unsigned long long* d_counters = nullptr;
+};
+
+}  // namespace pic
+
+struct pic_particles {
+    int prec = 4;
+    int64_t max_per_species = 0;
+    std::vector<pic::SpeciesStore> sp;
+    pic::CaseDev<float> hf{};
+    pic::CaseDev<double> hd{};
+    void* d_case = nullptr;        // CaseDev<T> on the device
+    pic::CaseState* d_state = nullptr;
+    int h_cur = 0;                 // host mirror of the buffer parity
+    pic::WorkItem* d_items = nullptr;   // scratch item table for store-level kernels
+    int items_alloc = 0;
+    int32_t* d_scratch = nullptr;  // counters / histogram scratch
+    int scratch_alloc = 0;
+    long long* d_ll = nullptr;
+    pic_sim* attached = nullptr;
+};
+
+struct pic_grid {
+    int m = 0, n = 0;
+    double X = 0, Y = 0, dx = 0, dy = 0;
+    double* rho = nullptr;
+    double* phi = nullptr;
+    double* E = nullptr;           // interleaved Ex, Ey
+    double* rhs = nullptr;
+};
+
+struct pic_walls {
+    double width = 0, height = 0;
+    int nx = 0, ny = 0;
+    std::vector<double> walls;     // domain units x1,y1,x2,y2
+    bool box = false;
+    bool compiled = false;
+    double* d_walls = nullptr;     // compile_walls(): device copy (domain units)
+};
+
+struct pic_xs {
+    struct Proc {
+        int species, type, product;
+        double thr, peak;
+        std::vector<double> e, s;  // tabulated; empty for analytic
+    };
+    double density = 0, t_gas = 0;
+    int points = 2048;
+    double emax = 200.0;
+    std::vector<Proc> procs;
+};
+
+struct pic_mg {
+    std::unique_ptr<pic::MGSolver> solver;
+};
+
+struct pic_sim {
+    pic_particles* p = nullptr;
+    pic_grid* g = nullptr;
+    pic_walls* w = nullptr;
+    pic_xs* xs = nullptr;
+    double dt = 0, eps0 = 0;
+    int bin_w = 16, bin_h = 16, halo = 2, mcc = 1, rebin_every = 0;
+    double wall_eps = 1e-10;
+    pic::MGParams mgp{};
+    std::unique_ptr<pic::MGSolver> mg;
+    std::vector<unsigned long long*> acc;
+    std::vector<void*> tabs;
+    std::vector<std::vector<double>> tab_host;
+    std::vector<pic_mcc_info> info;
+    double* d_walls_cell = nullptr;
+    pic::WorkItem* d_items = nullptr;
+    int n_items = 0;
+    int spill_chunk = 4096;
+    bool pending = false;
+    bool need_rebin = false;
+    int64_t steps = 0;
+    int64_t total_cycles = 0;
+    int last_cycles = 0;
+    double last_rel = 0;
+    int rebins = 0;
+    double ms_push = 0, ms_field = 0, ms_total = 0;
+    int* d_status = nullptr;
+    int* h_status = nullptr;
+    pic::ChargeArgs charge{};
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+};

@@ -1,0 +1,783 @@
+// push.hip -- particle kernels for gfx950.
+//
+//  push_kernel    one fused pass per step: CIC gather of E from an LDS tile (E is
+//                 computed from phi with the edge rules of kernels.cu:134-191),
+//                 kick (kernels.cu:71-131), null-collision MCC (replaces
+//                 mcc.cu:56-85), drift (kernels.cu:8-23), wall test
+//                 (collisions.cu:7-54), Q20 fixed-point CIC deposit into an LDS
+//                 tile flushed with one 64-bit atomic per node (replaces the FP64
+//                 atomics of kernels.cu:26-69) and wave-ballot compaction of the
+//                 survivors into the bin of their new cell (replaces
+//                 removeCollidedParticles, particles.cu:217-261).
+//  init_kernel    seeded uniform/Maxwellian particles (replaces main.cu:17-32).
+//  hist/scatter   re-binning of a species store (counting sort by tile).
+//  gather_kernel  flattening of a store for download.
+//  end_step       buffer-parity flip and step counter, one block per case.
+#include "pic_device.hpp"
+
+namespace pic {
+
+__device__ inline float dsqrt(float v) { return sqrtf(v); }
+__device__ inline double dsqrt(double v) { return sqrt(v); }
+__device__ inline float dfloor(float v) { return floorf(v); }
+__device__ inline double dfloor(double v) { return floor(v); }
+__device__ inline float dfma(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ inline double dfma(double a, double b, double c) { return fma(a, b, c); }
+
+// E = -grad(phi) at a node, kernels.cu:134-191 (per-axis one-sided at edges)
+__device__ inline double ex_node(const double* __restrict__ phi, int nx, int i, int j, double dx) {
+    const double* r = phi + (size_t)j * nx;
+    if (i == 0) return -(r[1] - r[0]) / dx;
+    if (i == nx - 1) return -(r[i] - r[i - 1]) / dx;
+    return -(r[i + 1] - r[i - 1]) / (2.0 * dx);
+}
+__device__ inline double ey_node(const double* __restrict__ phi, int nx, int ny, int i, int j, double dy) {
+    if (j == 0) return -(phi[(size_t)nx + i] - phi[i]) / dy;
+    if (j == ny - 1) return -(phi[(size_t)j * nx + i] - phi[(size_t)(j - 1) * nx + i]) / dy;
+    return -(phi[(size_t)(j + 1) * nx + i] - phi[(size_t)(j - 1) * nx + i]) / (2.0 * dy);
+}
+
+__device__ inline uint64_t lanemask_lt() {
+    const int lane = __lane_id();
+    return (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Wave-aggregated slot allocation: lanes that want a slot in bin `key` of the
+// output buffer get consecutive slots with one atomic per distinct key per
+// wave; slots beyond the bin capacity go to the spill region (index nb).
+// Returns the destination index, or -1 when the spill region is full as well.
+// Must be called by every lane of the wave.
+template <typename T>
+__device__ inline int alloc_slot(const SpeciesDev<T>& S, int out, bool want, int key) {
+    const int lane = __lane_id();
+    const uint64_t lt = lanemask_lt();
+    int slot = -1;
+    uint64_t act = __ballot(want);
+    while (act) {
+        const int leader = __ffsll((unsigned long long)act) - 1;
+        const int k = __shfl(key, leader);
+        const uint64_t m = __ballot(want && key == k) & act;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&S.count[out][k], __popcll(m));
+        base = __shfl(base, leader);
+        if ((m >> lane) & 1ull) slot = base + __popcll(m & lt);
+        act &= ~m;
+    }
+    int dst = -1;
+    bool over = false;
+    if (want) {
+        if (slot < S.cap[out][key]) dst = S.start[out][key] + slot;
+        else over = true;
+    }
+    const uint64_t om = __ballot(over);
+    if (om) {
+        const int nb = S.nb[out];
+        const int leader = __ffsll((unsigned long long)om) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&S.count[out][nb], __popcll(om));
+        base = __shfl(base, leader);
+        if (over) {
+            const int s2 = base + __popcll(om & lt);
+            dst = (s2 < S.cap[out][nb]) ? S.start[out][nb] + s2 : -1;
+        }
+    }
+    return dst;
+}
+
+template <typename T>
+__device__ inline int bin_key(const CaseDev<T>& C, int nb, T x, T y) {
+    if (nb == 0) return 0;
+    int ix = (int)dfloor(x), iy = (int)dfloor(y);
+    ix = min(max(ix, 0), C.nx - 2);
+    iy = min(max(iy, 0), C.ny - 2);
+    return (iy / C.bh) * C.nbx + (ix / C.bw);
+}
+
+template <typename T>
+__device__ inline bool hits_walls(const CaseDev<T>& C, T x0, T y0, T x1, T y1) {
+    if (C.box) {
+        if (x1 <= (T)0 || x1 >= (T)(C.nx - 1) || y1 <= (T)0 || y1 >= (T)(C.ny - 1)) return true;
+    }
+    if (C.n_walls > 0) {
+        const double X1 = x0, Y1 = y0, X2 = x1, Y2 = y1;
+        for (int j = 0; j < C.n_walls; ++j) {
+            const double* w = C.walls + 4 * j;
+            const double x3 = w[0], y3 = w[1], x4 = w[2], y4 = w[3];
+            const double den = (y4 - y3) * (X2 - X1) - (x4 - x3) * (Y2 - Y1);
+            if (fabs(den) < C.wall_eps) continue;
+            const double ua = ((x4 - x3) * (Y1 - y3) - (y4 - y3) * (X1 - x3)) / den;
+            const double ub = ((X2 - X1) * (Y1 - y3) - (Y2 - Y1) * (X1 - x3)) / den;
+            if (ua >= 0.0 && ua <= 1.0 && ub >= 0.0 && ub <= 1.0) return true;
+        }
+    }
+    return false;
+}
+
+// Q20 CIC deposit (cell test of kernels.cu:53).  LDS window if the four nodes
+// fall inside it, global 64-bit atomics otherwise.
+template <typename T>
+__device__ inline void deposit(T x, T y, int nx, int ny, bool use_lds, int wx0, int wy0, int aw, int ah,
+                               unsigned long long* lds, unsigned long long* gacc, unsigned* lcnt) {
+    const int ix = (int)dfloor(x), iy = (int)dfloor(y);
+    if (ix < 0 || ix >= nx - 1 || iy < 0 || iy >= ny - 1) return;
+    const T fx = x - (T)ix, fy = y - (T)iy;
+    const unsigned long long W = 1ull << PIC_QBITS;
+    const unsigned long long qx = (unsigned)(fx * (T)W), qy = (unsigned)(fy * (T)W);
+    const unsigned long long w00 = (W - qx) * (W - qy), w10 = qx * (W - qy);
+    const unsigned long long w01 = (W - qx) * qy, w11 = qx * qy;
+    const int lx = ix - wx0, ly = iy - wy0;
+    if (use_lds && lx >= 0 && lx < aw - 1 && ly >= 0 && ly < ah - 1) {
+        unsigned long long* b = lds + ly * aw + lx;
+        atomicAdd(b, w00);
+        atomicAdd(b + 1, w10);
+        atomicAdd(b + aw, w01);
+        atomicAdd(b + aw + 1, w11);
+    } else {
+        unsigned long long* b = gacc + (size_t)iy * nx + ix;
+        atomicAdd(b, w00);
+        atomicAdd(b + 1, w10);
+        atomicAdd(b + nx, w01);
+        atomicAdd(b + nx + 1, w11);
+        atomicAdd(lcnt + PIC_C_GLOBAL_DEPOSIT, 1u);
+    }
+}
+
+template <typename T>
+__device__ inline bool iso_dir(Stream& rs, T& c, T& s) {
+    for (int t = 0; t < 16; ++t) {
+        const T a = (T)2 * rs.uni<T>() - (T)1;
+        const T b = (T)2 * rs.uni<T>() - (T)1;
+        const T q = dfma(a, a, b * b);
+        if (q > (T)0 && q <= (T)1) {
+            c = dfma(a, a, -(b * b)) / q;
+            s = ((T)2 * a) * b / q;
+            return true;
+        }
+    }
+    c = (T)1;
+    s = (T)0;
+    return false;
+}
+
+template <typename T>
+__device__ inline bool gauss_pair(Stream& rs, double sigma, T& g1, T& g2) {
+    for (int t = 0; t < 16; ++t) {
+        const T a = (T)2 * rs.uni<T>() - (T)1;
+        const T b = (T)2 * rs.uni<T>() - (T)1;
+        const T q = dfma(a, a, b * b);
+        if (q > (T)0 && q < (T)1) {
+            const double qd = (double)q;
+            const double f = sqrt(-2.0 * det_log(qd) / qd);
+            g1 = (T)(sigma * ((double)a * f));
+            g2 = (T)(sigma * ((double)b * f));
+            return true;
+        }
+    }
+    g1 = (T)0;
+    g2 = (T)0;
+    return false;
+}
+
+template <typename T>
+struct Born {
+    T ex, ey, evx, evy, ix, iy, ivx, ivy;
+    uint32_t eid, iid;
+};
+
+// Null-collision MCC for one particle (replaces elasticCollisions,
+// mcc.cu:114-150, and its kernels kernels.cu:196-263, mcc.cu:56-85).
+template <typename T>
+__device__ inline bool mcc_collide(const SpeciesDev<T>& S, uint32_t id, uint32_t step, uint32_t case_id,
+                                   uint32_t seed, T x, T y, T& vx, T& vy, unsigned* lcnt, Born<T>& b) {
+    Stream rs(id, step, case_id, seed, S.key);
+    const T u0 = rs.uni<T>(), u1 = rs.uni<T>();
+    if (!(u0 < S.p_null)) return false;
+    atomicAdd(lcnt + PIC_C_NULL_CAND, 1u);
+    const T v2 = dfma(vx, vx, vy * vy);
+    const T E = S.efac * v2;
+    const T v = dsqrt(v2);
+    const T s = E * S.inv_de;
+    const int K = S.K;
+    int j;
+    T t;
+    if (s < (T)(K - 1)) { j = (int)s; t = s - (T)j; }   // false for NaN as well
+    else { j = K - 2; t = (T)1; }
+    const T R = u1 * S.nu_max;
+    T cum = (T)0;
+    int sel = -1;
+    for (int q = 0; q < S.n_proc; ++q) {
+        if (E < S.thr[q]) continue;
+        const T a = S.tab[(size_t)q * K + j], bb = S.tab[(size_t)q * K + j + 1];
+        const T sig = dfma(t, bb - a, a);
+        cum = dfma(sig, v, cum);
+        if (sel < 0 && R < cum) sel = q;
+    }
+    if (cum > S.nu_max) atomicAdd(lcnt + PIC_C_NUMAX_EXCEEDED, 1u);
+    if (sel < 0) return false;
+    atomicAdd(lcnt + PIC_C_PROC0 + sel, 1u);
+    const int ty = S.ptype[sel];
+    const T thr = S.thr[sel];
+    T c, sn;
+    if (ty == PIC_ELASTIC) {
+        if (!iso_dir(rs, c, sn)) atomicAdd(lcnt + PIC_C_REJECT_FALLBACK, 1u);
+        vx = v * c;
+        vy = v * sn;
+    } else if (ty == PIC_EXCITATION) {
+        const T e1 = E - thr;
+        const T v1 = dsqrt(e1 / S.efac);
+        if (!iso_dir(rs, c, sn)) atomicAdd(lcnt + PIC_C_REJECT_FALLBACK, 1u);
+        vx = v1 * c;
+        vy = v1 * sn;
+    } else if (ty == PIC_IONIZATION) {
+        const T er = E - thr;
+        const T us = rs.uni<T>();
+        const T e1 = er * us;
+        const T e2 = er - e1;
+        const T v1 = dsqrt(e1 / S.efac), v2s = dsqrt(e2 / S.efac);
+        if (!iso_dir(rs, c, sn)) atomicAdd(lcnt + PIC_C_REJECT_FALLBACK, 1u);
+        vx = v1 * c;
+        vy = v1 * sn;
+        if (!iso_dir(rs, c, sn)) atomicAdd(lcnt + PIC_C_REJECT_FALLBACK, 1u);
+        b.ex = x; b.ey = y; b.evx = v2s * c; b.evy = v2s * sn;
+        T g1, g2;
+        if (!gauss_pair(rs, S.gas_vth, g1, g2)) atomicAdd(lcnt + PIC_C_REJECT_FALLBACK, 1u);
+        b.ix = x; b.iy = y; b.ivx = g1; b.ivy = g2;
+        b.eid = rs.u32();
+        b.iid = rs.u32();
+        return true;
+    } else {  // charge exchange
+        T g1, g2;
+        if (!gauss_pair(rs, S.gas_vth, g1, g2)) atomicAdd(lcnt + PIC_C_REJECT_FALLBACK, 1u);
+        vx = g1;
+        vy = g2;
+    }
+    return false;
+}
+
+template <typename T>
+__device__ inline void item_range(const SpeciesDev<T>& S, int in, const WorkItem& it, int chunk, int& begin,
+                                  int& n) {
+    const int nb = S.nb[in];
+    if (it.bin < nb) {
+        begin = S.start[in][it.bin];
+        n = min(S.count[in][it.bin], S.cap[in][it.bin]);
+    } else {
+        const int tot = min(S.count[in][nb], S.cap[in][nb]);
+        begin = S.start[in][nb] + it.chunk * chunk;
+        n = min(chunk, tot - it.chunk * chunk);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// the fused particle kernel
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict__ cases,
+                                                   const CaseState* __restrict__ states,
+                                                   const WorkItem* __restrict__ items, int spill_chunk,
+                                                   int flags_arg, long long mcc_key) {
+    using T2 = typename Vec2<T>::type;
+    extern __shared__ unsigned long long smem_u64[];
+    const WorkItem it = items[blockIdx.x];
+    const CaseDev<T>& C = cases[it.case_idx];
+    const CaseState st = states[it.case_idx];
+    const SpeciesDev<T>& S = C.sp[it.species];
+    const int in = st.cur, out = in ^ 1;
+    int begin, n;
+    item_range(S, in, it, spill_chunk, begin, n);
+    if (n <= 0) return;
+
+    const int nx = C.nx, ny = C.ny;
+    const bool is_bin = it.bin < S.nb[in];
+    const int H = C.halo;
+    const int bx = is_bin ? it.bin % C.nbx : 0, by = is_bin ? it.bin / C.nbx : 0;
+    const int cx0 = bx * C.bw, cy0 = by * C.bh;
+    const int ew = C.bw + 1, eh = C.bh + 1;           // E window nodes
+    const int aw = C.bw + 1 + 2 * H, ah = C.bh + 1 + 2 * H;  // deposit window nodes
+    const int wx0 = cx0 - H, wy0 = cy0 - H;
+    unsigned long long* accw = smem_u64;
+    T2* ew_lds = reinterpret_cast<T2*>(smem_u64 + (is_bin ? aw * ah : 0));
+    unsigned* lcnt = reinterpret_cast<unsigned*>(ew_lds + (is_bin ? ew * eh : 0));
+    const int flags = flags_arg >= 0 ? flags_arg : st.flags;
+    const bool kick = flags & PIC_F_KICK, mcc = (flags & PIC_F_MCC) && S.n_proc > 0, drift = flags & PIC_F_DRIFT,
+               dep = flags & PIC_F_DEPOSIT;
+    // the kick and collisions of a pass complete step st.step - 1 (the drift
+    // starts step st.step), so collisions are keyed on st.step - 1
+    const uint32_t mcc_step = mcc_key >= 0 ? (uint32_t)mcc_key : st.step - 1u;
+
+    for (int k = threadIdx.x; k < PIC_NCOUNT; k += blockDim.x) lcnt[k] = 0u;
+    if (is_bin) {
+        for (int k = threadIdx.x; k < aw * ah; k += blockDim.x) accw[k] = 0ull;
+        if (kick) {
+            for (int k = threadIdx.x; k < ew * eh; k += blockDim.x) {
+                const int i = cx0 + k % ew, j = cy0 + k / ew;
+                T2 e;
+                if (i < nx && j < ny) {
+                    e.x = (T)ex_node(C.phi, nx, i, j, C.dx);
+                    e.y = (T)ey_node(C.phi, nx, ny, i, j, C.dy);
+                } else {
+                    e.x = (T)0;
+                    e.y = (T)0;
+                }
+                ew_lds[k] = e;
+            }
+        }
+    }
+    __syncthreads();
+
+    const T kq = S.kq, dtdx = C.dtdx, dtdy = C.dtdy;
+    const SpeciesDev<T>* O = (S.other >= 0) ? &C.sp[S.other] : nullptr;
+
+    for (int base = 0; base < n; base += blockDim.x) {
+        const int t = base + threadIdx.x;
+        const bool valid = t < n;
+        T x = (T)0, y = (T)0, vx = (T)0, vy = (T)0;
+        uint32_t id = 0;
+        if (valid) {
+            const int src = begin + t;
+            x = S.x[in][src];
+            y = S.y[in][src];
+            vx = S.vx[in][src];
+            vy = S.vy[in][src];
+            id = S.id[in][src];
+        }
+        bool alive = false, born = false, be_alive = false, bi_alive = false;
+        int key = 0, key_e = 0, key_i = 0;
+        Born<T> b;
+        if (valid) {
+            if (kick) {
+                const int ix = (int)dfloor(x), iy = (int)dfloor(y);
+                if (!(ix < 0 || ix >= nx - 1 || iy < 0 || iy >= ny - 1)) {
+                    const T fx = x - (T)ix, fy = y - (T)iy;
+                    const T gx = (T)1 - fx, gy = (T)1 - fy;
+                    const T w00 = gx * gy, w10 = fx * gy, w01 = gx * fy, w11 = fx * fy;
+                    T2 e00, e10, e01, e11;
+                    const int lx = ix - cx0, ly = iy - cy0;
+                    if (is_bin && lx >= 0 && lx < C.bw && ly >= 0 && ly < C.bh) {
+                        const T2* p = ew_lds + ly * ew + lx;
+                        e00 = p[0];
+                        e10 = p[1];
+                        e01 = p[ew];
+                        e11 = p[ew + 1];
+                    } else {
+                        e00.x = (T)ex_node(C.phi, nx, ix, iy, C.dx);
+                        e00.y = (T)ey_node(C.phi, nx, ny, ix, iy, C.dy);
+                        e10.x = (T)ex_node(C.phi, nx, ix + 1, iy, C.dx);
+                        e10.y = (T)ey_node(C.phi, nx, ny, ix + 1, iy, C.dy);
+                        e01.x = (T)ex_node(C.phi, nx, ix, iy + 1, C.dx);
+                        e01.y = (T)ey_node(C.phi, nx, ny, ix, iy + 1, C.dy);
+                        e11.x = (T)ex_node(C.phi, nx, ix + 1, iy + 1, C.dx);
+                        e11.y = (T)ey_node(C.phi, nx, ny, ix + 1, iy + 1, C.dy);
+                    }
+                    const T Ex = dfma(w11, e11.x, dfma(w01, e01.x, dfma(w10, e10.x, w00 * e00.x)));
+                    const T Ey = dfma(w11, e11.y, dfma(w01, e01.y, dfma(w10, e10.y, w00 * e00.y)));
+                    vx = dfma(kq, Ex, vx);
+                    vy = dfma(kq, Ey, vy);
+                }
+            }
+            if (mcc) born = mcc_collide(S, id, mcc_step, C.case_id, C.seed, x, y, vx, vy, lcnt, b);
+            const T x0 = x, y0 = y;
+            if (drift) {
+                x = dfma(vx, dtdx, x);
+                y = dfma(vy, dtdy, y);
+            }
+            if (hits_walls(C, x0, y0, x, y)) {
+                atomicAdd(lcnt + PIC_C_ABSORBED, 1u);
+            } else {
+                alive = true;
+                if (dep) deposit(x, y, nx, ny, is_bin, wx0, wy0, aw, ah, accw, S.acc, lcnt);
+                key = bin_key(C, S.nb[out], x, y);
+            }
+            if (born) {
+                // born particles drift from the collision point when the pass drifts
+                const T ex1 = drift ? dfma(b.evx, dtdx, b.ex) : b.ex, ey1 = drift ? dfma(b.evy, dtdy, b.ey) : b.ey;
+                if (hits_walls(C, b.ex, b.ey, ex1, ey1)) {
+                    atomicAdd(lcnt + PIC_C_BORN_ABSORBED, 1u);
+                } else {
+                    be_alive = true;
+                    b.ex = ex1;
+                    b.ey = ey1;
+                    if (dep) deposit(ex1, ey1, nx, ny, is_bin, wx0, wy0, aw, ah, accw, S.acc, lcnt);
+                    key_e = bin_key(C, S.nb[out], ex1, ey1);
+                    atomicAdd(lcnt + PIC_C_BORN_SELF, 1u);
+                }
+                const T ix1 = drift ? dfma(b.ivx, dtdx, b.ix) : b.ix, iy1 = drift ? dfma(b.ivy, dtdy, b.iy) : b.iy;
+                if (hits_walls(C, b.ix, b.iy, ix1, iy1)) {
+                    atomicAdd(lcnt + PIC_C_BORN_ABSORBED, 1u);
+                } else if (O) {
+                    bi_alive = true;
+                    b.ix = ix1;
+                    b.iy = iy1;
+                    if (dep) deposit(ix1, iy1, nx, ny, false, 0, 0, 0, 0, accw, O->acc, lcnt);
+                    key_i = bin_key(C, O->nb[out], ix1, iy1);
+                    atomicAdd(lcnt + PIC_C_BORN_OTHER, 1u);
+                }
+            }
+        }
+        const int dst = alloc_slot(S, out, alive, key);
+        if (alive) {
+            if (dst >= 0) {
+                S.x[out][dst] = x;
+                S.y[out][dst] = y;
+                S.vx[out][dst] = vx;
+                S.vy[out][dst] = vy;
+                S.id[out][dst] = id;
+                if (dst >= S.start[out][S.nb[out]]) atomicAdd(lcnt + PIC_C_SPILLED, 1u);
+            } else {
+                atomicAdd(lcnt + PIC_C_LOST, 1u);
+            }
+        }
+        if (__ballot(be_alive || bi_alive)) {
+            const int de = alloc_slot(S, out, be_alive, key_e);
+            if (be_alive) {
+                if (de >= 0) {
+                    S.x[out][de] = b.ex;
+                    S.y[out][de] = b.ey;
+                    S.vx[out][de] = b.evx;
+                    S.vy[out][de] = b.evy;
+                    S.id[out][de] = b.eid;
+                } else {
+                    atomicAdd(lcnt + PIC_C_LOST, 1u);
+                }
+            }
+            if (O) {
+                const int di = alloc_slot(*O, out, bi_alive, key_i);
+                if (bi_alive) {
+                    if (di >= 0) {
+                        O->x[out][di] = b.ix;
+                        O->y[out][di] = b.iy;
+                        O->vx[out][di] = b.ivx;
+                        O->vy[out][di] = b.ivy;
+                        O->id[out][di] = b.iid;
+                    } else {
+                        atomicAdd(lcnt + PIC_C_LOST, 1u);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (is_bin) {
+        for (int k = threadIdx.x; k < aw * ah; k += blockDim.x) {
+            const unsigned long long v = accw[k];
+            if (v) {
+                const int i = wx0 + k % aw, j = wy0 + k / aw;
+                if (i >= 0 && i < nx && j >= 0 && j < ny) atomicAdd(S.acc + (size_t)j * nx + i, v);
+            }
+        }
+    }
+    for (int k = threadIdx.x; k < PIC_NCOUNT; k += blockDim.x)
+        if (lcnt[k]) atomicAdd(S.counters + k, (unsigned long long)lcnt[k]);
+}
+
+// ---------------------------------------------------------------------------
+// seeded initial particles (uniform in the open box, Maxwellian velocities)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void init_kernel(T* x, T* y, T* vx, T* vy, uint32_t* idv, const int32_t* base_ptr, int n, int nx,
+                            int ny, double vth, uint32_t seed, uint32_t case_id, uint32_t species,
+                            uint32_t id0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int dst = *base_ptr + i;
+    const uint32_t pid = id0 + (uint32_t)i;
+    Stream rs(pid, 0xFFFFFFFFu, case_id, seed, species);
+    const uint32_t rx = rs.u32(), ry = rs.u32();
+    x[dst] = (T)(((double)rx + 0.5) * 0x1p-32 * (double)(nx - 1));
+    y[dst] = (T)(((double)ry + 0.5) * 0x1p-32 * (double)(ny - 1));
+    double g1 = 0.0, g2 = 0.0;
+    gauss_pair<double>(rs, vth, g1, g2);
+    vx[dst] = (T)g1;
+    vy[dst] = (T)g2;
+    idv[dst] = pid;
+}
+
+__global__ void add_count_kernel(int32_t* c, int n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *c += n;
+}
+
+// ---------------------------------------------------------------------------
+// re-binning: histogram of new bin keys, then scatter into the new layout
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline int key_for(int nxm2, int nym2, int bw, int bh, int nbx, int nb, T x, T y) {
+    if (nb == 0) return 0;
+    int ix = (int)dfloor(x), iy = (int)dfloor(y);
+    ix = min(max(ix, 0), nxm2);
+    iy = min(max(iy, 0), nym2);
+    return (iy / bh) * nbx + (ix / bw);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void hist_kernel(const CaseDev<T>* __restrict__ cases,
+                                                   const CaseState* __restrict__ states,
+                                                   const WorkItem* __restrict__ items, int chunk, int bw, int bh,
+                                                   int nbx, int nb_new, int32_t* hist) {
+    const WorkItem it = items[blockIdx.x];
+    const CaseDev<T>& C = cases[it.case_idx];
+    const SpeciesDev<T>& S = C.sp[it.species];
+    const int in = states[it.case_idx].cur;
+    int begin, n;
+    item_range(S, in, it, chunk, begin, n);
+    if (n <= 0) return;
+    const int lane = __lane_id();
+    for (int base = 0; base < n; base += blockDim.x) {
+        const int t = base + threadIdx.x;
+        const bool want = t < n;
+        int key = 0;
+        if (want) key = key_for(C.nx - 2, C.ny - 2, bw, bh, nbx, nb_new, S.x[in][begin + t], S.y[in][begin + t]);
+        uint64_t act = __ballot(want);
+        while (act) {
+            const int leader = __ffsll((unsigned long long)act) - 1;
+            const int k = __shfl(key, leader);
+            const uint64_t m = __ballot(want && key == k) & act;
+            if (lane == leader) atomicAdd(hist + k, __popcll(m));
+            act &= ~m;
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void scatter_kernel(const CaseDev<T>* __restrict__ cases,
+                                                      const CaseState* __restrict__ states,
+                                                      const WorkItem* __restrict__ items, int chunk, int bw,
+                                                      int bh, int nbx, unsigned long long* lost) {
+    const WorkItem it = items[blockIdx.x];
+    const CaseDev<T>& C = cases[it.case_idx];
+    const SpeciesDev<T>& S = C.sp[it.species];
+    const int in = states[it.case_idx].cur, out = in ^ 1;
+    int begin, n;
+    item_range(S, in, it, chunk, begin, n);
+    if (n <= 0) return;
+    for (int base = 0; base < n; base += blockDim.x) {
+        const int t = base + threadIdx.x;
+        const bool want = t < n;
+        int key = 0;
+        T x = 0, y = 0, vx = 0, vy = 0;
+        uint32_t id = 0;
+        if (want) {
+            const int s = begin + t;
+            x = S.x[in][s];
+            y = S.y[in][s];
+            vx = S.vx[in][s];
+            vy = S.vy[in][s];
+            id = S.id[in][s];
+            key = key_for(C.nx - 2, C.ny - 2, bw, bh, nbx, S.nb[out], x, y);
+        }
+        const int dst = alloc_slot(S, out, want, key);
+        if (want) {
+            if (dst >= 0) {
+                S.x[out][dst] = x;
+                S.y[out][dst] = y;
+                S.vx[out][dst] = vx;
+                S.vy[out][dst] = vy;
+                S.id[out][dst] = id;
+            } else {
+                atomicAdd(lost, 1ull);
+            }
+        }
+    }
+}
+
+// flatten the input layout of one species into buffer `out` from index 0
+template <typename T>
+__global__ __launch_bounds__(256) void gather_kernel(const CaseDev<T>* __restrict__ cases,
+                                                     const CaseState* __restrict__ states,
+                                                     const WorkItem* __restrict__ items, int chunk,
+                                                     int32_t* counter) {
+    const WorkItem it = items[blockIdx.x];
+    const CaseDev<T>& C = cases[it.case_idx];
+    const SpeciesDev<T>& S = C.sp[it.species];
+    const int in = states[it.case_idx].cur, out = in ^ 1;
+    int begin, n;
+    item_range(S, in, it, chunk, begin, n);
+    if (n <= 0) return;
+    const int lane = __lane_id();
+    for (int base = 0; base < n; base += blockDim.x) {
+        const int t = base + threadIdx.x;
+        const bool want = t < n;
+        const uint64_t m = __ballot(want);
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        int b0 = 0;
+        if (lane == leader) b0 = atomicAdd(counter, __popcll(m));
+        b0 = __shfl(b0, leader);
+        if (want) {
+            const int s = begin + t, d = b0 + __popcll(m & lanemask_lt());
+            S.x[out][d] = S.x[in][s];
+            S.y[out][d] = S.y[in][s];
+            S.vx[out][d] = S.vx[in][s];
+            S.vy[out][d] = S.vy[in][s];
+            S.id[out][d] = S.id[in][s];
+        }
+    }
+}
+
+// spill fill of every species of every case (read back by the host after the
+// Poisson solve to decide on re-binning)
+template <typename T>
+__global__ void status_kernel(const CaseDev<T>* __restrict__ cases, const CaseState* __restrict__ states,
+                              int n_cases, int* out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_cases) return;
+    const CaseDev<T>& C = cases[c];
+    const int cur = states[c].cur;
+    for (int s = 0; s < PIC_MAX_SPECIES; ++s) {
+        int v = 0;
+        if (s < C.ns) {
+            const SpeciesDev<T>& S = C.sp[s];
+            v = S.count[cur][S.nb[cur]];
+        }
+        out[c * PIC_MAX_SPECIES + s] = v;
+    }
+}
+
+// number of particles held by one species of one case (sum over bins of
+// min(count, cap) plus the spill region), for last_alive
+template <typename T>
+__global__ __launch_bounds__(256) void count_kernel(const CaseDev<T>* __restrict__ cases,
+                                                    const CaseState* __restrict__ states, int c, int s,
+                                                    long long* out) {
+    __shared__ long long sh[256];
+    const CaseDev<T>& C = cases[c];
+    const int cur = states[c].cur;
+    const SpeciesDev<T>& S = C.sp[s];
+    long long acc = 0;
+    for (int b = threadIdx.x; b <= S.nb[cur]; b += 256) acc += min(S.count[cur][b], S.cap[cur][b]);
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
+// one block per case: clear the counts of the buffer that becomes the next
+// output, flip the parity and advance the step counter
+template <typename T>
+__global__ __launch_bounds__(256) void end_step_kernel(const CaseDev<T>* __restrict__ cases, CaseState* states,
+                                                       int advance_step) {
+    const int c = blockIdx.x;
+    const CaseDev<T>& C = cases[c];
+    const int in = states[c].cur;
+    __syncthreads();
+    for (int s = 0; s < C.ns; ++s) {
+        const SpeciesDev<T>& S = C.sp[s];
+        const int nb = S.nb[in];
+        for (int k = threadIdx.x; k <= nb; k += blockDim.x) S.count[in][k] = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        states[c].cur = in ^ 1;
+        if (advance_step) states[c].step += 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <typename T>
+int launch_push(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items,
+                int spill_chunk, int flags, long long mcc_key, size_t lds, hipStream_t s) {
+    if (n_items <= 0) return 0;
+    hipLaunchKernelGGL(push_kernel<T>, dim3(n_items), dim3(256), lds, s, cases, states, items, spill_chunk,
+                       flags, mcc_key);
+    PIC_HIP(hipGetLastError());
+    return 0;
+}
+
+template <typename T>
+int launch_init(T* x, T* y, T* vx, T* vy, uint32_t* id, int32_t* count_ptr, int n, int nx, int ny, double vth,
+                uint32_t seed, uint32_t case_id, uint32_t species, uint32_t id0, hipStream_t s) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(init_kernel<T>, dim3(div_up(n, 256)), dim3(256), 0, s, x, y, vx, vy, id, count_ptr, n, nx,
+                       ny, vth, seed, case_id, species, id0);
+    hipLaunchKernelGGL(add_count_kernel, dim3(1), dim3(64), 0, s, count_ptr, n);
+    PIC_HIP(hipGetLastError());
+    return 0;
+}
+
+template <typename T>
+int launch_hist(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items, int chunk,
+                int bw, int bh, int nbx, int nb_new, int32_t* hist, hipStream_t s) {
+    if (n_items <= 0) return 0;
+    hipLaunchKernelGGL(hist_kernel<T>, dim3(n_items), dim3(256), 0, s, cases, states, items, chunk, bw, bh, nbx,
+                       nb_new, hist);
+    PIC_HIP(hipGetLastError());
+    return 0;
+}
+
+template <typename T>
+int launch_scatter(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items,
+                   int chunk, int bw, int bh, int nbx, unsigned long long* lost, hipStream_t s) {
+    if (n_items <= 0) return 0;
+    hipLaunchKernelGGL(scatter_kernel<T>, dim3(n_items), dim3(256), 0, s, cases, states, items, chunk, bw, bh,
+                       nbx, lost);
+    PIC_HIP(hipGetLastError());
+    return 0;
+}
+
+template <typename T>
+int launch_gather(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items, int chunk,
+                  int32_t* counter, hipStream_t s) {
+    if (n_items <= 0) return 0;
+    hipLaunchKernelGGL(gather_kernel<T>, dim3(n_items), dim3(256), 0, s, cases, states, items, chunk, counter);
+    PIC_HIP(hipGetLastError());
+    return 0;
+}
+
+template <typename T>
+int launch_end_step(const CaseDev<T>* cases, CaseState* states, int n_cases, int advance, hipStream_t s) {
+    hipLaunchKernelGGL(end_step_kernel<T>, dim3(n_cases), dim3(256), 0, s, cases, states, advance);
+    PIC_HIP(hipGetLastError());
+    return 0;
+}
+
+template <typename T>
+int launch_status(const CaseDev<T>* cases, const CaseState* states, int n_cases, int* out, hipStream_t s) {
+    hipLaunchKernelGGL(status_kernel<T>, dim3(div_up(n_cases, 64)), dim3(64), 0, s, cases, states, n_cases, out);
+    PIC_HIP(hipGetLastError());
+    return 0;
+}
+
+template <typename T>
+int launch_counts(const CaseDev<T>* cases, const CaseState* states, int c, int sp, long long* out, hipStream_t s) {
+    hipLaunchKernelGGL(count_kernel<T>, dim3(1), dim3(256), 0, s, cases, states, c, sp, out);
+    PIC_HIP(hipGetLastError());
+    return 0;
+}
+
+#define PIC_LAUNCH_INST(T)                                                                                       \
+    template int launch_push<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int, int, int, long long,  \
+                                size_t, hipStream_t);                                                            \
+    template int launch_status<T>(const CaseDev<T>*, const CaseState*, int, int*, hipStream_t);                  \
+    template int launch_counts<T>(const CaseDev<T>*, const CaseState*, int, int, long long*, hipStream_t);       \
+    template int launch_init<T>(T*, T*, T*, T*, uint32_t*, int32_t*, int, int, int, double, uint32_t, uint32_t,  \
+                                uint32_t, uint32_t, hipStream_t);                                                \
+    template int launch_hist<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int, int, int, int, int,   \
+                                int, int32_t*, hipStream_t);                                                     \
+    template int launch_scatter<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int, int, int, int, int, \
+                                   unsigned long long*, hipStream_t);                                            \
+    template int launch_gather<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int, int, int32_t*,       \
+                                  hipStream_t);                                                                  \
+    template int launch_end_step<T>(const CaseDev<T>*, CaseState*, int, int, hipStream_t);
+PIC_LAUNCH_INST(float)
+PIC_LAUNCH_INST(double)
+
+// instantiate
+#define PIC_INST(T)                                                                                          \
+    template __global__ void push_kernel<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int, int,  \
+                                            long long);                                                      \
+    template __global__ void init_kernel<T>(T*, T*, T*, T*, uint32_t*, const int32_t*, int, int, int, double, \
+                                            uint32_t, uint32_t, uint32_t, uint32_t);                         \
+    template __global__ void hist_kernel<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int, int,  \
+                                            int, int, int, int32_t*);                                        \
+    template __global__ void scatter_kernel<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int,    \
+                                               int, int, int, unsigned long long*);                          \
+    template __global__ void gather_kernel<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int,     \
+                                              int32_t*);                                                     \
+    template __global__ void end_step_kernel<T>(const CaseDev<T>*, CaseState*, int);
+PIC_INST(float)
+PIC_INST(double)
+
+}  // namespace pic

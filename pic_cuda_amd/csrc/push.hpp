@@ -1,0 +1,36 @@
+// push.hpp -- launchers of the particle kernels in push.hip.
+#pragma once
+#include "pic_device.hpp"
+
+namespace pic {
+
+template <typename T>
+int launch_push(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items,
+                int spill_chunk, int flags, long long mcc_key, size_t lds, hipStream_t s);
+template <typename T>
+int launch_status(const CaseDev<T>* cases, const CaseState* states, int n_cases, int* out, hipStream_t s);
+template <typename T>
+int launch_counts(const CaseDev<T>* cases, const CaseState* states, int c, int sp, long long* out, hipStream_t s);
+template <typename T>
+int launch_init(T* x, T* y, T* vx, T* vy, uint32_t* id, int32_t* count_ptr, int n, int nx, int ny, double vth,
+                uint32_t seed, uint32_t case_id, uint32_t species, uint32_t id0, hipStream_t s);
+template <typename T>
+int launch_hist(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items, int chunk,
+                int bw, int bh, int nbx, int nb_new, int32_t* hist, hipStream_t s);
+template <typename T>
+int launch_scatter(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items,
+                   int chunk, int bw, int bh, int nbx, unsigned long long* lost, hipStream_t s);
+template <typename T>
+int launch_gather(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items, int chunk,
+                  int32_t* counter, hipStream_t s);
+template <typename T>
+int launch_end_step(const CaseDev<T>* cases, CaseState* states, int n_cases, int advance, hipStream_t s);
+
+// dynamic LDS of push_kernel for a bin tile
+template <typename T>
+inline size_t push_lds_bytes(int bw, int bh, int halo) {
+    const size_t aw = bw + 1 + 2 * halo, ah = bh + 1 + 2 * halo;
+    return aw * ah * 8 + (size_t)(bw + 1) * (bh + 1) * 2 * sizeof(T) + PIC_NCOUNT * 4;
+}
+
+}  // namespace pic

@@ -1,0 +1,223 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Contract (bit-exact unless stated):
+  * initial particles, E field, fixed-point charge, event counters and the
+    particle set after every pass: bit-identical to the oracle fed the same
+    state (FP32 particles against the oracle's FP32 mirror, FP64 against FP64);
+  * FP64 multigrid: bit-identical to the serial solver for a fixed cycle count;
+  * mixed-precision multigrid: relative L2 error against the FP64 oracle
+    solution within the tolerance written in each test.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _sorted(x, y, vx, vy, ids):
+    o = np.lexsort((vy, vx, y, x, ids))
+    return x[o], y[o], vx[o], vy[o], ids[o]
+
+
+def _assert_same_particles(gpu, orc_sp, prec):
+    dt = np.float32 if prec == 4 else np.float64
+    g = _sorted(*(np.asarray(a, dtype=dt) if k < 4 else a for k, a in enumerate(gpu)))
+    o = _sorted(orc_sp.x, orc_sp.y, orc_sp.vx, orc_sp.vy, orc_sp.id)
+    assert len(g[0]) == len(o[0])
+    for a, b in zip(g, o):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_init_matches_oracle(pic, orc, prec):
+    p = pic.Particles2D(50000, prec)
+    p.init_maxwellian(0, 40000, 65, 33, 2.5e5, seed=3, case_id=2, id0=100)
+    x, y, vx, vy, ids = p.download(0)
+    ox, oy, ovx, ovy, oid = orc.init_particles(prec, 40000, 65, 33, 2.5e5, 3, 2, 0, 100)
+    o = np.argsort(ids)
+    dt = np.float32 if prec == 4 else np.float64
+    for a, b in ((x, ox), (y, oy), (vx, ovx), (vy, ovy)):
+        assert np.array_equal(a[o].astype(dt), b)
+    assert np.array_equal(ids[o], oid)
+    p.close()
+
+
+def test_efield_matches_oracle(pic, orc):
+    g = pic.Grid2D(97, 97, 0.3, 0.3)
+    rng = np.random.default_rng(1)
+    phi = rng.normal(size=(97, 97))
+    g.upload(pic.FIELD_PHI, phi)
+    ex, ey = g.computeElectricField()
+    oex, oey = orc.efield(phi, g.dx, g.dy)
+    assert np.array_equal(ex, oex) and np.array_equal(ey, oey)
+
+
+def _mg_gpu(pic, n, rhs, phi0, h, params):
+    g = pic.Grid2D(n, n, h * (n - 1), h * (n - 1))
+    g.upload(pic.FIELD_PHI, phi0)
+    g.upload(pic.FIELD_RHO, rhs)
+    rho_p, phi_p, _ = g.device_ptrs()
+    mg = pic.Multigrid(n, params)
+    st = mg.solve(phi_p, rho_p, g.dx)
+    return g.download(pic.FIELD_PHI), st
+
+
+@pytest.mark.parametrize("n", [33, 65, 129, 257])
+def test_mg_fp64_bit_identical(pic, orc, n):
+    rng = np.random.default_rng(n)
+    rhs = rng.normal(size=(n, n))
+    phi0 = np.zeros((n, n))
+    phi0[0, :] = rng.normal(size=n)          # Dirichlet values kept
+    h = 1.0 / (n - 1)
+    params = pic.mg_params(pic.PRESET_STEP, mode=pic.MG_FP64, tol=0.0, max_cycles=3)
+    phi_g, st = _mg_gpu(pic, n, rhs, phi0, h, params)
+    phi_o = phi0.copy()
+    ost = orc.mg_solve(phi_o, rhs, h, pre=2, post=2, coarse_sweeps=1, omega=1.0, tol=0.0, max_cycles=3)
+    assert st.cycles == ost.cycles == 3
+    assert np.array_equal(phi_g, phi_o)
+
+
+def test_mg_reference_preset_bit_identical(pic, orc):
+    n = 129
+    rng = np.random.default_rng(11)
+    rhs = rng.normal(size=(n, n))
+    h = 1.0 / (n - 1)
+    params = pic.mg_params(pic.PRESET_REFERENCE, levels=5, tol=0.0, max_cycles=4)
+    phi_g, st = _mg_gpu(pic, n, rhs, np.zeros((n, n)), h, params)
+    phi_o = np.zeros((n, n))
+    orc.mg_solve(phi_o, rhs, h, levels=5, pre=3, post=3, coarse_sweeps=20, omega=1.25, tol=0.0, max_cycles=4)
+    assert np.array_equal(phi_g, phi_o)
+
+
+@pytest.mark.parametrize("n", [129, 513])
+def test_mg_mixed_precision_accuracy(pic, orc, n):
+    rng = np.random.default_rng(2 * n)
+    rhs = rng.normal(size=(n, n))
+    h = 1.0 / (n - 1)
+    params = pic.mg_params(pic.PRESET_POISSON)          # FP32 cycles in FP64 refinement, 1e-10
+    phi_g, st = _mg_gpu(pic, n, rhs, np.zeros((n, n)), h, params)
+    assert st.rel_residual < 1e-10
+    phi_o = np.zeros((n, n))
+    orc.mg_solve(phi_o, rhs, h, tol=1e-13, max_cycles=100)
+    err = np.linalg.norm(phi_g - phi_o) / np.linalg.norm(phi_o)
+    assert err < 1e-9, err
+
+
+def test_solve_multigrid_reference_signature(pic, orc):
+    n = 65
+    rng = np.random.default_rng(5)
+    rhs = rng.normal(size=(n, n))
+    h = 1.0 / (n - 1)
+    g = pic.Grid2D(n, n, 1.0, 1.0)
+    g.upload(pic.FIELD_RHO, rhs)
+    rho_p, phi_p, _ = g.device_ptrs()
+    pic.solve_multigrid(phi_p, rho_p, n, n, 5, h, 1.25)
+    phi_g = g.download(pic.FIELD_PHI)
+    phi_o = np.zeros((n, n))
+    st = orc.mg_solve(phi_o, rhs, h, levels=5, pre=3, post=3, coarse_sweeps=20, omega=1.25, tol=1e-5,
+                      max_cycles=50)
+    assert st.rel_residual < 1e-5
+    assert np.array_equal(phi_g, phi_o)
+
+
+# ------------------------------------------------------------ full passes ----
+def _small_case(pic, prec, mcc, cells=32, n=20000, gas=1e22, seed=7, bin_w=16, te=8.0):
+    from pic_cuda_amd import configs
+    return configs.build(cells, n, prec, mcc=mcc, seed=seed, gas_density=gas, bin_w=bin_w, bin_h=bin_w, te=te)
+
+
+def _oracle_from(pic, orc, case, prec):
+    sim, parts, grid = case["sim"], case["particles"], case["grid"]
+    sp = []
+    for s in range(parts.species_num):
+        x, y, vx, vy, ids = parts.download(s)
+        dt = np.float32 if prec == 4 else np.float64
+        info = sim.mcc_info(s) if case["xs"] is not None else None
+        kq = pic.ELEMENTARY_CHARGE * (-1 if s == 0 else 1) * case["plasma"].dt / (
+            pic.ELECTRON_MASS if s == 0 else 39.948 * pic.AMU)
+        sp.append(orc.Species(prec, x.astype(dt), y.astype(dt), vx.astype(dt), vy.astype(dt), ids, kq,
+                              mcc=info if info and info["n_proc"] else None,
+                              other=info["product_species"] if info else -1))
+    pl = case["plasma"]
+    w = pl.density * pl.length * pl.length / len(sp[0].x)   # same expression as configs.build
+    qs = 1.0 / float(1 << 40)
+    cfac = [-(q * w * qs / (grid.dx * grid.dy)) / pic.EPS0 for q in (-pic.ELEMENTARY_CHARGE, pic.ELEMENTARY_CHARGE)]
+    return orc.OracleSim(sp, grid.m, grid.n, grid.dx, grid.dy, pl.dt, cfac, dict(tol=1e-8, max_cycles=100))
+
+
+@pytest.mark.parametrize("prec,mcc", [(4, True), (8, True), (4, False)])
+def test_step_passes_bit_identical(pic, orc, prec, mcc):
+    case = _small_case(pic, prec, mcc)
+    sim, grid = case["sim"], case["grid"]
+    o = _oracle_from(pic, orc, case, prec)
+    for k in range(6):
+        o.phi = grid.download(pic.FIELD_PHI)
+        sim.step(1)
+        o.step(1, solve=False)
+        rhs_g = grid.download(pic.FIELD_RHS)
+        assert np.array_equal(rhs_g, o.rhs), "charge differs at pass %d" % k
+        for s in range(2):
+            c = sim.counters(s)
+            assert np.array_equal(c[:11], o.sp[s].counters[:11]), (k, s, c, o.sp[s].counters)
+    o.phi = grid.download(pic.FIELD_PHI)
+    sim.sync()
+    o.sync()
+    parts = case["particles"]
+    for s in range(2):
+        _assert_same_particles(parts.download(s), o.sp[s], prec)
+        c = sim.counters(s)
+        assert c[pic.CNT["lost"]] == 0
+    if mcc:
+        c0 = sim.counters(0)
+        assert c0[2] > 0 and c0[3] > 0 and c0[4] > 0, c0    # elastic, excitation and ionization all happened
+
+
+def test_field_solve_in_step_matches_fp64(pic, orc):
+    case = _small_case(pic, 4, True, cells=64, n=40000)
+    sim, grid = case["sim"], case["grid"]
+    sim.step(3)
+    rhs = grid.download(pic.FIELD_RHS)
+    phi_g = grid.download(pic.FIELD_PHI)
+    phi_o = phi_g.copy()
+    orc.mg_solve(phi_o, rhs, grid.dx, tol=1e-13, max_cycles=100)
+    err = np.linalg.norm(phi_g - phi_o) / np.linalg.norm(phi_o)
+    assert err < 1e-6, err
+
+
+def test_rebinning_does_not_change_results(pic, orc):
+    a = _small_case(pic, 4, True, bin_w=16)
+    b = _small_case(pic, 4, True, bin_w=8)
+    a["sim"].step(4)
+    b["sim"].step(2)
+    b["sim"].rebin(4, 4)
+    b["sim"].step(2)
+    assert np.array_equal(a["grid"].download(pic.FIELD_PHI), b["grid"].download(pic.FIELD_PHI))
+    for s in range(2):
+        ga = _sorted(*a["particles"].download(s))
+        gb = _sorted(*b["particles"].download(s))
+        for u, v in zip(ga, gb):
+            assert np.array_equal(u, v)
+        assert np.array_equal(a["sim"].counters(s)[:11], b["sim"].counters(s)[:11])
+
+
+def test_plain_small_fp64_end_to_end(pic, orc):
+    """128^2 cells, 1e4 + 1e4 particles (the plain config at a tenth of its
+    particles), FP64, 100 steps, MG 2+2 GS to 1e-8: survivor and absorbed counts
+    bit-exact, potential within 1e-10 relative L2 of the FP64 oracle."""
+    from pic_cuda_amd import configs
+    case = configs.plain_small(scale=0.1)
+    sim, grid = case["sim"], case["grid"]
+    o = _oracle_from(pic, orc, case, 8)
+    steps = 100
+    sim.step(steps)
+    o.step(steps)
+    phi_g = grid.download(pic.FIELD_PHI)
+    err = np.linalg.norm(phi_g - o.phi) / np.linalg.norm(o.phi)
+    assert err < 1e-10, err
+    sim.sync()
+    o.sync()
+    for s in range(2):
+        assert sim.counters(s)[0] == o.sp[s].counters[0]
+        assert case["particles"].count(s) == o.sp[s].n

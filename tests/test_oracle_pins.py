@@ -1,0 +1,420 @@
+"""Pins of the CPU oracle: known-answer vectors, closed forms, invariants and
+brute force.  Each check is chosen so that a plausible transcription mistake in
+the oracle (a dropped term, a swapped index, a wrong constant) fails it.
+
+These tests run on the CPU only."""
+import json
+import math
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ---------------------------------------------------------------- Philox ----
+def test_philox_known_answers(orc):
+    kat = json.load(open(os.path.join(GOLDEN, "philox4x32_10_kat.json")))
+    for v in kat["vectors"]:
+        ctr = [int(h, 16) for h in v["ctr"]]
+        key = [int(h, 16) for h in v["key"]]
+        out = ["%08x" % w for w in orc.philox(ctr, key)]
+        assert out == v["out"]
+
+
+def _philox_py(ctr, key):
+    """Philox4x32-10 written from the SC'11 paper's round function."""
+    m0, m1, w0, w1 = 0xD2511F53, 0xCD9E8D57, 0x9E3779B9, 0xBB67AE85
+    c = list(ctr)
+    k = list(key)
+    for r in range(10):
+        if r:
+            k = [(k[0] + w0) & 0xFFFFFFFF, (k[1] + w1) & 0xFFFFFFFF]
+        p0 = m0 * c[0]
+        p1 = m1 * c[2]
+        c = [((p1 >> 32) ^ c[1] ^ k[0]) & 0xFFFFFFFF, p1 & 0xFFFFFFFF, ((p0 >> 32) ^ c[3] ^ k[1]) & 0xFFFFFFFF,
+             p0 & 0xFFFFFFFF]
+    return c
+
+
+def test_philox_random_counters(orc):
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        ctr = [int(v) for v in rng.integers(0, 2**32, 4)]
+        key = [int(v) for v in rng.integers(0, 2**32, 2)]
+        assert list(orc.philox(ctr, key)) == _philox_py(ctr, key)
+
+
+# ------------------------------------------------------------ det_log -------
+def test_det_log_against_libm(orc):
+    rng = np.random.default_rng(1)
+    xs = np.concatenate([rng.uniform(1e-12, 1.0, 4000), 10.0 ** rng.uniform(-300, 300, 4000),
+                         [1.0, 0.5, 2.0, math.e, 0.7071067811865476, 1.4142135623730951, 1e-300, 1e300]])
+    for x in xs:
+        ref = math.log(x)
+        got = orc.det_log(float(x))
+        if ref == 0.0:
+            assert got == 0.0
+        else:
+            assert abs(got - ref) <= 2 * math.ulp(ref) + 1e-300, (x, got, ref)
+
+
+# ------------------------------------------------------------- E field ------
+def test_efield_linear_potential_is_uniform(orc):
+    nx, ny, dx, dy = 17, 13, 0.25, 0.5
+    a, b = 3.0, -1.75
+    j, i = np.mgrid[0:ny, 0:nx]
+    phi = a * i * dx + b * j * dy
+    ex, ey = orc.efield(phi, dx, dy)
+    assert np.allclose(ex, -a, rtol=0, atol=1e-12)
+    assert np.allclose(ey, -b, rtol=0, atol=1e-12)
+
+
+def test_efield_quadratic_interior_and_one_sided_edges(orc):
+    nx, ny, dx = 11, 9, 0.5
+    j, i = np.mgrid[0:ny, 0:nx]
+    x = i * dx
+    phi = x * x
+    ex, ey = orc.efield(phi, dx, dx)
+    # central difference of x^2 is exact: -2x
+    assert np.allclose(ex[:, 1:-1], -2 * x[:, 1:-1], atol=1e-12)
+    # one-sided at the edges (kernels.cu:143-170): -(phi1-phi0)/dx = -dx, -(phi_n-phi_{n-1})/dx
+    assert np.allclose(ex[:, 0], -dx, atol=1e-12)
+    assert np.allclose(ex[:, -1], -(x[:, -1] ** 2 - x[:, -2] ** 2) / dx, atol=1e-12)
+    assert np.allclose(ey, 0.0)
+
+
+# ------------------------------------------------------------- deposit ------
+@pytest.mark.parametrize("prec", [4, 8])
+def test_deposit_conserves_charge_exactly(orc, prec):
+    rng = np.random.default_rng(2)
+    nx = ny = 33
+    n = 5000
+    x = rng.uniform(0, nx - 1, n)
+    y = rng.uniform(0, ny - 1, n)
+    acc = orc.deposit(x, y, nx, ny, prec)
+    assert int(acc.sum(dtype=np.uint64)) == n * (1 << (2 * orc.QBITS))
+
+
+def test_deposit_hand_worked_weights(orc):
+    nx = ny = 5
+    W = 1 << orc.QBITS
+    # on a node: everything there
+    acc = orc.deposit([2.0], [1.0], nx, ny, 8)
+    assert acc[1, 2] == W * W and acc.sum() == W * W
+    # cell centre: a quarter on each corner
+    acc = orc.deposit([1.5], [2.5], nx, ny, 8)
+    q = W * W // 4
+    assert acc[2, 1] == q and acc[2, 2] == q and acc[3, 1] == q and acc[3, 2] == q
+    # fx = 0.25, fy = 0.75 (exact binary fractions): x weights (3/4,1/4), y (1/4,3/4)
+    acc = orc.deposit([0.25], [0.75], nx, ny, 8)
+    assert acc[0, 0] == (3 * W // 4) * (W // 4)
+    assert acc[0, 1] == (W // 4) * (W // 4)
+    assert acc[1, 0] == (3 * W // 4) * (3 * W // 4)
+    assert acc[1, 1] == (W // 4) * (3 * W // 4)
+    # outside the grid (kernels.cu:53): skipped
+    acc = orc.deposit([-0.5, 4.0, 1.0], [1.0, 1.0, 4.2], nx, ny, 8)
+    assert acc.sum() == 0
+
+
+def test_deposit_is_linear(orc):
+    rng = np.random.default_rng(3)
+    x = rng.uniform(0, 16, 300)
+    y = rng.uniform(0, 16, 300)
+    a = orc.deposit(x[:100], y[:100], 17, 17, 4)
+    b = orc.deposit(x[100:], y[100:], 17, 17, 4)
+    c = orc.deposit(x, y, 17, 17, 4)
+    assert np.array_equal(a + b, c)
+
+
+def test_charge_to_rhs(orc):
+    acc0 = np.array([1, 2, 3], dtype=np.uint64)
+    acc1 = np.array([4, 0, 1 << 40], dtype=np.uint64)
+    rhs = orc.charge_to_rhs([acc0, acc1], [-2.0, 0.5])
+    assert np.array_equal(rhs, np.array([-2.0 + 2.0, -4.0, -6.0 + 0.5 * 2.0**40]))
+
+
+# ----------------------------------------------------------- multigrid ------
+def _eigen_problem(n):
+    h = 1.0 / (n - 1)
+    j, i = np.mgrid[0:n, 0:n]
+    phi_exact = np.sin(np.pi * i * h) * np.sin(np.pi * j * h)
+    lam = -(4.0 / h**2) * (np.sin(np.pi * h / 2) ** 2 + np.sin(np.pi * h / 2) ** 2)
+    return h, phi_exact, lam * phi_exact
+
+
+@pytest.mark.parametrize("n", [17, 65, 129])
+def test_mg_discrete_eigenfunction(orc, n):
+    h, exact, rhs = _eigen_problem(n)
+    phi = np.zeros((n, n))
+    st = orc.mg_solve(phi, rhs, h, tol=1e-11, max_cycles=60)
+    assert st.rel_residual < 1e-11
+    assert np.max(np.abs(phi - exact)) < 1e-9
+
+
+def _dense_laplacian_solve(rhs, h, bc):
+    n = rhs.shape[0]
+    m = n - 2
+    A = np.zeros((m * m, m * m))
+    b = rhs[1:-1, 1:-1].copy().ravel() * h * h
+    for jj in range(m):
+        for ii in range(m):
+            r = jj * m + ii
+            A[r, r] = -4.0
+            for di, dj in ((1, 0), (-1, 0), (0, 1), (0, -1)):
+                I, J = ii + di, jj + dj
+                if 0 <= I < m and 0 <= J < m:
+                    A[r, J * m + I] = 1.0
+                else:
+                    b[r] -= bc[J + 1, I + 1]
+    sol = np.linalg.solve(A, b)
+    out = bc.copy()
+    out[1:-1, 1:-1] = sol.reshape(m, m)
+    return out
+
+
+def test_mg_matches_dense_direct_solve_with_dirichlet_values(orc):
+    n = 17
+    rng = np.random.default_rng(4)
+    rhs = rng.normal(size=(n, n))
+    bc = np.zeros((n, n))
+    bc[0, :] = rng.normal(size=n)
+    bc[-1, :] = rng.normal(size=n)
+    bc[:, 0] = rng.normal(size=n)
+    bc[:, -1] = rng.normal(size=n)
+    h = 0.1
+    ref = _dense_laplacian_solve(rhs, h, bc)
+    phi = bc.copy()
+    st = orc.mg_solve(phi, rhs, h, tol=1e-13, max_cycles=80)
+    assert np.max(np.abs(phi - ref)) < 1e-10 * np.max(np.abs(ref))
+    # boundary untouched (Dirichlet)
+    assert np.array_equal(phi[0], bc[0]) and np.array_equal(phi[:, -1], bc[:, -1])
+    assert st.cycles > 0
+
+
+def test_mg_vcycle_contraction(orc):
+    n = 129
+    rng = np.random.default_rng(6)
+    rhs = rng.normal(size=(n, n))
+    h = 1.0 / (n - 1)
+    phi = np.zeros((n, n))
+    rels = []
+    for k in range(1, 6):
+        p = phi.copy()
+        st = orc.mg_solve(p, rhs, h, tol=0.0, max_cycles=k)
+        rels.append(st.rel_residual)
+    for a, b in zip(rels, rels[1:]):
+        assert b < 0.2 * a
+
+
+def test_mg_reference_preset_converges(orc):
+    # Multigrid.cu defaults: SOR 1.25, 3 pre / 3 post, 20 coarse sweeps, 5 levels at 129^2
+    n = 129
+    h, exact, rhs = _eigen_problem(n)
+    phi = np.zeros((n, n))
+    st = orc.mg_solve(phi, rhs, h, levels=5, pre=3, post=3, coarse_sweeps=20, omega=1.25, tol=1e-9,
+                      max_cycles=50)
+    assert st.rel_residual < 1e-9
+
+
+def test_mg_transfer_operators(orc):
+    # restriction of a constant interior field is the full-weighting average;
+    # prolongation of a bilinear coarse function is exact
+    nc = 9
+    nf = 2 * (nc - 1) + 1
+    J, I = np.mgrid[0:nc, 0:nc]
+    coarse = (2.0 * I + 3.0 * J + 1.0).astype(np.float64)
+    fine = np.zeros((nf, nf))
+    import ctypes
+    lib = orc.lib()
+    lib.orc_mg_prolong(fine.ctypes.data_as(ctypes.c_void_p), coarse.ctypes.data_as(ctypes.c_void_p), nc)
+    jf, i_f = np.mgrid[0:nf, 0:nf]
+    assert np.allclose(fine, 2.0 * (i_f / 2.0) + 3.0 * (jf / 2.0) + 1.0)
+    f = np.ones((nf, nf))
+    c = np.zeros((nc, nc))
+    lib.orc_mg_restrict(c.ctypes.data_as(ctypes.c_void_p), f.ctypes.data_as(ctypes.c_void_p), nf)
+    assert np.allclose(c[1:-1, 1:-1], 1.0)
+    assert np.isclose(c[0, 0], 9.0 / 16.0)      # corner keeps 4+2+2+1 of 16
+    assert np.allclose(c[0, 1:-1], 12.0 / 16.0)  # edge keeps 4+2+2+2+1+1 of 16
+
+
+# --------------------------------------------------------------- walls ------
+def _hits_exact(p1, p2, w):
+    """Segment intersection in exact rational arithmetic (inclusive)."""
+    x1, y1 = map(Fraction, p1)
+    x2, y2 = map(Fraction, p2)
+    x3, y3, x4, y4 = map(Fraction, w)
+    den = (y4 - y3) * (x2 - x1) - (x4 - x3) * (y2 - y1)
+    if den == 0:
+        return False
+    ua = ((x4 - x3) * (y1 - y3) - (y4 - y3) * (x1 - x3)) / den
+    ub = ((x2 - x1) * (y1 - y3) - (y2 - y1) * (x1 - x3)) / den
+    return 0 <= ua <= 1 and 0 <= ub <= 1
+
+
+def test_segment_test_brute_force(orc):
+    rng = np.random.default_rng(7)
+    # coordinates on a dyadic lattice: every intermediate is exact in FP64
+    for _ in range(3000):
+        p = rng.integers(0, 16, 8) / 4.0
+        w = p[4:]
+        got = orc.segment_hits(p[0], p[1], p[2], p[3], w, eps=1e-12)
+        assert got == _hits_exact(p[:2], p[2:4], w), p
+
+
+def test_segment_test_special_cases(orc):
+    wall = [1.0, 0.0, 1.0, 2.0]
+    assert orc.segment_hits(0.5, 1.0, 1.0, 1.0, wall)       # ends on the wall (ua = 1 inclusive)
+    assert orc.segment_hits(1.0, 1.0, 1.5, 1.0, wall)       # starts on the wall (ua = 0 inclusive)
+    assert orc.segment_hits(0.5, 0.5, 1.5, -0.5, wall)      # through the wall's end point (ub = 0)
+    assert not orc.segment_hits(0.5, 3.0, 1.5, 3.0, wall)   # passes beyond the wall
+    assert not orc.segment_hits(1.0, 0.5, 1.0, 1.5, wall)   # parallel (collinear) is cut
+    assert not orc.segment_hits(0.5, 0.0, 0.5, 2.0, wall)   # parallel
+
+
+# ----------------------------------------------------------------- push -----
+def _species(orc, prec, n, rng, nx, kq=1e-3, v=0.0):
+    x = rng.uniform(2, nx - 3, n)
+    y = rng.uniform(2, nx - 3, n)
+    vx = np.full(n, v)
+    vy = np.zeros(n)
+    return orc.Species(prec, x, y, vx, vy, np.arange(n, dtype=np.uint32), kq)
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_push_uniform_field_closed_form(orc, prec):
+    rng = np.random.default_rng(8)
+    nx = 33
+    dx = 0.01
+    E0 = 250.0
+    j, i = np.mgrid[0:nx, 0:nx]
+    phi = -E0 * i * dx
+    sp = _species(orc, prec, 500, rng, nx, kq=2e-4)
+    x0, vx0 = sp.x.copy(), sp.vx.copy()
+    dtdx = 1e-3
+    orc.push(sp, phi, nx, nx, dx, dx, dtdx, dtdx, dict(kick=True, drift=True), 0, 1, 0, 0)
+    order = np.argsort(sp.id)
+    tol = 1e-5 if prec == 4 else 1e-12
+    dv = 2e-4 * E0
+    assert np.allclose(sp.vx[order], vx0 + dv, rtol=tol, atol=tol * dv)
+    assert np.allclose(sp.x[order], x0 + (vx0 + dv) * dtdx, rtol=tol, atol=tol)
+    assert sp.counters[orc.C_ABSORBED] == 0
+
+
+def test_push_box_walls_absorb_and_compact_stably(orc):
+    nx = 17
+    x = np.array([1.0, 15.5, 8.0, 0.3, 10.0])
+    vx = np.array([0.0, 1000.0, 0.0, -500.0, 0.0])   # second and fourth leave the box
+    sp = orc.Species(8, x, np.full(5, 8.0), vx, np.zeros(5), np.arange(5, dtype=np.uint32), 0.0)
+    acc = np.zeros(nx * nx, dtype=np.uint64)
+    orc.push(sp, np.zeros((nx, nx)), nx, nx, 1.0, 1.0, 1e-3, 1e-3, dict(drift=True), 0, 1, 0, 0, acc_self=acc)
+    assert list(sp.id) == [0, 2, 4]                  # survivors in input order
+    assert sp.counters[orc.C_ABSORBED] == 2
+    assert int(acc.sum()) == 3 * (1 << 40)
+
+
+# ------------------------------------------------------------------ MCC -----
+def _mcc_const(orc, n_gas, sigma, m, dt, K=64, emax=100.0, types=(0,), thr=(0.0,), extra=None):
+    QE = 1.602176634e-19
+    de = emax / (K - 1)
+    tab = np.full((len(types), K), n_gas * sigma)
+    E = np.arange(K) * de
+    if extra is not None:
+        tab = extra(E)
+    efac = 0.5 * m / QE
+    vmax = math.sqrt(emax / efac)
+    numax = float(np.max(tab.sum(axis=0)) * vmax) * 1.001
+    return dict(n_proc=len(types), type=list(types), threshold=list(thr), table=tab.ravel(), K=K,
+                inv_de=1.0 / de, nu_max=numax, p_null=-math.expm1(-numax * dt), efac=efac, gas_vth=300.0)
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_mcc_elastic_rate_and_speed(orc, prec):
+    QE, ME = 1.602176634e-19, 9.1093837015e-31
+    n = 200000
+    dt = 2e-9
+    m = _mcc_const(orc, 1e21, 1e-19, ME, dt)
+    v = math.sqrt(2 * 10.0 * QE / ME)            # 10 eV electrons
+    sp = orc.Species(prec, np.full(n, 8.0), np.full(n, 8.0), np.full(n, v), np.zeros(n),
+                     np.arange(n, dtype=np.uint32), 0.0, mcc=m)
+    orc.push(sp, np.zeros((17, 17)), 17, 17, 1.0, 1.0, 0.0, 0.0, dict(mcc=True), 3, 9, 0, 0)
+    hits = int(sp.counters[orc.C_PROC0])
+    p = m["p_null"] * (1e21 * 1e-19 * v) / m["nu_max"]
+    mu, sd = n * p, math.sqrt(n * p * (1 - p))
+    assert abs(hits - mu) < 5 * sd
+    speed = np.hypot(sp.vx, sp.vy)
+    assert np.allclose(speed, v, rtol=3e-6 if prec == 4 else 1e-13)
+    scattered = sp.vy != 0
+    assert scattered.sum() == hits
+    ang = np.arctan2(sp.vy[scattered], sp.vx[scattered])
+    hist, _ = np.histogram(ang, bins=8, range=(-np.pi, np.pi))
+    assert np.all(np.abs(hist - hits / 8) < 6 * np.sqrt(hits / 8))
+
+
+def test_mcc_excitation_and_ionization_energy_bookkeeping(orc):
+    QE, ME = 1.602176634e-19, 9.1093837015e-31
+    n = 100000
+    dt = 5e-9
+    K, emax = 256, 100.0
+    E = np.arange(K) * emax / (K - 1)
+
+    def tables(Eg):
+        exc = np.where(Eg > 11.5, 1e21 * 4e-20, 0.0)
+        ion = np.where(Eg > 15.8, 1e21 * 6e-20, 0.0)
+        return np.stack([exc, ion])
+
+    m = _mcc_const(orc, 1e21, 0, ME, dt, K=K, emax=emax, types=(1, 2), thr=(11.5, 15.8), extra=tables)
+    e0 = 40.0
+    v = math.sqrt(2 * e0 * QE / ME)
+    sp = orc.Species(8, np.full(n, 8.0), np.full(n, 8.0), np.full(n, v), np.zeros(n), np.arange(n, dtype=np.uint32),
+                     0.0, mcc=m, other=1)
+    born = orc.push(sp, np.zeros((17, 17)), 17, 17, 1.0, 1.0, 0.0, 0.0, dict(mcc=True), 0, 4, 0, 0)
+    n_exc, n_ion = int(sp.counters[orc.C_PROC0]), int(sp.counters[orc.C_PROC0 + 1])
+    assert n_exc > 100 and n_ion > 100
+    assert sp.n == n + n_ion and born is not None and len(born[0]) == n_ion
+    energy = m["efac"] * (sp.vx**2 + sp.vy**2)
+    prim = energy[:n]
+    assert np.sum(np.isclose(prim, e0 - 11.5, rtol=1e-12)) == n_exc
+    # primary + secondary energy = E - threshold for every ionization
+    ids = sp.id[:n]
+    changed = ~np.isclose(prim, e0, rtol=1e-12) & ~np.isclose(prim, e0 - 11.5, rtol=1e-12)
+    assert changed.sum() == n_ion
+    assert np.isclose(prim[changed].sum() + energy[n:].sum(), n_ion * (e0 - 15.8), rtol=1e-9)
+    # born electrons and ions sit at the collision point (no drift in this pass)
+    assert np.all(born[0] == 8.0) and np.all(sp.x[n:] == 8.0)
+    assert len(set(ids.tolist())) == n
+
+
+def test_mcc_is_keyed_on_id_and_step(orc):
+    QE, ME = 1.602176634e-19, 9.1093837015e-31
+    m = _mcc_const(orc, 1e21, 1e-19, ME, 5e-9)
+    v = math.sqrt(2 * 5.0 * QE / ME)
+    n = 20000
+
+    def run(step, perm):
+        sp = orc.Species(4, np.full(n, 8.0), np.full(n, 8.0), np.full(n, v), np.zeros(n),
+                         np.arange(n, dtype=np.uint32)[perm], 0.0, mcc=m)
+        orc.push(sp, np.zeros((17, 17)), 17, 17, 1.0, 1.0, 0.0, 0.0, dict(mcc=True), step, 11, 0, 0)
+        o = np.argsort(sp.id)
+        return sp.vx[o], sp.vy[o]
+
+    a = run(5, np.arange(n))
+    b = run(5, np.random.default_rng(1).permutation(n))
+    c = run(6, np.arange(n))
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert not np.array_equal(a[1], c[1])
+
+
+# --------------------------------------------------------------- init -------
+def test_init_statistics(orc):
+    n = 200000
+    x, y, vx, vy, ids = orc.init_particles(8, n, 65, 65, 3.0, 1, 0, 0)
+    assert x.min() > 0 and x.max() < 64 and y.min() > 0 and y.max() < 64
+    assert abs(x.mean() - 32) < 0.2
+    assert abs(vx.std() - 3.0) < 0.03 and abs(vy.mean()) < 0.03
+    assert np.array_equal(ids, np.arange(n))
+    # kurtosis of a Gaussian is 3
+    k = np.mean(vx**4) / np.mean(vx**2) ** 2
+    assert abs(k - 3.0) < 0.05
