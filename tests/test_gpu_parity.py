@@ -128,7 +128,7 @@ def _small_case(pic, prec, mcc, cells=32, n=20000, gas=1e22, seed=7, bin_w=16, t
     return configs.build(cells, n, prec, mcc=mcc, seed=seed, gas_density=gas, bin_w=bin_w, bin_h=bin_w, te=te)
 
 
-def _oracle_from(pic, orc, case, prec):
+def _oracle_from(pic, orc, case, prec, seed=7):
     sim, parts, grid = case["sim"], case["particles"], case["grid"]
     sp = []
     for s in range(parts.species_num):
@@ -144,7 +144,8 @@ def _oracle_from(pic, orc, case, prec):
     w = pl.density * pl.length * pl.length / len(sp[0].x)   # same expression as configs.build
     qs = 1.0 / float(1 << 40)
     cfac = [-(q * w * qs / (grid.dx * grid.dy)) / pic.EPS0 for q in (-pic.ELEMENTARY_CHARGE, pic.ELEMENTARY_CHARGE)]
-    return orc.OracleSim(sp, grid.m, grid.n, grid.dx, grid.dy, pl.dt, cfac, dict(tol=1e-8, max_cycles=100))
+    return orc.OracleSim(sp, grid.m, grid.n, grid.dx, grid.dy, pl.dt, cfac, dict(tol=1e-8, max_cycles=100),
+                         seed=seed)
 
 
 @pytest.mark.parametrize("prec,mcc", [(4, True), (8, True), (4, False)])
@@ -209,7 +210,7 @@ def test_plain_small_fp64_end_to_end(pic, orc):
     from pic_cuda_amd import configs
     case = configs.plain_small(scale=0.1)
     sim, grid = case["sim"], case["grid"]
-    o = _oracle_from(pic, orc, case, 8)
+    o = _oracle_from(pic, orc, case, 8, seed=1)
     steps = 100
     sim.step(steps)
     o.step(steps)
