@@ -78,6 +78,141 @@ def cpu_baseline(gpu_case, budget_s=12.0):
                       "(same ppc, MCC and 1e-8 FP64 multigrid)" % steps}
 
 
+def poisson(n=2049, cpu=True):
+    """Mixed-precision Poisson alone: 2049^2 Dirichlet grid, 200 Gaussian blobs
+    + 1e-3 noise, FP32 V-cycles in FP64 refinement vs FP64-only V-cycles, both to
+    ||r||/||rhs|| < 1e-10 from a zero start."""
+    import numpy as np
+    import pic_cuda_amd as pic
+    from pic_cuda_amd import configs
+    rhs = configs.poisson_rhs(n)
+    h = 1.0 / (n - 1)
+    g = pic.Grid2D(n, n, 1.0, 1.0)
+    g.upload(pic.FIELD_RHO, rhs)
+    rho_p, phi_p, _ = g.device_ptrs()
+    out = {}
+    sols = {}
+    for name, prm in (("mixed", pic.mg_params(pic.PRESET_POISSON)),
+                      ("fp64", pic.mg_params(pic.PRESET_POISSON, mode=pic.MG_FP64))):
+        mg = pic.Multigrid(n, prm)
+        best = None
+        for _ in range(3):
+            g.upload(pic.FIELD_PHI, np.zeros((n, n)))
+            pic.synchronize()
+            st = mg.solve(phi_p, rho_p, h)
+            best = st if best is None or st.ms < best.ms else best
+        sols[name] = g.download(pic.FIELD_PHI)
+        cyc = best.cycles
+        byts = n * n * (32 * cyc + 40 * (cyc + 1)) if name == "mixed" else n * n * 64 * cyc
+        out[name] = {"ms": round(best.ms, 3), "cycles": cyc, "rel_residual": best.rel_residual,
+                     "launches": best.launches, "algorithmic_GBps": round(byts / (best.ms * 1e-3) / 1e9, 1)}
+    ref = sols["fp64"]
+    out["mixed_vs_fp64_rel_l2"] = float(np.linalg.norm(sols["mixed"] - ref) / np.linalg.norm(ref))
+    out["speedup_mixed_over_fp64"] = round(out["fp64"]["ms"] / out["mixed"]["ms"], 3)
+    if cpu:
+        from oracle import oracle as orc
+        phi = np.zeros((n, n))
+        t0 = time.perf_counter()
+        st = orc.mg_solve(phi, rhs, h, tol=1e-10, max_cycles=100)
+        el = time.perf_counter() - t0
+        out["cpu_fp64_oracle"] = {"ms": round(el * 1e3, 1), "cycles": st.cycles, "cores": 1}
+        out["mixed_vs_cpu_fp64_rel_l2"] = float(np.linalg.norm(sols["mixed"] - phi) / np.linalg.norm(phi))
+    return out
+
+
+def ensemble_cases(n_cases=64, seed=4):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    cases = []
+    for k in range(n_cases):
+        cells = int(max(64, min(512, round(np.exp(rng.uniform(np.log(64), np.log(512))) / 32) * 32)))
+        npart = int(min(4e6, max(1e4, 100 * cells * cells)))
+        te = float(rng.uniform(1.0, 5.0))
+        cases.append((cells, npart, te, 1000 + k))
+    return cases
+
+
+def ensemble(n_cases=64, steps=100, sequential=True):
+    """Batched ensemble: 64 plasmas, cells log-uniform 64..512 (multiples of 32),
+    clamp(100 cells^2, 1e4, 4e6) particles per species, Te ~ U(1,5) eV, MCC on;
+    one launch per stage for all cases, against the same cases one after
+    another through the single-plasma path."""
+    import pic_cuda_amd as pic
+    from pic_cuda_amd import configs
+    specs = ensemble_cases(n_cases)
+    res = {"cases": n_cases, "steps": steps,
+           "particles": int(sum(2 * s[1] for s in specs))}
+    if sequential:
+        t_seq = 0.0
+        for cells, npart, te, seed in specs:
+            c = configs.build(cells, npart, pic.PREC_FP32, mcc=True, seed=seed, te=te)
+            pic.synchronize()
+            t0 = time.perf_counter()
+            c["sim"].step(steps)
+            pic.synchronize()
+            t_seq += time.perf_counter() - t0
+            del c
+        res["sequential_s"] = round(t_seq, 3)
+        res["sequential_jobs_per_s"] = n_cases / t_seq
+    built = [configs.build(cells, npart, pic.PREC_FP32, mcc=True, seed=seed, te=te)
+             for cells, npart, te, seed in specs]
+    ens = pic.Ensemble()
+    for c in built:
+        ens.add(c["sim"])
+    ens.step(1)                      # builds the batch tables; counted below as a step
+    pic.synchronize()
+    t0 = time.perf_counter()
+    ens.step(steps - 1)
+    pic.synchronize()
+    el = time.perf_counter() - t0
+    t_batch = el * steps / (steps - 1)
+    res["batched_s"] = round(t_batch, 3)
+    res["batched_jobs_per_s"] = n_cases / t_batch
+    res["batched_ms_per_step"] = round(1e3 * el / (steps - 1), 3)
+    res["batched_particle_steps_per_s"] = res["particles"] * steps / t_batch
+    if sequential:
+        res["speedup_batched_over_sequential"] = round(t_seq / t_batch, 2)
+    ens.close()
+    return res
+
+
+def serve(n_jobs=24, rate=20.0, budget=200e9, seed=9, max_steps=2000):
+    """Serving: Poisson arrivals (rate jobs/s), cells log-uniform 64..1024
+    (multiples of 32), 100 particles per cell and species, steps U(200, max),
+    seed = job id, admission by an HBM budget, continuous batching."""
+    import numpy as np
+    import pic_cuda_amd as pic
+    rng = np.random.default_rng(seed)
+    t_arr = np.cumsum(rng.exponential(1.0 / rate, n_jobs))
+    sv = pic.Server(budget, 256, pic.PREC_FP32)
+    ids = []
+    for k in range(n_jobs):
+        cells = int(max(64, min(1024, round(np.exp(rng.uniform(np.log(64), np.log(1024))) / 32) * 32)))
+        steps = int(rng.integers(200, max_steps + 1))
+        ids.append(sv.submit(cells, 100.0, steps, seed=k, arrival_s=float(t_arr[k])))
+    t0 = time.perf_counter()
+    ticks = 0
+    while True:
+        now = time.perf_counter() - t0
+        run, wait = sv.tick(now)
+        ticks += 1
+        if run == 0 and wait == 0:
+            break
+        if run == 0:
+            time.sleep(max(0.0, min(0.05, float(min(t_arr[i] for i in range(n_jobs)
+                                                         if sv.result(ids[i]).state == 0)) - now)))
+    wall = time.perf_counter() - t0
+    results = [sv.result(j) for j in ids]
+    step_ms = np.concatenate([sv.step_ms(j) for j in ids])
+    comp = np.array([r.done_s - r.arrival_s for r in results])
+    psteps = sum(r.particles * r.steps_done for r in results)
+    sv.close()
+    return {"jobs": n_jobs, "rate_per_s": rate, "wall_s": round(wall, 3), "jobs_per_s": n_jobs / wall,
+            "ticks": ticks, "particle_steps_per_s": psteps / wall,
+            "step_ms_p50": float(np.percentile(step_ms, 50)), "step_ms_p99": float(np.percentile(step_ms, 99)),
+            "completion_s_p50": float(np.percentile(comp, 50)), "completion_s_p99": float(np.percentile(comp, 99))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,7 +222,17 @@ def main():
     ap.add_argument("--cells", type=int, default=1024)
     ap.add_argument("--bin", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="all", choices=["all", "step", "poisson", "ensemble", "serve"])
+    ap.add_argument("--ens-steps", type=int, default=100)
+    ap.add_argument("--serve-jobs", type=int, default=24)
     a = ap.parse_args()
+    if a.workload in ("poisson", "ensemble", "serve"):
+        import pic_cuda_amd as pic
+        pic.load()
+        fn = {"poisson": lambda: poisson(cpu=not a.no_cpu), "ensemble": lambda: ensemble(steps=a.ens_steps),
+              "serve": lambda: serve(n_jobs=a.serve_jobs)}[a.workload]
+        print(json.dumps({"workload": a.workload, **fn()}))
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; the multi-GPU decomposition is a design only "
               "(domain strips with particle migration and halo exchange)", file=sys.stderr)

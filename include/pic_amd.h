@@ -231,6 +231,56 @@ int pic_sim_mcc_info(pic_sim* s, int species, pic_mcc_info* out, double* table, 
 int pic_sim_rebin(pic_sim* s, int bin_w, int bin_h);
 int pic_sim_destroy(pic_sim* s);
 
+/* ---- ensembles of independent plasmas (no counterpart in the reference: it
+   steps one plasma).  Members step together, one launch per stage for all of
+   them; each member computes exactly what it computes alone. ------------- */
+typedef struct pic_ensemble pic_ensemble;
+int pic_ensemble_create(pic_ensemble** out);
+/* members need the same particle precision and multigrid cycle parameters */
+int pic_ensemble_add(pic_ensemble* e, pic_sim* s);
+int pic_ensemble_remove(pic_ensemble* e, pic_sim* s);
+int pic_ensemble_size(const pic_ensemble* e, int* n);
+/* n batched steps of updateSimulationCUDA (main.cu:37-140) for every member */
+int pic_ensemble_step(pic_ensemble* e, int n_steps);
+int pic_ensemble_destroy(pic_ensemble* e);
+
+/* ---- job server (no counterpart in the reference): continuous batching of
+   PIC jobs over an ensemble, admitted by HBM footprint. --------------------- */
+typedef struct pic_server pic_server;
+typedef struct {
+    int cells;           /* cells per side (grid of cells+1 nodes) */
+    double ppc;          /* macro-particles per cell and species */
+    int steps;
+    uint32_t seed;       /* Philox seed of the job */
+    double te_eV;        /* electron temperature (0: 2 eV) */
+    double density;      /* plasma density m^-3 (0: 1e15) */
+    double gas_density;  /* Ar background for MCC (0: 1e21) */
+    double arrival_s;    /* earliest admission time on the caller's clock */
+    int mcc;             /* null-collision MCC on */
+} pic_job_desc;
+typedef struct {
+    int state;           /* 0 waiting, 1 running, 2 finished, 3 released, -1 failed */
+    double arrival_s, admit_s, done_s;
+    int steps_done;
+    double step_ms_mean, step_ms_max;
+    int64_t particles, alive_end;
+    double footprint_bytes;
+} pic_job_result;
+/* base: step-driver parameters shared by all jobs (dt and seed are per job);
+   keep_finished: finished jobs keep their objects until pic_server_release */
+int pic_server_create(double hbm_budget_bytes, int max_active, int precision, const pic_sim_params* base,
+                      int keep_finished, pic_server** out);
+int pic_server_submit(pic_server* sv, const pic_job_desc* job, int* job_id);
+/* admit arrived jobs that fit, one batched step of every running job, retire
+   the finished ones; now_s is the caller's clock */
+int pic_server_tick(pic_server* sv, double now_s, int* n_running, int* n_waiting);
+int pic_server_job_result(pic_server* sv, int job_id, pic_job_result* out);
+int pic_server_job_step_ms(pic_server* sv, int job_id, int cap, float* out, int* n);
+int pic_server_job_handles(pic_server* sv, int job_id, pic_particles** p, pic_grid** g, pic_sim** s);
+int pic_server_release(pic_server* sv, int job_id);
+int pic_server_used_bytes(const pic_server* sv, double* used);
+int pic_server_destroy(pic_server* sv);
+
 #ifdef __cplusplus
 }
 #endif

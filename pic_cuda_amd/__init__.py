@@ -79,6 +79,19 @@ class MccInfo(ctypes.Structure):
                 ("product_species", ctypes.c_int)]
 
 
+class JobDesc(ctypes.Structure):
+    _fields_ = [("cells", ctypes.c_int), ("ppc", ctypes.c_double), ("steps", ctypes.c_int), ("seed", ctypes.c_uint32),
+                ("te_eV", ctypes.c_double), ("density", ctypes.c_double), ("gas_density", ctypes.c_double),
+                ("arrival_s", ctypes.c_double), ("mcc", ctypes.c_int)]
+
+
+class JobResult(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int), ("arrival_s", ctypes.c_double), ("admit_s", ctypes.c_double),
+                ("done_s", ctypes.c_double), ("steps_done", ctypes.c_int), ("step_ms_mean", ctypes.c_double),
+                ("step_ms_max", ctypes.c_double), ("particles", ctypes.c_int64), ("alive_end", ctypes.c_int64),
+                ("footprint_bytes", ctypes.c_double)]
+
+
 # every symbol declared in include/pic_amd.h, with its ctypes signature
 _vp, _i, _d, _u32, _i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_uint32, ctypes.c_int64
 _P = ctypes.POINTER
@@ -131,6 +144,21 @@ SIGNATURES = {
     "pic_sim_mcc_info": (_i, [_vp, _i, _P(MccInfo), _vp, _i]),
     "pic_sim_rebin": (_i, [_vp, _i, _i]),
     "pic_sim_destroy": (_i, [_vp]),
+    "pic_ensemble_create": (_i, [_P(_vp)]),
+    "pic_ensemble_add": (_i, [_vp, _vp]),
+    "pic_ensemble_remove": (_i, [_vp, _vp]),
+    "pic_ensemble_size": (_i, [_vp, _P(_i)]),
+    "pic_ensemble_step": (_i, [_vp, _i]),
+    "pic_ensemble_destroy": (_i, [_vp]),
+    "pic_server_create": (_i, [_d, _i, _i, _P(SimParams), _i, _P(_vp)]),
+    "pic_server_submit": (_i, [_vp, _P(JobDesc), _P(_i)]),
+    "pic_server_tick": (_i, [_vp, _d, _P(_i), _P(_i)]),
+    "pic_server_job_result": (_i, [_vp, _i, _P(JobResult)]),
+    "pic_server_job_step_ms": (_i, [_vp, _i, _i, _vp, _P(_i)]),
+    "pic_server_job_handles": (_i, [_vp, _i, _P(_vp), _P(_vp), _P(_vp)]),
+    "pic_server_release": (_i, [_vp, _i]),
+    "pic_server_used_bytes": (_i, [_vp, _P(_d)]),
+    "pic_server_destroy": (_i, [_vp]),
 }
 
 
@@ -186,6 +214,19 @@ class Particles2D:
     Species 0 exists with m = 1, q = 0.1 like the reference constructor.
     ``max_particles`` is the capacity of each species."""
 
+    _owned = True
+
+    @classmethod
+    def _view(cls, handle, precision, grid=None):
+        """Non-owning wrapper of a handle held elsewhere (e.g. by a job server)."""
+        o = cls.__new__(cls)
+        o._h = ctypes.c_void_p(handle)
+        o._owned = False
+        o.precision = precision
+        o.max_particles = 0
+        o.grid = grid
+        return o
+
     def __init__(self, max_particles, precision=PREC_FP32):
         self._h = ctypes.c_void_p()
         _check(load().pic_particles_create(int(max_particles), int(precision), ctypes.byref(self._h)))
@@ -200,7 +241,7 @@ class Particles2D:
 
     def __del__(self):
         try:
-            if self._h and _lib is not None:
+            if self._owned and self._h and _lib is not None:
                 _lib.pic_particles_destroy(self._h)
         except Exception:
             pass
@@ -276,17 +317,30 @@ class Particles2D:
 class Grid2D:
     """Grid2D(M, N, X, Y) -- grid.cu:10-20 (dx = X/(M-1)); fields start at zero."""
 
+    _owned = True
+
+    @classmethod
+    def _view(cls, handle):
+        o = cls.__new__(cls)
+        o._h = ctypes.c_void_p(handle)
+        o._owned = False
+        o._info()
+        return o
+
     def __init__(self, M, N, X, Y):
         self._h = ctypes.c_void_p()
         _check(load().pic_grid_create(int(M), int(N), float(X), float(Y), ctypes.byref(self._h)))
+        self._info()
+
+    def _info(self):
         m, n, dx, dy = ctypes.c_int(), ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
         _check(load().pic_grid_info(self._h, ctypes.byref(m), ctypes.byref(n), ctypes.byref(dx), ctypes.byref(dy)))
         self.m, self.n, self.dx, self.dy = m.value, n.value, dx.value, dy.value
-        self.x, self.y = float(X), float(Y)
+        self.x, self.y = (self.m - 1) * self.dx, (self.n - 1) * self.dy
 
     def __del__(self):
         try:
-            if self._h and _lib is not None:
+            if self._owned and self._h and _lib is not None:
                 _lib.pic_grid_destroy(self._h)
         except Exception:
             pass
@@ -443,6 +497,16 @@ def sim_params(**over):
 class Simulation:
     """The step driver (updateSimulationCUDA, main.cu:37-140) bound to its objects."""
 
+    _owned = True
+
+    @classmethod
+    def _view(cls, handle, particles, grid):
+        o = cls.__new__(cls)
+        o._h = ctypes.c_void_p(handle)
+        o._owned = False
+        o.particles, o.grid, o.walls, o.xs, o.params = particles, grid, None, None, None
+        return o
+
     def __init__(self, particles, grid, walls=None, xs=None, params=None, **over):
         self.params = params if params is not None else sim_params(**over)
         self._h = ctypes.c_void_p()
@@ -458,7 +522,7 @@ class Simulation:
 
     def __del__(self):
         try:
-            if self._h and _lib is not None:
+            if self._owned and self._h and _lib is not None:
                 _lib.pic_sim_destroy(self._h)
         except Exception:
             pass
@@ -497,3 +561,100 @@ def updateSimulation(sim, steps):
     """updateSimulationCUDA(particles, grid, boundaries, step, dt, steps, timing)."""
     sim.step(steps)
     return sim.stats()
+
+
+class Ensemble:
+    """Independent plasmas stepped together, one launch per stage for all."""
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+        _check(load().pic_ensemble_create(ctypes.byref(self._h)))
+        self.members = []
+
+    def add(self, sim):
+        _check(load().pic_ensemble_add(self._h, sim._h))
+        self.members.append(sim)
+
+    def remove(self, sim):
+        _check(load().pic_ensemble_remove(self._h, sim._h))
+        self.members.remove(sim)
+
+    def step(self, n=1):
+        _check(load().pic_ensemble_step(self._h, int(n)))
+
+    def close(self):
+        if self._h:
+            _check(load().pic_ensemble_destroy(self._h))
+            self._h = ctypes.c_void_p()
+            self.members = []
+
+    def __del__(self):
+        try:
+            if self._h and _lib is not None:
+                _lib.pic_ensemble_destroy(self._h)
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous batching of PIC jobs, admitted by HBM footprint."""
+
+    def __init__(self, hbm_budget_bytes, max_active=256, precision=PREC_FP32, params=None, keep_finished=False,
+                 **over):
+        self.params = params if params is not None else sim_params(**over)
+        self.precision = precision
+        self._h = ctypes.c_void_p()
+        _check(load().pic_server_create(float(hbm_budget_bytes), int(max_active), int(precision),
+                                        ctypes.byref(self.params), int(bool(keep_finished)), ctypes.byref(self._h)))
+
+    def submit(self, cells, ppc, steps, seed, arrival_s=0.0, mcc=True, te_eV=2.0, density=1e15, gas_density=1e21):
+        d = JobDesc(int(cells), float(ppc), int(steps), int(seed), float(te_eV), float(density), float(gas_density),
+                    float(arrival_s), int(bool(mcc)))
+        jid = ctypes.c_int()
+        _check(load().pic_server_submit(self._h, ctypes.byref(d), ctypes.byref(jid)))
+        return jid.value
+
+    def tick(self, now_s):
+        a, w = ctypes.c_int(), ctypes.c_int()
+        _check(load().pic_server_tick(self._h, float(now_s), ctypes.byref(a), ctypes.byref(w)))
+        return a.value, w.value
+
+    def result(self, jid):
+        r = JobResult()
+        _check(load().pic_server_job_result(self._h, int(jid), ctypes.byref(r)))
+        return r
+
+    def step_ms(self, jid):
+        n = ctypes.c_int()
+        _check(load().pic_server_job_step_ms(self._h, int(jid), 0, None, ctypes.byref(n)))
+        out = np.zeros(max(1, n.value), dtype=np.float32)
+        _check(load().pic_server_job_step_ms(self._h, int(jid), out.size, _ptr(out), ctypes.byref(n)))
+        return out[:n.value]
+
+    def handles(self, jid):
+        """(Particles2D, Grid2D, Simulation) views of a running or finished job."""
+        p, g, s = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        _check(load().pic_server_job_handles(self._h, int(jid), ctypes.byref(p), ctypes.byref(g), ctypes.byref(s)))
+        grid = Grid2D._view(g.value)
+        parts = Particles2D._view(p.value, self.precision, grid)
+        return parts, grid, Simulation._view(s.value, parts, grid)
+
+    def release(self, jid):
+        _check(load().pic_server_release(self._h, int(jid)))
+
+    def used_bytes(self):
+        u = ctypes.c_double()
+        _check(load().pic_server_used_bytes(self._h, ctypes.byref(u)))
+        return u.value
+
+    def close(self):
+        if self._h:
+            _check(load().pic_server_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            if self._h and _lib is not None:
+                _lib.pic_server_destroy(self._h)
+        except Exception:
+            pass

@@ -35,7 +35,7 @@ template <typename T> CaseDev<T>& hcase(pic_particles* p);
 template <> CaseDev<float>& hcase<float>(pic_particles* p) { return p->hf; }
 template <> CaseDev<double>& hcase<double>(pic_particles* p) { return p->hd; }
 
-static int upload_case(pic_particles* p) {
+int upload_case(pic_particles* p) {
     hipStream_t s = lib_stream();
     if (p->prec == 4) PIC_HIP(hipMemcpyAsync(p->d_case, &p->hf, sizeof(p->hf), hipMemcpyHostToDevice, s));
     else PIC_HIP(hipMemcpyAsync(p->d_case, &p->hd, sizeof(p->hd), hipMemcpyHostToDevice, s));
@@ -72,7 +72,7 @@ static int ensure_scratch(pic_particles* p, int n) {
     return 0;
 }
 
-static int upload_items(WorkItem** dptr, int* alloc, const std::vector<WorkItem>& items) {
+int upload_items(WorkItem** dptr, int* alloc, const std::vector<WorkItem>& items) {
     if ((int)items.size() > *alloc) {
         if (*dptr) PIC_HIP(hipFree(*dptr));
         *dptr = nullptr;
@@ -88,12 +88,12 @@ static int upload_items(WorkItem** dptr, int* alloc, const std::vector<WorkItem>
     return 0;
 }
 
-static void items_for_species(const pic_particles* p, int s, int chunk, std::vector<WorkItem>& out) {
+void items_for_species(const pic_particles* p, int s, int chunk, std::vector<WorkItem>& out, int case_idx) {
     const SpeciesStore& st = p->sp[s];
-    for (int b = 0; b < st.nb; ++b) out.push_back(WorkItem{0, s, b, 0});
+    for (int b = 0; b < st.nb; ++b) out.push_back(WorkItem{case_idx, s, b, 0});
     const int spill_cap = st.h_cap[st.nb];
     const int chunks = std::max(1, (spill_cap + chunk - 1) / chunk);
-    for (int k = 0; k < chunks; ++k) out.push_back(WorkItem{0, s, st.nb, k});
+    for (int k = 0; k < chunks; ++k) out.push_back(WorkItem{case_idx, s, st.nb, k});
 }
 
 static int alloc_layout_parity(SpeciesStore& st, int par, int entries) {
@@ -194,6 +194,8 @@ extern "C" int pic_particles_create(int64_t max_particles, int precision, pic_pa
         delete p;
         return PIC_ERR_HIP;
     }
+    p->own_case = p->d_case;
+    p->own_state = p->d_state;
     CaseState cs{0u, 0, 0, 0};
     if (hipMemcpy(p->d_state, &cs, sizeof(cs), hipMemcpyHostToDevice) != hipSuccess) {
         set_error("pic_particles_create: state upload failed");
@@ -225,8 +227,8 @@ extern "C" int pic_particles_destroy(pic_particles* p) {
         }
         (void)hipFree(st.d_counters);
     }
-    (void)hipFree(p->d_case);
-    (void)hipFree(p->d_state);
+    (void)hipFree(p->own_case);
+    (void)hipFree(p->own_state);
     (void)hipFree(p->d_items);
     (void)hipFree(p->d_scratch);
     (void)hipFree(p->d_ll);
@@ -806,7 +808,7 @@ static int build_mcc(const pic_xs* xs, const std::vector<SpeciesStore>& sp, int 
 // ============================================================================
 // step driver
 // ============================================================================
-static size_t cur_lds(pic_sim* S) {
+size_t cur_lds(pic_sim* S) {
     return S->p->prec == 4 ? push_lds_bytes<float>(S->bin_w, S->bin_h, S->halo)
                            : push_lds_bytes<double>(S->bin_w, S->bin_h, S->halo);
 }
@@ -935,7 +937,7 @@ static int rebin_t(pic_sim* S, int bw, int bh) {
     return 0;
 }
 
-static int sim_rebin(pic_sim* S, int bw, int bh) {
+int sim_rebin(pic_sim* S, int bw, int bh) {
     return S->p->prec == 4 ? rebin_t<float>(S, bw, bh) : rebin_t<double>(S, bw, bh);
 }
 
@@ -1100,7 +1102,7 @@ extern "C" int pic_sim_create(pic_particles* p, pic_grid* g, pic_walls* w, pic_x
     return 0;
 }
 
-static int mcc_active(pic_sim* S) {
+int mcc_active(pic_sim* S) {
     if (!S->xs || !S->mcc) return 0;
     for (auto& i : S->info)
         if (i.n_proc > 0) return 1;
@@ -1229,8 +1231,11 @@ extern "C" int pic_sim_rebin(pic_sim* S, int bin_w, int bin_h) {
     return sim_rebin(S, bw, bh);
 }
 
+int ensemble_forget(pic_sim* S);
+
 extern "C" int pic_sim_destroy(pic_sim* S) {
     if (!S) return 0;
+    (void)ensemble_forget(S);
     (void)hipStreamSynchronize(lib_stream());
     if (S->p && S->p->attached == S) {
         (void)sim_sync_impl(S);

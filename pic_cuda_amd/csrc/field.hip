@@ -67,6 +67,32 @@ int launch_charge(const ChargeArgs& a, size_t nn, hipStream_t s) {
     return 0;
 }
 
+__global__ __launch_bounds__(256) void charge_batch_kernel(const ChargeArgs* __restrict__ args,
+                                                           const long long* __restrict__ nns,
+                                                           const int4* __restrict__ chunks) {
+    const int4 c = chunks[blockIdx.x];
+    const ChargeArgs& a = args[c.x];
+    const size_t i = (size_t)c.y * 256 + threadIdx.x;
+    if (i >= (size_t)nns[c.x]) return;
+    double r = 0.0, q = 0.0;
+    for (int s = 0; s < a.ns; ++s) {
+        const double v = (double)a.acc[s][i];
+        r = fma(a.cfac[s], v, r);
+        q = fma(a.qfac[s], v, q);
+        if (a.zero) a.acc[s][i] = 0ull;
+    }
+    if (a.rho) a.rho[i] = q;
+    a.rhs[i] = r;
+}
+
+int launch_charge_batch(const ChargeArgs* args, const long long* nns, const int4* chunks, int n_chunks,
+                        hipStream_t s) {
+    if (n_chunks <= 0) return 0;
+    hipLaunchKernelGGL(charge_batch_kernel, dim3(n_chunks), dim3(256), 0, s, args, nns, chunks);
+    PIC_HIP(hipGetLastError());
+    return 0;
+}
+
 int launch_efield(const double* phi, double2* E, int nx, int ny, double dx, double dy, hipStream_t s) {
     hipLaunchKernelGGL(efield_kernel, dim3(div_up(nx, 16), div_up(ny, 16)), dim3(256), 0, s, phi, E, nx, ny, dx,
                        dy);

@@ -15,6 +15,11 @@ struct ChargeArgs {
 };
 
 int launch_charge(const ChargeArgs& a, size_t nn, hipStream_t s);
+
+// batched: one ChargeArgs per case, items (case, 256-node chunk) in `chunks`
+// (int4 {case, chunk, 0, 0}); nn per case in `nns`
+int launch_charge_batch(const ChargeArgs* args, const long long* nns, const int4* chunks, int n_chunks,
+                        hipStream_t s);
 int launch_efield(const double* phi, double2* E, int nx, int ny, double dx, double dy, hipStream_t s);
 int launch_deposit(int prec, const void* x, const void* y, int n, int nx, int ny, unsigned long long* acc,
                    hipStream_t s);

@@ -1,6 +1,6 @@
 // mg.hpp -- geometric multigrid Poisson solver for gfx950 (host side).
 //
-// Solves lap_h(phi) = rhs on a square vertex-centred grid with Dirichlet
+// Solves lap_h(phi) = rhs on square vertex-centred grids with Dirichlet
 // boundary values (the boundary nodes of phi are never changed), like
 // solve_multigrid (Multigrid.hpp:3-11, Multigrid.cu:573-642).
 //
@@ -8,6 +8,12 @@
 //   PIC_MG_FP64   V-cycles in FP64 directly on phi (the reference's scheme).
 //   PIC_MG_MIXED  FP64 iterative refinement: r = rhs - A phi in FP64 (stored
 //                 FP32), one FP32 V-cycle for A e = r, phi += e in FP64.
+//
+// MGSolver owns the level pyramid of one grid.  MGBatch runs the solves of
+// several MGSolvers together: every stage of the cycle (a level of the down
+// leg, the coarse cycle, a level of the up leg, the refinement kernels) is one
+// launch over a table of (case, tile) items, and every case stops on its own
+// device flag.  A single solve is a batch of one.
 #pragma once
 #include "pic_common.hpp"
 #include <vector>
@@ -32,34 +38,82 @@ struct MGStats {
     double rhs_norm;
 };
 
-// norms and the convergence flag, written by the reduction kernel
+// norms and the stop flag of one case, written by the reduction kernel
 struct MGNorm {
     double res_sq;
     double rhs_sq;
     double rel;
-    int converged;
-    int pad;
+    int converged;     // stop flag: tolerance met or max_cycles done
+    int cycles;        // cycles executed in this solve
+    int max_cycles;
+    int hit_tol;       // tolerance met
 };
+
+// one (case, level) entry of a tiled stage
+template <typename T>
+struct MGLevelTask {
+    const T* e_in;     // null: zero initial guess
+    const T* rhs;
+    T* e_out;
+    T* rhs_c;          // smooth stage: coarse rhs to restrict into (may be null)
+    const T* coarse;   // prolongation stage: coarse correction
+    int n;
+    int nu;            // sweeps
+    int H;             // halo
+    int pad;
+    T h2, inv_h2, omega;
+    const MGNorm* gate;
+};
+
+// one case of the single-workgroup coarse cycle
+template <typename T>
+struct MGCoarseTask {
+    const T* e_in;
+    const T* rhs;
+    T* e_out;
+    int n_top, Lc, pre, post, csweeps, pad;
+    T h_top, omega;
+    const MGNorm* gate;
+};
+
+// one case of the FP64 refinement / norm kernels
+struct MGFineTask {
+    double* phi;
+    const double* rhs;
+    float* r32;        // mixed mode: FP32 residual (the cycle's rhs)
+    const float* e32;  // mixed mode: the cycle's correction
+    double* partial;   // per-block sums of squares
+    MGNorm* norm;
+    MGNorm* mirror;    // contiguous copy of all norms of the batch
+    int n;
+    int gx;            // 16x16 blocks per row
+    int nblocks;       // residual partials of this case
+    int nchunks;       // 256-node chunks of this case
+    double inv_h2;
+};
+
+struct MGTile {
+    int task, tx, ty, pad;
+};
+
+class MGBatch;
 
 class MGSolver {
 public:
     MGSolver(int n, const MGParams& p, hipStream_t s);
     ~MGSolver();
-    int init();                                              // allocate (returns status)
+    int init();
     int solve(double* phi, const double* rhs, double h, MGStats* st);
     const MGParams& params() const { return p_; }
     int n() const { return n_; }
     int levels() const { return L_; }
-    void set_stream(hipStream_t s) { stream_ = s; }
-    // kernel launch counts of the last solve (for the measurement report)
     int last_launches() const { return launches_; }
+    void set_stream(hipStream_t s) { stream_ = s; }
 
 private:
-    template <typename T>
-    int vcycle(const T* e0_in, T* e0_out, T* e0_tmp, const T* rhs0, double h);
-    int reduce(const double* v_partials, int np, int which);
-
+    friend class MGBatch;
     int n_, L_, lc_;
+    int nc_ = 33;                  // largest level of the coarse kernel
     MGParams p_;
     hipStream_t stream_;
     std::vector<int> ns_;
@@ -71,9 +125,59 @@ private:
     double* partials_ = nullptr;
     int n_partials_ = 0;
     MGNorm* d_norm_ = nullptr;
-    MGNorm* h_norm_ = nullptr;   // pinned
     int launches_ = 0;
     bool ok_ = false;
+    MGBatch* solo_ = nullptr;      // the batch of one used by solve()
+    double* solo_phi_ = nullptr;
+    const double* solo_rhs_ = nullptr;
+    double solo_h_ = 0.0;
+};
+
+class MGBatch {
+public:
+    explicit MGBatch(hipStream_t s) : stream_(s) {}
+    ~MGBatch();
+    // members: solvers with identical parameters; phi/rhs/h per member
+    int build(const std::vector<MGSolver*>& members, const std::vector<double*>& phi,
+              const std::vector<const double*>& rhs, const std::vector<double>& h);
+    // solve every member (at least one V-cycle batch is enqueued per sync)
+    int solve();
+    int last_launches() const { return launches_; }
+    const MGNorm& norm(int k) const { return h_mirror_[k]; }
+    int size() const { return (int)m_.size(); }
+
+private:
+    struct Stage {
+        int kind = 0;      // 0 smooth, 1 prolong, 2 coarse, 3 copy
+        bool fp64 = false;
+        void* tasks = nullptr;
+        MGTile* tiles = nullptr;
+        int n_items = 0;   // launch blocks
+        size_t lds = 0;
+    };
+    template <typename T>
+    int build_cycle(const std::vector<double>& h);
+    template <typename T>
+    int add_level_stage(int kind, const std::vector<MGLevelTask<T>>& tasks, size_t lds);
+    int launch_stage(const Stage& st);
+    int enqueue_fine(int what, int which);
+    void free_all();
+
+    hipStream_t stream_;
+    std::vector<MGSolver*> m_;
+    std::vector<double*> phi_;
+    std::vector<const double*> rhs_;
+    MGParams p_{};
+    bool mixed_ = false;
+    std::vector<Stage> cycle_;          // stages of one V-cycle
+    MGFineTask* d_fine_ = nullptr;      // per-case refinement tasks
+    MGTile* d_chunk_tiles_ = nullptr;   // (case, chunk of 256 nodes)
+    MGTile* d_res_tiles_ = nullptr;     // (case, 16x16 block)
+    int n_chunk_ = 0, n_res_ = 0;
+    MGNorm* d_mirror_ = nullptr;
+    MGNorm* h_mirror_ = nullptr;        // pinned
+    int last_cycles_ = 1;
+    int launches_ = 0;
 };
 
 }  // namespace pic

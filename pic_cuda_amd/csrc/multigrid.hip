@@ -27,11 +27,15 @@
 #include "mg.hpp"
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 namespace pic {
 
 #define MG_TILE 32
-#define MG_NC 33     // largest level handled by the single-workgroup coarse kernel
+// largest level handled by the single-workgroup coarse kernel (its LDS stays
+// below 64 KB: FP32 65^2, FP64 33^2)
+#define MG_NC_F32 65
+#define MG_NC_F64 33
 
 template <typename T>
 __device__ inline T tfma(T a, T b, T c);
@@ -61,16 +65,24 @@ __device__ inline void rb_colour_lds(T* e, const T* r, int W, int Hh, int gi0, i
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void mg_smooth_tile(const T* __restrict__ e_in, const T* __restrict__ rhs,
-                                                      T* __restrict__ e_out, T* __restrict__ rhs_c, int n, T h2,
-                                                      T inv_h2, T omega, int nu, int H) {
+__global__ __launch_bounds__(256) void mg_smooth_stage(const MGLevelTask<T>* __restrict__ tasks,
+                                                       const MGTile* __restrict__ tiles) {
+    const MGTile tile = tiles[blockIdx.x];
+    const MGLevelTask<T>& tk = tasks[tile.task];
+    if (tk.gate && tk.gate->converged) return;   // cycle enqueued after convergence: no-op
+    const T* __restrict__ e_in = tk.e_in;
+    const T* __restrict__ rhs = tk.rhs;
+    T* __restrict__ e_out = tk.e_out;
+    T* __restrict__ rhs_c = tk.rhs_c;
+    const int n = tk.n, nu = tk.nu, H = tk.H;
+    const T h2 = tk.h2, inv_h2 = tk.inv_h2, omega = tk.omega;
     extern __shared__ unsigned char smem_mg[];
     const int W = MG_TILE + 2 * H;
     T* e = reinterpret_cast<T*>(smem_mg);
     T* r = e + W * W;
     T* res = r + W * W;
     const int tid = threadIdx.x;
-    const int i0 = blockIdx.x * MG_TILE, j0 = blockIdx.y * MG_TILE;
+    const int i0 = tile.tx * MG_TILE, j0 = tile.ty * MG_TILE;
     const int gi0 = i0 - H, gj0 = j0 - H;
     for (int k = tid; k < W * W; k += 256) {
         const int gi = gi0 + k % W, gj = gj0 + k / W;
@@ -144,16 +156,23 @@ __device__ inline T prolong_val(const T* __restrict__ coarse, int nc, int gi, in
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void mg_prolong_smooth_tile(const T* __restrict__ e_in,
-                                                              const T* __restrict__ coarse,
-                                                              const T* __restrict__ rhs, T* __restrict__ e_out,
-                                                              int n, T h2, T omega, int nu, int H) {
+__global__ __launch_bounds__(256) void mg_prolong_stage(const MGLevelTask<T>* __restrict__ tasks,
+                                                        const MGTile* __restrict__ tiles) {
+    const MGTile tile = tiles[blockIdx.x];
+    const MGLevelTask<T>& tk = tasks[tile.task];
+    if (tk.gate && tk.gate->converged) return;
+    const T* __restrict__ e_in = tk.e_in;
+    const T* __restrict__ coarse = tk.coarse;
+    const T* __restrict__ rhs = tk.rhs;
+    T* __restrict__ e_out = tk.e_out;
+    const int n = tk.n, nu = tk.nu, H = tk.H;
+    const T h2 = tk.h2, omega = tk.omega;
     extern __shared__ unsigned char smem_mg[];
     const int W = MG_TILE + 2 * H;
     T* e = reinterpret_cast<T*>(smem_mg);
     T* r = e + W * W;
     const int tid = threadIdx.x;
-    const int i0 = blockIdx.x * MG_TILE, j0 = blockIdx.y * MG_TILE;
+    const int i0 = tile.tx * MG_TILE, j0 = tile.ty * MG_TILE;
     const int gi0 = i0 - H, gj0 = j0 - H;
     const int nc = (n + 1) / 2;
     for (int k = tid; k < W * W; k += 256) {
@@ -192,9 +211,14 @@ __device__ inline void lds_smooth(T* e, const T* r, int n, T h, T omega, int ite
 }
 
 template <typename T>
-__global__ __launch_bounds__(1024) void mg_coarse_kernel(const T* __restrict__ e_in, const T* __restrict__ rhs_top,
-                                                         T* __restrict__ e_out, int n_top, int Lc, T h_top, int pre,
-                                                         int post, int csweeps, T omega) {
+__global__ __launch_bounds__(1024) void mg_coarse_stage(const MGCoarseTask<T>* __restrict__ tasks) {
+    const MGCoarseTask<T>& tk = tasks[blockIdx.x];
+    if (tk.gate && tk.gate->converged) return;
+    const T* __restrict__ e_in = tk.e_in;
+    const T* __restrict__ rhs_top = tk.rhs;
+    T* __restrict__ e_out = tk.e_out;
+    const int n_top = tk.n_top, Lc = tk.Lc, pre = tk.pre, post = tk.post, csweeps = tk.csweeps;
+    const T h_top = tk.h_top, omega = tk.omega;
     extern __shared__ unsigned char smem_mg[];
     T* base = reinterpret_cast<T*>(smem_mg);
     T* E[16];
@@ -281,67 +305,108 @@ __device__ inline double block_sum256(double v, double* sh) {
     return sh[0];
 }
 
+// FP64 residual r = rhs - A phi (zero on the boundary), optionally stored in
+// FP32 as the next cycle's rhs, and per-block sums of r^2.  which == 2: skipped
+// once the case has stopped.
 template <bool WRITE_R32>
-__global__ __launch_bounds__(256) void ir_residual_kernel(const double* __restrict__ phi,
-                                                          const double* __restrict__ f, float* __restrict__ r32,
-                                                          double* __restrict__ partial, int n, double inv_h2) {
+__global__ __launch_bounds__(256) void ir_residual_stage(const MGFineTask* __restrict__ tasks,
+                                                         const MGTile* __restrict__ tiles, int which) {
+    const MGTile tile = tiles[blockIdx.x];
+    const MGFineTask& tk = tasks[tile.task];
+    if (which == 2 && tk.norm->converged) return;
     __shared__ double sh[256];
-    const int i = blockIdx.x * 16 + threadIdx.x % 16, j = blockIdx.y * 16 + threadIdx.x / 16;
+    const int n = tk.n;
+    const double* __restrict__ phi = tk.phi;
+    const int i = tile.tx * 16 + threadIdx.x % 16, j = tile.ty * 16 + threadIdx.x / 16;
     double r = 0.0;
     if (i < n && j < n) {
         const size_t idx = (size_t)j * n + i;
         if (!(i == 0 || i == n - 1 || j == 0 || j == n - 1)) {
             const double c = phi[idx];
-            const double lap = (phi[idx - n] + phi[idx + n] + phi[idx - 1] + phi[idx + 1] - 4.0 * c) * inv_h2;
-            r = f[idx] - lap;
+            const double lap = (phi[idx - n] + phi[idx + n] + phi[idx - 1] + phi[idx + 1] - 4.0 * c) * tk.inv_h2;
+            r = tk.rhs[idx] - lap;
         }
-        if (WRITE_R32) r32[idx] = (float)r;
+        if (WRITE_R32) tk.r32[idx] = (float)r;
     }
     const double s = block_sum256(r * r, sh);
-    if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = s;
+    if (threadIdx.x == 0) tk.partial[tile.ty * tk.gx + tile.tx] = s;
 }
 
-__global__ __launch_bounds__(256) void sumsq_kernel(const double* __restrict__ v, size_t nn,
-                                                    double* __restrict__ partial) {
+__global__ __launch_bounds__(256) void sumsq_stage(const MGFineTask* __restrict__ tasks,
+                                                   const MGTile* __restrict__ tiles) {
+    const MGTile tile = tiles[blockIdx.x];
+    const MGFineTask& tk = tasks[tile.task];
     __shared__ double sh[256];
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    const double x = (i < nn) ? v[i] : 0.0;
+    const size_t nn = (size_t)tk.n * tk.n;
+    const size_t i = (size_t)tile.tx * 256 + threadIdx.x;
+    const double x = (i < nn) ? tk.rhs[i] : 0.0;
     const double s = block_sum256(x * x, sh);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    if (threadIdx.x == 0) tk.partial[tile.tx] = s;
 }
 
-__global__ __launch_bounds__(256) void ir_correct_kernel(double* __restrict__ phi, const float* __restrict__ e,
-                                                         size_t nn) {
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < nn) phi[i] = phi[i] + (double)e[i];
+__global__ __launch_bounds__(256) void ir_correct_stage(const MGFineTask* __restrict__ tasks,
+                                                        const MGTile* __restrict__ tiles) {
+    const MGTile tile = tiles[blockIdx.x];
+    const MGFineTask& tk = tasks[tile.task];
+    if (tk.norm->converged) return;
+    const size_t nn = (size_t)tk.n * tk.n;
+    const size_t i = (size_t)tile.tx * 256 + threadIdx.x;
+    if (i < nn) tk.phi[i] = tk.phi[i] + (double)tk.e32[i];
 }
 
-// which = 0: store the rhs norm; which = 1: residual norm, rel and flag
-__global__ __launch_bounds__(256) void norm_final_kernel(const double* __restrict__ partial, int np, MGNorm* out,
-                                                         int which, double tol) {
+// One workgroup per case: fixed-order sum of the block partials.  which = 0:
+// start of a solve (rhs norm, clear the flag and the cycle count); 1: initial
+// residual test; 2: residual test at the end of a cycle, counted, and a no-op
+// once the flag is set, so cycles enqueued after convergence change nothing.
+// Relative residual ||r||/||rhs|| (the reference divides by sum rhs^2,
+// Multigrid.cu:466).
+__global__ __launch_bounds__(256) void norm_final_stage(const MGFineTask* __restrict__ tasks, int which, double tol,
+                                                        int max_cycles) {
+    const MGFineTask& tk = tasks[blockIdx.x];
+    MGNorm* out = tk.norm;
+    if (which == 2 && out->converged) return;
     __shared__ double sh[256];
+    const int np = which == 0 ? tk.nchunks : tk.nblocks;
     double acc = 0.0;
-    for (int k = threadIdx.x; k < np; k += 256) acc += partial[k];
-    const double s = block_sum256(acc, sh);
+    for (int k = threadIdx.x; k < np; k += 256) acc += tk.partial[k];
+    const double tot = block_sum256(acc, sh);
     if (threadIdx.x == 0) {
         if (which == 0) {
-            out->rhs_sq = s;
+            out->rhs_sq = tot;
+            out->converged = 0;
+            out->cycles = 0;
+            out->max_cycles = max_cycles;
         } else {
-            out->res_sq = s;
-            const double rn = sqrt(s), fn = sqrt(out->rhs_sq);
+            out->res_sq = tot;
+            const double rn = sqrt(tot), fn = sqrt(out->rhs_sq);
             const double rel = fn > 0.0 ? rn / fn : rn;
             out->rel = rel;
-            out->converged = rel < tol ? 1 : 0;
+            if (which == 2) out->cycles += 1;
+            out->converged = (rel < tol || out->cycles >= out->max_cycles) ? 1 : 0;
+            out->hit_tol = rel < tol ? 1 : 0;
         }
+        *tk.mirror = *out;
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void mg_copy_stage(const MGLevelTask<T>* __restrict__ tasks,
+                                                     const MGTile* __restrict__ tiles) {
+    const MGTile tile = tiles[blockIdx.x];
+    const MGLevelTask<T>& tk = tasks[tile.task];
+    if (tk.gate && tk.gate->converged) return;
+    const size_t nn = (size_t)tk.n * tk.n;
+    const size_t i = (size_t)tile.tx * 256 + threadIdx.x;
+    if (i < nn) tk.e_out[i] = tk.e_in[i];
 }
 
 // ---------------------------------------------------------------------------
-// host side
+// MGSolver: the pyramid of one grid
 // ---------------------------------------------------------------------------
 MGSolver::MGSolver(int n, const MGParams& p, hipStream_t s) : n_(n), L_(0), lc_(0), p_(p), stream_(s) {}
 
 MGSolver::~MGSolver() {
+    delete solo_;
     for (int b = 0; b < 2; ++b) {
         for (auto* q : ef_[b]) if (q) (void)hipFree(q);
         for (auto* q : ed_[b]) if (q) (void)hipFree(q);
@@ -352,7 +417,6 @@ MGSolver::~MGSolver() {
     if (phi_tmp_) (void)hipFree(phi_tmp_);
     if (partials_) (void)hipFree(partials_);
     if (d_norm_) (void)hipFree(d_norm_);
-    if (h_norm_) (void)hipHostFree(h_norm_);
 }
 
 int MGSolver::init() {
@@ -363,6 +427,7 @@ int MGSolver::init() {
     } else {
         while (m > 3) { m = (m + 1) / 2; ++L; }
     }
+    PIC_REQUIRE(L <= 16, "multigrid: at most 16 levels");
     ns_.assign(L, 0);
     ns_[0] = n_;
     for (int l = 1; l < L; ++l) {
@@ -370,13 +435,14 @@ int MGSolver::init() {
         ns_[l] = (ns_[l - 1] + 1) / 2;
         PIC_REQUIRE(ns_[l] >= 2, "multigrid: too many levels for this grid");
     }
-    PIC_REQUIRE(L <= 16, "multigrid: at most 16 levels");
     PIC_REQUIRE(p_.pre >= 0 && p_.pre <= 3 && p_.post >= 0 && p_.post <= 3,
                 "multigrid: pre/post sweeps must be in [0,3]");
+    PIC_REQUIRE(p_.coarse_sweeps >= 0, "multigrid: coarse sweeps must be non-negative");
     L_ = L;
     lc_ = L - 1;
+    nc_ = p_.mode == PIC_MG_MIXED ? MG_NC_F32 : MG_NC_F64;
     for (int l = 0; l < L; ++l)
-        if (ns_[l] <= MG_NC) { lc_ = l; break; }
+        if (ns_[l] <= nc_) { lc_ = l; break; }
     for (int b = 0; b < 2; ++b) {
         ef_[b].assign(L, nullptr);
         ed_[b].assign(L, nullptr);
@@ -403,11 +469,38 @@ int MGSolver::init() {
     PIC_HIP(hipMalloc(&partials_, (size_t)n_partials_ * sizeof(double)));
     PIC_HIP(hipMalloc(&d_norm_, sizeof(MGNorm)));
     PIC_HIP(hipMemset(d_norm_, 0, sizeof(MGNorm)));
-    PIC_HIP(hipHostMalloc(&h_norm_, sizeof(MGNorm), hipHostMallocDefault));
     ok_ = true;
     return 0;
 }
 
+int MGSolver::solve(double* phi, const double* rhs, double h, MGStats* st) {
+    PIC_REQUIRE(ok_, "multigrid: solver not initialised");
+    if (!solo_ || phi != solo_phi_ || rhs != solo_rhs_ || h != solo_h_) {
+        delete solo_;
+        solo_ = new MGBatch(stream_);
+        if (int rc = solo_->build({this}, {phi}, {rhs}, {h})) {
+            delete solo_;
+            solo_ = nullptr;
+            return rc;
+        }
+        solo_phi_ = phi;
+        solo_rhs_ = rhs;
+        solo_h_ = h;
+    }
+    if (int rc = solo_->solve()) return rc;
+    launches_ = solo_->last_launches();
+    if (st) {
+        const MGNorm& nm = solo_->norm(0);
+        st->cycles = nm.cycles;
+        st->rel_residual = nm.rel;
+        st->rhs_norm = std::sqrt(nm.rhs_sq);
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// MGBatch: one launch per stage for all member grids
+// ---------------------------------------------------------------------------
 template <typename T>
 static size_t coarse_lds_bytes(int n_top, int Lc) {
     size_t tot = 0;
@@ -417,140 +510,346 @@ static size_t coarse_lds_bytes(int n_top, int Lc) {
     return tot * sizeof(T);
 }
 
-template <typename T>
-int MGSolver::vcycle(const T* e0_in, T* e0_out, T* e0_tmp, const T* rhs0, double h) {
-    const bool mixed = p_.mode == PIC_MG_MIXED;
-    auto E = [&](int b, int l) -> T* { return (T*)(mixed ? (void*)ef_[b][l] : (void*)ed_[b][l]); };
-    auto R = [&](int l) -> T* { return (T*)(mixed ? (void*)rf_[l] : (void*)rd_[l]); };
-    const T omega = (T)p_.omega;
-    std::vector<double> hs(L_);
-    hs[0] = h;
-    for (int l = 1; l < L_; ++l) hs[l] = hs[l - 1] * 2.0;
-    const int lc = lc_;
-    // down leg
-    for (int l = 0; l < lc; ++l) {
-        const int n = ns_[l];
-        const int H = 2 * p_.pre + 2;
-        const int W = MG_TILE + 2 * H;
-        const size_t lds = (2 * (size_t)W * W + (MG_TILE + 2) * (MG_TILE + 2)) * sizeof(T);
-        const T* ein = (l == 0) ? e0_in : nullptr;
-        const T* rhs = (l == 0) ? rhs0 : R(l);
-        T* eout = (l == 0) ? e0_tmp : E(1, l);
-        const double hh = hs[l];
-        hipLaunchKernelGGL(mg_smooth_tile<T>, dim3(div_up(n, MG_TILE), div_up(n, MG_TILE)), dim3(256), lds, stream_,
-                           ein, rhs, eout, R(l + 1), n, (T)(hh * hh), (T)(1.0 / (hh * hh)), omega, p_.pre, H);
-        ++launches_;
+static size_t smooth_lds(int H, size_t es) {
+    const size_t W = MG_TILE + 2 * H;
+    return (2 * W * W + (MG_TILE + 2) * (MG_TILE + 2)) * es;
+}
+
+MGBatch::~MGBatch() { free_all(); }
+
+void MGBatch::free_all() {
+    (void)hipStreamSynchronize(stream_);
+    for (auto& st : cycle_) {
+        if (st.tasks) (void)hipFree(st.tasks);
+        if (st.tiles) (void)hipFree(st.tiles);
     }
-    // coarse part
-    {
-        const int n = ns_[lc];
-        const T* ein = (lc == 0) ? e0_in : nullptr;
-        const T* rhs = (lc == 0) ? rhs0 : R(lc);
-        T* eout = (lc == 0) ? e0_out : E(0, lc);
-        if (n <= MG_NC) {
-            const size_t lds = coarse_lds_bytes<T>(n, L_ - lc);
-            hipLaunchKernelGGL(mg_coarse_kernel<T>, dim3(1), dim3(1024), lds, stream_, ein, rhs, eout, n, L_ - lc,
-                               (T)hs[lc], p_.pre, p_.post, p_.coarse_sweeps, omega);
-            ++launches_;
+    cycle_.clear();
+    if (d_fine_) (void)hipFree(d_fine_);
+    if (d_chunk_tiles_) (void)hipFree(d_chunk_tiles_);
+    if (d_res_tiles_) (void)hipFree(d_res_tiles_);
+    if (d_mirror_) (void)hipFree(d_mirror_);
+    if (h_mirror_) (void)hipHostFree(h_mirror_);
+    d_fine_ = nullptr;
+    d_chunk_tiles_ = d_res_tiles_ = nullptr;
+    d_mirror_ = nullptr;
+    h_mirror_ = nullptr;
+}
+
+template <typename T>
+static int upload_vec(void** dst, const std::vector<T>& v) {
+    PIC_HIP(hipMalloc(dst, std::max<size_t>(1, v.size()) * sizeof(T)));
+    if (!v.empty()) PIC_HIP(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+template <typename T>
+int MGBatch::add_level_stage(int kind, const std::vector<MGLevelTask<T>>& tasks, size_t lds) {
+    if (tasks.empty()) return 0;
+    std::vector<MGTile> tiles;
+    for (int k = 0; k < (int)tasks.size(); ++k) {
+        const int n = tasks[k].n;
+        if (kind == 3) {
+            const int chunks = (int)(((size_t)n * n + 255) / 256);
+            for (int c = 0; c < chunks; ++c) tiles.push_back(MGTile{k, c, 0, 0});
         } else {
-            // a large coarsest level: tiled sweeps in chunks of up to 3, ping-pong
-            T* bufs[2] = {eout, (lc == 0) ? e0_tmp : E(1, lc)};
-            const T* src = ein;
-            // never write the buffer a launch reads (tiles read their neighbours' halos)
-            int left = p_.coarse_sweeps, which = (src == bufs[0]) ? 1 : 0;
+            const int g = div_up(n, MG_TILE);
+            for (int ty = 0; ty < g; ++ty)
+                for (int tx = 0; tx < g; ++tx) tiles.push_back(MGTile{k, tx, ty, 0});
+        }
+    }
+    Stage st;
+    st.kind = kind;
+    st.fp64 = sizeof(T) == 8;
+    st.lds = lds;
+    st.n_items = (int)tiles.size();
+    if (int rc = upload_vec(&st.tasks, tasks)) return rc;
+    if (int rc = upload_vec((void**)&st.tiles, tiles)) return rc;
+    cycle_.push_back(st);
+    return 0;
+}
+
+template <typename T>
+int MGBatch::build_cycle(const std::vector<double>& h) {
+    const int M = (int)m_.size();
+    const T omega = (T)p_.omega;
+    // per member: level-0 input, output, scratch and rhs
+    auto E = [&](MGSolver* s, int b, int l) -> T* { return (T*)(mixed_ ? (void*)s->ef_[b][l] : (void*)s->ed_[b][l]); };
+    auto R = [&](MGSolver* s, int l) -> T* { return (T*)(mixed_ ? (void*)s->rf_[l] : (void*)s->rd_[l]); };
+    auto in0 = [&](int k) -> const T* { return mixed_ ? nullptr : (const T*)(void*)phi_[k]; };
+    auto out0 = [&](int k) -> T* { return mixed_ ? (T*)(void*)m_[k]->ef_[0][0] : (T*)(void*)phi_[k]; };
+    auto tmp0 = [&](int k) -> T* { return mixed_ ? (T*)(void*)m_[k]->ef_[1][0] : (T*)(void*)m_[k]->phi_tmp_; };
+    auto rhs0 = [&](int k) -> const T* { return mixed_ ? (const T*)(void*)m_[k]->r32_ : (const T*)(void*)rhs_[k]; };
+    auto hl = [&](int k, int l) { return h[k] * std::ldexp(1.0, l); };
+    int D = 0;
+    for (auto* s : m_) D = std::max(D, s->lc_);
+    // down leg
+    for (int d = 0; d < D; ++d) {
+        std::vector<MGLevelTask<T>> tasks;
+        for (int k = 0; k < M; ++k) {
+            MGSolver* s = m_[k];
+            if (s->lc_ <= d) continue;
+            const double hh = hl(k, d);
+            MGLevelTask<T> t{};
+            t.e_in = d == 0 ? in0(k) : nullptr;
+            t.rhs = d == 0 ? rhs0(k) : R(s, d);
+            t.e_out = d == 0 ? tmp0(k) : E(s, 1, d);
+            t.rhs_c = R(s, d + 1);
+            t.n = s->ns_[d];
+            t.nu = p_.pre;
+            t.H = 2 * p_.pre + 2;
+            t.h2 = (T)(hh * hh);
+            t.inv_h2 = (T)(1.0 / (hh * hh));
+            t.omega = omega;
+            t.gate = s->d_norm_;
+            tasks.push_back(t);
+        }
+        if (int rc = add_level_stage<T>(0, tasks, smooth_lds(2 * p_.pre + 2, sizeof(T)))) return rc;
+    }
+    // coarse part: the single-workgroup cycle, or tiled sweeps on a large coarsest level
+    {
+        std::vector<MGCoarseTask<T>> ct;
+        size_t lds = 0;
+        for (int k = 0; k < M; ++k) {
+            MGSolver* s = m_[k];
+            const int lc = s->lc_;
+            if (s->ns_[lc] > s->nc_) continue;
+            MGCoarseTask<T> t{};
+            t.e_in = lc == 0 ? in0(k) : nullptr;
+            t.rhs = lc == 0 ? rhs0(k) : R(s, lc);
+            t.e_out = lc == 0 ? out0(k) : E(s, 0, lc);
+            t.n_top = s->ns_[lc];
+            t.Lc = s->L_ - lc;
+            t.pre = p_.pre;
+            t.post = p_.post;
+            t.csweeps = p_.coarse_sweeps;
+            t.h_top = (T)hl(k, lc);
+            t.omega = omega;
+            t.gate = s->d_norm_;
+            lds = std::max(lds, coarse_lds_bytes<T>(t.n_top, t.Lc));
+            ct.push_back(t);
+        }
+        if (!ct.empty()) {
+            Stage st;
+            st.kind = 2;
+            st.fp64 = sizeof(T) == 8;
+            st.lds = lds;
+            st.n_items = (int)ct.size();
+            if (int rc = upload_vec(&st.tasks, ct)) return rc;
+            cycle_.push_back(st);
+        }
+        // large coarsest levels: chunks of up to 3 sweeps, ping-pong, then a copy
+        std::vector<int> big;
+        for (int k = 0; k < M; ++k)
+            if (m_[k]->ns_[m_[k]->lc_] > m_[k]->nc_) big.push_back(k);
+        if (!big.empty()) {
+            std::vector<const T*> src(M, nullptr);
+            std::vector<int> which(M, 0);
+            std::vector<T*> b0(M), b1(M);
+            for (int k : big) {
+                MGSolver* s = m_[k];
+                const int lc = s->lc_;
+                b0[k] = lc == 0 ? out0(k) : E(s, 0, lc);
+                b1[k] = lc == 0 ? tmp0(k) : E(s, 1, lc);
+                src[k] = lc == 0 ? in0(k) : nullptr;
+                which[k] = (src[k] == b0[k]) ? 1 : 0;   // never write the buffer a launch reads
+            }
+            int left = p_.coarse_sweeps;
             bool wrote = false;
-            const double hh = hs[lc];
             while (left > 0 || !wrote) {
                 const int nu = std::min(left, 3);
-                const int H = 2 * nu;
-                const int W = MG_TILE + 2 * H;
-                const size_t lds = (2 * (size_t)W * W + (MG_TILE + 2) * (MG_TILE + 2)) * sizeof(T);
-                hipLaunchKernelGGL(mg_smooth_tile<T>, dim3(div_up(n, MG_TILE), div_up(n, MG_TILE)), dim3(256), lds,
-                                   stream_, src, rhs, bufs[which], (T*)nullptr, n, (T)(hh * hh),
-                                   (T)(1.0 / (hh * hh)), omega, nu, H);
-                ++launches_;
-                src = bufs[which];
-                which ^= 1;
+                std::vector<MGLevelTask<T>> tasks;
+                for (int k : big) {
+                    MGSolver* s = m_[k];
+                    const int lc = s->lc_;
+                    const double hh = hl(k, lc);
+                    MGLevelTask<T> t{};
+                    t.e_in = src[k];
+                    t.rhs = lc == 0 ? rhs0(k) : R(s, lc);
+                    t.e_out = which[k] ? b1[k] : b0[k];
+                    t.n = s->ns_[lc];
+                    t.nu = nu;
+                    t.H = 2 * nu;
+                    t.h2 = (T)(hh * hh);
+                    t.inv_h2 = (T)(1.0 / (hh * hh));
+                    t.omega = omega;
+                    t.gate = s->d_norm_;
+                    tasks.push_back(t);
+                    src[k] = t.e_out;
+                    which[k] ^= 1;
+                }
+                if (int rc = add_level_stage<T>(0, tasks, smooth_lds(2 * nu, sizeof(T)))) return rc;
                 left -= nu;
                 wrote = true;
             }
-            if (src != eout) {
-                PIC_HIP(hipMemcpyAsync(eout, src, (size_t)n * n * sizeof(T), hipMemcpyDeviceToDevice, stream_));
+            std::vector<MGLevelTask<T>> copies;
+            for (int k : big) {
+                if (src[k] == b0[k]) continue;
+                MGLevelTask<T> t{};
+                t.e_in = src[k];
+                t.e_out = b0[k];
+                t.n = m_[k]->ns_[m_[k]->lc_];
+                t.gate = m_[k]->d_norm_;
+                copies.push_back(t);
             }
+            if (int rc = add_level_stage<T>(3, copies, 0)) return rc;
         }
     }
     // up leg
-    for (int l = lc - 1; l >= 0; --l) {
-        const int n = ns_[l];
-        const int H = 2 * p_.post;
-        const int W = MG_TILE + 2 * H;
-        const size_t lds = 2 * (size_t)W * W * sizeof(T);
-        const T* ein = (l == 0) ? e0_tmp : E(1, l);
-        const T* rhs = (l == 0) ? rhs0 : R(l);
-        T* eout = (l == 0) ? e0_out : E(0, l);
-        const double hh = hs[l];
-        hipLaunchKernelGGL(mg_prolong_smooth_tile<T>, dim3(div_up(n, MG_TILE), div_up(n, MG_TILE)), dim3(256), lds,
-                           stream_, ein, E(0, l + 1), rhs, eout, n, (T)(hh * hh), omega, p_.post, H);
-        ++launches_;
-    }
-    PIC_HIP(hipGetLastError());
-    return 0;
-}
-
-int MGSolver::reduce(const double* partials, int np, int which) {
-    hipLaunchKernelGGL(norm_final_kernel, dim3(1), dim3(256), 0, stream_, partials, np, d_norm_, which, p_.tol);
-    ++launches_;
-    return 0;
-}
-
-int MGSolver::solve(double* phi, const double* rhs, double h, MGStats* st) {
-    PIC_REQUIRE(ok_, "multigrid: solver not initialised");
-    launches_ = 0;
-    const size_t nn = (size_t)n_ * n_;
-    const int gb = div_up(n_, 16);
-    const double inv_h2 = 1.0 / (h * h);
-    const bool mixed = p_.mode == PIC_MG_MIXED;
-    // ||rhs||
-    hipLaunchKernelGGL(sumsq_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, stream_, rhs, nn, partials_);
-    ++launches_;
-    reduce(partials_, (int)((nn + 255) / 256), 0);
-    auto residual = [&]() {
-        if (mixed)
-            hipLaunchKernelGGL(ir_residual_kernel<true>, dim3(gb, gb), dim3(256), 0, stream_, phi, rhs, r32_,
-                               partials_, n_, inv_h2);
-        else
-            hipLaunchKernelGGL(ir_residual_kernel<false>, dim3(gb, gb), dim3(256), 0, stream_, phi, rhs,
-                               (float*)nullptr, partials_, n_, inv_h2);
-        ++launches_;
-        reduce(partials_, gb * gb, 1);
-    };
-    auto fetch = [&]() -> int {
-        PIC_HIP(hipMemcpyAsync(h_norm_, d_norm_, sizeof(MGNorm), hipMemcpyDeviceToHost, stream_));
-        PIC_HIP(hipStreamSynchronize(stream_));
-        return 0;
-    };
-    residual();
-    if (int rc = fetch()) return rc;
-    int cycles = 0;
-    while (!h_norm_->converged && cycles < p_.max_cycles) {
-        if (mixed) {
-            if (int rc = vcycle<float>(nullptr, ef_[0][0], ef_[1][0], r32_, h)) return rc;
-            hipLaunchKernelGGL(ir_correct_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, stream_, phi,
-                               ef_[0][0], nn);
-            ++launches_;
-        } else {
-            if (int rc = vcycle<double>(phi, phi, phi_tmp_, rhs, h)) return rc;
+    for (int d = D - 1; d >= 0; --d) {
+        std::vector<MGLevelTask<T>> tasks;
+        for (int k = 0; k < M; ++k) {
+            MGSolver* s = m_[k];
+            if (s->lc_ <= d) continue;
+            const double hh = hl(k, d);
+            MGLevelTask<T> t{};
+            t.e_in = d == 0 ? tmp0(k) : E(s, 1, d);
+            t.coarse = E(s, 0, d + 1);
+            t.rhs = d == 0 ? rhs0(k) : R(s, d);
+            t.e_out = d == 0 ? out0(k) : E(s, 0, d);
+            t.n = s->ns_[d];
+            t.nu = p_.post;
+            t.H = 2 * p_.post;
+            t.h2 = (T)(hh * hh);
+            t.inv_h2 = (T)(1.0 / (hh * hh));
+            t.omega = omega;
+            t.gate = s->d_norm_;
+            tasks.push_back(t);
         }
-        residual();
-        if (int rc = fetch()) return rc;
-        ++cycles;
+        const size_t W = MG_TILE + 2 * (2 * p_.post);
+        if (int rc = add_level_stage<T>(1, tasks, 2 * W * W * sizeof(T))) return rc;
+    }
+    return 0;
+}
+
+int MGBatch::build(const std::vector<MGSolver*>& members, const std::vector<double*>& phi,
+                   const std::vector<const double*>& rhs, const std::vector<double>& h) {
+    free_all();
+    PIC_REQUIRE(!members.empty() && members.size() == phi.size() && members.size() == rhs.size() &&
+                    members.size() == h.size(), "multigrid batch: inconsistent members");
+    m_ = members;
+    phi_ = phi;
+    rhs_ = rhs;
+    p_ = members[0]->p_;
+    for (auto* s : members) {
+        PIC_REQUIRE(s->ok_, "multigrid batch: member not initialised");
+        const MGParams& q = s->p_;
+        PIC_REQUIRE(q.mode == p_.mode && q.pre == p_.pre && q.post == p_.post && q.coarse_sweeps == p_.coarse_sweeps &&
+                        q.omega == p_.omega && q.tol == p_.tol && q.max_cycles == p_.max_cycles,
+                    "multigrid batch: members need identical cycle parameters");
+    }
+    mixed_ = p_.mode == PIC_MG_MIXED;
+    const int M = (int)m_.size();
+    PIC_HIP(hipMalloc(&d_mirror_, M * sizeof(MGNorm)));
+    PIC_HIP(hipMemset(d_mirror_, 0, M * sizeof(MGNorm)));
+    PIC_HIP(hipHostMalloc(&h_mirror_, M * sizeof(MGNorm), hipHostMallocDefault));
+    std::memset(h_mirror_, 0, M * sizeof(MGNorm));
+    std::vector<MGFineTask> fine(M);
+    std::vector<MGTile> chunks, res;
+    for (int k = 0; k < M; ++k) {
+        MGSolver* s = m_[k];
+        const int n = s->n_;
+        const size_t nn = (size_t)n * n;
+        MGFineTask& f = fine[k];
+        f.phi = phi[k];
+        f.rhs = rhs[k];
+        f.r32 = s->r32_;
+        f.e32 = mixed_ ? s->ef_[0][0] : nullptr;
+        f.partial = s->partials_;
+        f.norm = s->d_norm_;
+        f.mirror = d_mirror_ + k;
+        f.n = n;
+        f.gx = div_up(n, 16);
+        f.nblocks = f.gx * f.gx;
+        f.nchunks = (int)((nn + 255) / 256);
+        f.inv_h2 = 1.0 / (h[k] * h[k]);
+        for (int c = 0; c < f.nchunks; ++c) chunks.push_back(MGTile{k, c, 0, 0});
+        for (int ty = 0; ty < f.gx; ++ty)
+            for (int tx = 0; tx < f.gx; ++tx) res.push_back(MGTile{k, tx, ty, 0});
+    }
+    if (int rc = upload_vec((void**)&d_fine_, fine)) return rc;
+    if (int rc = upload_vec((void**)&d_chunk_tiles_, chunks)) return rc;
+    if (int rc = upload_vec((void**)&d_res_tiles_, res)) return rc;
+    n_chunk_ = (int)chunks.size();
+    n_res_ = (int)res.size();
+    return mixed_ ? build_cycle<float>(h) : build_cycle<double>(h);
+}
+
+int MGBatch::launch_stage(const Stage& st) {
+    const dim3 g(st.n_items), b(st.kind == 2 ? 1024 : 256);
+    if (st.fp64) {
+        auto* t = (const MGLevelTask<double>*)st.tasks;
+        if (st.kind == 0) hipLaunchKernelGGL(mg_smooth_stage<double>, g, b, st.lds, stream_, t, st.tiles);
+        else if (st.kind == 1) hipLaunchKernelGGL(mg_prolong_stage<double>, g, b, st.lds, stream_, t, st.tiles);
+        else if (st.kind == 2)
+            hipLaunchKernelGGL(mg_coarse_stage<double>, g, b, st.lds, stream_, (const MGCoarseTask<double>*)st.tasks);
+        else hipLaunchKernelGGL(mg_copy_stage<double>, g, b, 0, stream_, t, st.tiles);
+    } else {
+        auto* t = (const MGLevelTask<float>*)st.tasks;
+        if (st.kind == 0) hipLaunchKernelGGL(mg_smooth_stage<float>, g, b, st.lds, stream_, t, st.tiles);
+        else if (st.kind == 1) hipLaunchKernelGGL(mg_prolong_stage<float>, g, b, st.lds, stream_, t, st.tiles);
+        else if (st.kind == 2)
+            hipLaunchKernelGGL(mg_coarse_stage<float>, g, b, st.lds, stream_, (const MGCoarseTask<float>*)st.tasks);
+        else hipLaunchKernelGGL(mg_copy_stage<float>, g, b, 0, stream_, t, st.tiles);
+    }
+    ++launches_;
+    return 0;
+}
+
+// what: 0 rhs norm, 1 residual (which 1 or 2), 2 correction
+int MGBatch::enqueue_fine(int what, int which) {
+    const int M = (int)m_.size();
+    if (what == 0) {
+        hipLaunchKernelGGL(sumsq_stage, dim3(n_chunk_), dim3(256), 0, stream_, d_fine_, d_chunk_tiles_);
+        hipLaunchKernelGGL(norm_final_stage, dim3(M), dim3(256), 0, stream_, d_fine_, 0, p_.tol, p_.max_cycles);
+        launches_ += 2;
+    } else if (what == 1) {
+        if (mixed_)
+            hipLaunchKernelGGL(ir_residual_stage<true>, dim3(n_res_), dim3(256), 0, stream_, d_fine_, d_res_tiles_,
+                               which);
+        else
+            hipLaunchKernelGGL(ir_residual_stage<false>, dim3(n_res_), dim3(256), 0, stream_, d_fine_, d_res_tiles_,
+                               which);
+        hipLaunchKernelGGL(norm_final_stage, dim3(M), dim3(256), 0, stream_, d_fine_, which, p_.tol, p_.max_cycles);
+        launches_ += 2;
+    } else {
+        hipLaunchKernelGGL(ir_correct_stage, dim3(n_chunk_), dim3(256), 0, stream_, d_fine_, d_chunk_tiles_);
+        ++launches_;
+    }
+    return 0;
+}
+
+// The host enqueues cycles in batches (first batch: the previous solve's
+// largest cycle count) and reads the norms of all members once per batch.  The
+// device flags turn cycles beyond a member's convergence into no-ops, so each
+// result equals testing after every cycle, alone or in a batch.
+int MGBatch::solve() {
+    PIC_REQUIRE(!m_.empty(), "multigrid batch: not built");
+    launches_ = 0;
+    const int M = (int)m_.size();
+    enqueue_fine(0, 0);
+    enqueue_fine(1, 1);
+    int enqueued = 0;
+    int batch = std::max(1, last_cycles_);
+    for (;;) {
+        batch = std::min(batch, p_.max_cycles - enqueued);
+        for (int b = 0; b < batch; ++b) {
+            for (const auto& st : cycle_)
+                if (int rc = launch_stage(st)) return rc;
+            if (mixed_) enqueue_fine(2, 0);
+            enqueue_fine(1, 2);
+        }
+        enqueued += std::max(batch, 0);
+        PIC_HIP(hipMemcpyAsync(h_mirror_, d_mirror_, M * sizeof(MGNorm), hipMemcpyDeviceToHost, stream_));
+        PIC_HIP(hipStreamSynchronize(stream_));
+        bool all = true;
+        for (int k = 0; k < M; ++k) all = all && h_mirror_[k].converged;
+        if (all || enqueued >= p_.max_cycles) break;
+        batch = 1;
     }
     PIC_HIP(hipGetLastError());
-    if (st) {
-        st->cycles = cycles;
-        st->rel_residual = h_norm_->rel;
-        st->rhs_norm = std::sqrt(h_norm_->rhs_sq);
-    }
+    int mx = 0;
+    for (int k = 0; k < M; ++k) mx = std::max(mx, h_mirror_[k].cycles);
+    last_cycles_ = std::max(1, mx);
     return 0;
 }
 

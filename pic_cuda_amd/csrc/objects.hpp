@@ -42,8 +42,10 @@ struct pic_particles {
     std::vector<pic::SpeciesStore> sp;
     pic::CaseDev<float> hf{};
     pic::CaseDev<double> hd{};
-    void* d_case = nullptr;        // CaseDev<T> on the device
+    void* d_case = nullptr;        // CaseDev<T> on the device (own slot or an ensemble slot)
     pic::CaseState* d_state = nullptr;
+    void* own_case = nullptr;      // the slots allocated with the particles
+    pic::CaseState* own_state = nullptr;
     int h_cur = 0;                 // host mirror of the buffer parity
     pic::WorkItem* d_items = nullptr;   // scratch item table for store-level kernels
     int items_alloc = 0;
@@ -117,4 +119,15 @@ struct pic_sim {
     int* h_status = nullptr;
     pic::ChargeArgs charge{};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    struct pic_ensemble* ens = nullptr;   // batch this simulation belongs to
 };
+
+namespace pic {
+// shared helpers of capi.cpp used by ensemble.cpp
+void items_for_species(const pic_particles* p, int s, int chunk, std::vector<WorkItem>& out, int case_idx = 0);
+int upload_case(pic_particles* p);
+int upload_items(WorkItem** dptr, int* alloc, const std::vector<WorkItem>& items);
+}  // namespace pic
+size_t cur_lds(pic_sim* S);
+int mcc_active(pic_sim* S);
+int sim_rebin(pic_sim* S, int bw, int bh);
