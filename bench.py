@@ -120,12 +120,23 @@ def poisson(n=2049, cpu=True):
     return out
 
 
+# cells per side with a multigrid ladder down to 3x3 or 4x4 nodes (32*2^k or
+# 32*3*2^k): the log-uniform draws are rounded to the nearest in log space
+GRID_SIZES = (64, 96, 128, 192, 256, 384, 512, 768, 1024)
+
+
+def round_cells(c, lo, hi):
+    import numpy as np
+    sizes = [s for s in GRID_SIZES if lo <= s <= hi]
+    return min(sizes, key=lambda s: abs(np.log(s) - np.log(c)))
+
+
 def ensemble_cases(n_cases=64, seed=4):
     import numpy as np
     rng = np.random.default_rng(seed)
     cases = []
     for k in range(n_cases):
-        cells = int(max(64, min(512, round(np.exp(rng.uniform(np.log(64), np.log(512))) / 32) * 32)))
+        cells = round_cells(np.exp(rng.uniform(np.log(64), np.log(512))), 64, 512)
         npart = int(min(4e6, max(1e4, 100 * cells * cells)))
         te = float(rng.uniform(1.0, 5.0))
         cases.append((cells, npart, te, 1000 + k))
@@ -133,7 +144,7 @@ def ensemble_cases(n_cases=64, seed=4):
 
 
 def ensemble(n_cases=64, steps=100, sequential=True):
-    """Batched ensemble: 64 plasmas, cells log-uniform 64..512 (multiples of 32),
+    """Batched ensemble: 64 plasmas, cells log-uniform 64..512 (rounded to GRID_SIZES),
     clamp(100 cells^2, 1e4, 4e6) particles per species, Te ~ U(1,5) eV, MCC on;
     one launch per stage for all cases, against the same cases one after
     another through the single-plasma path."""
@@ -178,7 +189,7 @@ def ensemble(n_cases=64, steps=100, sequential=True):
 
 def serve(n_jobs=24, rate=20.0, budget=200e9, seed=9, max_steps=2000):
     """Serving: Poisson arrivals (rate jobs/s), cells log-uniform 64..1024
-    (multiples of 32), 100 particles per cell and species, steps U(200, max),
+    (rounded to GRID_SIZES), 100 particles per cell and species, steps U(200, max),
     seed = job id, admission by an HBM budget, continuous batching."""
     import numpy as np
     import pic_cuda_amd as pic
@@ -187,7 +198,7 @@ def serve(n_jobs=24, rate=20.0, budget=200e9, seed=9, max_steps=2000):
     sv = pic.Server(budget, 256, pic.PREC_FP32)
     ids = []
     for k in range(n_jobs):
-        cells = int(max(64, min(1024, round(np.exp(rng.uniform(np.log(64), np.log(1024))) / 32) * 32)))
+        cells = round_cells(np.exp(rng.uniform(np.log(64), np.log(1024))), 64, 1024)
         steps = int(rng.integers(200, max_steps + 1))
         ids.append(sv.submit(cells, 100.0, steps, seed=k, arrival_s=float(t_arr[k])))
     t0 = time.perf_counter()

@@ -219,10 +219,12 @@ static double sumsq(const double* v, size_t n) {
     return s;
 }
 
+/* automatic ladder: coarsen while the node count is odd and above 3; the
+   coarsest level is then 3x3 or 4x4 (checked by the caller) */
 static int mg_levels_for(int n, int requested) {
     if (requested > 0) return requested;
     int L = 1;
-    while (n > 3) { n = (n + 1) / 2; ++L; }
+    while (n > 3 && (n & 1)) { n = (n + 1) / 2; ++L; }
     return L;
 }
 
@@ -240,6 +242,7 @@ extern "C" int orc_mg_solve(double* phi, const double* rhs, int nx, int ny, doub
         if ((ns[l - 1] & 1) == 0) return -2;   /* ladder must stay odd */
         ns[l] = (ns[l - 1] + 1) / 2;
     }
+    if (p->levels <= 0 && ns[L - 1] != 3 && ns[L - 1] != 4) return -3;
     std::vector<std::vector<double>> ph(L), rh(L), rs(L);
     for (int l = 0; l < L; ++l) {
         size_t sz = (size_t)ns[l] * ns[l];
@@ -269,7 +272,9 @@ extern "C" int orc_mg_solve(double* phi, const double* rhs, int nx, int ny, doub
         {
             double* P = (L - 1) ? ph[L - 1].data() : phi;
             const double* R = (L - 1) ? rh[L - 1].data() : rhs;
-            orc_mg_smooth(P, R, ns[L - 1], hh, p->omega, p->coarse_sweeps);
+            /* a 4x4 coarsest level of an automatic ladder gets at least 32 sweeps */
+            const int cs = (p->levels <= 0 && ns[L - 1] == 4 && p->coarse_sweeps < 32) ? 32 : p->coarse_sweeps;
+            orc_mg_smooth(P, R, ns[L - 1], hh, p->omega, cs);
         }
         for (int l = L - 2; l >= 0; --l) {
             hh /= 2.0;

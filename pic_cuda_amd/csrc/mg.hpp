@@ -109,6 +109,13 @@ public:
     int levels() const { return L_; }
     int last_launches() const { return launches_; }
     void set_stream(hipStream_t s) { stream_ = s; }
+    // sweeps on the coarsest level: a 4x4 coarsest level of an automatic ladder
+    // (2x2 unknowns, Gauss-Seidel factor 1/4 per sweep) gets at least 32, which
+    // solves it to rounding; a 3x3 level is solved exactly by one sweep
+    int coarse_sweeps_eff() const {
+        return (p_.levels <= 0 && !ns_.empty() && ns_.back() == 4) ? (p_.coarse_sweeps > 32 ? p_.coarse_sweeps : 32)
+                                                                 : p_.coarse_sweeps;
+    }
 
 private:
     friend class MGBatch;

@@ -425,7 +425,10 @@ int MGSolver::init() {
     if (p_.levels > 0) {
         L = p_.levels;
     } else {
-        while (m > 3) { m = (m + 1) / 2; ++L; }
+        // coarsen while the level has an odd node count above 3; the coarsest
+        // level must then be 3x3 (one unknown) or 4x4 (2x2 unknowns)
+        while (m > 3 && (m & 1)) { m = (m + 1) / 2; ++L; }
+        PIC_REQUIRE(m == 3 || m == 4, "multigrid: with automatic levels the grid needs 2^k*{1,3}+1 nodes per side");
     }
     PIC_REQUIRE(L <= 16, "multigrid: at most 16 levels");
     ns_.assign(L, 0);
@@ -621,7 +624,7 @@ int MGBatch::build_cycle(const std::vector<double>& h) {
             t.Lc = s->L_ - lc;
             t.pre = p_.pre;
             t.post = p_.post;
-            t.csweeps = p_.coarse_sweeps;
+            t.csweeps = s->coarse_sweeps_eff();
             t.h_top = (T)hl(k, lc);
             t.omega = omega;
             t.gate = s->d_norm_;

@@ -64,7 +64,7 @@ def _mg_gpu(pic, n, rhs, phi0, h, params):
     return g.download(pic.FIELD_PHI), st
 
 
-@pytest.mark.parametrize("n", [33, 65, 129, 257])
+@pytest.mark.parametrize("n", [33, 65, 97, 129, 257])
 def test_mg_fp64_bit_identical(pic, orc, n):
     rng = np.random.default_rng(n)
     rhs = rng.normal(size=(n, n))

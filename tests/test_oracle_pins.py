@@ -194,6 +194,30 @@ def test_mg_matches_dense_direct_solve_with_dirichlet_values(orc):
     assert st.cycles > 0
 
 
+def test_mg_ladder_ending_at_4x4(orc):
+    # 97 -> 49 -> 25 -> 13 -> 7 -> 4: the 4x4 coarsest level (2x2 unknowns) is
+    # swept 32 times; the solve still matches the dense direct solution
+    n = 97
+    rng = np.random.default_rng(12)
+    rhs = rng.normal(size=(n, n))
+    rhs[0, :] = rhs[-1, :] = rhs[:, 0] = rhs[:, -1] = 0
+    h = 0.05
+    phi = np.zeros((n, n))
+    st = orc.mg_solve(phi, rhs, h, tol=1e-12, max_cycles=60)
+    assert st.rel_residual < 1e-12 and st.cycles < 20
+    import ctypes
+    res = np.zeros((n, n))
+    orc.lib().orc_mg_residual(res.ctypes.data_as(ctypes.c_void_p), phi.ctypes.data_as(ctypes.c_void_p),
+                              rhs.ctypes.data_as(ctypes.c_void_p), n, h)
+    assert np.linalg.norm(res) < 1e-12 * np.linalg.norm(rhs)
+
+
+def test_mg_rejects_unsupported_ladder(orc):
+    n = 41          # 41 -> 21 -> 11 -> 6: coarsest 6x6 is not supported
+    with pytest.raises(RuntimeError):
+        orc.mg_solve(np.zeros((n, n)), np.ones((n, n)), 0.1)
+
+
 def test_mg_vcycle_contraction(orc):
     n = 129
     rng = np.random.default_rng(6)
