@@ -56,6 +56,7 @@ static void fill_species_ptrs(pic_particles* p, int s) {
         S.start[par] = st.d_start[par];
         S.cap[par] = st.d_cap[par];
         S.count[par] = st.d_count[par];
+        S.mcount[par] = st.d_mcount[par];
         S.nb[par] = st.nb;
     }
     S.counters = st.d_counters;
@@ -100,11 +101,14 @@ static int alloc_layout_parity(SpeciesStore& st, int par, int entries) {
     if (st.d_start[par]) PIC_HIP(hipFree(st.d_start[par]));
     if (st.d_cap[par]) PIC_HIP(hipFree(st.d_cap[par]));
     if (st.d_count[par]) PIC_HIP(hipFree(st.d_count[par]));
-    st.d_start[par] = st.d_cap[par] = st.d_count[par] = nullptr;
+    if (st.d_mcount[par]) PIC_HIP(hipFree(st.d_mcount[par]));
+    st.d_start[par] = st.d_cap[par] = st.d_count[par] = st.d_mcount[par] = nullptr;
     PIC_HIP(hipMalloc(&st.d_start[par], (size_t)entries * sizeof(int32_t)));
     PIC_HIP(hipMalloc(&st.d_cap[par], (size_t)entries * sizeof(int32_t)));
     PIC_HIP(hipMalloc(&st.d_count[par], (size_t)entries * sizeof(int32_t)));
+    PIC_HIP(hipMalloc(&st.d_mcount[par], (size_t)entries * sizeof(int32_t)));
     PIC_HIP(hipMemset(st.d_count[par], 0, (size_t)entries * sizeof(int32_t)));
+    PIC_HIP(hipMemset(st.d_mcount[par], 0, (size_t)entries * sizeof(int32_t)));
     return 0;
 }
 
@@ -115,7 +119,8 @@ static int create_species(pic_particles* p, double m, double q) {
     st.m = m;
     st.q = q;
     st.w = 1.0;
-    const int64_t cap = p->max_per_species + p->max_per_species / 4 + 65536;
+    // bins get 1.5x their population plus 64 slots at every re-binning
+    const int64_t cap = p->max_per_species + p->max_per_species / 2 + p->max_per_species / 8 + (1 << 20);
     PIC_REQUIRE(cap < (int64_t)INT32_MAX, "capacity per species must stay below 2^31 slots");
     st.capacity = cap;
     const size_t eb = (size_t)p->prec;
@@ -224,6 +229,7 @@ extern "C" int pic_particles_destroy(pic_particles* p) {
             (void)hipFree(st.d_start[par]);
             (void)hipFree(st.d_cap[par]);
             (void)hipFree(st.d_count[par]);
+            (void)hipFree(st.d_mcount[par]);
         }
         (void)hipFree(st.d_counters);
     }
@@ -864,9 +870,20 @@ static int rebin_t(pic_sim* S, int bw, int bh) {
             set_error("re-binning: particle count exceeds the species capacity");
             return PIC_ERR_CAPACITY;
         }
-        const int64_t avail = C_ - total - spill_min;
-        int64_t want = 0;
-        for (int b = 0; b < nb_new; ++b) want += hist[b] / 4 + 32 + 15;
+        // every bin keeps at least its population in the front part (3/4 of
+        // the bin); the slack targets 1.5x the population + 64, 16-aligned
+        auto cmin = [](int64_t h) { return (4 * h + 2) / 3 + 4; };
+        auto extra = [&](int64_t h) { return std::max<int64_t>(0, h + h / 2 + 64 - cmin(h)); };
+        int64_t need = 0, want = 0;
+        for (int b = 0; b < nb_new; ++b) {
+            need += cmin(hist[b]);
+            want += extra(hist[b]) + 15;
+        }
+        if (need + spill_min > C_) {
+            set_error("re-binning: particle count exceeds the species capacity");
+            return PIC_ERR_CAPACITY;
+        }
+        const int64_t avail = C_ - spill_min - need;
         const bool full = want <= avail;
         const double scale = full ? 1.0 : (double)avail / (double)std::max<int64_t>(want, 1);
         auto& ns_ = new_start[s];
@@ -876,8 +893,8 @@ static int rebin_t(pic_sim* S, int bw, int bh) {
         int64_t pos = 0;
         for (int b = 0; b < nb_new; ++b) {
             int64_t c;
-            if (full) c = (hist[b] + hist[b] / 4 + 32 + 15) / 16 * 16;
-            else c = hist[b] + (int64_t)std::floor((hist[b] / 4 + 32) * scale);
+            if (full) c = (cmin(hist[b]) + extra(hist[b]) + 15) / 16 * 16;
+            else c = cmin(hist[b]) + (int64_t)std::floor(extra(hist[b]) * scale);
             ns_[b] = (int32_t)pos;
             nc_[b] = (int32_t)c;
             pos += c;
@@ -893,6 +910,7 @@ static int rebin_t(pic_sim* S, int bw, int bh) {
         D.start[out] = st.d_start[out];
         D.cap[out] = st.d_cap[out];
         D.count[out] = st.d_count[out];
+        D.mcount[out] = st.d_mcount[out];
         D.nb[out] = nb_new;
     }
     if (int rc = upload_case(p)) return rc;
@@ -918,6 +936,7 @@ static int rebin_t(pic_sim* S, int bw, int bh) {
         D.start[in] = st.d_start[in];
         D.cap[in] = st.d_cap[in];
         D.count[in] = st.d_count[in];
+        D.mcount[in] = st.d_mcount[in];
         D.nb[in] = nb_new;
     }
     C.bw = bw;

@@ -28,6 +28,7 @@ struct SpeciesStore {
     int32_t* d_start[2] = {nullptr, nullptr};
     int32_t* d_cap[2] = {nullptr, nullptr};
     int32_t* d_count[2] = {nullptr, nullptr};
+    int32_t* d_mcount[2] = {nullptr, nullptr};
     int layout_alloc = 0;          // entries allocated in the layout arrays
     int nb = 0;                    // bins of the current layout (spill is index nb)
     std::vector<int32_t> h_start, h_cap;

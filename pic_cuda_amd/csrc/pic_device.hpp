@@ -18,7 +18,8 @@ struct SpeciesDev {
     uint32_t* id[2];
     const int32_t* start[2];   // nb[p]+1 entries
     const int32_t* cap[2];
-    int32_t* count[2];
+    int32_t* count[2];         // front part of each bin (survivors that stayed); spill fill at nb
+    int32_t* mcount[2];        // back part of each bin (particles that moved in)
     int nb[2];
     // physics
     T kq;                      // q dt / m
@@ -64,6 +65,10 @@ struct CaseState {
 };
 
 enum { PIC_F_KICK = 1, PIC_F_MCC = 2, PIC_F_DRIFT = 4, PIC_F_DEPOSIT = 8 };
+
+// entries of the LDS stage of particles leaving their bin (a pass of 256
+// particles adds at most 512: 256 movers and 256 born particles)
+#define PIC_MSTAGE 768
 
 // work item of the particle kernels: a bin of a species of a case, or a chunk
 // of its spill region (bin == nb)

@@ -13,6 +13,15 @@
 //  hist/scatter   re-binning of a species store (counting sort by tile).
 //  gather_kernel  flattening of a store for download.
 //  end_step       buffer-parity flip and step counter, one block per case.
+//
+// Bin layout.  Bin b of a buffer is [start, start+cap): survivors that stay in
+// their bin fill the front part [start, start+cap-cap/4) from slots handed out
+// by a counter in LDS (only the block that owns the bin writes there); particles
+// that move in from other bins, and born particles, fill the back part downwards
+// from start+cap-1 with one global atomic per destination bin per wave.  Movers
+// are staged in LDS and written in a flush phase, so no global atomic sits on
+// the path of a particle that stays.  Overflow of either part goes to the spill
+// region (index nb).
 #include "pic_device.hpp"
 
 namespace pic {
@@ -42,41 +51,50 @@ __device__ inline uint64_t lanemask_lt() {
     return (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 }
 
-// Wave-aggregated slot allocation: lanes that want a slot in bin `key` of the
-// output buffer get consecutive slots with one atomic per distinct key per
-// wave; slots beyond the bin capacity go to the spill region (index nb).
-// Returns the destination index, or -1 when the spill region is full as well.
-// Must be called by every lane of the wave.
+__device__ inline int front_cap(int cap) { return cap - (cap >> 2); }
+__device__ inline int back_cap(int cap) { return cap >> 2; }
+
+// Wave-aggregated slot allocation in the output buffer.  back == false: the
+// front part of bin `key` (re-binning); back == true: the back part (movers and
+// born particles).  Overflow goes to the spill region; -1 when that is full too.
+// key == nb addresses the spill region directly.  Every lane of the wave must
+// call it.
 template <typename T>
-__device__ inline int alloc_slot(const SpeciesDev<T>& S, int out, bool want, int key) {
+__device__ inline int alloc_slot(const SpeciesDev<T>& S, int out, bool want, int key, bool back) {
     const int lane = __lane_id();
     const uint64_t lt = lanemask_lt();
+    const int nb = S.nb[out];
     int slot = -1;
-    uint64_t act = __ballot(want);
+    uint64_t act = __ballot(want && key != nb);
     while (act) {
         const int leader = __ffsll((unsigned long long)act) - 1;
         const int k = __shfl(key, leader);
         const uint64_t m = __ballot(want && key == k) & act;
         int base = 0;
-        if (lane == leader) base = atomicAdd(&S.count[out][k], __popcll(m));
+        if (lane == leader) base = atomicAdd(back ? &S.mcount[out][k] : &S.count[out][k], __popcll(m));
         base = __shfl(base, leader);
         if ((m >> lane) & 1ull) slot = base + __popcll(m & lt);
         act &= ~m;
     }
     int dst = -1;
-    bool over = false;
-    if (want) {
-        if (slot < S.cap[out][key]) dst = S.start[out][key] + slot;
-        else over = true;
+    bool spill = want && key == nb;
+    if (want && key != nb) {
+        const int cap = S.cap[out][key];
+        if (back) {
+            if (slot < back_cap(cap)) dst = S.start[out][key] + cap - 1 - slot;
+            else spill = true;
+        } else {
+            if (slot < front_cap(cap)) dst = S.start[out][key] + slot;
+            else spill = true;
+        }
     }
-    const uint64_t om = __ballot(over);
+    const uint64_t om = __ballot(spill);
     if (om) {
-        const int nb = S.nb[out];
         const int leader = __ffsll((unsigned long long)om) - 1;
         int base = 0;
         if (lane == leader) base = atomicAdd(&S.count[out][nb], __popcll(om));
         base = __shfl(base, leader);
-        if (over) {
+        if (spill) {
             const int s2 = base + __popcll(om & lt);
             dst = (s2 < S.cap[out][nb]) ? S.start[out][nb] + s2 : -1;
         }
@@ -254,17 +272,88 @@ __device__ inline bool mcc_collide(const SpeciesDev<T>& S, uint32_t id, uint32_t
     return false;
 }
 
+// The input of a work item: bin b is its front part [start, start+n1) and its
+// back part [start+cap-n2, start+cap); a spill chunk is one range.
+struct Range {
+    int b1, n1, b2, n2;
+    __device__ inline int n() const { return n1 + n2; }
+    __device__ inline int at(int t) const { return t < n1 ? b1 + t : b2 + (t - n1); }
+};
+
 template <typename T>
-__device__ inline void item_range(const SpeciesDev<T>& S, int in, const WorkItem& it, int chunk, int& begin,
-                                  int& n) {
+__device__ inline Range item_range(const SpeciesDev<T>& S, int in, const WorkItem& it, int chunk) {
     const int nb = S.nb[in];
+    Range r;
     if (it.bin < nb) {
-        begin = S.start[in][it.bin];
-        n = min(S.count[in][it.bin], S.cap[in][it.bin]);
+        const int start = S.start[in][it.bin], cap = S.cap[in][it.bin];
+        r.n1 = min(S.count[in][it.bin], front_cap(cap));
+        r.n2 = min(S.mcount[in][it.bin], back_cap(cap));
+        r.b1 = start;
+        r.b2 = start + cap - r.n2;
     } else {
         const int tot = min(S.count[in][nb], S.cap[in][nb]);
-        begin = S.start[in][nb] + it.chunk * chunk;
-        n = min(chunk, tot - it.chunk * chunk);
+        r.b1 = S.start[in][nb] + it.chunk * chunk;
+        r.n1 = max(0, min(chunk, tot - it.chunk * chunk));
+        r.b2 = 0;
+        r.n2 = 0;
+    }
+    return r;
+}
+
+// LDS staging of the particles that leave their bin (and of born particles of
+// the same species) until a flush writes them to their destination bins
+
+template <typename T>
+struct Stage {
+    T* x;
+    T* y;
+    T* vx;
+    T* vy;
+    uint32_t* id;
+    int* key;
+};
+
+template <typename T>
+__device__ inline void flush_stage(const SpeciesDev<T>& S, int out, const Stage<T>& sg, int nm, unsigned* lcnt) {
+    const int wave = threadIdx.x >> 6, lane = __lane_id();
+    for (int base = wave * 64; base < nm; base += blockDim.x) {
+        const int e = base + lane;
+        const bool want = e < nm;
+        const int key = want ? sg.key[e] : 0;
+        const int dst = alloc_slot(S, out, want, key, true);
+        if (want) {
+            if (dst >= 0) {
+                S.x[out][dst] = sg.x[e];
+                S.y[out][dst] = sg.y[e];
+                S.vx[out][dst] = sg.vx[e];
+                S.vy[out][dst] = sg.vy[e];
+                S.id[out][dst] = sg.id[e];
+                if (dst >= S.start[out][S.nb[out]]) atomicAdd(lcnt + PIC_C_SPILLED, 1u);
+            } else {
+                atomicAdd(lcnt + PIC_C_LOST, 1u);
+            }
+        }
+    }
+}
+
+// append the lanes with `want` to the stage (wave-aggregated LDS atomic)
+template <typename T>
+__device__ inline void stage_put(const Stage<T>& sg, int* nm, bool want, T x, T y, T vx, T vy, uint32_t id, int key) {
+    const uint64_t m = __ballot(want);
+    if (!m) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(nm, __popcll(m));
+    base = __shfl(base, leader);
+    if (want) {
+        const int e = base + __popcll(m & lanemask_lt());
+        sg.x[e] = x;
+        sg.y[e] = y;
+        sg.vx[e] = vx;
+        sg.vy[e] = vy;
+        sg.id[e] = id;
+        sg.key[e] = key;
     }
 }
 
@@ -278,26 +367,37 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
                                                    int flags_arg, long long mcc_key) {
     using T2 = typename Vec2<T>::type;
     extern __shared__ unsigned long long smem_u64[];
+    __shared__ int sh_stay, sh_nm;
     const WorkItem it = items[blockIdx.x];
     const CaseDev<T>& C = cases[it.case_idx];
     const CaseState st = states[it.case_idx];
     const SpeciesDev<T>& S = C.sp[it.species];
     const int in = st.cur, out = in ^ 1;
-    int begin, n;
-    item_range(S, in, it, spill_chunk, begin, n);
+    const Range rg = item_range(S, in, it, spill_chunk);
+    const int n = rg.n();
+    const bool is_bin = it.bin < S.nb[in];
+    // an empty bin still publishes its (zero) stay count; nothing else to do
     if (n <= 0) return;
 
     const int nx = C.nx, ny = C.ny;
-    const bool is_bin = it.bin < S.nb[in];
     const int H = C.halo;
     const int bx = is_bin ? it.bin % C.nbx : 0, by = is_bin ? it.bin / C.nbx : 0;
     const int cx0 = bx * C.bw, cy0 = by * C.bh;
-    const int ew = C.bw + 1, eh = C.bh + 1;           // E window nodes
+    const int ew = C.bw + 1, eh = C.bh + 1;                  // E window nodes
     const int aw = C.bw + 1 + 2 * H, ah = C.bh + 1 + 2 * H;  // deposit window nodes
     const int wx0 = cx0 - H, wy0 = cy0 - H;
+    // LDS: deposit window (u64), E window (T2), stage (4 T + id + key), counters
     unsigned long long* accw = smem_u64;
     T2* ew_lds = reinterpret_cast<T2*>(smem_u64 + (is_bin ? aw * ah : 0));
-    unsigned* lcnt = reinterpret_cast<unsigned*>(ew_lds + (is_bin ? ew * eh : 0));
+    T* stage_base = reinterpret_cast<T*>(ew_lds + (is_bin ? ew * eh : 0));
+    Stage<T> sg;
+    sg.x = stage_base;
+    sg.y = sg.x + PIC_MSTAGE;
+    sg.vx = sg.y + PIC_MSTAGE;
+    sg.vy = sg.vx + PIC_MSTAGE;
+    sg.id = reinterpret_cast<uint32_t*>(sg.vy + PIC_MSTAGE);
+    sg.key = reinterpret_cast<int*>(sg.id + PIC_MSTAGE);
+    unsigned* lcnt = reinterpret_cast<unsigned*>(sg.key + PIC_MSTAGE);
     const int flags = flags_arg >= 0 ? flags_arg : st.flags;
     const bool kick = flags & PIC_F_KICK, mcc = (flags & PIC_F_MCC) && S.n_proc > 0, drift = flags & PIC_F_DRIFT,
                dep = flags & PIC_F_DEPOSIT;
@@ -306,6 +406,10 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
     const uint32_t mcc_step = mcc_key >= 0 ? (uint32_t)mcc_key : st.step - 1u;
 
     for (int k = threadIdx.x; k < PIC_NCOUNT; k += blockDim.x) lcnt[k] = 0u;
+    if (threadIdx.x == 0) {
+        sh_stay = 0;
+        sh_nm = 0;
+    }
     if (is_bin) {
         for (int k = threadIdx.x; k < aw * ah; k += blockDim.x) accw[k] = 0ull;
         if (kick) {
@@ -327,6 +431,9 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
 
     const T kq = S.kq, dtdx = C.dtdx, dtdy = C.dtdy;
     const SpeciesDev<T>* O = (S.other >= 0) ? &C.sp[S.other] : nullptr;
+    const int home_start = is_bin ? S.start[out][it.bin] : 0;
+    const int home_front = is_bin ? front_cap(S.cap[out][it.bin]) : 0;
+    const int lane = __lane_id();
 
     for (int base = 0; base < n; base += blockDim.x) {
         const int t = base + threadIdx.x;
@@ -334,7 +441,7 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
         T x = (T)0, y = (T)0, vx = (T)0, vy = (T)0;
         uint32_t id = 0;
         if (valid) {
-            const int src = begin + t;
+            const int src = rg.at(t);
             x = S.x[in][src];
             y = S.y[in][src];
             vx = S.vx[in][src];
@@ -414,48 +521,58 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
                 }
             }
         }
-        const int dst = alloc_slot(S, out, alive, key);
-        if (alive) {
-            if (dst >= 0) {
-                S.x[out][dst] = x;
-                S.y[out][dst] = y;
-                S.vx[out][dst] = vx;
-                S.vy[out][dst] = vy;
-                S.id[out][dst] = id;
-                if (dst >= S.start[out][S.nb[out]]) atomicAdd(lcnt + PIC_C_SPILLED, 1u);
-            } else {
-                atomicAdd(lcnt + PIC_C_LOST, 1u);
+        // survivors that stay in this bin: front slots from the LDS counter
+        bool stay = alive && is_bin && key == it.bin;
+        const uint64_t sm = __ballot(stay);
+        if (sm) {
+            const int leader = __ffsll((unsigned long long)sm) - 1;
+            int sbase = 0;
+            if (lane == leader) sbase = atomicAdd(&sh_stay, __popcll(sm));
+            sbase = __shfl(sbase, leader);
+            if (stay) {
+                const int slot = sbase + __popcll(sm & lanemask_lt());
+                if (slot < home_front) {
+                    const int dst = home_start + slot;
+                    S.x[out][dst] = x;
+                    S.y[out][dst] = y;
+                    S.vx[out][dst] = vx;
+                    S.vy[out][dst] = vy;
+                    S.id[out][dst] = id;
+                } else {
+                    stay = false;   // front part full: goes with the movers
+                }
             }
         }
-        if (__ballot(be_alive || bi_alive)) {
-            const int de = alloc_slot(S, out, be_alive, key_e);
-            if (be_alive) {
-                if (de >= 0) {
-                    S.x[out][de] = b.ex;
-                    S.y[out][de] = b.ey;
-                    S.vx[out][de] = b.evx;
-                    S.vy[out][de] = b.evy;
-                    S.id[out][de] = b.eid;
+        // movers and born particles of this species: staged
+        stage_put(sg, &sh_nm, alive && !stay, x, y, vx, vy, id, key);
+        stage_put(sg, &sh_nm, be_alive, b.ex, b.ey, b.evx, b.evy, b.eid, key_e);
+        // born ions of the other species (rare): written directly
+        if (O && __ballot(bi_alive)) {
+            const int di = alloc_slot(*O, out, bi_alive, key_i, true);
+            if (bi_alive) {
+                if (di >= 0) {
+                    O->x[out][di] = b.ix;
+                    O->y[out][di] = b.iy;
+                    O->vx[out][di] = b.ivx;
+                    O->vy[out][di] = b.ivy;
+                    O->id[out][di] = b.iid;
                 } else {
                     atomicAdd(lcnt + PIC_C_LOST, 1u);
                 }
             }
-            if (O) {
-                const int di = alloc_slot(*O, out, bi_alive, key_i);
-                if (bi_alive) {
-                    if (di >= 0) {
-                        O->x[out][di] = b.ix;
-                        O->y[out][di] = b.iy;
-                        O->vx[out][di] = b.ivx;
-                        O->vy[out][di] = b.ivy;
-                        O->id[out][di] = b.iid;
-                    } else {
-                        atomicAdd(lcnt + PIC_C_LOST, 1u);
-                    }
-                }
-            }
+        }
+        __syncthreads();
+        const int nm = sh_nm;
+        if (nm > PIC_MSTAGE - 512) {
+            flush_stage(S, out, sg, nm, lcnt);
+            __syncthreads();
+            if (threadIdx.x == 0) sh_nm = 0;
+            __syncthreads();
         }
     }
+    const int nm = sh_nm;
+    if (nm > 0) flush_stage(S, out, sg, nm, lcnt);
+    if (is_bin && threadIdx.x == 0) S.count[out][it.bin] = min(sh_stay, home_front);
     __syncthreads();
     if (is_bin) {
         for (int k = threadIdx.x; k < aw * ah; k += blockDim.x) {
@@ -517,15 +634,18 @@ __global__ __launch_bounds__(256) void hist_kernel(const CaseDev<T>* __restrict_
     const CaseDev<T>& C = cases[it.case_idx];
     const SpeciesDev<T>& S = C.sp[it.species];
     const int in = states[it.case_idx].cur;
-    int begin, n;
-    item_range(S, in, it, chunk, begin, n);
+    const Range rg = item_range(S, in, it, chunk);
+    const int n = rg.n();
     if (n <= 0) return;
     const int lane = __lane_id();
     for (int base = 0; base < n; base += blockDim.x) {
         const int t = base + threadIdx.x;
         const bool want = t < n;
         int key = 0;
-        if (want) key = key_for(C.nx - 2, C.ny - 2, bw, bh, nbx, nb_new, S.x[in][begin + t], S.y[in][begin + t]);
+        if (want) {
+            const int s = rg.at(t);
+            key = key_for(C.nx - 2, C.ny - 2, bw, bh, nbx, nb_new, S.x[in][s], S.y[in][s]);
+        }
         uint64_t act = __ballot(want);
         while (act) {
             const int leader = __ffsll((unsigned long long)act) - 1;
@@ -546,8 +666,8 @@ __global__ __launch_bounds__(256) void scatter_kernel(const CaseDev<T>* __restri
     const CaseDev<T>& C = cases[it.case_idx];
     const SpeciesDev<T>& S = C.sp[it.species];
     const int in = states[it.case_idx].cur, out = in ^ 1;
-    int begin, n;
-    item_range(S, in, it, chunk, begin, n);
+    const Range rg = item_range(S, in, it, chunk);
+    const int n = rg.n();
     if (n <= 0) return;
     for (int base = 0; base < n; base += blockDim.x) {
         const int t = base + threadIdx.x;
@@ -556,7 +676,7 @@ __global__ __launch_bounds__(256) void scatter_kernel(const CaseDev<T>* __restri
         T x = 0, y = 0, vx = 0, vy = 0;
         uint32_t id = 0;
         if (want) {
-            const int s = begin + t;
+            const int s = rg.at(t);
             x = S.x[in][s];
             y = S.y[in][s];
             vx = S.vx[in][s];
@@ -564,7 +684,7 @@ __global__ __launch_bounds__(256) void scatter_kernel(const CaseDev<T>* __restri
             id = S.id[in][s];
             key = key_for(C.nx - 2, C.ny - 2, bw, bh, nbx, S.nb[out], x, y);
         }
-        const int dst = alloc_slot(S, out, want, key);
+        const int dst = alloc_slot(S, out, want, key, false);
         if (want) {
             if (dst >= 0) {
                 S.x[out][dst] = x;
@@ -589,20 +709,21 @@ __global__ __launch_bounds__(256) void gather_kernel(const CaseDev<T>* __restric
     const CaseDev<T>& C = cases[it.case_idx];
     const SpeciesDev<T>& S = C.sp[it.species];
     const int in = states[it.case_idx].cur, out = in ^ 1;
-    int begin, n;
-    item_range(S, in, it, chunk, begin, n);
+    const Range rg = item_range(S, in, it, chunk);
+    const int n = rg.n();
     if (n <= 0) return;
     const int lane = __lane_id();
     for (int base = 0; base < n; base += blockDim.x) {
         const int t = base + threadIdx.x;
         const bool want = t < n;
         const uint64_t m = __ballot(want);
+        if (!m) continue;
         const int leader = __ffsll((unsigned long long)m) - 1;
         int b0 = 0;
         if (lane == leader) b0 = atomicAdd(counter, __popcll(m));
         b0 = __shfl(b0, leader);
         if (want) {
-            const int s = begin + t, d = b0 + __popcll(m & lanemask_lt());
+            const int s = rg.at(t), d = b0 + __popcll(m & lanemask_lt());
             S.x[out][d] = S.x[in][s];
             S.y[out][d] = S.y[in][s];
             S.vx[out][d] = S.vx[in][s];
@@ -631,8 +752,8 @@ __global__ void status_kernel(const CaseDev<T>* __restrict__ cases, const CaseSt
     }
 }
 
-// number of particles held by one species of one case (sum over bins of
-// min(count, cap) plus the spill region), for last_alive
+// number of particles held by one species of one case (both parts of every
+// bin plus the spill region), for last_alive
 template <typename T>
 __global__ __launch_bounds__(256) void count_kernel(const CaseDev<T>* __restrict__ cases,
                                                     const CaseState* __restrict__ states, int c, int s,
@@ -641,8 +762,13 @@ __global__ __launch_bounds__(256) void count_kernel(const CaseDev<T>* __restrict
     const CaseDev<T>& C = cases[c];
     const int cur = states[c].cur;
     const SpeciesDev<T>& S = C.sp[s];
+    const int nb = S.nb[cur];
     long long acc = 0;
-    for (int b = threadIdx.x; b <= S.nb[cur]; b += 256) acc += min(S.count[cur][b], S.cap[cur][b]);
+    for (int b = threadIdx.x; b <= nb; b += 256) {
+        const int cap = S.cap[cur][b];
+        if (b < nb) acc += min(S.count[cur][b], front_cap(cap)) + min(S.mcount[cur][b], back_cap(cap));
+        else acc += min(S.count[cur][b], cap);
+    }
     sh[threadIdx.x] = acc;
     __syncthreads();
     for (int k = 128; k > 0; k >>= 1) {
@@ -664,7 +790,10 @@ __global__ __launch_bounds__(256) void end_step_kernel(const CaseDev<T>* __restr
     for (int s = 0; s < C.ns; ++s) {
         const SpeciesDev<T>& S = C.sp[s];
         const int nb = S.nb[in];
-        for (int k = threadIdx.x; k <= nb; k += blockDim.x) S.count[in][k] = 0;
+        for (int k = threadIdx.x; k <= nb; k += blockDim.x) {
+            S.count[in][k] = 0;
+            S.mcount[in][k] = 0;
+        }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -680,8 +809,8 @@ template <typename T>
 int launch_push(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items,
                 int spill_chunk, int flags, long long mcc_key, size_t lds, hipStream_t s) {
     if (n_items <= 0) return 0;
-    hipLaunchKernelGGL(push_kernel<T>, dim3(n_items), dim3(256), lds, s, cases, states, items, spill_chunk,
-                       flags, mcc_key);
+    hipLaunchKernelGGL(push_kernel<T>, dim3(n_items), dim3(256), lds, s, cases, states, items, spill_chunk, flags,
+                       mcc_key);
     PIC_HIP(hipGetLastError());
     return 0;
 }
@@ -711,8 +840,8 @@ template <typename T>
 int launch_scatter(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items,
                    int chunk, int bw, int bh, int nbx, unsigned long long* lost, hipStream_t s) {
     if (n_items <= 0) return 0;
-    hipLaunchKernelGGL(scatter_kernel<T>, dim3(n_items), dim3(256), 0, s, cases, states, items, chunk, bw, bh,
-                       nbx, lost);
+    hipLaunchKernelGGL(scatter_kernel<T>, dim3(n_items), dim3(256), 0, s, cases, states, items, chunk, bw, bh, nbx,
+                       lost);
     PIC_HIP(hipGetLastError());
     return 0;
 }
@@ -763,21 +892,5 @@ int launch_counts(const CaseDev<T>* cases, const CaseState* states, int c, int s
     template int launch_end_step<T>(const CaseDev<T>*, CaseState*, int, int, hipStream_t);
 PIC_LAUNCH_INST(float)
 PIC_LAUNCH_INST(double)
-
-// instantiate
-#define PIC_INST(T)                                                                                          \
-    template __global__ void push_kernel<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int, int,  \
-                                            long long);                                                      \
-    template __global__ void init_kernel<T>(T*, T*, T*, T*, uint32_t*, const int32_t*, int, int, int, double, \
-                                            uint32_t, uint32_t, uint32_t, uint32_t);                         \
-    template __global__ void hist_kernel<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int, int,  \
-                                            int, int, int, int32_t*);                                        \
-    template __global__ void scatter_kernel<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int,    \
-                                               int, int, int, unsigned long long*);                          \
-    template __global__ void gather_kernel<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int,     \
-                                              int32_t*);                                                     \
-    template __global__ void end_step_kernel<T>(const CaseDev<T>*, CaseState*, int);
-PIC_INST(float)
-PIC_INST(double)
 
 }  // namespace pic
