@@ -25,7 +25,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-steps/s at 1024^2+4e7 ptcls; MG-solve ms; ensemble jobs/s, p99 lat"
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_PARTICLE = 36.0   # x,y,vx,vy FP32 read + write, id read
+BYTES_PER_PARTICLE = 40.0   # x, y, vx, vy (FP32) and id (u32), read from one buffer, written to the other
 
 
 def read_traffic():
