@@ -6,11 +6,12 @@ Workload (one step = one pass of the hot path): 1024^2 cells (1025^2 nodes),
 n = 1e15 m^-3, dx = 0.5 Debye lengths, dt = 0.2/omega_pe), absorbing box walls
 with compaction every step, null-collision MCC (e-Ar elastic, excitation,
 ionization; Ar+ charge exchange) on Ar at 1e21 m^-3, FP32 particles, Poisson
-solve by FP32 V-cycles inside FP64 iterative refinement to ||r||/||rhs|| < 1e-8.
+solve by FP32 V-cycles (2+2 red-black SOR, factor 1.2) inside FP64 iterative
+refinement to ||r||/||rhs|| < 1e-8.
 Inputs are generated on the device and stay resident in HBM.
 
 Prints one JSON line: particle-steps/s (sum of live particles over the timed
-steps / wall time), the roofline of the dominant kernel (push_kernel: 36
+steps / wall time), the roofline of the dominant kernel (push_kernel: 40
 algorithmic bytes per particle, timed with HIP events on the library's stream)
 and a CPU baseline (the serial oracle on a bounded sample of the same workload).
 """
@@ -63,7 +64,10 @@ def cpu_baseline(gpu_case, budget_s=12.0):
     w = pl.density * pl.length * pl.length / n
     qs = 1.0 / float(1 << 40)
     cfac = [-(q * w * qs / (pl.dx * pl.dx)) / pic.EPS0 for q in (-pic.ELEMENTARY_CHARGE, pic.ELEMENTARY_CHARGE)]
-    o = orc.OracleSim(sp, nn, nn, pl.dx, pl.dx, pl.dt, cfac, dict(tol=1e-8, max_cycles=100))
+    mg = gpu_case["mg"]
+    o = orc.OracleSim(sp, nn, nn, pl.dx, pl.dx, pl.dt, cfac,
+                      dict(tol=mg.tol, max_cycles=mg.max_cycles, pre=mg.pre, post=mg.post,
+                           coarse_sweeps=mg.coarse_sweeps, omega=mg.omega))
     o.step(1)                      # first pass has no kick; time full steps after it
     t0 = time.perf_counter()
     steps = 0
@@ -113,7 +117,9 @@ def poisson(n=2049, cpu=True):
         from oracle import oracle as orc
         phi = np.zeros((n, n))
         t0 = time.perf_counter()
-        st = orc.mg_solve(phi, rhs, h, tol=1e-10, max_cycles=100)
+        p64 = pic.mg_params(pic.PRESET_POISSON, mode=pic.MG_FP64)
+        st = orc.mg_solve(phi, rhs, h, pre=p64.pre, post=p64.post, coarse_sweeps=p64.coarse_sweeps,
+                          omega=p64.omega, tol=p64.tol, max_cycles=p64.max_cycles)
         el = time.perf_counter() - t0
         out["cpu_fp64_oracle"] = {"ms": round(el * 1e3, 1), "cycles": st.cycles, "cores": 1}
         out["mixed_vs_cpu_fp64_rel_l2"] = float(np.linalg.norm(sols["mixed"] - phi) / np.linalg.norm(phi))
@@ -284,7 +290,7 @@ def main():
         "warmup": a.warmup, "ms_per_step": 1e3 * el / a.steps, "higher_is_better": True, "scaling": "strong",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "full PIC step: %d^2 cells, %.3g e- + %.3g Ar+, null-collision MCC on Ar 1e21 m^-3, "
-                               "absorbing box walls, FP32 particles, FP32-in-FP64 multigrid to 1e-8"
+                               "absorbing box walls, FP32 particles, FP32-in-FP64 multigrid (2+2 red-black SOR 1.2) to 1e-8"
                                % (a.cells, alive_mean / 2, alive_mean / 2),
                    "cells": a.cells, "particles": int(st0.alive), "bin": a.bin},
         "roofline": roof, "cpu_baseline": cpu,

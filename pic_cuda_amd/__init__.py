@@ -37,7 +37,7 @@ PREC_FP32, PREC_FP64 = 4, 8
 ELASTIC, EXCITATION, IONIZATION, CHARGE_EXCHANGE = 0, 1, 2, 3
 FIELD_RHO, FIELD_PHI, FIELD_EX, FIELD_EY, FIELD_RHS = 0, 1, 2, 3, 4
 MG_FP64, MG_MIXED = 0, 1
-PRESET_REFERENCE, PRESET_STEP, PRESET_POISSON = 0, 1, 2
+PRESET_REFERENCE, PRESET_STEP, PRESET_POISSON, PRESET_STEP_SOR = 0, 1, 2, 3
 CNT = dict(absorbed=0, null_candidates=1, process0=2, process1=3, process2=4, process3=5, born_self=6,
            born_other=7, born_absorbed=8, numax_exceeded=9, reject_fallback=10, spilled=11, lost=12,
            global_deposit=13)

@@ -18,7 +18,7 @@ import numpy as np
 
 from . import (AMU, ELEMENTARY_CHARGE as QE, ELECTRON_MASS as ME, EPS0, Boundaries2D, CrossSections, Grid2D,
                Particles2D, Simulation, mg_params, sim_params, PREC_FP32, PREC_FP64, ELASTIC, EXCITATION,
-               IONIZATION, CHARGE_EXCHANGE, MG_FP64, MG_MIXED, PRESET_STEP)
+               IONIZATION, CHARGE_EXCHANGE, MG_FP64, MG_MIXED, PRESET_STEP, PRESET_STEP_SOR)
 
 M_AR = 39.948 * AMU
 
@@ -69,14 +69,17 @@ def build(cells, n_per_species, precision, mcc, seed=1, case_id=0, mg=None, bin_
         xs = CrossSections(gas_density, gas_t)
         add_argon_processes(xs, 0, ion)
     if mg is None:
-        mg = mg_params(PRESET_STEP, mode=MG_MIXED if precision == PREC_FP32 else MG_FP64)
+        mg = mg_params(PRESET_STEP_SOR, mode=MG_MIXED if precision == PREC_FP32 else MG_FP64)
     sim = Simulation(parts, grid, walls, xs, sim_params(dt=pl.dt, seed=seed, case_id=case_id, bin_w=bin_w,
                                                         bin_h=bin_h, halo=halo, mcc=1 if mcc else 0, mg=mg))
-    return dict(plasma=pl, grid=grid, particles=parts, walls=walls, xs=xs, sim=sim)
+    return dict(plasma=pl, grid=grid, particles=parts, walls=walls, xs=xs, sim=sim, mg=mg)
 
 
 def plain_small(scale=1.0, **kw):
+    """128^2 cells, 1e5 + 1e5 particles, FP64, no collisions, multigrid 2+2
+    red-black Gauss-Seidel to 1e-8."""
     n = max(1000, int(1e5 * scale))
+    kw.setdefault("mg", mg_params(PRESET_STEP, mode=MG_FP64))
     return build(128, n, PREC_FP64, mcc=False, **kw)
 
 

@@ -581,7 +581,8 @@ extern "C" int pic_mg_default_params(int preset, pic_mg_params* out) {
     switch (preset) {
         case PIC_MG_PRESET_REFERENCE: *out = pic_mg_params{0, 3, 3, 20, 1.25, 1e-5, 50, PIC_MG_FP64}; break;
         case PIC_MG_PRESET_STEP: *out = pic_mg_params{0, 2, 2, 1, 1.0, 1e-8, 100, PIC_MG_MIXED}; break;
-        case PIC_MG_PRESET_POISSON: *out = pic_mg_params{0, 2, 2, 1, 1.0, 1e-10, 100, PIC_MG_MIXED}; break;
+        case PIC_MG_PRESET_POISSON: *out = pic_mg_params{0, 2, 2, 1, 1.2, 1e-10, 100, PIC_MG_MIXED}; break;
+        case PIC_MG_PRESET_STEP_SOR: *out = pic_mg_params{0, 2, 2, 1, 1.2, 1e-8, 100, PIC_MG_MIXED}; break;
         default: set_error("unknown multigrid preset"); return PIC_ERR_ARG;
     }
     return 0;
@@ -1065,7 +1066,7 @@ extern "C" int pic_sim_default_params(pic_sim_params* out) {
     out->mcc = 1;
     out->rebin_every = 0;
     out->wall_eps = 1e-10;
-    return pic_mg_default_params(PIC_MG_PRESET_STEP, &out->mg);
+    return pic_mg_default_params(PIC_MG_PRESET_STEP_SOR, &out->mg);
 }
 
 extern "C" int pic_sim_create(pic_particles* p, pic_grid* g, pic_walls* w, pic_xs* xs, const pic_sim_params* prm,

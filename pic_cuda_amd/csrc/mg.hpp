@@ -79,14 +79,15 @@ struct MGCoarseTask {
 // one case of the FP64 refinement / norm kernels
 struct MGFineTask {
     double* phi;
+    double* phi_tmp;   // mixed mode: phi alternates between phi and phi_tmp
     const double* rhs;
     float* r32;        // mixed mode: FP32 residual (the cycle's rhs)
     const float* e32;  // mixed mode: the cycle's correction
-    double* partial;   // per-block sums of squares
+    double* partial;   // per-tile sums of squares
     MGNorm* norm;
     MGNorm* mirror;    // contiguous copy of all norms of the batch
     int n;
-    int gx;            // 16x16 blocks per row
+    int gx;            // residual tiles per row
     int nblocks;       // residual partials of this case
     int nchunks;       // 256-node chunks of this case
     double inv_h2;
@@ -156,6 +157,8 @@ public:
 private:
     struct Stage {
         int kind = 0;      // 0 smooth, 1 prolong, 2 coarse, 3 copy
+        int nu = 0;        // sweeps of a tiled stage
+        bool restr = false;   // smooth stage also restricts the residual
         bool fp64 = false;
         void* tasks = nullptr;
         MGTile* tiles = nullptr;
@@ -165,9 +168,9 @@ private:
     template <typename T>
     int build_cycle(const std::vector<double>& h);
     template <typename T>
-    int add_level_stage(int kind, const std::vector<MGLevelTask<T>>& tasks, size_t lds);
+    int add_level_stage(int kind, const std::vector<MGLevelTask<T>>& tasks, int nu, bool restr);
     int launch_stage(const Stage& st);
-    int enqueue_fine(int what, int which);
+    int enqueue_fine(int what, int which, int parity);
     void free_all();
 
     hipStream_t stream_;
@@ -179,7 +182,7 @@ private:
     std::vector<Stage> cycle_;          // stages of one V-cycle
     MGFineTask* d_fine_ = nullptr;      // per-case refinement tasks
     MGTile* d_chunk_tiles_ = nullptr;   // (case, chunk of 256 nodes)
-    MGTile* d_res_tiles_ = nullptr;     // (case, 16x16 block)
+    MGTile* d_res_tiles_ = nullptr;     // (case, 64x16 residual tile)
     int n_chunk_ = 0, n_res_ = 0;
     MGNorm* d_mirror_ = nullptr;
     MGNorm* h_mirror_ = nullptr;        // pinned

@@ -9,21 +9,25 @@
 //
 // gfx950 structure (one launch per level and leg, no grid-wide barrier, which
 // replaces the cooperative smoother of :229-267 that fails above the
-// co-residency limit):
-//   mg_smooth_tile         32x32 output tile + halo 2*nu(+2) staged in LDS;
-//                          nu sweeps in LDS (the halo shrinks by one node per
-//                          half-sweep), then residual and restriction of the
-//                          coarse nodes the tile owns.
-//   mg_prolong_smooth_tile prolongation-add over tile + halo 2*nu, nu sweeps.
-//   mg_coarse_kernel       one workgroup runs the whole sub-cycle of the levels
-//                          with n <= 33 in LDS.
-//   ir_residual_kernel     FP64 residual (stored FP32 for the next FP32 cycle)
-//                          and per-block sums of squares.
-//   ir_correct_kernel      phi += e in FP64.
-//   norm_final_kernel      fixed-order reduction of the block sums, relative
-//                          residual and the convergence flag (build rule
-//                          ||r||/||rhs||; the reference divides by sum rhs^2,
-//                          Multigrid.cu:466).
+// co-residency limit; a device-wide barrier costs more than a launch here):
+//   mg_smooth_stage   32x32 output tile + halo 2*nu(+2) staged in LDS; nu
+//                     sweeps in LDS, each half-sweep skipping the outer ring
+//                     that can no longer be exact; then residual and
+//                     restriction of the coarse nodes the tile owns.  The
+//                     sweep count is a template parameter, so the tile width
+//                     is a constant and every global load of the tile is in
+//                     flight before the first wait.
+//   mg_prolong_stage  coarse patch + fine tile + halo 2*nu in LDS,
+//                     prolongation-add, nu sweeps.
+//   mg_coarse_stage   one workgroup runs the whole sub-cycle of the levels
+//                     with n <= 65 (FP32) / 33 (FP64) in LDS.
+//   ir_residual_stage FP64 residual on 64x16 tiles, per-tile sums of r^2; in
+//                     mixed mode fused with the correction phi += e of the
+//                     previous FP32 cycle (phi alternates between two buffers).
+//   norm_final_stage  fixed-order reduction of the tile sums, relative
+//                     residual and the convergence flag (build rule
+//                     ||r||/||rhs||; the reference divides by sum rhs^2,
+//                     Multigrid.cu:466).
 #include "mg.hpp"
 #include <algorithm>
 #include <cmath>
@@ -36,37 +40,57 @@ namespace pic {
 // below 64 KB: FP32 65^2, FP64 33^2)
 #define MG_NC_F32 65
 #define MG_NC_F64 33
+// residual tiles: 64 columns x 16 rows, 256 threads with 4 rows each
+#define MG_RX 64
+#define MG_RY 16
 
 template <typename T>
 __device__ inline T tfma(T a, T b, T c);
 template <> __device__ inline float tfma(float a, float b, float c) { return fmaf(a, b, c); }
 template <> __device__ inline double tfma(double a, double b, double c) { return fma(a, b, c); }
 
-// one colour of red-black SOR over a (rows x cols) LDS array whose element
-// (0,0) sits at global node (gi0, gj0); only global interior nodes change
+// the SOR update of one node (the reference's expression)
 template <typename T>
-__device__ inline void rb_colour_lds(T* e, const T* r, int W, int Hh, int gi0, int gj0, int n, T h2, T omega,
-                                     int colour, int tid, int nthr) {
-    const int half = (W - 1) / 2;
-    const T om1 = (T)1 - omega;
-    for (int k = tid; k < (Hh - 2) * half; k += nthr) {
-        const int lj = 1 + k / half, m = k % half;
+__device__ inline T sor_node(const T* e, const T* r, int idx, int stride, T h2, T omega) {
+    const T c = e[idx];
+    const T s = e[idx + stride] + e[idx - stride] + e[idx + 1] + e[idx - 1];
+    const T nv = (T)0.25 * (s - h2 * r[idx]);
+    return tfma(omega, nv, ((T)1 - omega) * c);
+}
+
+// One colour of red-black SOR on a W x W LDS tile whose element (0,0) is the
+// global node (gi0, gj0).  Only rows and columns [lo, W-1-lo] are updated: after
+// s half-sweeps the outer s rings are stale, so half-sweep s uses lo = s + 1.
+// Colour c updates the nodes with (gi + gj) odd for c = 1, even for c = 0.
+template <typename T, int W>
+__device__ inline void tile_colour(T* e, const T* r, int gi0, int gj0, int n, T h2, T omega, int colour, int lo) {
+    const int span = W - 2 * lo;
+    const int halfc = (span + 1) / 2;
+    const int items = span * halfc;
+    for (int k = threadIdx.x; k < items; k += 256) {
+        const int q = k / halfc;
+        const int lj = lo + q, m = k - q * halfc;
         const int gj = gj0 + lj;
-        const int li = 1 + ((colour ^ (gi0 + 1 + gj)) & 1) + 2 * m;
-        if (li > W - 2) continue;
+        const int li = lo + ((colour + gi0 + gj + lo) & 1) + 2 * m;
+        if (li > W - 1 - lo) continue;
         const int gi = gi0 + li;
         if (gi <= 0 || gi >= n - 1 || gj <= 0 || gj >= n - 1) continue;
         const int idx = lj * W + li;
-        const T c = e[idx];
-        const T s = e[idx + W] + e[idx - W] + e[idx + 1] + e[idx - 1];
-        const T nv = (T)0.25 * (s - h2 * r[idx]);
-        e[idx] = tfma(omega, nv, om1 * c);
+        e[idx] = sor_node(e, r, idx, W, h2, omega);
     }
 }
 
-template <typename T>
+// Down-leg stage: load tile + halo, NU sweeps, store the tile; with RESTRICT
+// also the residual on tile + 1 ring and the restriction to the coarse nodes
+// (2ic, 2jc) inside the tile.  Without RESTRICT (tiled sweeps of a large
+// coarsest level) the halo is 2*NU.
+template <typename T, int NU, bool RESTRICT>
 __global__ __launch_bounds__(256) void mg_smooth_stage(const MGLevelTask<T>* __restrict__ tasks,
                                                        const MGTile* __restrict__ tiles) {
+    constexpr int H = RESTRICT ? 2 * NU + 2 : 2 * NU;
+    constexpr int W = MG_TILE + 2 * H;
+    constexpr int WW = W * W;
+    constexpr int NL = (WW + 255) / 256;
     const MGTile tile = tiles[blockIdx.x];
     const MGLevelTask<T>& tk = tasks[tile.task];
     if (tk.gate && tk.gate->converged) return;   // cycle enqueued after convergence: no-op
@@ -74,59 +98,73 @@ __global__ __launch_bounds__(256) void mg_smooth_stage(const MGLevelTask<T>* __r
     const T* __restrict__ rhs = tk.rhs;
     T* __restrict__ e_out = tk.e_out;
     T* __restrict__ rhs_c = tk.rhs_c;
-    const int n = tk.n, nu = tk.nu, H = tk.H;
+    const int n = tk.n;
     const T h2 = tk.h2, inv_h2 = tk.inv_h2, omega = tk.omega;
     extern __shared__ unsigned char smem_mg[];
-    const int W = MG_TILE + 2 * H;
     T* e = reinterpret_cast<T*>(smem_mg);
-    T* r = e + W * W;
-    T* res = r + W * W;
+    T* r = e + WW;
     const int tid = threadIdx.x;
     const int i0 = tile.tx * MG_TILE, j0 = tile.ty * MG_TILE;
     const int gi0 = i0 - H, gj0 = j0 - H;
-    for (int k = tid; k < W * W; k += 256) {
-        const int gi = gi0 + k % W, gj = gj0 + k / W;
-        T ev = (T)0, rv = (T)0;
-        if (gi >= 0 && gi < n && gj >= 0 && gj < n) {
+    T ev[NL], rv[NL];
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+        const int k = tid + 256 * q;
+        const int lj = k / W, li = k - lj * W;
+        const int gi = gi0 + li, gj = gj0 + lj;
+        ev[q] = (T)0;
+        rv[q] = (T)0;
+        if (k < WW && gi >= 0 && gi < n && gj >= 0 && gj < n) {
             const size_t g = (size_t)gj * n + gi;
-            if (e_in) ev = e_in[g];
-            rv = rhs[g];
+            if (e_in) ev[q] = e_in[g];
+            rv[q] = rhs[g];
         }
-        e[k] = ev;
-        r[k] = rv;
+    }
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+        const int k = tid + 256 * q;
+        if (k < WW) {
+            e[k] = ev[q];
+            r[k] = rv[q];
+        }
     }
     __syncthreads();
-    for (int it = 0; it < nu; ++it)
-        for (int colour = 1; colour >= 0; --colour) {
-            rb_colour_lds(e, r, W, W, gi0, gj0, n, h2, omega, colour, tid, 256);
-            __syncthreads();
-        }
-    for (int k = tid; k < MG_TILE * MG_TILE; k += 256) {
+#pragma unroll
+    for (int s = 0; s < 2 * NU; ++s) {
+        tile_colour<T, W>(e, r, gi0, gj0, n, h2, omega, 1 - (s & 1), s + 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < (MG_TILE * MG_TILE) / 256; ++q) {
+        const int k = tid + 256 * q;
         const int gi = i0 + k % MG_TILE, gj = j0 + k / MG_TILE;
         if (gi < n && gj < n) e_out[(size_t)gj * n + gi] = e[(gj - gj0) * W + (gi - gi0)];
     }
-    if (!rhs_c) return;
-    const int RW = MG_TILE + 2;
-    for (int k = tid; k < RW * RW; k += 256) {
-        const int gi = i0 - 1 + k % RW, gj = j0 - 1 + k / RW;
-        T v = (T)0;
-        if (gi > 0 && gi < n - 1 && gj > 0 && gj < n - 1) {
-            const int idx = (gj - gj0) * W + (gi - gi0);
-            const T c = e[idx];
-            const T lap = (e[idx - W] + e[idx + W] + e[idx - 1] + e[idx + 1] - (T)4 * c) * inv_h2;
-            v = r[idx] - lap;
+    if constexpr (RESTRICT) {
+        if (!rhs_c) return;
+        constexpr int RW = MG_TILE + 2;
+        T* res = r + WW;
+        for (int k = tid; k < RW * RW; k += 256) {
+            const int q = k / RW;
+            const int gi = i0 - 1 + (k - q * RW), gj = j0 - 1 + q;
+            T v = (T)0;
+            if (gi > 0 && gi < n - 1 && gj > 0 && gj < n - 1) {
+                const int idx = (gj - gj0) * W + (gi - gi0);
+                const T c = e[idx];
+                const T lap = (e[idx - W] + e[idx + W] + e[idx - 1] + e[idx + 1] - (T)4 * c) * inv_h2;
+                v = r[idx] - lap;
+            }
+            res[k] = v;
         }
-        res[k] = v;
-    }
-    __syncthreads();
-    const int nc = (n + 1) / 2, TC = MG_TILE / 2;
-    for (int k = tid; k < TC * TC; k += 256) {
-        const int ic = i0 / 2 + k % TC, jc = j0 / 2 + k / TC;
-        if (ic >= nc || jc >= nc) continue;
+        __syncthreads();
+        const int nc = (n + 1) / 2;
+        constexpr int TC = MG_TILE / 2;
+        const int ic = i0 / 2 + tid % TC, jc = j0 / 2 + tid / TC;   // TC*TC == 256
+        if (ic >= nc || jc >= nc) return;
         const int i = 2 * ic, j = 2 * jc;
         if (i >= n || j >= n) {
             rhs_c[(size_t)jc * nc + ic] = (T)0;
-            continue;
+            return;
         }
         const int f = (j - j0 + 1) * RW + (i - i0 + 1);
         T s = (T)4 * res[f];
@@ -142,22 +180,30 @@ __global__ __launch_bounds__(256) void mg_smooth_stage(const MGLevelTask<T>* __r
     }
 }
 
+// bilinear prolongation of the coarse value at fine node (gi, gj), with the
+// coarse patch in LDS: element (0,0) of the CW-wide patch is coarse (ic0, jc0)
 template <typename T>
-__device__ inline T prolong_val(const T* __restrict__ coarse, int nc, int gi, int gj) {
-    const int ic = gi / 2, jc = gj / 2;
-    const size_t c = (size_t)jc * nc + ic;
+__device__ inline T prolong_lds(const T* C, int CW, int ic0, int jc0, int nc, int gi, int gj) {
+    const int ic = gi >> 1, jc = gj >> 1;
+    const int c = (jc - jc0) * CW + (ic - ic0);
     const bool io = gi & 1, jo = gj & 1;
-    if (!io && !jo) return coarse[c];
-    if (io && !jo) return (ic + 1 < nc) ? (T)0.5 * (coarse[c] + coarse[c + 1]) : coarse[c];
-    if (!io && jo) return (jc + 1 < nc) ? (T)0.5 * (coarse[c] + coarse[c + nc]) : coarse[c];
-    return (ic + 1 < nc && jc + 1 < nc)
-               ? (T)0.25 * (coarse[c] + coarse[c + 1] + coarse[c + nc] + coarse[c + nc + 1])
-               : coarse[c];
+    if (!io && !jo) return C[c];
+    if (io && !jo) return (ic + 1 < nc) ? (T)0.5 * (C[c] + C[c + 1]) : C[c];
+    if (!io && jo) return (jc + 1 < nc) ? (T)0.5 * (C[c] + C[c + CW]) : C[c];
+    return (ic + 1 < nc && jc + 1 < nc) ? (T)0.25 * (C[c] + C[c + 1] + C[c + CW] + C[c + CW + 1]) : C[c];
 }
 
-template <typename T>
+// Up-leg stage: e = e_in + P(coarse) on tile + halo 2*NU, NU sweeps, store the tile.
+template <typename T, int NU>
 __global__ __launch_bounds__(256) void mg_prolong_stage(const MGLevelTask<T>* __restrict__ tasks,
                                                         const MGTile* __restrict__ tiles) {
+    constexpr int H = 2 * NU;
+    constexpr int W = MG_TILE + 2 * H;
+    constexpr int WW = W * W;
+    constexpr int NL = (WW + 255) / 256;
+    constexpr int CW = W / 2 + 2;
+    constexpr int CC = CW * CW;
+    constexpr int NCL = (CC + 255) / 256;
     const MGTile tile = tiles[blockIdx.x];
     const MGLevelTask<T>& tk = tasks[tile.task];
     if (tk.gate && tk.gate->converged) return;
@@ -165,47 +211,88 @@ __global__ __launch_bounds__(256) void mg_prolong_stage(const MGLevelTask<T>* __
     const T* __restrict__ coarse = tk.coarse;
     const T* __restrict__ rhs = tk.rhs;
     T* __restrict__ e_out = tk.e_out;
-    const int n = tk.n, nu = tk.nu, H = tk.H;
+    const int n = tk.n;
     const T h2 = tk.h2, omega = tk.omega;
     extern __shared__ unsigned char smem_mg[];
-    const int W = MG_TILE + 2 * H;
     T* e = reinterpret_cast<T*>(smem_mg);
-    T* r = e + W * W;
+    T* r = e + WW;
+    T* C = r + WW;
     const int tid = threadIdx.x;
     const int i0 = tile.tx * MG_TILE, j0 = tile.ty * MG_TILE;
-    const int gi0 = i0 - H, gj0 = j0 - H;
+    const int gi0 = i0 - H, gj0 = j0 - H;   // even: H and i0 are
     const int nc = (n + 1) / 2;
-    for (int k = tid; k < W * W; k += 256) {
-        const int gi = gi0 + k % W, gj = gj0 + k / W;
-        T ev = (T)0, rv = (T)0;
-        if (gi >= 0 && gi < n && gj >= 0 && gj < n) {
+    const int ic0 = gi0 >> 1, jc0 = gj0 >> 1;
+    T ev[NL], rv[NL], cv[NCL];
+#pragma unroll
+    for (int q = 0; q < NCL; ++q) {
+        const int k = tid + 256 * q;
+        const int cj = k / CW, ci = k - cj * CW;
+        const int ic = ic0 + ci, jc = jc0 + cj;
+        cv[q] = (T)0;
+        if (k < CC && ic >= 0 && ic < nc && jc >= 0 && jc < nc) cv[q] = coarse[(size_t)jc * nc + ic];
+    }
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+        const int k = tid + 256 * q;
+        const int lj = k / W, li = k - lj * W;
+        const int gi = gi0 + li, gj = gj0 + lj;
+        ev[q] = (T)0;
+        rv[q] = (T)0;
+        if (k < WW && gi >= 0 && gi < n && gj >= 0 && gj < n) {
             const size_t g = (size_t)gj * n + gi;
-            ev = e_in[g];
-            rv = rhs[g];
-            if (gi / 2 < nc && gj / 2 < nc) ev = ev + prolong_val(coarse, nc, gi, gj);
+            ev[q] = e_in[g];
+            rv[q] = rhs[g];
         }
-        e[k] = ev;
-        r[k] = rv;
+    }
+#pragma unroll
+    for (int q = 0; q < NCL; ++q) {
+        const int k = tid + 256 * q;
+        if (k < CC) C[k] = cv[q];
     }
     __syncthreads();
-    for (int it = 0; it < nu; ++it)
-        for (int colour = 1; colour >= 0; --colour) {
-            rb_colour_lds(e, r, W, W, gi0, gj0, n, h2, omega, colour, tid, 256);
-            __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+        const int k = tid + 256 * q;
+        if (k < WW) {
+            const int lj = k / W, li = k - lj * W;
+            const int gi = gi0 + li, gj = gj0 + lj;
+            T v = ev[q];
+            if (gi >= 0 && gi < n && gj >= 0 && gj < n && (gi >> 1) < nc && (gj >> 1) < nc)
+                v = v + prolong_lds(C, CW, ic0, jc0, nc, gi, gj);
+            e[k] = v;
+            r[k] = rv[q];
         }
-    for (int k = tid; k < MG_TILE * MG_TILE; k += 256) {
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 2 * NU; ++s) {
+        tile_colour<T, W>(e, r, gi0, gj0, n, h2, omega, 1 - (s & 1), s + 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < (MG_TILE * MG_TILE) / 256; ++q) {
+        const int k = tid + 256 * q;
         const int gi = i0 + k % MG_TILE, gj = j0 + k / MG_TILE;
         if (gi < n && gj < n) e_out[(size_t)gj * n + gi] = e[(gj - gj0) * W + (gi - gi0)];
     }
 }
 
 // ---- the single-workgroup sub-cycle for the small levels ---------------------
+// 1024 threads as 32 x 32: thread (tx, ty) takes rows ty, ty+32, ... and
+// columns tx, tx+32, ... (or every other column for one colour).
 template <typename T>
-__device__ inline void lds_smooth(T* e, const T* r, int n, T h, T omega, int iters) {
+__device__ inline void lds_sweeps(T* e, const T* r, int n, T h, T omega, int iters) {
     const T h2 = h * h;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     for (int it = 0; it < iters; ++it)
         for (int colour = 1; colour >= 0; --colour) {
-            rb_colour_lds(e, r, n, n, 0, 0, n, h2, omega, colour, threadIdx.x, blockDim.x);
+            for (int j = 1 + ty; j < n - 1; j += 32) {
+                const int ib = 1 + ((colour + j + 1) & 1);   // (i + j) & 1 == colour
+                for (int i = ib + 2 * tx; i < n - 1; i += 64) {
+                    const int idx = j * n + i;
+                    e[idx] = sor_node(e, r, idx, n, h2, omega);
+                }
+            }
             __syncthreads();
         }
 }
@@ -221,76 +308,88 @@ __global__ __launch_bounds__(1024) void mg_coarse_stage(const MGCoarseTask<T>* _
     const T h_top = tk.h_top, omega = tk.omega;
     extern __shared__ unsigned char smem_mg[];
     T* base = reinterpret_cast<T*>(smem_mg);
-    T* E[16];
-    T* R[16];
-    int N[16];
-    int off = 0, n = n_top;
-    for (int l = 0; l < Lc; ++l) {
-        N[l] = n;
-        E[l] = base + off;
-        off += n * n;
-        R[l] = base + off;
-        off += n * n;
-        n = (n + 1) / 2;
+    // level l: n = s_n[l], e at base + s_off[l], rhs right after it; the
+    // residual scratch follows the last level (s_off[Lc])
+    __shared__ int s_n[17], s_off[17];
+    if ((int)threadIdx.x <= Lc) {
+        int n = n_top, off = 0;
+        for (int l = 0; l < (int)threadIdx.x; ++l) {
+            off += 2 * n * n;
+            n = (n + 1) / 2;
+        }
+        s_n[threadIdx.x] = n;
+        s_off[threadIdx.x] = off;
     }
-    T* res = base + off;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    for (int k = tid; k < n_top * n_top; k += nt) {
-        E[0][k] = e_in ? e_in[k] : (T)0;
-        R[0][k] = rhs_top[k];
-    }
+    auto E = [&](int l) { return base + s_off[l]; };
+    auto R = [&](int l) { return base + s_off[l] + s_n[l] * s_n[l]; };
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int j = ty; j < n_top; j += 32)
+        for (int i = tx; i < n_top; i += 32) {
+            const int k = j * n_top + i;
+            base[k] = e_in ? e_in[k] : (T)0;
+            base[n_top * n_top + k] = rhs_top[k];
+        }
     __syncthreads();
     T h = h_top;
     for (int l = 0; l < Lc - 1; ++l) {
-        const int nf = N[l], ncs = N[l + 1];
-        lds_smooth(E[l], R[l], nf, h, omega, pre);
+        const int nf = s_n[l], ncs = s_n[l + 1];
+        lds_sweeps(E(l), R(l), nf, h, omega, pre);
         const T inv_h2 = (T)1 / (h * h);
-        for (int k = tid; k < nf * nf; k += nt) {
-            const int i = k % nf, j = k / nf;
-            T v = (T)0;
-            if (!(i == 0 || i == nf - 1 || j == 0 || j == nf - 1)) {
-                const T* e = E[l];
-                const T lap = (e[k - nf] + e[k + nf] + e[k - 1] + e[k + 1] - (T)4 * e[k]) * inv_h2;
-                v = R[l][k] - lap;
+        const T* e = E(l);
+        const T* rl = R(l);
+        T* res = base + s_off[Lc];
+        for (int j = ty; j < nf; j += 32)
+            for (int i = tx; i < nf; i += 32) {
+                const int k = j * nf + i;
+                T v = (T)0;
+                if (!(i == 0 || i == nf - 1 || j == 0 || j == nf - 1)) {
+                    const T lap = (e[k - nf] + e[k + nf] + e[k - 1] + e[k + 1] - (T)4 * e[k]) * inv_h2;
+                    v = rl[k] - lap;
+                }
+                res[k] = v;
             }
-            res[k] = v;
-        }
         __syncthreads();
-        for (int k = tid; k < ncs * ncs; k += nt) {
-            const int ic = k % ncs, jc = k / ncs;
-            const int i = 2 * ic, j = 2 * jc;
-            T s = (T)0;
-            if (i < nf && j < nf) {
-                const int f = j * nf + i;
-                s = (T)4 * res[f];
-                if (i > 0) s += (T)2 * res[f - 1];
-                if (i < nf - 1) s += (T)2 * res[f + 1];
-                if (j > 0) s += (T)2 * res[f - nf];
-                if (j < nf - 1) s += (T)2 * res[f + nf];
-                if (i > 0 && j > 0) s += res[f - nf - 1];
-                if (i < nf - 1 && j > 0) s += res[f - nf + 1];
-                if (i > 0 && j < nf - 1) s += res[f + nf - 1];
-                if (i < nf - 1 && j < nf - 1) s += res[f + nf + 1];
-                s = s * (T)0.0625;
+        for (int jc = ty; jc < ncs; jc += 32)
+            for (int ic = tx; ic < ncs; ic += 32) {
+                const int i = 2 * ic, j = 2 * jc;
+                T s = (T)0;
+                if (i < nf && j < nf) {
+                    const int f = j * nf + i;
+                    s = (T)4 * res[f];
+                    if (i > 0) s += (T)2 * res[f - 1];
+                    if (i < nf - 1) s += (T)2 * res[f + 1];
+                    if (j > 0) s += (T)2 * res[f - nf];
+                    if (j < nf - 1) s += (T)2 * res[f + nf];
+                    if (i > 0 && j > 0) s += res[f - nf - 1];
+                    if (i < nf - 1 && j > 0) s += res[f - nf + 1];
+                    if (i > 0 && j < nf - 1) s += res[f + nf - 1];
+                    if (i < nf - 1 && j < nf - 1) s += res[f + nf + 1];
+                    s = s * (T)0.0625;
+                }
+                const int kc = jc * ncs + ic;
+                R(l + 1)[kc] = s;
+                E(l + 1)[kc] = (T)0;
             }
-            R[l + 1][k] = s;
-            E[l + 1][k] = (T)0;
-        }
         __syncthreads();
         h = h * (T)2;
     }
-    lds_smooth(E[Lc - 1], R[Lc - 1], N[Lc - 1], h, omega, csweeps);
+    lds_sweeps(E(Lc - 1), R(Lc - 1), s_n[Lc - 1], h, omega, csweeps);
     for (int l = Lc - 2; l >= 0; --l) {
         h = h / (T)2;
-        const int nf = N[l], ncs = N[l + 1];
-        for (int k = tid; k < nf * nf; k += nt) {
-            const int i = k % nf, j = k / nf;
-            if (i / 2 < ncs && j / 2 < ncs) E[l][k] = E[l][k] + prolong_val(E[l + 1], ncs, i, j);
-        }
+        const int nf = s_n[l], ncs = s_n[l + 1];
+        const T* C = E(l + 1);
+        T* el = E(l);
+        for (int j = ty; j < nf; j += 32)
+            for (int i = tx; i < nf; i += 32) {
+                if ((i >> 1) >= ncs || (j >> 1) >= ncs) continue;
+                const int k = j * nf + i;
+                el[k] = el[k] + prolong_lds(C, ncs, 0, 0, ncs, i, j);
+            }
         __syncthreads();
-        lds_smooth(E[l], R[l], nf, h, omega, post);
+        lds_sweeps(el, R(l), nf, h, omega, post);
     }
-    for (int k = tid; k < n_top * n_top; k += nt) e_out[k] = E[0][k];
+    for (int j = ty; j < n_top; j += 32)
+        for (int i = tx; i < n_top; i += 32) e_out[j * n_top + i] = base[j * n_top + i];
 }
 
 // ---- FP64 refinement and norms --------------------------------------------------
@@ -305,30 +404,69 @@ __device__ inline double block_sum256(double v, double* sh) {
     return sh[0];
 }
 
-// FP64 residual r = rhs - A phi (zero on the boundary), optionally stored in
-// FP32 as the next cycle's rhs, and per-block sums of r^2.  which == 2: skipped
-// once the case has stopped.
-template <bool WRITE_R32>
+// FP64 residual r = rhs - A phi (zero on the boundary) on a 64x16 tile, and
+// the tile's sum of r^2.  Thread (tx, ty) owns column tx, rows 4ty..4ty+3.
+//   MODE 0: phi as is (FP64 V-cycles).
+//   MODE 1: phi as is, r stored FP32 as the next FP32 cycle's rhs.
+//   MODE 2: the correction of the FP32 cycle just done is applied first:
+//           phi_new = phi_src + e (every node value, neighbours included, is
+//           formed the same way), written to the other phi buffer (parity 0:
+//           phi -> phi_tmp, 1: phi_tmp -> phi); r stored FP32.
+// which == 2: a residual at the end of a cycle, a no-op once the case stopped.
+template <int MODE>
 __global__ __launch_bounds__(256) void ir_residual_stage(const MGFineTask* __restrict__ tasks,
-                                                         const MGTile* __restrict__ tiles, int which) {
+                                                         const MGTile* __restrict__ tiles, int which, int parity) {
     const MGTile tile = tiles[blockIdx.x];
     const MGFineTask& tk = tasks[tile.task];
     if (which == 2 && tk.norm->converged) return;
     __shared__ double sh[256];
     const int n = tk.n;
-    const double* __restrict__ phi = tk.phi;
-    const int i = tile.tx * 16 + threadIdx.x % 16, j = tile.ty * 16 + threadIdx.x / 16;
-    double r = 0.0;
-    if (i < n && j < n) {
-        const size_t idx = (size_t)j * n + i;
-        if (!(i == 0 || i == n - 1 || j == 0 || j == n - 1)) {
-            const double c = phi[idx];
-            const double lap = (phi[idx - n] + phi[idx + n] + phi[idx - 1] + phi[idx + 1] - 4.0 * c) * tk.inv_h2;
-            r = tk.rhs[idx] - lap;
+    const double* __restrict__ src = (MODE == 2 && parity) ? tk.phi_tmp : tk.phi;
+    double* __restrict__ dst = (MODE == 2) ? (parity ? tk.phi : tk.phi_tmp) : nullptr;
+    const float* __restrict__ e32 = tk.e32;
+    const double* __restrict__ rhs = tk.rhs;
+    const double inv_h2 = tk.inv_h2;
+    const int i = tile.tx * MG_RX + (threadIdx.x & (MG_RX - 1));
+    const int jb = tile.ty * MG_RY + (threadIdx.x / MG_RX) * 4;
+    auto node = [&](int ii, int jj) -> double {
+        const size_t g = (size_t)jj * n + ii;
+        double v = src[g];
+        if (MODE == 2) v = v + (double)e32[g];
+        return v;
+    };
+    double acc = 0.0;
+    if (i < n) {
+        double col[6], lf[4], rt[4], rh[4];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const int j = jb - 1 + q;
+            col[q] = (j >= 0 && j < n) ? node(i, j) : 0.0;
         }
-        if (WRITE_R32) tk.r32[idx] = (float)r;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int j = jb + q;
+            const bool in = j > 0 && j < n - 1 && i > 0 && i < n - 1;
+            lf[q] = in ? node(i - 1, j) : 0.0;
+            rt[q] = in ? node(i + 1, j) : 0.0;
+            rh[q] = in ? rhs[(size_t)j * n + i] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int j = jb + q;
+            if (j >= n) break;
+            const size_t g = (size_t)j * n + i;
+            const double c = col[q + 1];
+            double r = 0.0;
+            if (!(i == 0 || i == n - 1 || j == 0 || j == n - 1)) {
+                const double lap = (col[q] + col[q + 2] + lf[q] + rt[q] - 4.0 * c) * inv_h2;
+                r = rh[q] - lap;
+            }
+            if (MODE == 2) dst[g] = c;
+            if (MODE >= 1) tk.r32[g] = (float)r;
+            acc += r * r;
+        }
     }
-    const double s = block_sum256(r * r, sh);
+    const double s = block_sum256(acc, sh);
     if (threadIdx.x == 0) tk.partial[tile.ty * tk.gx + tile.tx] = s;
 }
 
@@ -344,14 +482,16 @@ __global__ __launch_bounds__(256) void sumsq_stage(const MGFineTask* __restrict_
     if (threadIdx.x == 0) tk.partial[tile.tx] = s;
 }
 
-__global__ __launch_bounds__(256) void ir_correct_stage(const MGFineTask* __restrict__ tasks,
-                                                        const MGTile* __restrict__ tiles) {
+// End of a mixed-mode solve: after an odd number of executed cycles the
+// solution sits in phi_tmp; copy it back.
+__global__ __launch_bounds__(256) void ir_finish_stage(const MGFineTask* __restrict__ tasks,
+                                                       const MGTile* __restrict__ tiles) {
     const MGTile tile = tiles[blockIdx.x];
     const MGFineTask& tk = tasks[tile.task];
-    if (tk.norm->converged) return;
+    if (!(tk.norm->cycles & 1)) return;
     const size_t nn = (size_t)tk.n * tk.n;
     const size_t i = (size_t)tile.tx * 256 + threadIdx.x;
-    if (i < nn) tk.phi[i] = tk.phi[i] + (double)tk.e32[i];
+    if (i < nn) tk.phi[i] = tk.phi_tmp[i];
 }
 
 // One workgroup per case: fixed-order sum of the block partials.  which = 0:
@@ -466,9 +606,8 @@ int MGSolver::init() {
     }
     const size_t nn = (size_t)n_ * n_;
     if (mixed) PIC_HIP(hipMalloc(&r32_, nn * sizeof(float)));
-    else PIC_HIP(hipMalloc(&phi_tmp_, nn * sizeof(double)));
-    const int gb = div_up(n_, 16);
-    n_partials_ = std::max(gb * gb, (int)((nn + 255) / 256));
+    PIC_HIP(hipMalloc(&phi_tmp_, nn * sizeof(double)));
+    n_partials_ = std::max(div_up(n_, MG_RX) * div_up(n_, MG_RY), (int)((nn + 255) / 256));
     PIC_HIP(hipMalloc(&partials_, (size_t)n_partials_ * sizeof(double)));
     PIC_HIP(hipMalloc(&d_norm_, sizeof(MGNorm)));
     PIC_HIP(hipMemset(d_norm_, 0, sizeof(MGNorm)));
@@ -513,9 +652,15 @@ static size_t coarse_lds_bytes(int n_top, int Lc) {
     return tot * sizeof(T);
 }
 
-static size_t smooth_lds(int H, size_t es) {
+// LDS of a tiled stage: e and rhs over tile + halo, plus the residual patch
+// (smooth with restriction) or the coarse patch (prolongation)
+static size_t tiled_lds(int kind, int nu, bool restr, size_t es) {
+    const size_t H = (kind == 0 && restr) ? 2 * nu + 2 : 2 * nu;
     const size_t W = MG_TILE + 2 * H;
-    return (2 * W * W + (MG_TILE + 2) * (MG_TILE + 2)) * es;
+    size_t tot = 2 * W * W;
+    if (kind == 0 && restr) tot += (MG_TILE + 2) * (MG_TILE + 2);
+    if (kind == 1) tot += (W / 2 + 2) * (W / 2 + 2);
+    return tot * es;
 }
 
 MGBatch::~MGBatch() { free_all(); }
@@ -546,7 +691,7 @@ static int upload_vec(void** dst, const std::vector<T>& v) {
 }
 
 template <typename T>
-int MGBatch::add_level_stage(int kind, const std::vector<MGLevelTask<T>>& tasks, size_t lds) {
+int MGBatch::add_level_stage(int kind, const std::vector<MGLevelTask<T>>& tasks, int nu, bool restr) {
     if (tasks.empty()) return 0;
     std::vector<MGTile> tiles;
     for (int k = 0; k < (int)tasks.size(); ++k) {
@@ -562,8 +707,10 @@ int MGBatch::add_level_stage(int kind, const std::vector<MGLevelTask<T>>& tasks,
     }
     Stage st;
     st.kind = kind;
+    st.nu = nu;
+    st.restr = restr;
     st.fp64 = sizeof(T) == 8;
-    st.lds = lds;
+    st.lds = kind == 3 ? 0 : tiled_lds(kind, nu, restr, sizeof(T));
     st.n_items = (int)tiles.size();
     if (int rc = upload_vec(&st.tasks, tasks)) return rc;
     if (int rc = upload_vec((void**)&st.tiles, tiles)) return rc;
@@ -606,7 +753,7 @@ int MGBatch::build_cycle(const std::vector<double>& h) {
             t.gate = s->d_norm_;
             tasks.push_back(t);
         }
-        if (int rc = add_level_stage<T>(0, tasks, smooth_lds(2 * p_.pre + 2, sizeof(T)))) return rc;
+        if (int rc = add_level_stage<T>(0, tasks, p_.pre, true)) return rc;
     }
     // coarse part: the single-workgroup cycle, or tiled sweeps on a large coarsest level
     {
@@ -680,7 +827,7 @@ int MGBatch::build_cycle(const std::vector<double>& h) {
                     src[k] = t.e_out;
                     which[k] ^= 1;
                 }
-                if (int rc = add_level_stage<T>(0, tasks, smooth_lds(2 * nu, sizeof(T)))) return rc;
+                if (int rc = add_level_stage<T>(0, tasks, nu, false)) return rc;
                 left -= nu;
                 wrote = true;
             }
@@ -694,7 +841,7 @@ int MGBatch::build_cycle(const std::vector<double>& h) {
                 t.gate = m_[k]->d_norm_;
                 copies.push_back(t);
             }
-            if (int rc = add_level_stage<T>(3, copies, 0)) return rc;
+            if (int rc = add_level_stage<T>(3, copies, 0, false)) return rc;
         }
     }
     // up leg
@@ -718,8 +865,7 @@ int MGBatch::build_cycle(const std::vector<double>& h) {
             t.gate = s->d_norm_;
             tasks.push_back(t);
         }
-        const size_t W = MG_TILE + 2 * (2 * p_.post);
-        if (int rc = add_level_stage<T>(1, tasks, 2 * W * W * sizeof(T))) return rc;
+        if (int rc = add_level_stage<T>(1, tasks, p_.post, false)) return rc;
     }
     return 0;
 }
@@ -754,6 +900,7 @@ int MGBatch::build(const std::vector<MGSolver*>& members, const std::vector<doub
         const size_t nn = (size_t)n * n;
         MGFineTask& f = fine[k];
         f.phi = phi[k];
+        f.phi_tmp = s->phi_tmp_;
         f.rhs = rhs[k];
         f.r32 = s->r32_;
         f.e32 = mixed_ ? s->ef_[0][0] : nullptr;
@@ -761,12 +908,13 @@ int MGBatch::build(const std::vector<MGSolver*>& members, const std::vector<doub
         f.norm = s->d_norm_;
         f.mirror = d_mirror_ + k;
         f.n = n;
-        f.gx = div_up(n, 16);
-        f.nblocks = f.gx * f.gx;
+        f.gx = div_up(n, MG_RX);
+        const int gy = div_up(n, MG_RY);
+        f.nblocks = f.gx * gy;
         f.nchunks = (int)((nn + 255) / 256);
         f.inv_h2 = 1.0 / (h[k] * h[k]);
         for (int c = 0; c < f.nchunks; ++c) chunks.push_back(MGTile{k, c, 0, 0});
-        for (int ty = 0; ty < f.gx; ++ty)
+        for (int ty = 0; ty < gy; ++ty)
             for (int tx = 0; tx < f.gx; ++tx) res.push_back(MGTile{k, tx, ty, 0});
     }
     if (int rc = upload_vec((void**)&d_fine_, fine)) return rc;
@@ -777,45 +925,71 @@ int MGBatch::build(const std::vector<MGSolver*>& members, const std::vector<doub
     return mixed_ ? build_cycle<float>(h) : build_cycle<double>(h);
 }
 
+template <typename T, int NU>
+static void launch_tiled_nu(int kind, bool restr, dim3 g, size_t lds, hipStream_t s, const MGLevelTask<T>* t,
+                            const MGTile* tiles) {
+    if (kind == 1) hipLaunchKernelGGL((mg_prolong_stage<T, NU>), g, dim3(256), lds, s, t, tiles);
+    else if (restr) hipLaunchKernelGGL((mg_smooth_stage<T, NU, true>), g, dim3(256), lds, s, t, tiles);
+    else hipLaunchKernelGGL((mg_smooth_stage<T, NU, false>), g, dim3(256), lds, s, t, tiles);
+}
+
+template <typename T>
+static void launch_tiled(const void* tasks, int kind, int nu, bool restr, dim3 g, size_t lds, hipStream_t s,
+                         const MGTile* tiles) {
+    auto* t = (const MGLevelTask<T>*)tasks;
+    switch (nu) {
+    case 0: launch_tiled_nu<T, 0>(kind, restr, g, lds, s, t, tiles); break;
+    case 1: launch_tiled_nu<T, 1>(kind, restr, g, lds, s, t, tiles); break;
+    case 2: launch_tiled_nu<T, 2>(kind, restr, g, lds, s, t, tiles); break;
+    default: launch_tiled_nu<T, 3>(kind, restr, g, lds, s, t, tiles); break;
+    }
+}
+
 int MGBatch::launch_stage(const Stage& st) {
-    const dim3 g(st.n_items), b(st.kind == 2 ? 1024 : 256);
-    if (st.fp64) {
-        auto* t = (const MGLevelTask<double>*)st.tasks;
-        if (st.kind == 0) hipLaunchKernelGGL(mg_smooth_stage<double>, g, b, st.lds, stream_, t, st.tiles);
-        else if (st.kind == 1) hipLaunchKernelGGL(mg_prolong_stage<double>, g, b, st.lds, stream_, t, st.tiles);
-        else if (st.kind == 2)
-            hipLaunchKernelGGL(mg_coarse_stage<double>, g, b, st.lds, stream_, (const MGCoarseTask<double>*)st.tasks);
-        else hipLaunchKernelGGL(mg_copy_stage<double>, g, b, 0, stream_, t, st.tiles);
+    const dim3 g(st.n_items);
+    if (st.kind == 0 || st.kind == 1) {
+        if (st.fp64) launch_tiled<double>(st.tasks, st.kind, st.nu, st.restr, g, st.lds, stream_, st.tiles);
+        else launch_tiled<float>(st.tasks, st.kind, st.nu, st.restr, g, st.lds, stream_, st.tiles);
+    } else if (st.kind == 2) {
+        if (st.fp64)
+            hipLaunchKernelGGL(mg_coarse_stage<double>, g, dim3(1024), st.lds, stream_,
+                               (const MGCoarseTask<double>*)st.tasks);
+        else
+            hipLaunchKernelGGL(mg_coarse_stage<float>, g, dim3(1024), st.lds, stream_,
+                               (const MGCoarseTask<float>*)st.tasks);
     } else {
-        auto* t = (const MGLevelTask<float>*)st.tasks;
-        if (st.kind == 0) hipLaunchKernelGGL(mg_smooth_stage<float>, g, b, st.lds, stream_, t, st.tiles);
-        else if (st.kind == 1) hipLaunchKernelGGL(mg_prolong_stage<float>, g, b, st.lds, stream_, t, st.tiles);
-        else if (st.kind == 2)
-            hipLaunchKernelGGL(mg_coarse_stage<float>, g, b, st.lds, stream_, (const MGCoarseTask<float>*)st.tasks);
-        else hipLaunchKernelGGL(mg_copy_stage<float>, g, b, 0, stream_, t, st.tiles);
+        if (st.fp64)
+            hipLaunchKernelGGL(mg_copy_stage<double>, g, dim3(256), 0, stream_, (const MGLevelTask<double>*)st.tasks,
+                               st.tiles);
+        else
+            hipLaunchKernelGGL(mg_copy_stage<float>, g, dim3(256), 0, stream_, (const MGLevelTask<float>*)st.tasks,
+                               st.tiles);
     }
     ++launches_;
     return 0;
 }
 
-// what: 0 rhs norm, 1 residual (which 1 or 2), 2 correction
-int MGBatch::enqueue_fine(int what, int which) {
+// what: 0 rhs norm; 1 residual and its norm (which 1: before the first cycle,
+// 2: after a cycle, whose index has the given parity); 3 mixed mode: the
+// solution back into phi after an odd number of cycles
+int MGBatch::enqueue_fine(int what, int which, int parity) {
     const int M = (int)m_.size();
     if (what == 0) {
         hipLaunchKernelGGL(sumsq_stage, dim3(n_chunk_), dim3(256), 0, stream_, d_fine_, d_chunk_tiles_);
         hipLaunchKernelGGL(norm_final_stage, dim3(M), dim3(256), 0, stream_, d_fine_, 0, p_.tol, p_.max_cycles);
         launches_ += 2;
     } else if (what == 1) {
-        if (mixed_)
-            hipLaunchKernelGGL(ir_residual_stage<true>, dim3(n_res_), dim3(256), 0, stream_, d_fine_, d_res_tiles_,
-                               which);
+        const dim3 g(n_res_), b(256);
+        if (!mixed_)
+            hipLaunchKernelGGL(ir_residual_stage<0>, g, b, 0, stream_, d_fine_, d_res_tiles_, which, 0);
+        else if (which == 1)
+            hipLaunchKernelGGL(ir_residual_stage<1>, g, b, 0, stream_, d_fine_, d_res_tiles_, which, 0);
         else
-            hipLaunchKernelGGL(ir_residual_stage<false>, dim3(n_res_), dim3(256), 0, stream_, d_fine_, d_res_tiles_,
-                               which);
+            hipLaunchKernelGGL(ir_residual_stage<2>, g, b, 0, stream_, d_fine_, d_res_tiles_, which, parity);
         hipLaunchKernelGGL(norm_final_stage, dim3(M), dim3(256), 0, stream_, d_fine_, which, p_.tol, p_.max_cycles);
         launches_ += 2;
     } else {
-        hipLaunchKernelGGL(ir_correct_stage, dim3(n_chunk_), dim3(256), 0, stream_, d_fine_, d_chunk_tiles_);
+        hipLaunchKernelGGL(ir_finish_stage, dim3(n_chunk_), dim3(256), 0, stream_, d_fine_, d_chunk_tiles_);
         ++launches_;
     }
     return 0;
@@ -824,13 +998,15 @@ int MGBatch::enqueue_fine(int what, int which) {
 // The host enqueues cycles in batches (first batch: the previous solve's
 // largest cycle count) and reads the norms of all members once per batch.  The
 // device flags turn cycles beyond a member's convergence into no-ops, so each
-// result equals testing after every cycle, alone or in a batch.
+// result equals testing after every cycle, alone or in a batch.  The executed
+// cycles of a member are a prefix of the enqueued ones, so in mixed mode the
+// buffer holding its solution follows from its own cycle count.
 int MGBatch::solve() {
     PIC_REQUIRE(!m_.empty(), "multigrid batch: not built");
     launches_ = 0;
     const int M = (int)m_.size();
-    enqueue_fine(0, 0);
-    enqueue_fine(1, 1);
+    enqueue_fine(0, 0, 0);
+    enqueue_fine(1, 1, 0);
     int enqueued = 0;
     int batch = std::max(1, last_cycles_);
     for (;;) {
@@ -838,8 +1014,7 @@ int MGBatch::solve() {
         for (int b = 0; b < batch; ++b) {
             for (const auto& st : cycle_)
                 if (int rc = launch_stage(st)) return rc;
-            if (mixed_) enqueue_fine(2, 0);
-            enqueue_fine(1, 2);
+            enqueue_fine(1, 2, (enqueued + b) & 1);
         }
         enqueued += std::max(batch, 0);
         PIC_HIP(hipMemcpyAsync(h_mirror_, d_mirror_, M * sizeof(MGNorm), hipMemcpyDeviceToHost, stream_));
@@ -849,6 +1024,7 @@ int MGBatch::solve() {
         if (all || enqueued >= p_.max_cycles) break;
         batch = 1;
     }
+    if (mixed_) enqueue_fine(3, 0, 0);
     PIC_HIP(hipGetLastError());
     int mx = 0;
     for (int k = 0; k < M; ++k) mx = std::max(mx, h_mirror_[k].cycles);

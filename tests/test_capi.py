@@ -68,6 +68,9 @@ def test_mg_presets(pic):
         (3, 3, 20, 1.25, 1e-5, 50, pic.MG_FP64)
     st = pic.mg_params(pic.PRESET_STEP)
     assert (st.pre, st.post, st.omega, st.tol, st.mode) == (2, 2, 1.0, 1e-8, pic.MG_MIXED)
+    sor = pic.mg_params(pic.PRESET_STEP_SOR)
+    assert (sor.pre, sor.post, sor.omega, sor.tol, sor.mode) == (2, 2, 1.2, 1e-8, pic.MG_MIXED)
+    assert pic.mg_params(pic.PRESET_POISSON).omega == 1.2
     with pytest.raises(pic.PicError):
         pic.mg_params(7)
 
@@ -75,6 +78,7 @@ def test_mg_presets(pic):
 def test_sim_default_params(pic):
     p = pic.sim_params()
     assert p.bin_w == 16 and p.halo == 2 and p.eps0 == 8.8541878128e-12
+    assert p.mg.omega == 1.2 and p.mg.tol == 1e-8
 
 
 def test_last_error_reports_message(pic):

@@ -144,8 +144,10 @@ def _oracle_from(pic, orc, case, prec, seed=7):
     w = pl.density * pl.length * pl.length / len(sp[0].x)   # same expression as configs.build
     qs = 1.0 / float(1 << 40)
     cfac = [-(q * w * qs / (grid.dx * grid.dy)) / pic.EPS0 for q in (-pic.ELEMENTARY_CHARGE, pic.ELEMENTARY_CHARGE)]
-    return orc.OracleSim(sp, grid.m, grid.n, grid.dx, grid.dy, pl.dt, cfac, dict(tol=1e-8, max_cycles=100),
-                         seed=seed)
+    mg = case["mg"]
+    return orc.OracleSim(sp, grid.m, grid.n, grid.dx, grid.dy, pl.dt, cfac,
+                         dict(tol=mg.tol, max_cycles=mg.max_cycles, pre=mg.pre, post=mg.post,
+                              coarse_sweeps=mg.coarse_sweeps, omega=mg.omega), seed=seed)
 
 
 @pytest.mark.parametrize("prec,mcc", [(4, True), (8, True), (4, False)])
