@@ -164,6 +164,9 @@ private:
         MGTile* tiles = nullptr;
         int n_items = 0;   // launch blocks
         size_t lds = 0;
+        int n_tasks = 0;
+        int gx = 0;        // single task: tiles per row
+        alignas(16) unsigned char t0[128];   // single task: passed by value
     };
     template <typename T>
     int build_cycle(const std::vector<double>& h);
@@ -181,6 +184,7 @@ private:
     bool mixed_ = false;
     std::vector<Stage> cycle_;          // stages of one V-cycle
     MGFineTask* d_fine_ = nullptr;      // per-case refinement tasks
+    std::vector<MGFineTask> h_fine_;
     MGTile* d_chunk_tiles_ = nullptr;   // (case, chunk of 256 nodes)
     MGTile* d_res_tiles_ = nullptr;     // (case, 64x16 residual tile)
     int n_chunk_ = 0, n_res_ = 0;

@@ -49,6 +49,46 @@ __device__ inline T tfma(T a, T b, T c);
 template <> __device__ inline float tfma(float a, float b, float c) { return fmaf(a, b, c); }
 template <> __device__ inline double tfma(double a, double b, double c) { return fma(a, b, c); }
 
+// Work-item addressing of a stage.  A single-grid stage carries its task by
+// value and derives its tile from blockIdx, so no dependent load precedes the
+// data; a batched stage reads its (case, tile) entry and then the case's task.
+template <typename Task>
+struct StageRef {
+    Task t0;
+    const Task* tasks;     // null: the single task t0
+    const MGTile* tiles;   // batched tiled stages
+    int gx;                // single task: tiles per row (0: one tile per block, tile.tx = blockIdx.x)
+};
+
+template <typename Task>
+__device__ inline void stage_item(const StageRef<Task>& a, Task& tk, MGTile& tile) {
+    if (a.tasks) {
+        tile = a.tiles[blockIdx.x];
+        tk = a.tasks[tile.task];
+    } else {
+        tk = a.t0;
+        tile.task = 0;
+        tile.tx = a.gx ? (int)blockIdx.x % a.gx : (int)blockIdx.x;
+        tile.ty = a.gx ? (int)blockIdx.x / a.gx : 0;
+    }
+}
+
+// Stop flag of a task without one: always zero.  Reading the flag through an
+// unconditional load lets its latency overlap the tile's data loads.
+__device__ const int g_never_stop = 0;
+
+template <typename T>
+__device__ inline const int* stop_flag(const MGLevelTask<T>& tk) {
+    return tk.gate ? &tk.gate->converged : &g_never_stop;
+}
+
+// one task per block (coarse cycle, norms)
+template <typename Task>
+__device__ inline void stage_case(const StageRef<Task>& a, Task& tk) {
+    if (a.tasks) tk = a.tasks[blockIdx.x];
+    else tk = a.t0;
+}
+
 // the SOR update of one node (the reference's expression)
 template <typename T>
 __device__ inline T sor_node(const T* e, const T* r, int idx, int stride, T h2, T omega) {
@@ -85,19 +125,21 @@ __device__ inline void tile_colour(T* e, const T* r, int gi0, int gj0, int n, T 
 // (2ic, 2jc) inside the tile.  Without RESTRICT (tiled sweeps of a large
 // coarsest level) the halo is 2*NU.
 template <typename T, int NU, bool RESTRICT>
-__global__ __launch_bounds__(256) void mg_smooth_stage(const MGLevelTask<T>* __restrict__ tasks,
-                                                       const MGTile* __restrict__ tiles) {
+__global__ __launch_bounds__(256) void mg_smooth_stage(const StageRef<MGLevelTask<T>> a) {
     constexpr int H = RESTRICT ? 2 * NU + 2 : 2 * NU;
     constexpr int W = MG_TILE + 2 * H;
     constexpr int WW = W * W;
     constexpr int NL = (WW + 255) / 256;
-    const MGTile tile = tiles[blockIdx.x];
-    const MGLevelTask<T>& tk = tasks[tile.task];
-    if (tk.gate && tk.gate->converged) return;   // cycle enqueued after convergence: no-op
-    const T* __restrict__ e_in = tk.e_in;
-    const T* __restrict__ rhs = tk.rhs;
-    T* __restrict__ e_out = tk.e_out;
-    T* __restrict__ rhs_c = tk.rhs_c;
+    MGTile tile;
+    MGLevelTask<T> tk;
+    stage_item(a, tk, tile);
+    // stop flag (a cycle enqueued after convergence is a no-op): read here,
+    // tested once the tile's loads are issued
+    const int stop = *G(stop_flag(tk));
+    const gptr<const T> e_in = G(tk.e_in);
+    const gptr<const T> rhs = G(tk.rhs);
+    const gptr<T> e_out = G(tk.e_out);
+    const gptr<T> rhs_c = G(tk.rhs_c);
     const int n = tk.n;
     const T h2 = tk.h2, inv_h2 = tk.inv_h2, omega = tk.omega;
     extern __shared__ unsigned char smem_mg[];
@@ -120,6 +162,7 @@ __global__ __launch_bounds__(256) void mg_smooth_stage(const MGLevelTask<T>* __r
             rv[q] = rhs[g];
         }
     }
+    if (stop) return;
 #pragma unroll
     for (int q = 0; q < NL; ++q) {
         const int k = tid + 256 * q;
@@ -195,8 +238,7 @@ __device__ inline T prolong_lds(const T* C, int CW, int ic0, int jc0, int nc, in
 
 // Up-leg stage: e = e_in + P(coarse) on tile + halo 2*NU, NU sweeps, store the tile.
 template <typename T, int NU>
-__global__ __launch_bounds__(256) void mg_prolong_stage(const MGLevelTask<T>* __restrict__ tasks,
-                                                        const MGTile* __restrict__ tiles) {
+__global__ __launch_bounds__(256) void mg_prolong_stage(const StageRef<MGLevelTask<T>> a) {
     constexpr int H = 2 * NU;
     constexpr int W = MG_TILE + 2 * H;
     constexpr int WW = W * W;
@@ -204,13 +246,14 @@ __global__ __launch_bounds__(256) void mg_prolong_stage(const MGLevelTask<T>* __
     constexpr int CW = W / 2 + 2;
     constexpr int CC = CW * CW;
     constexpr int NCL = (CC + 255) / 256;
-    const MGTile tile = tiles[blockIdx.x];
-    const MGLevelTask<T>& tk = tasks[tile.task];
-    if (tk.gate && tk.gate->converged) return;
-    const T* __restrict__ e_in = tk.e_in;
-    const T* __restrict__ coarse = tk.coarse;
-    const T* __restrict__ rhs = tk.rhs;
-    T* __restrict__ e_out = tk.e_out;
+    MGTile tile;
+    MGLevelTask<T> tk;
+    stage_item(a, tk, tile);
+    const int stop = *G(stop_flag(tk));   // tested once the loads are issued
+    const gptr<const T> e_in = G(tk.e_in);
+    const gptr<const T> coarse = G(tk.coarse);
+    const gptr<const T> rhs = G(tk.rhs);
+    const gptr<T> e_out = G(tk.e_out);
     const int n = tk.n;
     const T h2 = tk.h2, omega = tk.omega;
     extern __shared__ unsigned char smem_mg[];
@@ -244,6 +287,7 @@ __global__ __launch_bounds__(256) void mg_prolong_stage(const MGLevelTask<T>* __
             rv[q] = rhs[g];
         }
     }
+    if (stop) return;
 #pragma unroll
     for (int q = 0; q < NCL; ++q) {
         const int k = tid + 256 * q;
@@ -298,12 +342,13 @@ __device__ inline void lds_sweeps(T* e, const T* r, int n, T h, T omega, int ite
 }
 
 template <typename T>
-__global__ __launch_bounds__(1024) void mg_coarse_stage(const MGCoarseTask<T>* __restrict__ tasks) {
-    const MGCoarseTask<T>& tk = tasks[blockIdx.x];
-    if (tk.gate && tk.gate->converged) return;
-    const T* __restrict__ e_in = tk.e_in;
-    const T* __restrict__ rhs_top = tk.rhs;
-    T* __restrict__ e_out = tk.e_out;
+__global__ __launch_bounds__(1024) void mg_coarse_stage(const StageRef<MGCoarseTask<T>> a) {
+    MGCoarseTask<T> tk;
+    stage_case(a, tk);
+    if (tk.gate && *G(&tk.gate->converged)) return;
+    const gptr<const T> e_in = G(tk.e_in);
+    const gptr<const T> rhs_top = G(tk.rhs);
+    const gptr<T> e_out = G(tk.e_out);
     const int n_top = tk.n_top, Lc = tk.Lc, pre = tk.pre, post = tk.post, csweeps = tk.csweeps;
     const T h_top = tk.h_top, omega = tk.omega;
     extern __shared__ unsigned char smem_mg[];
@@ -414,17 +459,18 @@ __device__ inline double block_sum256(double v, double* sh) {
 //           phi -> phi_tmp, 1: phi_tmp -> phi); r stored FP32.
 // which == 2: a residual at the end of a cycle, a no-op once the case stopped.
 template <int MODE>
-__global__ __launch_bounds__(256) void ir_residual_stage(const MGFineTask* __restrict__ tasks,
-                                                         const MGTile* __restrict__ tiles, int which, int parity) {
-    const MGTile tile = tiles[blockIdx.x];
-    const MGFineTask& tk = tasks[tile.task];
-    if (which == 2 && tk.norm->converged) return;
+__global__ __launch_bounds__(256) void ir_residual_stage(const StageRef<MGFineTask> a, int which, int parity) {
+    MGTile tile;
+    MGFineTask tk;
+    stage_item(a, tk, tile);
+    if (which == 2 && *G(&tk.norm->converged)) return;
     __shared__ double sh[256];
     const int n = tk.n;
-    const double* __restrict__ src = (MODE == 2 && parity) ? tk.phi_tmp : tk.phi;
-    double* __restrict__ dst = (MODE == 2) ? (parity ? tk.phi : tk.phi_tmp) : nullptr;
-    const float* __restrict__ e32 = tk.e32;
-    const double* __restrict__ rhs = tk.rhs;
+    const gptr<const double> src = G((MODE == 2 && parity) ? tk.phi_tmp : tk.phi);
+    const gptr<double> dst = G((MODE == 2) ? (parity ? tk.phi : tk.phi_tmp) : nullptr);
+    const gptr<const float> e32 = G(tk.e32);
+    const gptr<const double> rhs = G(tk.rhs);
+    const gptr<float> r32 = G(tk.r32);
     const double inv_h2 = tk.inv_h2;
     const int i = tile.tx * MG_RX + (threadIdx.x & (MG_RX - 1));
     const int jb = tile.ty * MG_RY + (threadIdx.x / MG_RX) * 4;
@@ -462,36 +508,36 @@ __global__ __launch_bounds__(256) void ir_residual_stage(const MGFineTask* __res
                 r = rh[q] - lap;
             }
             if (MODE == 2) dst[g] = c;
-            if (MODE >= 1) tk.r32[g] = (float)r;
+            if (MODE >= 1) r32[g] = (float)r;
             acc += r * r;
         }
     }
     const double s = block_sum256(acc, sh);
-    if (threadIdx.x == 0) tk.partial[tile.ty * tk.gx + tile.tx] = s;
+    if (threadIdx.x == 0) G(tk.partial)[tile.ty * tk.gx + tile.tx] = s;
 }
 
-__global__ __launch_bounds__(256) void sumsq_stage(const MGFineTask* __restrict__ tasks,
-                                                   const MGTile* __restrict__ tiles) {
-    const MGTile tile = tiles[blockIdx.x];
-    const MGFineTask& tk = tasks[tile.task];
+__global__ __launch_bounds__(256) void sumsq_stage(const StageRef<MGFineTask> a) {
+    MGTile tile;
+    MGFineTask tk;
+    stage_item(a, tk, tile);
     __shared__ double sh[256];
     const size_t nn = (size_t)tk.n * tk.n;
     const size_t i = (size_t)tile.tx * 256 + threadIdx.x;
-    const double x = (i < nn) ? tk.rhs[i] : 0.0;
+    const double x = (i < nn) ? G(tk.rhs)[i] : 0.0;
     const double s = block_sum256(x * x, sh);
-    if (threadIdx.x == 0) tk.partial[tile.tx] = s;
+    if (threadIdx.x == 0) G(tk.partial)[tile.tx] = s;
 }
 
 // End of a mixed-mode solve: after an odd number of executed cycles the
 // solution sits in phi_tmp; copy it back.
-__global__ __launch_bounds__(256) void ir_finish_stage(const MGFineTask* __restrict__ tasks,
-                                                       const MGTile* __restrict__ tiles) {
-    const MGTile tile = tiles[blockIdx.x];
-    const MGFineTask& tk = tasks[tile.task];
-    if (!(tk.norm->cycles & 1)) return;
+__global__ __launch_bounds__(256) void ir_finish_stage(const StageRef<MGFineTask> a) {
+    MGTile tile;
+    MGFineTask tk;
+    stage_item(a, tk, tile);
+    if (!(*G(&tk.norm->cycles) & 1)) return;
     const size_t nn = (size_t)tk.n * tk.n;
     const size_t i = (size_t)tile.tx * 256 + threadIdx.x;
-    if (i < nn) tk.phi[i] = tk.phi_tmp[i];
+    if (i < nn) G(tk.phi)[i] = G(tk.phi_tmp)[i];
 }
 
 // One workgroup per case: fixed-order sum of the block partials.  which = 0:
@@ -500,15 +546,16 @@ __global__ __launch_bounds__(256) void ir_finish_stage(const MGFineTask* __restr
 // once the flag is set, so cycles enqueued after convergence change nothing.
 // Relative residual ||r||/||rhs|| (the reference divides by sum rhs^2,
 // Multigrid.cu:466).
-__global__ __launch_bounds__(256) void norm_final_stage(const MGFineTask* __restrict__ tasks, int which, double tol,
+__global__ __launch_bounds__(256) void norm_final_stage(const StageRef<MGFineTask> a, int which, double tol,
                                                         int max_cycles) {
-    const MGFineTask& tk = tasks[blockIdx.x];
+    MGFineTask tk;
+    stage_case(a, tk);
     MGNorm* out = tk.norm;
     if (which == 2 && out->converged) return;
     __shared__ double sh[256];
     const int np = which == 0 ? tk.nchunks : tk.nblocks;
     double acc = 0.0;
-    for (int k = threadIdx.x; k < np; k += 256) acc += tk.partial[k];
+    for (int k = threadIdx.x; k < np; k += 256) acc += G(tk.partial)[k];
     const double tot = block_sum256(acc, sh);
     if (threadIdx.x == 0) {
         if (which == 0) {
@@ -530,14 +577,14 @@ __global__ __launch_bounds__(256) void norm_final_stage(const MGFineTask* __rest
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void mg_copy_stage(const MGLevelTask<T>* __restrict__ tasks,
-                                                     const MGTile* __restrict__ tiles) {
-    const MGTile tile = tiles[blockIdx.x];
-    const MGLevelTask<T>& tk = tasks[tile.task];
+__global__ __launch_bounds__(256) void mg_copy_stage(const StageRef<MGLevelTask<T>> a) {
+    MGTile tile;
+    MGLevelTask<T> tk;
+    stage_item(a, tk, tile);
     if (tk.gate && tk.gate->converged) return;
     const size_t nn = (size_t)tk.n * tk.n;
     const size_t i = (size_t)tile.tx * 256 + threadIdx.x;
-    if (i < nn) tk.e_out[i] = tk.e_in[i];
+    if (i < nn) G(tk.e_out)[i] = G(tk.e_in)[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -712,6 +759,10 @@ int MGBatch::add_level_stage(int kind, const std::vector<MGLevelTask<T>>& tasks,
     st.fp64 = sizeof(T) == 8;
     st.lds = kind == 3 ? 0 : tiled_lds(kind, nu, restr, sizeof(T));
     st.n_items = (int)tiles.size();
+    st.n_tasks = (int)tasks.size();
+    st.gx = kind == 3 ? 0 : div_up(tasks[0].n, MG_TILE);
+    static_assert(sizeof(MGLevelTask<T>) <= sizeof(st.t0), "task does not fit the by-value slot");
+    std::memcpy(st.t0, &tasks[0], sizeof(MGLevelTask<T>));
     if (int rc = upload_vec(&st.tasks, tasks)) return rc;
     if (int rc = upload_vec((void**)&st.tiles, tiles)) return rc;
     cycle_.push_back(st);
@@ -784,6 +835,9 @@ int MGBatch::build_cycle(const std::vector<double>& h) {
             st.fp64 = sizeof(T) == 8;
             st.lds = lds;
             st.n_items = (int)ct.size();
+            st.n_tasks = (int)ct.size();
+            static_assert(sizeof(MGCoarseTask<T>) <= sizeof(st.t0), "task does not fit the by-value slot");
+            std::memcpy(st.t0, &ct[0], sizeof(MGCoarseTask<T>));
             if (int rc = upload_vec(&st.tasks, ct)) return rc;
             cycle_.push_back(st);
         }
@@ -918,6 +972,7 @@ int MGBatch::build(const std::vector<MGSolver*>& members, const std::vector<doub
             for (int tx = 0; tx < f.gx; ++tx) res.push_back(MGTile{k, tx, ty, 0});
     }
     if (int rc = upload_vec((void**)&d_fine_, fine)) return rc;
+    h_fine_ = fine;
     if (int rc = upload_vec((void**)&d_chunk_tiles_, chunks)) return rc;
     if (int rc = upload_vec((void**)&d_res_tiles_, res)) return rc;
     n_chunk_ = (int)chunks.size();
@@ -925,46 +980,66 @@ int MGBatch::build(const std::vector<MGSolver*>& members, const std::vector<doub
     return mixed_ ? build_cycle<float>(h) : build_cycle<double>(h);
 }
 
+struct MGBatchStageView {
+    int kind, nu;
+    bool restr;
+    const void* tasks;
+    const MGTile* tiles;
+    int n_items;
+    size_t lds;
+    int n_tasks, gx;
+    const unsigned char* t0;
+};
+
+template <typename Task>
+static StageRef<Task> make_ref(const void* t0, const void* tasks, const MGTile* tiles, int n_tasks, int gx) {
+    StageRef<Task> r;
+    std::memcpy(&r.t0, t0, sizeof(Task));
+    const bool single = n_tasks == 1;
+    r.tasks = single ? nullptr : (const Task*)tasks;
+    r.tiles = tiles;
+    r.gx = gx;
+    return r;
+}
+
 template <typename T, int NU>
-static void launch_tiled_nu(int kind, bool restr, dim3 g, size_t lds, hipStream_t s, const MGLevelTask<T>* t,
-                            const MGTile* tiles) {
-    if (kind == 1) hipLaunchKernelGGL((mg_prolong_stage<T, NU>), g, dim3(256), lds, s, t, tiles);
-    else if (restr) hipLaunchKernelGGL((mg_smooth_stage<T, NU, true>), g, dim3(256), lds, s, t, tiles);
-    else hipLaunchKernelGGL((mg_smooth_stage<T, NU, false>), g, dim3(256), lds, s, t, tiles);
+static void launch_tiled_nu(int kind, bool restr, dim3 g, size_t lds, hipStream_t s, const StageRef<MGLevelTask<T>>& r) {
+    if (kind == 1) hipLaunchKernelGGL((mg_prolong_stage<T, NU>), g, dim3(256), lds, s, r);
+    else if (restr) hipLaunchKernelGGL((mg_smooth_stage<T, NU, true>), g, dim3(256), lds, s, r);
+    else hipLaunchKernelGGL((mg_smooth_stage<T, NU, false>), g, dim3(256), lds, s, r);
 }
 
 template <typename T>
-static void launch_tiled(const void* tasks, int kind, int nu, bool restr, dim3 g, size_t lds, hipStream_t s,
-                         const MGTile* tiles) {
-    auto* t = (const MGLevelTask<T>*)tasks;
+static void launch_tiled(int kind, int nu, bool restr, dim3 g, size_t lds, hipStream_t s,
+                         const StageRef<MGLevelTask<T>>& r) {
     switch (nu) {
-    case 0: launch_tiled_nu<T, 0>(kind, restr, g, lds, s, t, tiles); break;
-    case 1: launch_tiled_nu<T, 1>(kind, restr, g, lds, s, t, tiles); break;
-    case 2: launch_tiled_nu<T, 2>(kind, restr, g, lds, s, t, tiles); break;
-    default: launch_tiled_nu<T, 3>(kind, restr, g, lds, s, t, tiles); break;
+    case 0: launch_tiled_nu<T, 0>(kind, restr, g, lds, s, r); break;
+    case 1: launch_tiled_nu<T, 1>(kind, restr, g, lds, s, r); break;
+    case 2: launch_tiled_nu<T, 2>(kind, restr, g, lds, s, r); break;
+    default: launch_tiled_nu<T, 3>(kind, restr, g, lds, s, r); break;
+    }
+}
+
+template <typename T>
+static void launch_stage_t(const MGBatchStageView& st, hipStream_t s) {
+    const dim3 g(st.n_items);
+    if (st.kind == 0 || st.kind == 1) {
+        const auto r = make_ref<MGLevelTask<T>>(st.t0, st.tasks, st.tiles, st.n_tasks, st.gx);
+        launch_tiled<T>(st.kind, st.nu, st.restr, g, st.lds, s, r);
+    } else if (st.kind == 2) {
+        const auto r = make_ref<MGCoarseTask<T>>(st.t0, st.tasks, nullptr, st.n_tasks, 0);
+        hipLaunchKernelGGL(mg_coarse_stage<T>, g, dim3(1024), st.lds, s, r);
+    } else {
+        const auto r = make_ref<MGLevelTask<T>>(st.t0, st.tasks, st.tiles, st.n_tasks, 0);
+        hipLaunchKernelGGL(mg_copy_stage<T>, g, dim3(256), 0, s, r);
     }
 }
 
 int MGBatch::launch_stage(const Stage& st) {
-    const dim3 g(st.n_items);
-    if (st.kind == 0 || st.kind == 1) {
-        if (st.fp64) launch_tiled<double>(st.tasks, st.kind, st.nu, st.restr, g, st.lds, stream_, st.tiles);
-        else launch_tiled<float>(st.tasks, st.kind, st.nu, st.restr, g, st.lds, stream_, st.tiles);
-    } else if (st.kind == 2) {
-        if (st.fp64)
-            hipLaunchKernelGGL(mg_coarse_stage<double>, g, dim3(1024), st.lds, stream_,
-                               (const MGCoarseTask<double>*)st.tasks);
-        else
-            hipLaunchKernelGGL(mg_coarse_stage<float>, g, dim3(1024), st.lds, stream_,
-                               (const MGCoarseTask<float>*)st.tasks);
-    } else {
-        if (st.fp64)
-            hipLaunchKernelGGL(mg_copy_stage<double>, g, dim3(256), 0, stream_, (const MGLevelTask<double>*)st.tasks,
-                               st.tiles);
-        else
-            hipLaunchKernelGGL(mg_copy_stage<float>, g, dim3(256), 0, stream_, (const MGLevelTask<float>*)st.tasks,
-                               st.tiles);
-    }
+    const MGBatchStageView v{st.kind, st.nu, st.restr, st.tasks, st.tiles, st.n_items, st.lds, st.n_tasks, st.gx,
+                             st.t0};
+    if (st.fp64) launch_stage_t<double>(v, stream_);
+    else launch_stage_t<float>(v, stream_);
     ++launches_;
     return 0;
 }
@@ -974,22 +1049,22 @@ int MGBatch::launch_stage(const Stage& st) {
 // solution back into phi after an odd number of cycles
 int MGBatch::enqueue_fine(int what, int which, int parity) {
     const int M = (int)m_.size();
+    const auto chunks = make_ref<MGFineTask>(&h_fine_[0], d_fine_, d_chunk_tiles_, M, 0);
+    const auto cases = make_ref<MGFineTask>(&h_fine_[0], d_fine_, nullptr, M, 0);
     if (what == 0) {
-        hipLaunchKernelGGL(sumsq_stage, dim3(n_chunk_), dim3(256), 0, stream_, d_fine_, d_chunk_tiles_);
-        hipLaunchKernelGGL(norm_final_stage, dim3(M), dim3(256), 0, stream_, d_fine_, 0, p_.tol, p_.max_cycles);
+        hipLaunchKernelGGL(sumsq_stage, dim3(n_chunk_), dim3(256), 0, stream_, chunks);
+        hipLaunchKernelGGL(norm_final_stage, dim3(M), dim3(256), 0, stream_, cases, 0, p_.tol, p_.max_cycles);
         launches_ += 2;
     } else if (what == 1) {
+        const auto res = make_ref<MGFineTask>(&h_fine_[0], d_fine_, d_res_tiles_, M, h_fine_[0].gx);
         const dim3 g(n_res_), b(256);
-        if (!mixed_)
-            hipLaunchKernelGGL(ir_residual_stage<0>, g, b, 0, stream_, d_fine_, d_res_tiles_, which, 0);
-        else if (which == 1)
-            hipLaunchKernelGGL(ir_residual_stage<1>, g, b, 0, stream_, d_fine_, d_res_tiles_, which, 0);
-        else
-            hipLaunchKernelGGL(ir_residual_stage<2>, g, b, 0, stream_, d_fine_, d_res_tiles_, which, parity);
-        hipLaunchKernelGGL(norm_final_stage, dim3(M), dim3(256), 0, stream_, d_fine_, which, p_.tol, p_.max_cycles);
+        if (!mixed_) hipLaunchKernelGGL(ir_residual_stage<0>, g, b, 0, stream_, res, which, 0);
+        else if (which == 1) hipLaunchKernelGGL(ir_residual_stage<1>, g, b, 0, stream_, res, which, 0);
+        else hipLaunchKernelGGL(ir_residual_stage<2>, g, b, 0, stream_, res, which, parity);
+        hipLaunchKernelGGL(norm_final_stage, dim3(M), dim3(256), 0, stream_, cases, which, p_.tol, p_.max_cycles);
         launches_ += 2;
     } else {
-        hipLaunchKernelGGL(ir_finish_stage, dim3(n_chunk_), dim3(256), 0, stream_, d_fine_, d_chunk_tiles_);
+        hipLaunchKernelGGL(ir_finish_stage, dim3(n_chunk_), dim3(256), 0, stream_, chunks);
         ++launches_;
     }
     return 0;

@@ -135,4 +135,19 @@ template <> struct Vec2<double> { using type = double2; };
 
 __host__ __device__ inline int div_up(int a, int b) { return (a + b - 1) / b; }
 
+// Explicit global-address-space access.  A pointer read out of a struct is
+// generic to the compiler, which then emits flat memory instructions; flat
+// loads count against the LDS wait counter too, so every wait for an LDS access
+// would also wait for the loads in flight.  Device buffers here are always
+// global memory.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ inline gptr<T> G(T* p) { return (gptr<T>)p; }
+// relaxed device-scope atomic add (the semantics of atomicAdd) on global memory
+template <typename T>
+__device__ inline T gatomic_add(T* p, T v) {
+    return __hip_atomic_fetch_add(G(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 }  // namespace pic

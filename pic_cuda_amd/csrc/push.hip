@@ -34,13 +34,14 @@ __device__ inline float dfma(float a, float b, float c) { return fmaf(a, b, c); 
 __device__ inline double dfma(double a, double b, double c) { return fma(a, b, c); }
 
 // E = -grad(phi) at a node, kernels.cu:134-191 (per-axis one-sided at edges)
-__device__ inline double ex_node(const double* __restrict__ phi, int nx, int i, int j, double dx) {
-    const double* r = phi + (size_t)j * nx;
+__device__ inline double ex_node(const double* __restrict__ phi_, int nx, int i, int j, double dx) {
+    const gptr<const double> r = G(phi_) + (size_t)j * nx;
     if (i == 0) return -(r[1] - r[0]) / dx;
     if (i == nx - 1) return -(r[i] - r[i - 1]) / dx;
     return -(r[i + 1] - r[i - 1]) / (2.0 * dx);
 }
-__device__ inline double ey_node(const double* __restrict__ phi, int nx, int ny, int i, int j, double dy) {
+__device__ inline double ey_node(const double* __restrict__ phi_, int nx, int ny, int i, int j, double dy) {
+    const gptr<const double> phi = G(phi_);
     if (j == 0) return -(phi[(size_t)nx + i] - phi[i]) / dy;
     if (j == ny - 1) return -(phi[(size_t)j * nx + i] - phi[(size_t)(j - 1) * nx + i]) / dy;
     return -(phi[(size_t)(j + 1) * nx + i] - phi[(size_t)(j - 1) * nx + i]) / (2.0 * dy);
@@ -71,7 +72,7 @@ __device__ inline int alloc_slot(const SpeciesDev<T>& S, int out, bool want, int
         const int k = __shfl(key, leader);
         const uint64_t m = __ballot(want && key == k) & act;
         int base = 0;
-        if (lane == leader) base = atomicAdd(back ? &S.mcount[out][k] : &S.count[out][k], __popcll(m));
+        if (lane == leader) base = gatomic_add(back ? &S.mcount[out][k] : &S.count[out][k], (int)__popcll(m));
         base = __shfl(base, leader);
         if ((m >> lane) & 1ull) slot = base + __popcll(m & lt);
         act &= ~m;
@@ -79,12 +80,13 @@ __device__ inline int alloc_slot(const SpeciesDev<T>& S, int out, bool want, int
     int dst = -1;
     bool spill = want && key == nb;
     if (want && key != nb) {
-        const int cap = S.cap[out][key];
+        const int cap = G(S.cap[out])[key];
+        const int start = G(S.start[out])[key];
         if (back) {
-            if (slot < back_cap(cap)) dst = S.start[out][key] + cap - 1 - slot;
+            if (slot < back_cap(cap)) dst = start + cap - 1 - slot;
             else spill = true;
         } else {
-            if (slot < front_cap(cap)) dst = S.start[out][key] + slot;
+            if (slot < front_cap(cap)) dst = start + slot;
             else spill = true;
         }
     }
@@ -92,11 +94,11 @@ __device__ inline int alloc_slot(const SpeciesDev<T>& S, int out, bool want, int
     if (om) {
         const int leader = __ffsll((unsigned long long)om) - 1;
         int base = 0;
-        if (lane == leader) base = atomicAdd(&S.count[out][nb], __popcll(om));
+        if (lane == leader) base = gatomic_add(&S.count[out][nb], (int)__popcll(om));
         base = __shfl(base, leader);
         if (spill) {
             const int s2 = base + __popcll(om & lt);
-            dst = (s2 < S.cap[out][nb]) ? S.start[out][nb] + s2 : -1;
+            dst = (s2 < G(S.cap[out])[nb]) ? G(S.start[out])[nb] + s2 : -1;
         }
     }
     return dst;
@@ -152,10 +154,10 @@ __device__ inline void deposit(T x, T y, int nx, int ny, bool use_lds, int wx0, 
         atomicAdd(b + aw + 1, w11);
     } else {
         unsigned long long* b = gacc + (size_t)iy * nx + ix;
-        atomicAdd(b, w00);
-        atomicAdd(b + 1, w10);
-        atomicAdd(b + nx, w01);
-        atomicAdd(b + nx + 1, w11);
+        gatomic_add(b, w00);
+        gatomic_add(b + 1, w10);
+        gatomic_add(b + nx, w01);
+        gatomic_add(b + nx + 1, w11);
         atomicAdd(lcnt + PIC_C_GLOBAL_DEPOSIT, 1u);
     }
 }
@@ -225,7 +227,7 @@ __device__ inline bool mcc_collide(const SpeciesDev<T>& S, uint32_t id, uint32_t
     int sel = -1;
     for (int q = 0; q < S.n_proc; ++q) {
         if (E < S.thr[q]) continue;
-        const T a = S.tab[(size_t)q * K + j], bb = S.tab[(size_t)q * K + j + 1];
+        const T a = G(S.tab)[(size_t)q * K + j], bb = G(S.tab)[(size_t)q * K + j + 1];
         const T sig = dfma(t, bb - a, a);
         cum = dfma(sig, v, cum);
         if (sel < 0 && R < cum) sel = q;
@@ -285,14 +287,14 @@ __device__ inline Range item_range(const SpeciesDev<T>& S, int in, const WorkIte
     const int nb = S.nb[in];
     Range r;
     if (it.bin < nb) {
-        const int start = S.start[in][it.bin], cap = S.cap[in][it.bin];
-        r.n1 = min(S.count[in][it.bin], front_cap(cap));
-        r.n2 = min(S.mcount[in][it.bin], back_cap(cap));
+        const int start = G(S.start[in])[it.bin], cap = G(S.cap[in])[it.bin];
+        r.n1 = min(G(S.count[in])[it.bin], front_cap(cap));
+        r.n2 = min(G(S.mcount[in])[it.bin], back_cap(cap));
         r.b1 = start;
         r.b2 = start + cap - r.n2;
     } else {
-        const int tot = min(S.count[in][nb], S.cap[in][nb]);
-        r.b1 = S.start[in][nb] + it.chunk * chunk;
+        const int tot = min(G(S.count[in])[nb], G(S.cap[in])[nb]);
+        r.b1 = G(S.start[in])[nb] + it.chunk * chunk;
         r.n1 = max(0, min(chunk, tot - it.chunk * chunk));
         r.b2 = 0;
         r.n2 = 0;
@@ -323,12 +325,12 @@ __device__ inline void flush_stage(const SpeciesDev<T>& S, int out, const Stage<
         const int dst = alloc_slot(S, out, want, key, true);
         if (want) {
             if (dst >= 0) {
-                S.x[out][dst] = sg.x[e];
-                S.y[out][dst] = sg.y[e];
-                S.vx[out][dst] = sg.vx[e];
-                S.vy[out][dst] = sg.vy[e];
-                S.id[out][dst] = sg.id[e];
-                if (dst >= S.start[out][S.nb[out]]) atomicAdd(lcnt + PIC_C_SPILLED, 1u);
+                G(S.x[out])[dst] = sg.x[e];
+                G(S.y[out])[dst] = sg.y[e];
+                G(S.vx[out])[dst] = sg.vx[e];
+                G(S.vy[out])[dst] = sg.vy[e];
+                G(S.id[out])[dst] = sg.id[e];
+                if (dst >= G(S.start[out])[S.nb[out]]) atomicAdd(lcnt + PIC_C_SPILLED, 1u);
             } else {
                 atomicAdd(lcnt + PIC_C_LOST, 1u);
             }
@@ -431,8 +433,8 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
 
     const T kq = S.kq, dtdx = C.dtdx, dtdy = C.dtdy;
     const SpeciesDev<T>* O = (S.other >= 0) ? &C.sp[S.other] : nullptr;
-    const int home_start = is_bin ? S.start[out][it.bin] : 0;
-    const int home_front = is_bin ? front_cap(S.cap[out][it.bin]) : 0;
+    const int home_start = is_bin ? G(S.start[out])[it.bin] : 0;
+    const int home_front = is_bin ? front_cap(G(S.cap[out])[it.bin]) : 0;
     const int lane = __lane_id();
 
     for (int base = 0; base < n; base += blockDim.x) {
@@ -442,11 +444,11 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
         uint32_t id = 0;
         if (valid) {
             const int src = rg.at(t);
-            x = S.x[in][src];
-            y = S.y[in][src];
-            vx = S.vx[in][src];
-            vy = S.vy[in][src];
-            id = S.id[in][src];
+            x = G(S.x[in])[src];
+            y = G(S.y[in])[src];
+            vx = G(S.vx[in])[src];
+            vy = G(S.vy[in])[src];
+            id = G(S.id[in])[src];
         }
         bool alive = false, born = false, be_alive = false, bi_alive = false;
         int key = 0, key_e = 0, key_i = 0;
@@ -533,11 +535,11 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
                 const int slot = sbase + __popcll(sm & lanemask_lt());
                 if (slot < home_front) {
                     const int dst = home_start + slot;
-                    S.x[out][dst] = x;
-                    S.y[out][dst] = y;
-                    S.vx[out][dst] = vx;
-                    S.vy[out][dst] = vy;
-                    S.id[out][dst] = id;
+                    G(S.x[out])[dst] = x;
+                    G(S.y[out])[dst] = y;
+                    G(S.vx[out])[dst] = vx;
+                    G(S.vy[out])[dst] = vy;
+                    G(S.id[out])[dst] = id;
                 } else {
                     stay = false;   // front part full: goes with the movers
                 }
@@ -551,11 +553,11 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
             const int di = alloc_slot(*O, out, bi_alive, key_i, true);
             if (bi_alive) {
                 if (di >= 0) {
-                    O->x[out][di] = b.ix;
-                    O->y[out][di] = b.iy;
-                    O->vx[out][di] = b.ivx;
-                    O->vy[out][di] = b.ivy;
-                    O->id[out][di] = b.iid;
+                    G(O->x[out])[di] = b.ix;
+                    G(O->y[out])[di] = b.iy;
+                    G(O->vx[out])[di] = b.ivx;
+                    G(O->vy[out])[di] = b.ivy;
+                    G(O->id[out])[di] = b.iid;
                 } else {
                     atomicAdd(lcnt + PIC_C_LOST, 1u);
                 }
@@ -572,19 +574,19 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
     }
     const int nm = sh_nm;
     if (nm > 0) flush_stage(S, out, sg, nm, lcnt);
-    if (is_bin && threadIdx.x == 0) S.count[out][it.bin] = min(sh_stay, home_front);
+    if (is_bin && threadIdx.x == 0) G(S.count[out])[it.bin] = min(sh_stay, home_front);
     __syncthreads();
     if (is_bin) {
         for (int k = threadIdx.x; k < aw * ah; k += blockDim.x) {
             const unsigned long long v = accw[k];
             if (v) {
                 const int i = wx0 + k % aw, j = wy0 + k / aw;
-                if (i >= 0 && i < nx && j >= 0 && j < ny) atomicAdd(S.acc + (size_t)j * nx + i, v);
+                if (i >= 0 && i < nx && j >= 0 && j < ny) gatomic_add(S.acc + (size_t)j * nx + i, v);
             }
         }
     }
     for (int k = threadIdx.x; k < PIC_NCOUNT; k += blockDim.x)
-        if (lcnt[k]) atomicAdd(S.counters + k, (unsigned long long)lcnt[k]);
+        if (lcnt[k]) gatomic_add(S.counters + k, (unsigned long long)lcnt[k]);
 }
 
 // ---------------------------------------------------------------------------
