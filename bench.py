@@ -13,7 +13,8 @@ Inputs are generated on the device and stay resident in HBM.
 Prints one JSON line: particle-steps/s (sum of live particles over the timed
 steps / wall time), the roofline of the dominant kernel (push_kernel: 40
 algorithmic bytes per particle, timed with HIP events on the library's stream)
-and a CPU baseline (the serial oracle on a bounded sample of the same workload).
+and a CPU baseline (the oracle, OpenMP build and serial build, on a bounded
+sample of the same workload).
 """
 import argparse
 import json
@@ -42,8 +43,10 @@ def read_traffic():
 
 
 def cpu_baseline(gpu_case, budget_s=12.0):
-    """The serial CPU oracle on a 256^2-cell, 1.25e6 + 1.25e6 particle sample of
-    the same workload (same particles per cell, physics and solver settings)."""
+    """The CPU oracle on a 256^2-cell, 1.25e6 + 1.25e6 particle sample of the
+    same workload (same particles per cell, physics and solver settings): the
+    OpenMP build on all the host threads OpenMP is given, and the serial build
+    on one core beside it."""
     import numpy as np
     from oracle import oracle as orc
     from pic_cuda_amd import configs
@@ -54,32 +57,40 @@ def cpu_baseline(gpu_case, budget_s=12.0):
     pl = configs.Plasma(cells=cells)
     nn = pl.nodes
     sim = gpu_case["sim"]
-    sp = []
-    for s in range(2):
-        info = sim.mcc_info(s)
-        vth = pl.vte if s == 0 else pl.vti
-        x, y, vx, vy, ids = orc.init_particles(4, n, nn, nn, vth, 1, 0, s, 0)
-        kq = pic.ELEMENTARY_CHARGE * (-1 if s == 0 else 1) * pl.dt / (pic.ELECTRON_MASS if s == 0 else configs.M_AR)
-        sp.append(orc.Species(4, x, y, vx, vy, ids, kq, mcc=info, other=info["product_species"]))
+    mg = gpu_case["mg"]
     w = pl.density * pl.length * pl.length / n
     qs = 1.0 / float(1 << 40)
     cfac = [-(q * w * qs / (pl.dx * pl.dx)) / pic.EPS0 for q in (-pic.ELEMENTARY_CHARGE, pic.ELEMENTARY_CHARGE)]
-    mg = gpu_case["mg"]
-    o = orc.OracleSim(sp, nn, nn, pl.dx, pl.dx, pl.dt, cfac,
-                      dict(tol=mg.tol, max_cycles=mg.max_cycles, pre=mg.pre, post=mg.post,
-                           coarse_sweeps=mg.coarse_sweeps, omega=mg.omega))
-    o.step(1)                      # first pass has no kick; time full steps after it
-    t0 = time.perf_counter()
-    steps = 0
-    psteps = 0
-    while steps < 40 and (time.perf_counter() - t0) < budget_s:
-        psteps += sum(s.n for s in o.sp)
-        o.step(1)
-        steps += 1
-    el = time.perf_counter() - t0
-    return {"value": psteps / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": "serial oracle, %d full steps of a 256^2-cell, 1.25e6 e- + 1.25e6 Ar+ sample "
-                      "(same ppc, MCC and 1e-8 FP64 multigrid)" % steps}
+
+    def run(omp):
+        sp = []
+        for s in range(2):
+            info = sim.mcc_info(s)
+            vth = pl.vte if s == 0 else pl.vti
+            x, y, vx, vy, ids = orc.init_particles(4, n, nn, nn, vth, 1, 0, s, 0)
+            kq = pic.ELEMENTARY_CHARGE * (-1 if s == 0 else 1) * pl.dt / (
+                pic.ELECTRON_MASS if s == 0 else configs.M_AR)
+            sp.append(orc.Species(4, x, y, vx, vy, ids, kq, mcc=info, other=info["product_species"]))
+        o = orc.OracleSim(sp, nn, nn, pl.dx, pl.dx, pl.dt, cfac,
+                          dict(tol=mg.tol, max_cycles=mg.max_cycles, pre=mg.pre, post=mg.post,
+                               coarse_sweeps=mg.coarse_sweeps, omega=mg.omega), omp=omp)
+        o.step(1)                      # first pass has no kick; time full steps after it
+        t0 = time.perf_counter()
+        steps = 0
+        psteps = 0
+        while steps < 40 and (time.perf_counter() - t0) < budget_s:
+            psteps += sum(s.n for s in o.sp)
+            o.step(1)
+            steps += 1
+        return psteps / (time.perf_counter() - t0), steps
+
+    threads = orc.lib(True).orc_num_threads()
+    v_omp, s_omp = run(True)
+    v_ser, s_ser = run(False)
+    return {"value": v_omp, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": "oracle (OpenMP, %d threads), %d full steps of a 256^2-cell, 1.25e6 e- + 1.25e6 Ar+ sample "
+                      "(same ppc, MCC and 1e-8 FP64 multigrid)" % (threads, s_omp),
+            "serial_value": v_ser, "serial_steps": s_ser}
 
 
 def poisson(n=2049, cpu=True):

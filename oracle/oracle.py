@@ -144,14 +144,15 @@ def charge_to_rhs(accs, cfacs):
     return rhs
 
 
-def mg_solve(phi, rhs, h, levels=0, pre=2, post=2, coarse_sweeps=1, omega=1.0, tol=1e-8, max_cycles=100):
-    """solve_multigrid restated (Multigrid.cu:573-642); phi updated in place."""
+def mg_solve(phi, rhs, h, levels=0, pre=2, post=2, coarse_sweeps=1, omega=1.0, tol=1e-8, max_cycles=100, omp=False):
+    """solve_multigrid restated (Multigrid.cu:573-642); phi updated in place.
+    omp: the OpenMP build (colour sweeps split over threads; same values)."""
     assert phi.dtype == np.float64 and phi.flags.c_contiguous
     rhs = np.ascontiguousarray(rhs, dtype=np.float64)
     n = phi.shape[0]
     p = MGParams(levels, pre, post, coarse_sweeps, omega, tol, max_cycles)
     st = MGStats()
-    rc = lib().orc_mg_solve(_p(phi), _p(rhs), n, n, h, ctypes.byref(p), ctypes.byref(st))
+    rc = lib(omp).orc_mg_solve(_p(phi), _p(rhs), n, n, h, ctypes.byref(p), ctypes.byref(st))
     if rc != 0:
         raise RuntimeError("orc_mg_solve failed: %d" % rc)
     return st
@@ -187,10 +188,41 @@ class Species:
         self.mcc = mcc            # dict of the null-collision constants or None
         self.other = other
         self.counters = np.zeros(16, dtype=np.uint64)
+        self._bufs = None         # capacity arrays that x..id are views of after a pass
+        self._born = None
 
     @property
     def n(self):
         return len(self.x)
+
+    def _buffers(self, need):
+        """Arrays of capacity >= need holding the particles in front; x..id
+        become views of them (no copy when they already are)."""
+        n = self.n
+        b = self._bufs
+        fresh = b is None or len(b[0]) < need or not all(
+            v.base is buf and v.ctypes.data == buf.ctypes.data for v, buf in zip(self._views(), b))
+        if fresh:
+            cap = max(need, int(need * 1.25))
+            b = [np.empty(cap, dtype=self.dt) for _ in range(4)] + [np.empty(cap, dtype=np.uint32)]
+            for v, buf in zip(self._views(), b):
+                buf[:n] = v
+            self._bufs = b
+        return b
+
+    def _views(self):
+        return (self.x, self.y, self.vx, self.vy, self.id)
+
+    def _set_len(self, k):
+        b = self._bufs
+        self.x, self.y, self.vx, self.vy, self.id = (buf[:k] for buf in b)
+
+    def append(self, x, y, vx, vy, ids):
+        n, m = self.n, len(x)
+        b = self._buffers(n + m)
+        for buf, v in zip(b, (x, y, vx, vy, ids)):
+            buf[n:n + m] = v
+        self._set_len(n + m)
 
 
 def push(sp, phi, nx, ny, dx, dy, dtdx, dtdy, flags, step, seed, case_id, species_idx, box=True, walls=None,
@@ -201,15 +233,14 @@ def push(sp, phi, nx, ny, dx, dy, dtdx, dtdy, flags, step, seed, case_id, specie
     n = sp.n
     born_cap = n + 16
     cap = n + born_cap
-    arrs = []
-    for a in (sp.x, sp.y, sp.vx, sp.vy):
-        b = np.zeros(cap, dtype=sp.dt)
-        b[:n] = a
-        arrs.append(b)
-    idb = np.zeros(cap, dtype=np.uint32)
-    idb[:n] = sp.id
-    xo, yo, vxo, vyo = (np.zeros(born_cap, dtype=sp.dt) for _ in range(4))
-    ido = np.zeros(born_cap, dtype=np.uint32)
+    arrs = sp._buffers(cap)
+    cap = len(arrs[0])
+    idb = arrs[4]
+    if sp._born is None or len(sp._born[0]) < born_cap:
+        bc = max(born_cap, int(born_cap * 1.25))
+        sp._born = [np.empty(bc, dtype=sp.dt) for _ in range(4)] + [np.empty(bc, dtype=np.uint32)]
+    xo, yo, vxo, vyo, ido = sp._born
+    born_cap = len(xo)
     if acc_self is None:
         acc_self = np.zeros(nx * ny, dtype=np.uint64)
     if acc_other is None:
@@ -242,8 +273,7 @@ def push(sp, phi, nx, ny, dx, dy, dtdx, dtdy, flags, step, seed, case_id, specie
     k = lib(omp).orc_push(ctypes.byref(p), _p(phi), _p(arrs[0]), _p(arrs[1]), _p(arrs[2]), _p(arrs[3]), _p(idb),
                           n, cap, _p(acc_self), _p(xo), _p(yo), _p(vxo), _p(vyo), _p(ido), born_cap,
                           ctypes.byref(n_other), _p(acc_other), _p(sp.counters))
-    sp.x, sp.y, sp.vx, sp.vy = (a[:k].copy() for a in arrs)
-    sp.id = idb[:k].copy()
+    sp._set_len(k)
     no = n_other.value
     if no:
         return (xo[:no].copy(), yo[:no].copy(), vxo[:no].copy(), vyo[:no].copy(), ido[:no].copy())
@@ -285,11 +315,7 @@ class OracleSim:
         for s, lst in born.items():
             sp = self.sp[s]
             for (x, y, vx, vy, ids) in lst:
-                sp.x = np.concatenate([sp.x, x])
-                sp.y = np.concatenate([sp.y, y])
-                sp.vx = np.concatenate([sp.vx, vx])
-                sp.vy = np.concatenate([sp.vy, vy])
-                sp.id = np.concatenate([sp.id, ids])
+                sp.append(x, y, vx, vy, ids)
         return accs
 
     def step(self, n=1, solve=True):
@@ -300,7 +326,7 @@ class OracleSim:
             self.step_no += 1
             self.rhs = charge_to_rhs(accs, self.cfac).reshape(self.ny, self.nx)
             if solve:
-                self.last_mg = mg_solve(self.phi, self.rhs, self.dx, **self.mg)
+                self.last_mg = mg_solve(self.phi, self.rhs, self.dx, omp=self.omp, **self.mg)
             self.pending = True
 
     def sync(self):

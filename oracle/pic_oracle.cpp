@@ -140,6 +140,8 @@ extern "C" void orc_charge_to_rhs(const uint64_t* const* acc, const double* cfac
 /* ------------------------------------------------------------------------ */
 /* One colour of red-black SOR, Multigrid.cu:192-227 (same expression form). */
 static void rb_colour(double* phi, const double* rhs, int n, double h2, double omega, int colour) {
+    /* the nodes of one colour are independent: any order gives the same values */
+#pragma omp parallel for schedule(static) if (n > 256)
     for (int j = 1; j < n - 1; ++j)
         for (int i = 1; i < n - 1; ++i) {
             if (((i + j) & 1) != colour) continue;
@@ -386,146 +388,218 @@ static inline bool absorbed(const orc_push_params* p, T x0, T y0, T x1, T y1) {
     return false;
 }
 
+/* One particle of a pass: kick, collisions, drift, walls, deposit.  emit(kind,
+   particle) receives the survivor (kind 0), a born particle of the same species
+   (1) and a born particle of the other species (2), in that order. */
+template <typename T, typename Emit>
+static inline void push_one(const orc_push_params* p, const double* phi, const T* tab, PP<T> q,
+                            uint64_t* acc_self, uint64_t* acc_other, uint64_t* cnt, Emit&& emit) {
+    const int nx = p->nx, ny = p->ny;
+    const T kq = (T)p->kq, dtdx = (T)p->dtdx, dtdy = (T)p->dtdy;
+    const T efac = (T)p->efac, inv_de = (T)p->inv_de, nu_max = (T)p->nu_max, p_null = (T)p->p_null;
+    const int K = p->K;
+    T x = q.x, y = q.y, vx = q.vx, vy = q.vy;
+    const uint32_t id = q.id;
+    if (p->do_kick) {
+        int ix = (int)std::floor(x), iy = (int)std::floor(y);
+        if (!(ix < 0 || ix >= nx - 1 || iy < 0 || iy >= ny - 1)) {
+            T fx = x - (T)ix, fy = y - (T)iy;
+            T gx = (T)1 - fx, gy = (T)1 - fy;
+            T w00 = gx * gy, w10 = fx * gy, w01 = gx * fy, w11 = fx * fy;
+            T e[4][2];
+            const int ci[4] = {ix, ix + 1, ix, ix + 1}, cj[4] = {iy, iy, iy + 1, iy + 1};
+            for (int c = 0; c < 4; ++c) {
+                e[c][0] = (T)ex_at(phi, nx, ci[c], cj[c], p->dx);
+                e[c][1] = (T)ey_at(phi, nx, ny, ci[c], cj[c], p->dy);
+            }
+            T Ex = std::fma(w11, e[3][0], std::fma(w01, e[2][0], std::fma(w10, e[1][0], w00 * e[0][0])));
+            T Ey = std::fma(w11, e[3][1], std::fma(w01, e[2][1], std::fma(w10, e[1][1], w00 * e[0][1])));
+            vx = std::fma(kq, Ex, vx);
+            vy = std::fma(kq, Ey, vy);
+        }
+    }
+    bool have_born = false;
+    PP<T> be{}, bi{};
+    if (p->do_mcc && p->n_proc > 0) {
+        Stream rs(id, p->step, p->case_id, p->seed, p->species);
+        T u0 = rs.uni<T>(), u1 = rs.uni<T>();
+        if (u0 < p_null) {
+            cnt[ORC_C_NULL_CAND]++;
+            T v2 = std::fma(vx, vx, vy * vy);
+            T E = efac * v2;
+            T v = std::sqrt(v2);
+            T s = E * inv_de;
+            int j;
+            T t;
+            if (s < (T)(K - 1)) { j = (int)s; t = s - (T)j; }
+            else { j = K - 2; t = (T)1; }
+            T R = u1 * nu_max;
+            T cum = (T)0;
+            int sel = -1;
+            for (int qq = 0; qq < p->n_proc; ++qq) {
+                if (E < (T)p->proc_thr[qq]) continue;
+                T a = tab[(size_t)qq * K + j], b = tab[(size_t)qq * K + j + 1];
+                T sig = std::fma(t, b - a, a);
+                cum = std::fma(sig, v, cum);
+                if (sel < 0 && R < cum) sel = qq;
+            }
+            if (cum > nu_max) cnt[ORC_C_NUMAX_EXCEEDED]++;
+            if (sel >= 0) {
+                cnt[ORC_C_PROC0 + sel]++;
+                int ty = p->proc_type[sel];
+                T thr = (T)p->proc_thr[sel];
+                T c, sn;
+                if (ty == ORC_ELASTIC) {
+                    if (!iso_dir(rs, c, sn)) cnt[ORC_C_REJECT_FALLBACK]++;
+                    vx = v * c; vy = v * sn;
+                } else if (ty == ORC_EXCITATION) {
+                    T e1 = E - thr;
+                    T v1 = std::sqrt(e1 / efac);
+                    if (!iso_dir(rs, c, sn)) cnt[ORC_C_REJECT_FALLBACK]++;
+                    vx = v1 * c; vy = v1 * sn;
+                } else if (ty == ORC_IONIZATION) {
+                    T er = E - thr;
+                    T us = rs.uni<T>();
+                    T e1 = er * us;
+                    T e2 = er - e1;
+                    T v1 = std::sqrt(e1 / efac), v2s = std::sqrt(e2 / efac);
+                    if (!iso_dir(rs, c, sn)) cnt[ORC_C_REJECT_FALLBACK]++;
+                    vx = v1 * c; vy = v1 * sn;
+                    if (!iso_dir(rs, c, sn)) cnt[ORC_C_REJECT_FALLBACK]++;
+                    be.x = x; be.y = y; be.vx = v2s * c; be.vy = v2s * sn;
+                    T g1, g2;
+                    if (!gauss_pair(rs, p->gas_vth, g1, g2)) cnt[ORC_C_REJECT_FALLBACK]++;
+                    bi.x = x; bi.y = y; bi.vx = g1; bi.vy = g2;
+                    be.id = rs.u32();
+                    bi.id = rs.u32();
+                    have_born = true;
+                } else { /* charge exchange: take a gas-atom velocity */
+                    T g1, g2;
+                    if (!gauss_pair(rs, p->gas_vth, g1, g2)) cnt[ORC_C_REJECT_FALLBACK]++;
+                    vx = g1; vy = g2;
+                }
+            }
+        }
+    }
+    T x0 = x, y0 = y;
+    if (p->do_drift) {
+        x = std::fma(vx, dtdx, x);
+        y = std::fma(vy, dtdy, y);
+    }
+    if (absorbed(p, x0, y0, x, y)) {
+        cnt[ORC_C_ABSORBED]++;
+    } else {
+        deposit_one(x, y, nx, ny, acc_self);
+        emit(0, PP<T>{x, y, vx, vy, id});
+    }
+    if (have_born) {
+        /* born particles drift from the collision point when the pass drifts */
+        PP<T> b[2] = {be, bi};
+        for (int w = 0; w < 2; ++w) {
+            T bx = p->do_drift ? std::fma(b[w].vx, dtdx, b[w].x) : b[w].x;
+            T by = p->do_drift ? std::fma(b[w].vy, dtdy, b[w].y) : b[w].y;
+            if (absorbed(p, b[w].x, b[w].y, bx, by)) { cnt[ORC_C_BORN_ABSORBED]++; continue; }
+            b[w].x = bx; b[w].y = by;
+            if (w == 0) {
+                deposit_one(bx, by, nx, ny, acc_self);
+                emit(1, b[w]);
+                cnt[ORC_C_BORN_SELF]++;
+            } else {
+                deposit_one(bx, by, nx, ny, acc_other);
+                emit(2, b[w]);
+                cnt[ORC_C_BORN_OTHER]++;
+            }
+        }
+    }
+}
+
+/* A pass over one species.  Survivors keep their order and are followed by the
+   born particles of this species in the order of creation; born particles of
+   the other species go to the *O arrays.  The OpenMP build splits the input into
+   one contiguous range per thread, collects each range's results privately and
+   concatenates them in range order, so its output equals the serial one
+   (fixed-point charge is summed exactly). */
 template <typename T>
 static int64_t push_t(const orc_push_params* p, const double* phi,
                       T* X, T* Y, T* VX, T* VY, uint32_t* ID, int64_t n, int64_t cap,
                       uint64_t* acc_self, T* XO, T* YO, T* VXO, T* VYO, uint32_t* IDO,
                       int64_t cap_o, int64_t* n_other, uint64_t* acc_other, uint64_t* cnt) {
-    const int nx = p->nx, ny = p->ny;
-    const T kq = (T)p->kq, dtdx = (T)p->dtdx, dtdy = (T)p->dtdy;
-    const T efac = (T)p->efac, inv_de = (T)p->inv_de, nu_max = (T)p->nu_max, p_null = (T)p->p_null;
     const int K = p->K;
     std::vector<T> tab;
     if (p->do_mcc && p->n_proc > 0) {
         tab.resize((size_t)p->n_proc * K);
         for (size_t i = 0; i < tab.size(); ++i) tab[i] = (T)p->tab[i];
     }
-    std::vector<PP<T>> born_self;
     int64_t no = 0, k = 0;
-
-    for (int64_t i = 0; i < n; ++i) {
-        T x = X[i], y = Y[i], vx = VX[i], vy = VY[i];
-        uint32_t id = ID[i];
-        if (p->do_kick) {
-            int ix = (int)std::floor(x), iy = (int)std::floor(y);
-            if (!(ix < 0 || ix >= nx - 1 || iy < 0 || iy >= ny - 1)) {
-                T fx = x - (T)ix, fy = y - (T)iy;
-                T gx = (T)1 - fx, gy = (T)1 - fy;
-                T w00 = gx * gy, w10 = fx * gy, w01 = gx * fy, w11 = fx * fy;
-                T e[4][2];
-                const int ci[4] = {ix, ix + 1, ix, ix + 1}, cj[4] = {iy, iy, iy + 1, iy + 1};
-                for (int c = 0; c < 4; ++c) {
-                    e[c][0] = (T)ex_at(phi, nx, ci[c], cj[c], p->dx);
-                    e[c][1] = (T)ey_at(phi, nx, ny, ci[c], cj[c], p->dy);
-                }
-                T Ex = std::fma(w11, e[3][0], std::fma(w01, e[2][0], std::fma(w10, e[1][0], w00 * e[0][0])));
-                T Ey = std::fma(w11, e[3][1], std::fma(w01, e[2][1], std::fma(w10, e[1][1], w00 * e[0][1])));
-                vx = std::fma(kq, Ex, vx);
-                vy = std::fma(kq, Ey, vy);
-            }
+    auto put_other = [&](const PP<T>& b) {
+        if (no < cap_o) { XO[no] = b.x; YO[no] = b.y; VXO[no] = b.vx; VYO[no] = b.vy; IDO[no] = b.id; }
+        ++no;
+    };
+    int nt = 1;
+#ifdef _OPENMP
+    nt = omp_get_max_threads();
+    if (n < 65536) nt = 1;
+#endif
+    if (nt == 1) {
+        std::vector<PP<T>> born_self;
+        for (int64_t i = 0; i < n; ++i) {
+            push_one(p, phi, tab.data(), PP<T>{X[i], Y[i], VX[i], VY[i], ID[i]}, acc_self, acc_other, cnt,
+                     [&](int kind, const PP<T>& r) {
+                         if (kind == 0) { X[k] = r.x; Y[k] = r.y; VX[k] = r.vx; VY[k] = r.vy; ID[k] = r.id; ++k; }
+                         else if (kind == 1) born_self.push_back(r);
+                         else put_other(r);
+                     });
         }
-        bool have_born = false;
-        PP<T> be{}, bi{};
-        if (p->do_mcc && p->n_proc > 0) {
-            Stream rs(id, p->step, p->case_id, p->seed, p->species);
-            T u0 = rs.uni<T>(), u1 = rs.uni<T>();
-            if (u0 < p_null) {
-                cnt[ORC_C_NULL_CAND]++;
-                T v2 = std::fma(vx, vx, vy * vy);
-                T E = efac * v2;
-                T v = std::sqrt(v2);
-                T s = E * inv_de;
-                int j;
-                T t;
-                if (s < (T)(K - 1)) { j = (int)s; t = s - (T)j; }
-                else { j = K - 2; t = (T)1; }
-                T R = u1 * nu_max;
-                T cum = (T)0;
-                int sel = -1;
-                for (int q = 0; q < p->n_proc; ++q) {
-                    if (E < (T)p->proc_thr[q]) continue;
-                    T a = tab[(size_t)q * K + j], b = tab[(size_t)q * K + j + 1];
-                    T sig = std::fma(t, b - a, a);
-                    cum = std::fma(sig, v, cum);
-                    if (sel < 0 && R < cum) sel = q;
-                }
-                if (cum > nu_max) cnt[ORC_C_NUMAX_EXCEEDED]++;
-                if (sel >= 0) {
-                    cnt[ORC_C_PROC0 + sel]++;
-                    int ty = p->proc_type[sel];
-                    T thr = (T)p->proc_thr[sel];
-                    T c, sn;
-                    if (ty == ORC_ELASTIC) {
-                        if (!iso_dir(rs, c, sn)) cnt[ORC_C_REJECT_FALLBACK]++;
-                        vx = v * c; vy = v * sn;
-                    } else if (ty == ORC_EXCITATION) {
-                        T e1 = E - thr;
-                        T v1 = std::sqrt(e1 / efac);
-                        if (!iso_dir(rs, c, sn)) cnt[ORC_C_REJECT_FALLBACK]++;
-                        vx = v1 * c; vy = v1 * sn;
-                    } else if (ty == ORC_IONIZATION) {
-                        T er = E - thr;
-                        T us = rs.uni<T>();
-                        T e1 = er * us;
-                        T e2 = er - e1;
-                        T v1 = std::sqrt(e1 / efac), v2s = std::sqrt(e2 / efac);
-                        if (!iso_dir(rs, c, sn)) cnt[ORC_C_REJECT_FALLBACK]++;
-                        vx = v1 * c; vy = v1 * sn;
-                        if (!iso_dir(rs, c, sn)) cnt[ORC_C_REJECT_FALLBACK]++;
-                        be.x = x; be.y = y; be.vx = v2s * c; be.vy = v2s * sn;
-                        T g1, g2;
-                        if (!gauss_pair(rs, p->gas_vth, g1, g2)) cnt[ORC_C_REJECT_FALLBACK]++;
-                        bi.x = x; bi.y = y; bi.vx = g1; bi.vy = g2;
-                        be.id = rs.u32();
-                        bi.id = rs.u32();
-                        have_born = true;
-                    } else { /* charge exchange: take a gas-atom velocity */
-                        T g1, g2;
-                        if (!gauss_pair(rs, p->gas_vth, g1, g2)) cnt[ORC_C_REJECT_FALLBACK]++;
-                        vx = g1; vy = g2;
-                    }
-                }
-            }
-        }
-        T x0 = x, y0 = y;
-        if (p->do_drift) {
-            x = std::fma(vx, dtdx, x);
-            y = std::fma(vy, dtdy, y);
-        }
-        if (absorbed(p, x0, y0, x, y)) {
-            cnt[ORC_C_ABSORBED]++;
-        } else {
-            deposit_one(x, y, nx, ny, acc_self);
-            X[k] = x; Y[k] = y; VX[k] = vx; VY[k] = vy; ID[k] = id;
+        for (auto& b : born_self) {
+            if (k < cap) { X[k] = b.x; Y[k] = b.y; VX[k] = b.vx; VY[k] = b.vy; ID[k] = b.id; }
             ++k;
         }
-        if (have_born) {
-            /* born particles drift from the collision point when the pass drifts */
-            PP<T> b[2] = {be, bi};
-            for (int w = 0; w < 2; ++w) {
-                T bx = p->do_drift ? std::fma(b[w].vx, dtdx, b[w].x) : b[w].x;
-                T by = p->do_drift ? std::fma(b[w].vy, dtdy, b[w].y) : b[w].y;
-                if (absorbed(p, b[w].x, b[w].y, bx, by)) { cnt[ORC_C_BORN_ABSORBED]++; continue; }
-                b[w].x = bx; b[w].y = by;
-                if (w == 0) {
-                    deposit_one(bx, by, nx, ny, acc_self);
-                    born_self.push_back(b[w]);
-                    cnt[ORC_C_BORN_SELF]++;
-                } else {
-                    deposit_one(bx, by, nx, ny, acc_other);
-                    if (no < cap_o) {
-                        XO[no] = bx; YO[no] = by; VXO[no] = b[w].vx; VYO[no] = b[w].vy; IDO[no] = b[w].id;
-                    }
-                    ++no;
-                    cnt[ORC_C_BORN_OTHER]++;
-                }
+        if (n_other) *n_other = no;
+        return k;
+    }
+#ifdef _OPENMP
+    const size_t nn = (size_t)p->nx * p->ny;
+    struct Part {
+        std::vector<PP<T>> out[3];
+        std::vector<uint64_t> acc[2];
+        uint64_t cnt[ORC_NCOUNT];
+    };
+    std::vector<Part> parts(nt);
+#pragma omp parallel num_threads(nt)
+    {
+        const int t = omp_get_thread_num();
+        Part& pt = parts[t];
+        pt.acc[0].assign(nn, 0);
+        if (acc_other) pt.acc[1].assign(nn, 0);
+        std::fill(pt.cnt, pt.cnt + ORC_NCOUNT, 0ull);
+        const int64_t i0 = n * t / nt, i1 = n * (t + 1) / nt;
+        pt.out[0].reserve(i1 - i0);
+        for (int64_t i = i0; i < i1; ++i)
+            push_one(p, phi, tab.data(), PP<T>{X[i], Y[i], VX[i], VY[i], ID[i]}, pt.acc[0].data(),
+                     acc_other ? pt.acc[1].data() : nullptr, pt.cnt,
+                     [&](int kind, const PP<T>& r) { pt.out[kind].push_back(r); });
+    }
+    for (int kind = 0; kind < 2; ++kind)
+        for (int t = 0; t < nt; ++t)
+            for (const auto& r : parts[t].out[kind]) {
+                if (k < cap) { X[k] = r.x; Y[k] = r.y; VX[k] = r.vx; VY[k] = r.vy; ID[k] = r.id; }
+                ++k;
             }
+    for (int t = 0; t < nt; ++t)
+        for (const auto& r : parts[t].out[2]) put_other(r);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)nn; ++i) {
+        uint64_t a = 0, b = 0;
+        for (int t = 0; t < nt; ++t) {
+            a += parts[t].acc[0][i];
+            if (acc_other) b += parts[t].acc[1][i];
         }
+        acc_self[i] += a;
+        if (acc_other) acc_other[i] += b;
     }
-    for (auto& b : born_self) {
-        if (k < cap) { X[k] = b.x; Y[k] = b.y; VX[k] = b.vx; VY[k] = b.vy; ID[k] = b.id; }
-        ++k;
-    }
+    for (int t = 0; t < nt; ++t)
+        for (int c = 0; c < ORC_NCOUNT; ++c) cnt[c] += parts[t].cnt[c];
+#endif
     if (n_other) *n_other = no;
     return k;
 }

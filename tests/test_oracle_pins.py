@@ -442,3 +442,44 @@ def test_init_statistics(orc):
     # kurtosis of a Gaussian is 3
     k = np.mean(vx**4) / np.mean(vx**2) ** 2
     assert abs(k - 3.0) < 0.05
+
+
+def test_openmp_oracle_pass_equals_serial(orc):
+    """The OpenMP build of the oracle (used only for the CPU baseline) gives the
+    serial result: same survivors in the same order, same born particles, same
+    fixed-point charge and counters."""
+    QE, ME = 1.602176634e-19, 9.1093837015e-31
+    n, nx = 120000, 33
+    dt = 5e-9
+    K, emax = 256, 100.0
+
+    def tables(Eg):
+        el = np.full_like(Eg, 1e21 * 2e-20)
+        ion = np.where(Eg > 15.8, 1e21 * 6e-20, 0.0)
+        return np.stack([el, ion])
+
+    m = _mcc_const(orc, 1e21, 0, ME, dt, K=K, emax=emax, types=(0, 2), thr=(0.0, 15.8), extra=tables)
+    rng = np.random.default_rng(5)
+    x0 = rng.uniform(0.5, nx - 1.5, n)
+    y0 = rng.uniform(0.5, nx - 1.5, n)
+    v = math.sqrt(2 * 30.0 * QE / ME)
+    ang = rng.uniform(0, 2 * np.pi, n)
+    phi = rng.normal(size=(nx, nx))
+    out = []
+    for omp in (False, True):
+        sp = orc.Species(4, x0.astype(np.float32), y0.astype(np.float32), (v * np.cos(ang)).astype(np.float32),
+                         (v * np.sin(ang)).astype(np.float32), np.arange(n, dtype=np.uint32), -1e3, mcc=m, other=1)
+        acc = np.zeros(nx * nx, dtype=np.uint64)
+        acc_o = np.zeros(nx * nx, dtype=np.uint64)
+        born = orc.push(sp, phi, nx, nx, 1.0, 1.0, 1e-7, 1e-7, dict(kick=True, mcc=True, drift=True), 2, 3, 0, 0,
+                        acc_self=acc, acc_other=acc_o, omp=omp)
+        out.append((sp, acc, acc_o, born))
+    (a, acc_a, acco_a, born_a), (b, acc_b, acco_b, born_b) = out
+    assert orc.lib(True).orc_num_threads() >= 1
+    assert a.n == b.n and a.n > n // 2
+    for u, w in ((a.x, b.x), (a.y, b.y), (a.vx, b.vx), (a.vy, b.vy), (a.id, b.id)):
+        assert np.array_equal(u, w)
+    assert np.array_equal(acc_a, acc_b) and np.array_equal(acco_a, acco_b)
+    assert np.array_equal(a.counters, b.counters) and a.counters[orc.C_PROC0 + 1] > 0
+    for u, w in zip(born_a, born_b):
+        assert np.array_equal(u, w)
