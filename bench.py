@@ -200,8 +200,52 @@ def ensemble(n_cases=64, steps=100, sequential=True):
     res["batched_particle_steps_per_s"] = res["particles"] * steps / t_batch
     if sequential:
         res["speedup_batched_over_sequential"] = round(t_seq / t_batch, 2)
+    # the same ensemble continued with one graph launch per step
+    ens.use_graph(True)
+    ens.step(1)                      # captures the graph
+    pic.synchronize()
+    t0 = time.perf_counter()
+    ens.step(steps - 1)
+    pic.synchronize()
+    el = time.perf_counter() - t0
+    res["batched_graph_ms_per_step"] = round(1e3 * el / (steps - 1), 3)
+    res["batched_graph_jobs_per_s"] = n_cases / (el * steps / (steps - 1))
+    if sequential:
+        res["speedup_batched_graph_over_sequential"] = round(t_seq / (el * steps / (steps - 1)), 2)
     ens.close()
     return res
+
+
+def graph(steps=100):
+    """Eager launches against one graph launch per step: the plain config
+    (128^2 cells, 1e5 + 1e5 particles, FP64, GS 2+2 to 1e-8) where launch
+    overhead is a large part of a step, and the full step at scale."""
+    import pic_cuda_amd as pic
+    from pic_cuda_amd import configs
+    out = {}
+    for name, make, n in (("plain_128", lambda: configs.plain_small(), steps),
+                          ("full_1024", lambda: configs.full_step(), 40)):
+        res = {}
+        for mode in ("eager", "graph"):
+            c = make()
+            sim = c["sim"]
+            if mode == "graph":
+                sim.use_graph(True)
+            sim.step(3)
+            pic.synchronize()
+            st0 = sim.stats()
+            t0 = time.perf_counter()
+            sim.step(n)
+            pic.synchronize()
+            el = time.perf_counter() - t0
+            st1 = sim.stats()
+            res[mode] = {"ms_per_step": round(1e3 * el / n, 4),
+                         "mg_cycles_per_step": (st1.total_mg_cycles - st0.total_mg_cycles) / n,
+                         "graph_steps": st1.graph_steps - st0.graph_steps}
+            del c
+        res["speedup_graph_over_eager"] = round(res["eager"]["ms_per_step"] / res["graph"]["ms_per_step"], 3)
+        out[name] = res
+    return out
 
 
 def serve(n_jobs=24, rate=20.0, budget=200e9, seed=9, max_steps=2000):
@@ -250,15 +294,15 @@ def main():
     ap.add_argument("--cells", type=int, default=1024)
     ap.add_argument("--bin", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="all", choices=["all", "step", "poisson", "ensemble", "serve"])
-    ap.add_argument("--ens-steps", type=int, default=100)
+    ap.add_argument("--workload", default="all", choices=["all", "step", "poisson", "ensemble", "serve", "graph"])
+    ap.add_argument("--ens-steps", type=int, default=500)
     ap.add_argument("--serve-jobs", type=int, default=24)
     a = ap.parse_args()
-    if a.workload in ("poisson", "ensemble", "serve"):
+    if a.workload in ("poisson", "ensemble", "serve", "graph"):
         import pic_cuda_amd as pic
         pic.load()
         fn = {"poisson": lambda: poisson(cpu=not a.no_cpu), "ensemble": lambda: ensemble(steps=a.ens_steps),
-              "serve": lambda: serve(n_jobs=a.serve_jobs)}[a.workload]
+              "serve": lambda: serve(n_jobs=a.serve_jobs), "graph": lambda: graph()}[a.workload]
         print(json.dumps({"workload": a.workload, **fn()}))
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

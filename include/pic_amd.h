@@ -204,6 +204,9 @@ typedef struct {
     int64_t total_mg_cycles;
     int rebins;
     double ms_push, ms_field, ms_total;
+    int64_t graph_steps;     /* steps run as a graph launch */
+    int64_t graph_captures;  /* graph (re)captures */
+    double ms_graph;         /* device time of the graph steps */
 } pic_sim_report;
 
 /* null-collision constants of a species (inputs of the oracle's MCC) */
@@ -232,6 +235,12 @@ int pic_sim_stats(pic_sim* s, pic_sim_report* out);
 int pic_sim_mcc_info(pic_sim* s, int species, pic_mcc_info* out, double* table, int cap);
 /* re-bin now with a new tile (0 keeps the current one) */
 int pic_sim_rebin(pic_sim* s, int bin_w, int bin_h);
+/* graph mode (no counterpart in the reference, which launches every kernel
+   from the host): each step after the first is one hipGraph launch (push,
+   charge, a multigrid solve of the last step's cycle count) followed by one
+   host synchronization; a solve that needs more cycles gets them eagerly.
+   Results equal the eager path's.  Off by default. */
+int pic_sim_use_graph(pic_sim* s, int on);
 int pic_sim_destroy(pic_sim* s);
 
 /* ---- ensembles of independent plasmas (no counterpart in the reference: it
@@ -245,6 +254,8 @@ int pic_ensemble_remove(pic_ensemble* e, pic_sim* s);
 int pic_ensemble_size(const pic_ensemble* e, int* n);
 /* n batched steps of updateSimulationCUDA (main.cu:37-140) for every member */
 int pic_ensemble_step(pic_ensemble* e, int n_steps);
+/* graph mode for the batched step, as pic_sim_use_graph */
+int pic_ensemble_use_graph(pic_ensemble* e, int on);
 int pic_ensemble_destroy(pic_ensemble* e);
 
 /* ---- job server (no counterpart in the reference): continuous batching of

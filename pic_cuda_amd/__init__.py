@@ -69,7 +69,8 @@ class SimParams(ctypes.Structure):
 class SimReport(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int64), ("alive", ctypes.c_int64), ("last_mg_cycles", ctypes.c_int),
                 ("last_mg_rel", ctypes.c_double), ("total_mg_cycles", ctypes.c_int64), ("rebins", ctypes.c_int),
-                ("ms_push", ctypes.c_double), ("ms_field", ctypes.c_double), ("ms_total", ctypes.c_double)]
+                ("ms_push", ctypes.c_double), ("ms_field", ctypes.c_double), ("ms_total", ctypes.c_double),
+                ("graph_steps", ctypes.c_int64), ("graph_captures", ctypes.c_int64), ("ms_graph", ctypes.c_double)]
 
 
 class MccInfo(ctypes.Structure):
@@ -143,12 +144,14 @@ SIGNATURES = {
     "pic_sim_stats": (_i, [_vp, _P(SimReport)]),
     "pic_sim_mcc_info": (_i, [_vp, _i, _P(MccInfo), _vp, _i]),
     "pic_sim_rebin": (_i, [_vp, _i, _i]),
+    "pic_sim_use_graph": (_i, [_vp, _i]),
     "pic_sim_destroy": (_i, [_vp]),
     "pic_ensemble_create": (_i, [_P(_vp)]),
     "pic_ensemble_add": (_i, [_vp, _vp]),
     "pic_ensemble_remove": (_i, [_vp, _vp]),
     "pic_ensemble_size": (_i, [_vp, _P(_i)]),
     "pic_ensemble_step": (_i, [_vp, _i]),
+    "pic_ensemble_use_graph": (_i, [_vp, _i]),
     "pic_ensemble_destroy": (_i, [_vp]),
     "pic_server_create": (_i, [_d, _i, _i, _P(SimParams), _i, _P(_vp)]),
     "pic_server_submit": (_i, [_vp, _P(JobDesc), _P(_i)]),
@@ -556,6 +559,10 @@ class Simulation:
     def rebin(self, bin_w=0, bin_h=0):
         _check(load().pic_sim_rebin(self._h, int(bin_w), int(bin_h)))
 
+    def use_graph(self, on=True):
+        """Run each step after the first as one graph launch."""
+        _check(load().pic_sim_use_graph(self._h, 1 if on else 0))
+
 
 def updateSimulation(sim, steps):
     """updateSimulationCUDA(particles, grid, boundaries, step, dt, steps, timing)."""
@@ -581,6 +588,9 @@ class Ensemble:
 
     def step(self, n=1):
         _check(load().pic_ensemble_step(self._h, int(n)))
+
+    def use_graph(self, on=True):
+        _check(load().pic_ensemble_use_graph(self._h, 1 if on else 0))
 
     def close(self):
         if self._h:

@@ -1129,6 +1129,67 @@ int mcc_active(pic_sim* S) {
     return 0;
 }
 
+// The work of one step after the host decisions: push, end of step, spill
+// status to the host, charge, and a multigrid solve of `cycles` cycles with no
+// host synchronization.  Graph mode captures exactly this.
+template <typename T>
+static int enqueue_step_body(pic_sim* S, int flags, size_t lds, int cycles) {
+    pic_particles* p = S->p;
+    hipStream_t sm = lib_stream();
+    const size_t nn = (size_t)S->g->m * S->g->n;
+    if (int rc = launch_push<T>((CaseDev<T>*)p->d_case, p->d_state, S->d_items, S->n_items, S->spill_chunk, flags, -1,
+                                lds, sm))
+        return rc;
+    if (int rc = launch_end_step<T>((CaseDev<T>*)p->d_case, p->d_state, 1, 1, sm)) return rc;
+    if (int rc = launch_status<T>((CaseDev<T>*)p->d_case, p->d_state, 1, S->d_status, sm)) return rc;
+    PIC_HIP(hipMemcpyAsync(S->h_status, S->d_status, PIC_MAX_SPECIES * sizeof(int), hipMemcpyDeviceToHost, sm));
+    if (int rc = launch_charge(S->charge, nn, sm)) return rc;
+    return S->mg->enqueue_fixed(cycles);
+}
+
+static void drop_graph(pic_sim* S) {
+    if (S->gexec) (void)hipGraphExecDestroy(S->gexec);
+    S->gexec = nullptr;
+}
+
+// One step as a graph launch.  The graph is captured again when the layout
+// (work items, LDS size), the pass flags or the cycle count change.
+template <typename T>
+static int graph_step(pic_sim* S, int flags, size_t lds, MGStats* st) {
+    hipStream_t sm = lib_stream();
+    const int cycles = std::max(1, S->last_cycles);
+    if (S->gexec && (S->g_items != S->d_items || S->g_n_items != S->n_items || S->g_lds != lds ||
+                     S->g_flags != flags || S->g_cycles != cycles))
+        drop_graph(S);
+    if (!S->gexec) {
+        if (int rc = S->mg->prepare(S->g->phi, S->g->rhs, S->g->dx)) return rc;   // allocates: not in a capture
+        PIC_HIP(hipStreamBeginCapture(sm, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_step_body<T>(S, flags, lds, cycles);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(sm, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        PIC_HIP(ec);
+        const hipError_t ei = hipGraphInstantiate(&S->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        PIC_HIP(ei);
+        S->g_items = S->d_items;
+        S->g_n_items = S->n_items;
+        S->g_lds = lds;
+        S->g_flags = flags;
+        S->g_cycles = cycles;
+        S->graph_captures++;
+    }
+    PIC_HIP(hipEventRecord(S->ev[0], sm));
+    PIC_HIP(hipGraphLaunch(S->gexec, sm));
+    PIC_HIP(hipEventRecord(S->ev[2], sm));
+    PIC_HIP(hipEventSynchronize(S->ev[2]));
+    S->graph_steps++;
+    return S->mg->complete_fixed(st);
+}
+
 template <typename T>
 static int step_t(pic_sim* S, int n) {
     pic_particles* p = S->p;
@@ -1141,6 +1202,23 @@ static int step_t(pic_sim* S, int n) {
             if (int rc = sim_rebin(S, S->bin_w, S->bin_h)) return rc;
         }
         const int flags = PIC_F_DRIFT | PIC_F_DEPOSIT | (S->pending ? (PIC_F_KICK | (mcc ? PIC_F_MCC : 0)) : 0);
+        if (S->use_graph && S->pending && S->last_cycles > 0) {
+            MGStats st{};
+            if (int rc = graph_step<T>(S, flags, cur_lds(S), &st)) return rc;
+            p->h_cur ^= 1;
+            float a = 0;
+            (void)hipEventElapsedTime(&a, S->ev[0], S->ev[2]);
+            S->ms_graph += a;
+            S->steps++;
+            S->last_cycles = st.cycles;
+            S->last_rel = st.rel_residual;
+            S->total_cycles += st.cycles;
+            for (int s = 0; s < (int)p->sp.size(); ++s) {
+                const SpeciesStore& ss = p->sp[s];
+                if (S->h_status[s] > ss.h_cap[ss.nb] / 2) S->need_rebin = true;
+            }
+            continue;
+        }
         PIC_HIP(hipEventRecord(S->ev[0], sm));
         if (int rc = launch_push<T>((CaseDev<T>*)p->d_case, p->d_state, S->d_items, S->n_items, S->spill_chunk, flags,
                                     -1, lds, sm))
@@ -1229,6 +1307,9 @@ extern "C" int pic_sim_stats(pic_sim* S, pic_sim_report* out) {
     out->ms_push = S->ms_push;
     out->ms_field = S->ms_field;
     out->ms_total = S->ms_total;
+    out->graph_steps = S->graph_steps;
+    out->graph_captures = S->graph_captures;
+    out->ms_graph = S->ms_graph;
     return 0;
 }
 
@@ -1253,10 +1334,20 @@ extern "C" int pic_sim_rebin(pic_sim* S, int bin_w, int bin_h) {
 
 int ensemble_forget(pic_sim* S);
 
+extern "C" int pic_sim_use_graph(pic_sim* S, int on) {
+    PIC_REQUIRE(S, "sim is NULL");
+    PIC_REQUIRE(!S->ens, "the simulation is in an ensemble: set graph mode on the ensemble");
+    PIC_HIP(hipStreamSynchronize(lib_stream()));
+    S->use_graph = on != 0;
+    if (!S->use_graph) drop_graph(S);
+    return 0;
+}
+
 extern "C" int pic_sim_destroy(pic_sim* S) {
     if (!S) return 0;
     (void)ensemble_forget(S);
     (void)hipStreamSynchronize(lib_stream());
+    drop_graph(S);
     if (S->p && S->p->attached == S) {
         (void)sim_sync_impl(S);
         S->p->attached = nullptr;

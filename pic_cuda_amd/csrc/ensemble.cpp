@@ -41,7 +41,18 @@ struct pic_ensemble {
     size_t lds = 0;
     int64_t steps = 0;
     double ms = 0.0;
+    // graph mode: the instantiated batched step and what it was captured for
+    bool use_graph = false;
+    hipGraphExec_t gexec = nullptr;
+    int64_t gen = 0, g_gen = -1;   // layout generation (bumped by every rebuild)
+    int g_cycles = 0;
+    int64_t graph_steps = 0, graph_captures = 0;
 };
+
+static void drop_graph(pic_ensemble* e) {
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    e->gexec = nullptr;
+}
 
 static void free_arrays(pic_ensemble* e) {
     (void)hipStreamSynchronize(lib_stream());
@@ -69,6 +80,8 @@ static void free_arrays(pic_ensemble* e) {
 static int rebuild(pic_ensemble* e) {
     hipStream_t sm = lib_stream();
     PIC_HIP(hipStreamSynchronize(sm));
+    drop_graph(e);
+    e->gen++;
     const int M = (int)e->m.size();
     void* old_cases = e->d_cases;
     CaseState* old_states = e->d_states;
@@ -186,6 +199,23 @@ extern "C" int pic_ensemble_size(const pic_ensemble* e, int* n) {
     return 0;
 }
 
+// push, end of step, spill status to the host, charge; then the multigrid
+// solve: with cycles > 0 a fixed-cycle solve with no host synchronization (the
+// body graph mode captures), with 0 the adaptive solve
+template <typename T>
+static int enqueue_body(pic_ensemble* e, int cycles) {
+    hipStream_t sm = lib_stream();
+    const int M = (int)e->m.size();
+    if (int rc = launch_push<T>((CaseDev<T>*)e->d_cases, e->d_states, e->d_items, e->n_items, e->m[0]->spill_chunk,
+                                -1, -1, e->lds, sm))
+        return rc;
+    if (int rc = launch_end_step<T>((CaseDev<T>*)e->d_cases, e->d_states, M, 1, sm)) return rc;
+    if (int rc = launch_status<T>((CaseDev<T>*)e->d_cases, e->d_states, M, e->d_status, sm)) return rc;
+    PIC_HIP(hipMemcpyAsync(e->h_status, e->d_status, M * PIC_MAX_SPECIES * sizeof(int), hipMemcpyDeviceToHost, sm));
+    if (int rc = launch_charge_batch(e->d_charge, e->d_nns, e->d_chunks, e->n_chunks, sm)) return rc;
+    return cycles > 0 ? e->mg->enqueue_fixed(cycles) : e->mg->solve();
+}
+
 template <typename T>
 static int ens_step_t(pic_ensemble* e) {
     hipStream_t sm = lib_stream();
@@ -208,15 +238,38 @@ static int ens_step_t(pic_ensemble* e) {
             e->flags[k] = f;
         }
     }
-    if (int rc = launch_push<T>((CaseDev<T>*)e->d_cases, e->d_states, e->d_items, e->n_items, e->m[0]->spill_chunk,
-                                -1, -1, e->lds, sm))
-        return rc;
-    if (int rc = launch_end_step<T>((CaseDev<T>*)e->d_cases, e->d_states, M, 1, sm)) return rc;
+    bool all_pending = true;
+    for (auto* S : e->m) all_pending = all_pending && S->pending;
+    if (e->use_graph && all_pending) {
+        // one graph launch: the captured body below with a solve of the last
+        // step's cycle count, then completion of members that need more
+        const int cycles = e->mg->last_cycles();
+        if (e->gexec && (e->g_gen != e->gen || e->g_cycles != cycles)) drop_graph(e);
+        if (!e->gexec) {
+            PIC_HIP(hipStreamBeginCapture(sm, hipStreamCaptureModeThreadLocal));
+            const int rc = enqueue_body<T>(e, cycles);
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(sm, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            PIC_HIP(ec);
+            const hipError_t ei = hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            PIC_HIP(ei);
+            e->g_gen = e->gen;
+            e->g_cycles = cycles;
+            e->graph_captures++;
+        }
+        PIC_HIP(hipGraphLaunch(e->gexec, sm));
+        PIC_HIP(hipStreamSynchronize(sm));
+        e->graph_steps++;
+        if (int rc = e->mg->complete_fixed()) return rc;
+    } else {
+        if (int rc = enqueue_body<T>(e, 0)) return rc;
+    }
     for (auto* S : e->m) S->p->h_cur ^= 1;
-    if (int rc = launch_status<T>((CaseDev<T>*)e->d_cases, e->d_states, M, e->d_status, sm)) return rc;
-    PIC_HIP(hipMemcpyAsync(e->h_status, e->d_status, M * PIC_MAX_SPECIES * sizeof(int), hipMemcpyDeviceToHost, sm));
-    if (int rc = launch_charge_batch(e->d_charge, e->d_nns, e->d_chunks, e->n_chunks, sm)) return rc;
-    if (int rc = e->mg->solve()) return rc;
     for (int k = 0; k < M; ++k) {
         pic_sim* S = e->m[k];
         const MGNorm& nm = e->mg->norm(k);
@@ -247,8 +300,17 @@ extern "C" int pic_ensemble_step(pic_ensemble* e, int n_steps) {
     return 0;
 }
 
+extern "C" int pic_ensemble_use_graph(pic_ensemble* e, int on) {
+    PIC_REQUIRE(e, "ensemble is NULL");
+    PIC_HIP(hipStreamSynchronize(lib_stream()));
+    e->use_graph = on != 0;
+    if (!e->use_graph) drop_graph(e);
+    return 0;
+}
+
 extern "C" int pic_ensemble_destroy(pic_ensemble* e) {
     if (!e) return 0;
+    drop_graph(e);
     while (!e->m.empty())
         if (int rc = pic_ensemble_remove(e, e->m.back())) return rc;
     free_arrays(e);

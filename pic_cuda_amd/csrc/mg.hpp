@@ -105,6 +105,13 @@ public:
     ~MGSolver();
     int init();
     int solve(double* phi, const double* rhs, double h, MGStats* st);
+    // the same solve split for graph capture: prepare (allocations) before
+    // capture, enqueue_fixed (no host synchronization) inside it, and
+    // complete_fixed once the stream has run it
+    int prepare(double* phi, const double* rhs, double h);
+    int enqueue_fixed(int cycles);
+    int complete_fixed(MGStats* st);
+    int last_cycles() const;
     const MGParams& params() const { return p_; }
     int n() const { return n_; }
     int levels() const { return L_; }
@@ -150,6 +157,12 @@ public:
               const std::vector<const double*>& rhs, const std::vector<double>& h);
     // solve every member (at least one V-cycle batch is enqueued per sync)
     int solve();
+    // a solve of a fixed number of cycles with no host synchronization (it can
+    // be captured in a graph), and the completion after the stream has run it:
+    // members that have not stopped get further cycles until they do
+    int enqueue_fixed(int cycles);
+    int complete_fixed();
+    int last_cycles() const { return last_cycles_; }
     int last_launches() const { return launches_; }
     const MGNorm& norm(int k) const { return h_mirror_[k]; }
     int size() const { return (int)m_.size(); }
@@ -191,6 +204,7 @@ private:
     MGNorm* d_mirror_ = nullptr;
     MGNorm* h_mirror_ = nullptr;        // pinned
     int last_cycles_ = 1;
+    int fixed_cycles_ = 0;
     int launches_ = 0;
 };
 

@@ -662,7 +662,7 @@ int MGSolver::init() {
     return 0;
 }
 
-int MGSolver::solve(double* phi, const double* rhs, double h, MGStats* st) {
+int MGSolver::prepare(double* phi, const double* rhs, double h) {
     PIC_REQUIRE(ok_, "multigrid: solver not initialised");
     if (!solo_ || phi != solo_phi_ || rhs != solo_rhs_ || h != solo_h_) {
         delete solo_;
@@ -676,16 +676,38 @@ int MGSolver::solve(double* phi, const double* rhs, double h, MGStats* st) {
         solo_rhs_ = rhs;
         solo_h_ = h;
     }
-    if (int rc = solo_->solve()) return rc;
-    launches_ = solo_->last_launches();
-    if (st) {
-        const MGNorm& nm = solo_->norm(0);
-        st->cycles = nm.cycles;
-        st->rel_residual = nm.rel;
-        st->rhs_norm = std::sqrt(nm.rhs_sq);
-    }
     return 0;
 }
+
+static void fill_stats(const MGNorm& nm, MGStats* st) {
+    if (!st) return;
+    st->cycles = nm.cycles;
+    st->rel_residual = nm.rel;
+    st->rhs_norm = std::sqrt(nm.rhs_sq);
+}
+
+int MGSolver::solve(double* phi, const double* rhs, double h, MGStats* st) {
+    if (int rc = prepare(phi, rhs, h)) return rc;
+    if (int rc = solo_->solve()) return rc;
+    launches_ = solo_->last_launches();
+    fill_stats(solo_->norm(0), st);
+    return 0;
+}
+
+int MGSolver::enqueue_fixed(int cycles) {
+    PIC_REQUIRE(solo_, "multigrid: prepare() first");
+    return solo_->enqueue_fixed(cycles);
+}
+
+int MGSolver::complete_fixed(MGStats* st) {
+    PIC_REQUIRE(solo_, "multigrid: prepare() first");
+    if (int rc = solo_->complete_fixed()) return rc;
+    launches_ = solo_->last_launches();
+    fill_stats(solo_->norm(0), st);
+    return 0;
+}
+
+int MGSolver::last_cycles() const { return solo_ ? solo_->last_cycles() : 1; }
 
 // ---------------------------------------------------------------------------
 // MGBatch: one launch per stage for all member grids
@@ -1100,6 +1122,49 @@ int MGBatch::solve() {
         batch = 1;
     }
     if (mixed_) enqueue_fine(3, 0, 0);
+    PIC_HIP(hipGetLastError());
+    int mx = 0;
+    for (int k = 0; k < M; ++k) mx = std::max(mx, h_mirror_[k].cycles);
+    last_cycles_ = std::max(1, mx);
+    return 0;
+}
+
+int MGBatch::enqueue_fixed(int cycles) {
+    PIC_REQUIRE(!m_.empty(), "multigrid batch: not built");
+    launches_ = 0;
+    const int M = (int)m_.size();
+    const int K = std::max(1, std::min(cycles, p_.max_cycles));
+    enqueue_fine(0, 0, 0);
+    enqueue_fine(1, 1, 0);
+    for (int b = 0; b < K; ++b) {
+        for (const auto& st : cycle_)
+            if (int rc = launch_stage(st)) return rc;
+        enqueue_fine(1, 2, b & 1);
+    }
+    if (mixed_) enqueue_fine(3, 0, 0);
+    PIC_HIP(hipMemcpyAsync(h_mirror_, d_mirror_, M * sizeof(MGNorm), hipMemcpyDeviceToHost, stream_));
+    fixed_cycles_ = K;
+    return 0;
+}
+
+// The mirror holds the norms after the fixed cycles.  A member that has not
+// stopped continues with cycle index `enqueued`: its solution is in phi (the
+// copy-back ran after an odd count) or it is the cycle's source by parity.
+int MGBatch::complete_fixed() {
+    const int M = (int)m_.size();
+    int enqueued = fixed_cycles_;
+    for (;;) {
+        bool all = true;
+        for (int k = 0; k < M; ++k) all = all && h_mirror_[k].converged;
+        if (all || enqueued >= p_.max_cycles) break;
+        for (const auto& st : cycle_)
+            if (int rc = launch_stage(st)) return rc;
+        enqueue_fine(1, 2, enqueued & 1);
+        ++enqueued;
+        PIC_HIP(hipMemcpyAsync(h_mirror_, d_mirror_, M * sizeof(MGNorm), hipMemcpyDeviceToHost, stream_));
+        PIC_HIP(hipStreamSynchronize(stream_));
+    }
+    if (enqueued > fixed_cycles_ && mixed_) enqueue_fine(3, 0, 0);
     PIC_HIP(hipGetLastError());
     int mx = 0;
     for (int k = 0; k < M; ++k) mx = std::max(mx, h_mirror_[k].cycles);

@@ -115,12 +115,20 @@ struct pic_sim {
     int last_cycles = 0;
     double last_rel = 0;
     int rebins = 0;
-    double ms_push = 0, ms_field = 0, ms_total = 0;
+    double ms_push = 0, ms_field = 0, ms_total = 0, ms_graph = 0;
     int* d_status = nullptr;
     int* h_status = nullptr;
     pic::ChargeArgs charge{};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     struct pic_ensemble* ens = nullptr;   // batch this simulation belongs to
+    // graph mode: the instantiated step and the launch parameters it was
+    // captured with (a change of any of them captures it again)
+    bool use_graph = false;
+    hipGraphExec_t gexec = nullptr;
+    const void* g_items = nullptr;
+    int g_n_items = 0, g_flags = 0, g_cycles = 0;
+    size_t g_lds = 0;
+    int64_t graph_steps = 0, graph_captures = 0;
 };
 
 namespace pic {
