@@ -248,6 +248,37 @@ def graph(steps=100):
     return out
 
 
+def tune(steps=40):
+    """The full step at scale before and after the autotuner (bin tile, SOR
+    factor, sweeps per multigrid level)."""
+    import pic_cuda_amd as pic
+    from pic_cuda_amd import configs
+    c = configs.full_step()
+    sim = c["sim"]
+
+    def timed():
+        sim.step(3)
+        pic.synchronize()
+        st0 = sim.stats()
+        t0 = time.perf_counter()
+        sim.step(steps)
+        pic.synchronize()
+        el = time.perf_counter() - t0
+        st1 = sim.stats()
+        return {"ms_per_step": round(1e3 * el / steps, 4),
+                "push_ms": round((st1.ms_push - st0.ms_push) / steps, 4),
+                "field_ms": round((st1.ms_field - st0.ms_field) / steps, 4),
+                "mg_cycles_per_step": (st1.total_mg_cycles - st0.total_mg_cycles) / steps}
+
+    before = timed()
+    t0 = time.perf_counter()
+    chosen = sim.autotune()
+    tune_s = time.perf_counter() - t0
+    after = timed()
+    return {"before": before, "after": after, "chosen": chosen, "tune_s": round(tune_s, 2),
+            "speedup": round(before["ms_per_step"] / after["ms_per_step"], 3)}
+
+
 def serve(n_jobs=24, rate=20.0, budget=200e9, seed=9, max_steps=2000):
     """Serving: Poisson arrivals (rate jobs/s), cells log-uniform 64..1024
     (rounded to GRID_SIZES), 100 particles per cell and species, steps U(200, max),
@@ -294,15 +325,16 @@ def main():
     ap.add_argument("--cells", type=int, default=1024)
     ap.add_argument("--bin", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="all", choices=["all", "step", "poisson", "ensemble", "serve", "graph"])
+    ap.add_argument("--workload", default="all",
+                    choices=["all", "step", "poisson", "ensemble", "serve", "graph", "tune"])
     ap.add_argument("--ens-steps", type=int, default=500)
     ap.add_argument("--serve-jobs", type=int, default=24)
     a = ap.parse_args()
-    if a.workload in ("poisson", "ensemble", "serve", "graph"):
+    if a.workload in ("poisson", "ensemble", "serve", "graph", "tune"):
         import pic_cuda_amd as pic
         pic.load()
         fn = {"poisson": lambda: poisson(cpu=not a.no_cpu), "ensemble": lambda: ensemble(steps=a.ens_steps),
-              "serve": lambda: serve(n_jobs=a.serve_jobs), "graph": lambda: graph()}[a.workload]
+              "serve": lambda: serve(n_jobs=a.serve_jobs), "graph": lambda: graph(), "tune": lambda: tune()}[a.workload]
         print(json.dumps({"workload": a.workload, **fn()}))
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

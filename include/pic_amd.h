@@ -243,6 +243,31 @@ int pic_sim_rebin(pic_sim* s, int bin_w, int bin_h);
 int pic_sim_use_graph(pic_sim* s, int on);
 int pic_sim_destroy(pic_sim* s);
 
+/* ---- tuning (no counterpart in the reference, whose tiles and sweeps are
+   fixed in the source) --------------------------------------------------- */
+#define PIC_TUNE_BINS 1    /* particle bin tile (the push kernel's LDS tiles) */
+#define PIC_TUNE_MG 2      /* SOR factor and sweeps per multigrid level */
+typedef struct {
+    int bin_w, bin_h;        /* bin tile chosen */
+    double push_ms;          /* its trial push time */
+    double omega;            /* SOR factor chosen */
+    int levels;              /* multigrid levels of the grid */
+    int pre[16], post[16];   /* sweeps chosen per level, finest first (the
+                                coarsest level uses coarse_sweeps) */
+    double solve_ms;         /* a solve of the current charge from a zero
+                                interior with them */
+    int solve_cycles;
+    int trials;              /* configurations timed */
+} pic_tune_report;
+/* Times candidate bin tiles with trial pushes that leave the simulation's
+   state unchanged, and multigrid settings with solves of the current charge
+   into scratch memory, and keeps the fastest.  Not for ensemble members. */
+int pic_sim_autotune(pic_sim* s, int what, pic_tune_report* out);
+/* sweeps of each multigrid level (finest first, n entries, each in [0,3]);
+   levels from n on keep the uniform pre/post */
+int pic_sim_set_level_sweeps(pic_sim* s, int n, const int* pre, const int* post);
+int pic_mg_set_level_sweeps(pic_mg* m, int n, const int* pre, const int* post);
+
 /* ---- ensembles of independent plasmas (no counterpart in the reference: it
    steps one plasma).  Members step together, one launch per stage for all of
    them; each member computes exactly what it computes alone. ------------- */

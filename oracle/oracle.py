@@ -47,7 +47,8 @@ class PushParams(ctypes.Structure):
 class MGParams(ctypes.Structure):
     _fields_ = [("levels", ctypes.c_int), ("pre", ctypes.c_int), ("post", ctypes.c_int),
                 ("coarse_sweeps", ctypes.c_int), ("omega", ctypes.c_double), ("tol", ctypes.c_double),
-                ("max_cycles", ctypes.c_int)]
+                ("max_cycles", ctypes.c_int), ("per_level", ctypes.c_int), ("pre_level", ctypes.c_int * 16),
+                ("post_level", ctypes.c_int * 16)]
 
 
 class MGStats(ctypes.Structure):
@@ -144,13 +145,21 @@ def charge_to_rhs(accs, cfacs):
     return rhs
 
 
-def mg_solve(phi, rhs, h, levels=0, pre=2, post=2, coarse_sweeps=1, omega=1.0, tol=1e-8, max_cycles=100, omp=False):
+def mg_solve(phi, rhs, h, levels=0, pre=2, post=2, coarse_sweeps=1, omega=1.0, tol=1e-8, max_cycles=100, omp=False,
+             pre_levels=None, post_levels=None):
     """solve_multigrid restated (Multigrid.cu:573-642); phi updated in place.
-    omp: the OpenMP build (colour sweeps split over threads; same values)."""
+    omp: the OpenMP build (colour sweeps split over threads; same values).
+    pre_levels/post_levels: sweeps of each level above the coarsest (finest
+    first) instead of pre/post."""
     assert phi.dtype == np.float64 and phi.flags.c_contiguous
     rhs = np.ascontiguousarray(rhs, dtype=np.float64)
     n = phi.shape[0]
     p = MGParams(levels, pre, post, coarse_sweeps, omega, tol, max_cycles)
+    if pre_levels is not None or post_levels is not None:
+        p.per_level = 1
+        for l in range(16):
+            p.pre_level[l] = pre_levels[l] if pre_levels is not None and l < len(pre_levels) else pre
+            p.post_level[l] = post_levels[l] if post_levels is not None and l < len(post_levels) else post
     st = MGStats()
     rc = lib(omp).orc_mg_solve(_p(phi), _p(rhs), n, n, h, ctypes.byref(p), ctypes.byref(st))
     if rc != 0:

@@ -265,7 +265,7 @@ extern "C" int orc_mg_solve(double* phi, const double* rhs, int nx, int ny, doub
         for (int l = 0; l < L - 1; ++l) {
             double* P = l ? ph[l].data() : phi;
             const double* R = l ? rh[l].data() : rhs;
-            orc_mg_smooth(P, R, ns[l], hh, p->omega, p->pre);
+            orc_mg_smooth(P, R, ns[l], hh, p->omega, p->per_level ? p->pre_level[l] : p->pre);
             orc_mg_residual(rs[l].data(), P, R, ns[l], hh);
             orc_mg_restrict(rh[l + 1].data(), rs[l].data(), ns[l]);
             std::fill(ph[l + 1].begin(), ph[l + 1].end(), 0.0);
@@ -283,7 +283,7 @@ extern "C" int orc_mg_solve(double* phi, const double* rhs, int nx, int ny, doub
             double* P = l ? ph[l].data() : phi;
             const double* R = l ? rh[l].data() : rhs;
             orc_mg_prolong(P, ph[l + 1].data(), ns[l + 1]);
-            orc_mg_smooth(P, R, ns[l], hh, p->omega, p->post);
+            orc_mg_smooth(P, R, ns[l], hh, p->omega, p->per_level ? p->post_level[l] : p->post);
         }
         ++cyc;
         rel = rel_res();

@@ -66,6 +66,9 @@ typedef struct {
     double omega;        /* SOR factor (reference: 1.25) */
     double tol;          /* relative residual ||r||/||rhs|| */
     int max_cycles;      /* reference: 50 */
+    int per_level;       /* 1: pre_level/post_level give the sweeps of each level
+                            above the coarsest (index 0 = finest) */
+    int pre_level[16], post_level[16];
 } orc_mg_params;
 
 typedef struct {

@@ -38,6 +38,7 @@ ELASTIC, EXCITATION, IONIZATION, CHARGE_EXCHANGE = 0, 1, 2, 3
 FIELD_RHO, FIELD_PHI, FIELD_EX, FIELD_EY, FIELD_RHS = 0, 1, 2, 3, 4
 MG_FP64, MG_MIXED = 0, 1
 PRESET_REFERENCE, PRESET_STEP, PRESET_POISSON, PRESET_STEP_SOR = 0, 1, 2, 3
+TUNE_BINS, TUNE_MG = 1, 2
 CNT = dict(absorbed=0, null_candidates=1, process0=2, process1=3, process2=4, process3=5, born_self=6,
            born_other=7, born_absorbed=8, numax_exceeded=9, reject_fallback=10, spilled=11, lost=12,
            global_deposit=13)
@@ -71,6 +72,13 @@ class SimReport(ctypes.Structure):
                 ("last_mg_rel", ctypes.c_double), ("total_mg_cycles", ctypes.c_int64), ("rebins", ctypes.c_int),
                 ("ms_push", ctypes.c_double), ("ms_field", ctypes.c_double), ("ms_total", ctypes.c_double),
                 ("graph_steps", ctypes.c_int64), ("graph_captures", ctypes.c_int64), ("ms_graph", ctypes.c_double)]
+
+
+class TuneReport(ctypes.Structure):
+    _fields_ = [("bin_w", ctypes.c_int), ("bin_h", ctypes.c_int), ("push_ms", ctypes.c_double),
+                ("omega", ctypes.c_double), ("levels", ctypes.c_int), ("pre", ctypes.c_int * 16),
+                ("post", ctypes.c_int * 16), ("solve_ms", ctypes.c_double), ("solve_cycles", ctypes.c_int),
+                ("trials", ctypes.c_int)]
 
 
 class MccInfo(ctypes.Structure):
@@ -145,6 +153,9 @@ SIGNATURES = {
     "pic_sim_mcc_info": (_i, [_vp, _i, _P(MccInfo), _vp, _i]),
     "pic_sim_rebin": (_i, [_vp, _i, _i]),
     "pic_sim_use_graph": (_i, [_vp, _i]),
+    "pic_sim_autotune": (_i, [_vp, _i, ctypes.POINTER(TuneReport)]),
+    "pic_sim_set_level_sweeps": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    "pic_mg_set_level_sweeps": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "pic_sim_destroy": (_i, [_vp]),
     "pic_ensemble_create": (_i, [_P(_vp)]),
     "pic_ensemble_add": (_i, [_vp, _vp]),
@@ -435,6 +446,13 @@ class Multigrid:
         _check(load().pic_mg_solve(self._h, _devptr(phi_dev), _devptr(rhs_dev), float(h), ctypes.byref(st)))
         return st
 
+    def set_level_sweeps(self, pre, post):
+        """Sweeps of each level, finest first."""
+        n = len(pre)
+        a = (ctypes.c_int * max(1, n))(*pre)
+        b = (ctypes.c_int * max(1, n))(*post)
+        _check(load().pic_mg_set_level_sweeps(self._h, n, a, b))
+
 
 def solve_multigrid(phi_d, rho_d, nx, ny, levels, h, omega=1.25):
     """solve_multigrid(phi, rho, nx, ny, levels, h, omega) -- Multigrid.hpp:3-11 (device tensors)."""
@@ -562,6 +580,24 @@ class Simulation:
     def use_graph(self, on=True):
         """Run each step after the first as one graph launch."""
         _check(load().pic_sim_use_graph(self._h, 1 if on else 0))
+
+    def autotune(self, bins=True, multigrid=True):
+        """Time candidate bin tiles and multigrid settings on the device and
+        keep the fastest; returns what was chosen."""
+        r = TuneReport()
+        what = (TUNE_BINS if bins else 0) | (TUNE_MG if multigrid else 0)
+        _check(load().pic_sim_autotune(self._h, what, ctypes.byref(r)))
+        nl = max(0, r.levels - 1)
+        return {"bin": (r.bin_w, r.bin_h), "push_ms": r.push_ms, "omega": r.omega, "levels": r.levels,
+                "pre": list(r.pre)[:nl], "post": list(r.post)[:nl], "solve_ms": r.solve_ms,
+                "solve_cycles": r.solve_cycles, "trials": r.trials}
+
+    def set_level_sweeps(self, pre, post):
+        """Sweeps of each multigrid level, finest first."""
+        n = len(pre)
+        a = (ctypes.c_int * max(1, n))(*pre)
+        b = (ctypes.c_int * max(1, n))(*post)
+        _check(load().pic_sim_set_level_sweeps(self._h, n, a, b))
 
 
 def updateSimulation(sim, steps):

@@ -1104,7 +1104,8 @@ extern "C" int pic_sim_create(pic_particles* p, pic_grid* g, pic_walls* w, pic_x
     if (rc) return fail(rc);
     S->mg.reset(new MGSolver(g->m, S->mgp, lib_stream()));
     if ((rc = S->mg->init())) return fail(rc);
-    if (hipMalloc(&S->d_status, PIC_MAX_SPECIES * sizeof(int)) != hipSuccess ||
+    if (hipMalloc(&S->phi_prev, (size_t)g->m * g->n * sizeof(double)) != hipSuccess ||
+        hipMalloc(&S->d_status, PIC_MAX_SPECIES * sizeof(int)) != hipSuccess ||
         hipHostMalloc(&S->h_status, PIC_MAX_SPECIES * sizeof(int), hipHostMallocDefault) != hipSuccess) {
         set_error("status buffers");
         return fail(PIC_ERR_HIP);
@@ -1144,12 +1145,57 @@ static int enqueue_step_body(pic_sim* S, int flags, size_t lds, int cycles) {
     if (int rc = launch_status<T>((CaseDev<T>*)p->d_case, p->d_state, 1, S->d_status, sm)) return rc;
     PIC_HIP(hipMemcpyAsync(S->h_status, S->d_status, PIC_MAX_SPECIES * sizeof(int), hipMemcpyDeviceToHost, sm));
     if (int rc = launch_charge(S->charge, nn, sm)) return rc;
+    PIC_HIP(hipMemcpyAsync(S->phi_prev, S->g->phi, nn * sizeof(double), hipMemcpyDeviceToDevice, sm));
     return S->mg->enqueue_fixed(cycles);
 }
 
 static void drop_graph(pic_sim* S) {
     if (S->gexec) (void)hipGraphExecDestroy(S->gexec);
     S->gexec = nullptr;
+}
+
+void sim_drop_graph(pic_sim* S) { drop_graph(S); }
+
+int sim_set_mg(pic_sim* S, const MGParams& prm) {
+    PIC_HIP(hipStreamSynchronize(lib_stream()));
+    std::unique_ptr<MGSolver> mg(new MGSolver(S->g->m, prm, lib_stream()));
+    if (int rc = mg->init()) return rc;
+    drop_graph(S);
+    S->mg = std::move(mg);
+    S->mgp = prm;
+    S->last_cycles = 0;
+    return 0;
+}
+
+static int level_params(MGParams& p, int n, const int* pre, const int* post) {
+    PIC_REQUIRE(n >= 0 && n <= 16 && (n == 0 || (pre && post)), "level sweeps: bad arguments");
+    p.per_level = 1;
+    for (int l = 0; l < 16; ++l) {
+        p.pre_l[l] = l < n ? pre[l] : p.pre;
+        p.post_l[l] = l < n ? post[l] : p.post;
+        PIC_REQUIRE(p.pre_l[l] >= 0 && p.pre_l[l] <= 3 && p.post_l[l] >= 0 && p.post_l[l] <= 3,
+                    "level sweeps must be in [0,3]");
+    }
+    return 0;
+}
+
+extern "C" int pic_sim_set_level_sweeps(pic_sim* S, int n, const int* pre, const int* post) {
+    PIC_REQUIRE(S, "sim is NULL");
+    PIC_REQUIRE(!S->ens, "the simulation is in an ensemble");
+    MGParams p = S->mgp;
+    if (int rc = level_params(p, n, pre, post)) return rc;
+    return sim_set_mg(S, p);
+}
+
+extern "C" int pic_mg_set_level_sweeps(pic_mg* m, int n, const int* pre, const int* post) {
+    PIC_REQUIRE(m, "solver is NULL");
+    MGParams p = m->solver->params();
+    if (int rc = level_params(p, n, pre, post)) return rc;
+    PIC_HIP(hipStreamSynchronize(lib_stream()));
+    std::unique_ptr<MGSolver> s(new MGSolver(m->solver->n(), p, lib_stream()));
+    if (int rc = s->init()) return rc;
+    m->solver = std::move(s);
+    return 0;
 }
 
 // One step as a graph launch.  The graph is captured again when the layout
@@ -1205,6 +1251,7 @@ static int step_t(pic_sim* S, int n) {
         if (S->use_graph && S->pending && S->last_cycles > 0) {
             MGStats st{};
             if (int rc = graph_step<T>(S, flags, cur_lds(S), &st)) return rc;
+            S->have_prev = true;
             p->h_cur ^= 1;
             float a = 0;
             (void)hipEventElapsedTime(&a, S->ev[0], S->ev[2]);
@@ -1229,6 +1276,8 @@ static int step_t(pic_sim* S, int n) {
         if (int rc = launch_status<T>((CaseDev<T>*)p->d_case, p->d_state, 1, S->d_status, sm)) return rc;
         PIC_HIP(hipMemcpyAsync(S->h_status, S->d_status, PIC_MAX_SPECIES * sizeof(int), hipMemcpyDeviceToHost, sm));
         if (int rc = launch_charge(S->charge, nn, sm)) return rc;
+        PIC_HIP(hipMemcpyAsync(S->phi_prev, S->g->phi, nn * sizeof(double), hipMemcpyDeviceToDevice, sm));
+        S->have_prev = true;
         MGStats st{};
         if (int rc = S->mg->solve(S->g->phi, S->g->rhs, S->g->dx, &st)) return rc;
         PIC_HIP(hipEventRecord(S->ev[2], sm));
@@ -1357,6 +1406,7 @@ extern "C" int pic_sim_destroy(pic_sim* S) {
     (void)hipFree(S->d_walls_cell);
     (void)hipFree(S->d_items);
     (void)hipFree(S->d_status);
+    (void)hipFree(S->phi_prev);
     if (S->h_status) (void)hipHostFree(S->h_status);
     for (auto& e : S->ev)
         if (e) (void)hipEventDestroy(e);

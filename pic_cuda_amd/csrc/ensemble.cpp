@@ -168,6 +168,7 @@ extern "C" int pic_ensemble_add(pic_ensemble* e, pic_sim* S) {
         PIC_REQUIRE(S->spill_chunk == e->m[0]->spill_chunk, "ensemble members need the same spill chunk");
     }
     e->prec = S->p->prec;
+    S->have_prev = false;   // batched steps do not record the member's pre-solve potential
     e->m.push_back(S);
     S->ens = e;
     e->dirty = true;

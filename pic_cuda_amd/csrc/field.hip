@@ -100,6 +100,20 @@ int launch_efield(const double* phi, double2* E, int nx, int ny, double dx, doub
     return 0;
 }
 
+__global__ __launch_bounds__(256) void zero_interior_kernel(double* __restrict__ f, int n) {
+    const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= (size_t)n * n) return;
+    const int i = (int)(k % n), j = (int)(k / n);
+    if (i > 0 && i < n - 1 && j > 0 && j < n - 1) f[k] = 0.0;
+}
+
+int launch_zero_interior(double* f, int n, hipStream_t s) {
+    const size_t nn = (size_t)n * n;
+    hipLaunchKernelGGL(zero_interior_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, f, n);
+    PIC_HIP(hipGetLastError());
+    return 0;
+}
+
 int launch_deposit(int prec, const void* x, const void* y, int n, int nx, int ny, unsigned long long* acc,
                    hipStream_t s) {
     if (n <= 0) return 0;

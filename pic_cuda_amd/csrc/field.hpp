@@ -21,6 +21,8 @@ int launch_charge(const ChargeArgs& a, size_t nn, hipStream_t s);
 int launch_charge_batch(const ChargeArgs* args, const long long* nns, const int4* chunks, int n_chunks,
                         hipStream_t s);
 int launch_efield(const double* phi, double2* E, int nx, int ny, double dx, double dy, hipStream_t s);
+// zero the interior nodes of an n x n field (the boundary keeps its values)
+int launch_zero_interior(double* f, int n, hipStream_t s);
 int launch_deposit(int prec, const void* x, const void* y, int n, int nx, int ny, unsigned long long* acc,
                    hipStream_t s);
 

@@ -30,6 +30,10 @@ struct MGParams {
     double tol;         // ||r|| / ||rhs||
     int max_cycles;
     int mode;
+    int per_level = 0;  // 1: pre_l/post_l give each level's sweeps (index 0 = finest)
+    int pre_l[16] = {}, post_l[16] = {};
+    int pre_at(int l) const { return per_level ? pre_l[l] : pre; }
+    int post_at(int l) const { return per_level ? post_l[l] : post; }
 };
 
 struct MGStats {
@@ -71,7 +75,8 @@ struct MGCoarseTask {
     const T* e_in;
     const T* rhs;
     T* e_out;
-    int n_top, Lc, pre, post, csweeps, pad;
+    int n_top, Lc, csweeps, pad;
+    signed char pre_l[16], post_l[16];   // sweeps of the kernel's levels (0 = its top)
     T h_top, omega;
     const MGNorm* gate;
 };

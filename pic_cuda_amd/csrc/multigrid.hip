@@ -349,7 +349,7 @@ __global__ __launch_bounds__(1024) void mg_coarse_stage(const StageRef<MGCoarseT
     const gptr<const T> e_in = G(tk.e_in);
     const gptr<const T> rhs_top = G(tk.rhs);
     const gptr<T> e_out = G(tk.e_out);
-    const int n_top = tk.n_top, Lc = tk.Lc, pre = tk.pre, post = tk.post, csweeps = tk.csweeps;
+    const int n_top = tk.n_top, Lc = tk.Lc, csweeps = tk.csweeps;
     const T h_top = tk.h_top, omega = tk.omega;
     extern __shared__ unsigned char smem_mg[];
     T* base = reinterpret_cast<T*>(smem_mg);
@@ -378,7 +378,7 @@ __global__ __launch_bounds__(1024) void mg_coarse_stage(const StageRef<MGCoarseT
     T h = h_top;
     for (int l = 0; l < Lc - 1; ++l) {
         const int nf = s_n[l], ncs = s_n[l + 1];
-        lds_sweeps(E(l), R(l), nf, h, omega, pre);
+        lds_sweeps(E(l), R(l), nf, h, omega, (int)tk.pre_l[l]);
         const T inv_h2 = (T)1 / (h * h);
         const T* e = E(l);
         const T* rl = R(l);
@@ -431,7 +431,7 @@ __global__ __launch_bounds__(1024) void mg_coarse_stage(const StageRef<MGCoarseT
                 el[k] = el[k] + prolong_lds(C, ncs, 0, 0, ncs, i, j);
             }
         __syncthreads();
-        lds_sweeps(el, R(l), nf, h, omega, post);
+        lds_sweeps(el, R(l), nf, h, omega, (int)tk.post_l[l]);
     }
     for (int j = ty; j < n_top; j += 32)
         for (int i = tx; i < n_top; i += 32) e_out[j * n_top + i] = base[j * n_top + i];
@@ -627,6 +627,9 @@ int MGSolver::init() {
     }
     PIC_REQUIRE(p_.pre >= 0 && p_.pre <= 3 && p_.post >= 0 && p_.post <= 3,
                 "multigrid: pre/post sweeps must be in [0,3]");
+    for (int l = 0; l < 16 && p_.per_level; ++l)
+        PIC_REQUIRE(p_.pre_l[l] >= 0 && p_.pre_l[l] <= 3 && p_.post_l[l] >= 0 && p_.post_l[l] <= 3,
+                    "multigrid: per-level sweeps must be in [0,3]");
     PIC_REQUIRE(p_.coarse_sweeps >= 0, "multigrid: coarse sweeps must be non-negative");
     L_ = L;
     lc_ = L - 1;
@@ -818,15 +821,15 @@ int MGBatch::build_cycle(const std::vector<double>& h) {
             t.e_out = d == 0 ? tmp0(k) : E(s, 1, d);
             t.rhs_c = R(s, d + 1);
             t.n = s->ns_[d];
-            t.nu = p_.pre;
-            t.H = 2 * p_.pre + 2;
+            t.nu = p_.pre_at(d);
+            t.H = 2 * t.nu + 2;
             t.h2 = (T)(hh * hh);
             t.inv_h2 = (T)(1.0 / (hh * hh));
             t.omega = omega;
             t.gate = s->d_norm_;
             tasks.push_back(t);
         }
-        if (int rc = add_level_stage<T>(0, tasks, p_.pre, true)) return rc;
+        if (int rc = add_level_stage<T>(0, tasks, p_.pre_at(d), true)) return rc;
     }
     // coarse part: the single-workgroup cycle, or tiled sweeps on a large coarsest level
     {
@@ -842,8 +845,10 @@ int MGBatch::build_cycle(const std::vector<double>& h) {
             t.e_out = lc == 0 ? out0(k) : E(s, 0, lc);
             t.n_top = s->ns_[lc];
             t.Lc = s->L_ - lc;
-            t.pre = p_.pre;
-            t.post = p_.post;
+            for (int l = 0; l < 16; ++l) {
+                t.pre_l[l] = (signed char)(lc + l < 16 ? p_.pre_at(lc + l) : p_.pre);
+                t.post_l[l] = (signed char)(lc + l < 16 ? p_.post_at(lc + l) : p_.post);
+            }
             t.csweeps = s->coarse_sweeps_eff();
             t.h_top = (T)hl(k, lc);
             t.omega = omega;
@@ -933,15 +938,15 @@ int MGBatch::build_cycle(const std::vector<double>& h) {
             t.rhs = d == 0 ? rhs0(k) : R(s, d);
             t.e_out = d == 0 ? out0(k) : E(s, 0, d);
             t.n = s->ns_[d];
-            t.nu = p_.post;
-            t.H = 2 * p_.post;
+            t.nu = p_.post_at(d);
+            t.H = 2 * t.nu;
             t.h2 = (T)(hh * hh);
             t.inv_h2 = (T)(1.0 / (hh * hh));
             t.omega = omega;
             t.gate = s->d_norm_;
             tasks.push_back(t);
         }
-        if (int rc = add_level_stage<T>(1, tasks, p_.post, false)) return rc;
+        if (int rc = add_level_stage<T>(1, tasks, p_.post_at(d), false)) return rc;
     }
     return 0;
 }
@@ -958,8 +963,11 @@ int MGBatch::build(const std::vector<MGSolver*>& members, const std::vector<doub
     for (auto* s : members) {
         PIC_REQUIRE(s->ok_, "multigrid batch: member not initialised");
         const MGParams& q = s->p_;
+        bool same_levels = q.per_level == p_.per_level;
+        for (int l = 0; l < 16 && same_levels && q.per_level; ++l)
+            same_levels = q.pre_l[l] == p_.pre_l[l] && q.post_l[l] == p_.post_l[l];
         PIC_REQUIRE(q.mode == p_.mode && q.pre == p_.pre && q.post == p_.post && q.coarse_sweeps == p_.coarse_sweeps &&
-                        q.omega == p_.omega && q.tol == p_.tol && q.max_cycles == p_.max_cycles,
+                        q.omega == p_.omega && q.tol == p_.tol && q.max_cycles == p_.max_cycles && same_levels,
                     "multigrid batch: members need identical cycle parameters");
     }
     mixed_ = p_.mode == PIC_MG_MIXED;

@@ -123,6 +123,10 @@ struct pic_sim {
     struct pic_ensemble* ens = nullptr;   // batch this simulation belongs to
     // graph mode: the instantiated step and the launch parameters it was
     // captured with (a change of any of them captures it again)
+    // potential before the last step's solve: with the charge, the last
+    // warm-started solve, which the tuner times
+    double* phi_prev = nullptr;
+    bool have_prev = false;
     bool use_graph = false;
     hipGraphExec_t gexec = nullptr;
     const void* g_items = nullptr;
@@ -140,3 +144,6 @@ int upload_items(WorkItem** dptr, int* alloc, const std::vector<WorkItem>& items
 size_t cur_lds(pic_sim* S);
 int mcc_active(pic_sim* S);
 int sim_rebin(pic_sim* S, int bw, int bh);
+void sim_drop_graph(pic_sim* S);
+// replace the simulation's multigrid solver (new parameters)
+int sim_set_mg(pic_sim* S, const pic::MGParams& prm);

@@ -804,6 +804,24 @@ __global__ __launch_bounds__(256) void end_step_kernel(const CaseDev<T>* __restr
     }
 }
 
+// Undo the bin counts a push left in the output parity (trial pushes of the
+// tuner): the state of every case stays what it was before the push.
+template <typename T>
+__global__ __launch_bounds__(256) void clear_out_kernel(const CaseDev<T>* __restrict__ cases,
+                                                        const CaseState* __restrict__ states) {
+    const int c = blockIdx.x;
+    const CaseDev<T>& C = cases[c];
+    const int out = states[c].cur ^ 1;
+    for (int s = 0; s < C.ns; ++s) {
+        const SpeciesDev<T>& S = C.sp[s];
+        const int nb = S.nb[out];
+        for (int k = threadIdx.x; k <= nb; k += blockDim.x) {
+            G(S.count[out])[k] = 0;
+            G(S.mcount[out])[k] = 0;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -865,6 +883,13 @@ int launch_end_step(const CaseDev<T>* cases, CaseState* states, int n_cases, int
 }
 
 template <typename T>
+int launch_clear_out(const CaseDev<T>* cases, const CaseState* states, int n_cases, hipStream_t s) {
+    hipLaunchKernelGGL(clear_out_kernel<T>, dim3(n_cases), dim3(256), 0, s, cases, states);
+    PIC_HIP(hipGetLastError());
+    return 0;
+}
+
+template <typename T>
 int launch_status(const CaseDev<T>* cases, const CaseState* states, int n_cases, int* out, hipStream_t s) {
     hipLaunchKernelGGL(status_kernel<T>, dim3(div_up(n_cases, 64)), dim3(64), 0, s, cases, states, n_cases, out);
     PIC_HIP(hipGetLastError());
@@ -891,7 +916,8 @@ int launch_counts(const CaseDev<T>* cases, const CaseState* states, int c, int s
                                    unsigned long long*, hipStream_t);                                            \
     template int launch_gather<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int, int, int32_t*,       \
                                   hipStream_t);                                                                  \
-    template int launch_end_step<T>(const CaseDev<T>*, CaseState*, int, int, hipStream_t);
+    template int launch_end_step<T>(const CaseDev<T>*, CaseState*, int, int, hipStream_t);                       \
+    template int launch_clear_out<T>(const CaseDev<T>*, const CaseState*, int, hipStream_t);
 PIC_LAUNCH_INST(float)
 PIC_LAUNCH_INST(double)
 

@@ -25,6 +25,9 @@ int launch_gather(const CaseDev<T>* cases, const CaseState* states, const WorkIt
                   int32_t* counter, hipStream_t s);
 template <typename T>
 int launch_end_step(const CaseDev<T>* cases, CaseState* states, int n_cases, int advance, hipStream_t s);
+// clear the bin counts of the output parity (after a trial push)
+template <typename T>
+int launch_clear_out(const CaseDev<T>* cases, const CaseState* states, int n_cases, hipStream_t s);
 
 // dynamic LDS of push_kernel for a bin tile
 template <typename T>
