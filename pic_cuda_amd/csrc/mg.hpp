@@ -76,7 +76,9 @@ struct MGCoarseTask {
     const T* rhs;
     T* e_out;
     int n_top, Lc, csweeps, pad;
-    signed char pre_l[16], post_l[16];   // sweeps of the kernel's levels (0 = its top)
+    // sweeps of the kernel's level l (0 = its top) in bits 2l..2l+1 (packed so
+    // that no array is indexed at run time)
+    unsigned pre_bits, post_bits;
     T h_top, omega;
     const MGNorm* gate;
 };

@@ -378,7 +378,7 @@ __global__ __launch_bounds__(1024) void mg_coarse_stage(const StageRef<MGCoarseT
     T h = h_top;
     for (int l = 0; l < Lc - 1; ++l) {
         const int nf = s_n[l], ncs = s_n[l + 1];
-        lds_sweeps(E(l), R(l), nf, h, omega, (int)tk.pre_l[l]);
+        lds_sweeps(E(l), R(l), nf, h, omega, (int)((tk.pre_bits >> (2 * l)) & 3u));
         const T inv_h2 = (T)1 / (h * h);
         const T* e = E(l);
         const T* rl = R(l);
@@ -431,7 +431,7 @@ __global__ __launch_bounds__(1024) void mg_coarse_stage(const StageRef<MGCoarseT
                 el[k] = el[k] + prolong_lds(C, ncs, 0, 0, ncs, i, j);
             }
         __syncthreads();
-        lds_sweeps(el, R(l), nf, h, omega, (int)tk.post_l[l]);
+        lds_sweeps(el, R(l), nf, h, omega, (int)((tk.post_bits >> (2 * l)) & 3u));
     }
     for (int j = ty; j < n_top; j += 32)
         for (int i = tx; i < n_top; i += 32) e_out[j * n_top + i] = base[j * n_top + i];
@@ -845,9 +845,12 @@ int MGBatch::build_cycle(const std::vector<double>& h) {
             t.e_out = lc == 0 ? out0(k) : E(s, 0, lc);
             t.n_top = s->ns_[lc];
             t.Lc = s->L_ - lc;
+            t.pre_bits = t.post_bits = 0u;
             for (int l = 0; l < 16; ++l) {
-                t.pre_l[l] = (signed char)(lc + l < 16 ? p_.pre_at(lc + l) : p_.pre);
-                t.post_l[l] = (signed char)(lc + l < 16 ? p_.post_at(lc + l) : p_.post);
+                const unsigned a = (unsigned)(lc + l < 16 ? p_.pre_at(lc + l) : p_.pre);
+                const unsigned b = (unsigned)(lc + l < 16 ? p_.post_at(lc + l) : p_.post);
+                t.pre_bits |= (a & 3u) << (2 * l);
+                t.post_bits |= (b & 3u) << (2 * l);
             }
             t.csweeps = s->coarse_sweeps_eff();
             t.h_top = (T)hl(k, lc);
