@@ -209,18 +209,25 @@ typedef struct {
     double ms_graph;         /* device time of the graph steps */
 } pic_sim_report;
 
-/* null-collision constants of a species (inputs of the oracle's MCC) */
+/* null-collision constants of a species (inputs of the oracle's MCC).  The
+   majorant is taken per energy band: band 0 holds E < emax*2^-14, band b in
+   1..14 holds emax*2^(b-15) <= E < emax*2^(b-14), band 15 holds E >= emax/2;
+   a particle is a collision candidate when its first uniform is below the
+   p_null of its band. */
 typedef struct {
     int n_proc;
     int K;
     int type[4];
     double threshold[4];
     double inv_de;
-    double nu_max;
-    double p_null;
+    double nu_max;       /* largest band majorant */
+    double p_null;       /* 1 - exp(-nu_max dt) */
     double efac;
     double gas_vth;
     int product_species;
+    double emax;                 /* table end, eV */
+    double band_nu_max[16];      /* majorant of each band (x1.001) */
+    double band_p_null[16];      /* 1 - exp(-band_nu_max dt) */
 } pic_mcc_info;
 
 int pic_sim_default_params(pic_sim_params* out);

@@ -41,6 +41,7 @@ class PushParams(ctypes.Structure):
         ("inv_de", ctypes.c_double), ("nu_max", ctypes.c_double), ("p_null", ctypes.c_double),
         ("efac", ctypes.c_double), ("gas_vth", ctypes.c_double),
         ("ion_species", ctypes.c_uint32), ("ion_efac_unused", ctypes.c_double),
+        ("inv_emax", ctypes.c_double), ("band_nu_max", ctypes.c_double * 16), ("band_p_null", ctypes.c_double * 16),
     ]
 
 
@@ -278,6 +279,11 @@ def push(sp, phi, nx, ny, dx, dy, dtdx, dtdy, flags, step, seed, case_id, specie
         p.tab = tab.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         p.K, p.inv_de, p.nu_max, p.p_null = m["K"], m["inv_de"], m["nu_max"], m["p_null"]
         p.efac, p.gas_vth = m["efac"], m["gas_vth"]
+        # energy-banded majorant; a dict without bands has one majorant for all
+        p.inv_emax = 1.0 / m["emax"] if m.get("emax") else 1.0
+        for b in range(16):
+            p.band_nu_max[b] = m["band_nu_max"][b] if "band_nu_max" in m else m["nu_max"]
+            p.band_p_null[b] = m["band_p_null"][b] if "band_p_null" in m else m["p_null"]
     n_other = ctypes.c_int64(0)
     k = lib(omp).orc_push(ctypes.byref(p), _p(phi), _p(arrs[0]), _p(arrs[1]), _p(arrs[2]), _p(arrs[3]), _p(idb),
                           n, cap, _p(acc_self), _p(xo), _p(yo), _p(vxo), _p(vyo), _p(ido), born_cap,

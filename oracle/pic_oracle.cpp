@@ -388,6 +388,16 @@ static inline bool absorbed(const orc_push_params* p, T x0, T y0, T x1, T y1) {
     return false;
 }
 
+/* energy band of the null-collision majorant (same operations as the kernels) */
+template <typename T>
+static inline int mcc_band(T E, T inv_emax) {
+    const T r = E * inv_emax;
+    if (!(r >= (T)0x1p-14)) return 0;
+    int e = 0;
+    (void)std::frexp(r, &e);
+    return std::min(14 + e, 15);
+}
+
 /* One particle of a pass: kick, collisions, drift, walls, deposit.  emit(kind,
    particle) receives the survivor (kind 0), a born particle of the same species
    (1) and a born particle of the other species (2), in that order. */
@@ -396,7 +406,7 @@ static inline void push_one(const orc_push_params* p, const double* phi, const T
                             uint64_t* acc_self, uint64_t* acc_other, uint64_t* cnt, Emit&& emit) {
     const int nx = p->nx, ny = p->ny;
     const T kq = (T)p->kq, dtdx = (T)p->dtdx, dtdy = (T)p->dtdy;
-    const T efac = (T)p->efac, inv_de = (T)p->inv_de, nu_max = (T)p->nu_max, p_null = (T)p->p_null;
+    const T efac = (T)p->efac, inv_de = (T)p->inv_de;
     const int K = p->K;
     T x = q.x, y = q.y, vx = q.vx, vy = q.vy;
     const uint32_t id = q.id;
@@ -423,10 +433,12 @@ static inline void push_one(const orc_push_params* p, const double* phi, const T
     if (p->do_mcc && p->n_proc > 0) {
         Stream rs(id, p->step, p->case_id, p->seed, p->species);
         T u0 = rs.uni<T>(), u1 = rs.uni<T>();
-        if (u0 < p_null) {
+        T v2 = std::fma(vx, vx, vy * vy);
+        T E = efac * v2;
+        const int band = mcc_band(E, (T)p->inv_emax);
+        if (u0 < (T)p->band_p_null[band]) {
+            const T nu_max = (T)p->band_nu_max[band];
             cnt[ORC_C_NULL_CAND]++;
-            T v2 = std::fma(vx, vx, vy * vy);
-            T E = efac * v2;
             T v = std::sqrt(v2);
             T s = E * inv_de;
             int j;

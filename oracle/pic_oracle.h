@@ -112,6 +112,11 @@ typedef struct {
     /* products of ionization */
     uint32_t ion_species;
     double ion_efac_unused;
+    /* energy-banded majorant: band 0 below emax*2^-14, b = 15 + floor(log2(E/emax))
+       above, 15 from emax/2 on; a candidate has u0 < band_p_null[band] */
+    double inv_emax;
+    double band_nu_max[16];
+    double band_p_null[16];
 } orc_push_params;
 
 enum {

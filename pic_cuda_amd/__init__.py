@@ -85,7 +85,8 @@ class MccInfo(ctypes.Structure):
     _fields_ = [("n_proc", ctypes.c_int), ("K", ctypes.c_int), ("type", ctypes.c_int * 4),
                 ("threshold", ctypes.c_double * 4), ("inv_de", ctypes.c_double), ("nu_max", ctypes.c_double),
                 ("p_null", ctypes.c_double), ("efac", ctypes.c_double), ("gas_vth", ctypes.c_double),
-                ("product_species", ctypes.c_int)]
+                ("product_species", ctypes.c_int), ("emax", ctypes.c_double), ("band_nu_max", ctypes.c_double * 16),
+                ("band_p_null", ctypes.c_double * 16)]
 
 
 class JobDesc(ctypes.Structure):
@@ -572,7 +573,8 @@ class Simulation:
         return dict(n_proc=info.n_proc, K=info.K, type=list(info.type)[:info.n_proc],
                     threshold=list(info.threshold)[:info.n_proc], inv_de=info.inv_de, nu_max=info.nu_max,
                     p_null=info.p_null, efac=info.efac, gas_vth=info.gas_vth,
-                    product_species=info.product_species, table=tab[:info.n_proc * info.K])
+                    product_species=info.product_species, table=tab[:info.n_proc * info.K], emax=info.emax,
+                    band_nu_max=list(info.band_nu_max), band_p_null=list(info.band_p_null))
 
     def rebin(self, bin_w=0, bin_h=0):
         _check(load().pic_sim_rebin(self._h, int(bin_w), int(bin_h)))

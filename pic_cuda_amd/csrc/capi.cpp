@@ -761,8 +761,9 @@ struct MccBuild {
     std::vector<double> tab;   // n_proc * K of n_gas * sigma
 };
 
-// null-collision constants of one species: uniform table, nu_max over the
-// table (16 samples per interval of the interpolant), p_null = 1-exp(-nu_max dt)
+// null-collision constants of one species: uniform table, and the majorant of
+// each energy band (16 samples per table interval of the interpolant inside
+// the band, and the band edges, x1.001), p_null = 1-exp(-nu_max dt)
 static int build_mcc(const pic_xs* xs, const std::vector<SpeciesStore>& sp, int s, double dt, MccBuild& out) {
     std::vector<const pic_xs::Proc*> ps;
     for (auto& p : xs->procs)
@@ -792,22 +793,43 @@ static int build_mcc(const pic_xs* xs, const std::vector<SpeciesStore>& sp, int 
     out.tab.assign((size_t)I.n_proc * K, 0.0);
     for (int q = 0; q < I.n_proc; ++q)
         for (int j = 0; j < K; ++j) out.tab[(size_t)q * K + j] = xs->density * proc_sigma(*ps[q], j * de);
-    double numax = 0.0;
     const double m = sp[s].m;
-    for (int j = 0; j < K - 1; ++j)
-        for (int k = 0; k <= 16; ++k) {
-            const double t = k / 16.0;
-            const double E = (j + t) * de;
-            const double v = std::sqrt(2.0 * E * kElementaryCharge / m);
-            double nu = 0.0;
-            for (int q = 0; q < I.n_proc; ++q) {
-                if (E < I.threshold[q]) continue;
-                const double a = out.tab[(size_t)q * K + j], b = out.tab[(size_t)q * K + j + 1];
-                nu += (a + t * (b - a)) * v;
-            }
-            numax = std::max(numax, nu);
+    // collision frequency of the interpolated tables at energy E
+    auto nu_at = [&](double E) {
+        double x = E / de;
+        int j = (int)std::floor(x);
+        double t = x - j;
+        if (j >= K - 1) {
+            j = K - 2;
+            t = 1.0;
         }
-    I.nu_max = numax * 1.001;
+        const double v = std::sqrt(2.0 * E * kElementaryCharge / m);
+        double nu = 0.0;
+        for (int q = 0; q < I.n_proc; ++q) {
+            if (E < I.threshold[q]) continue;
+            const double a = out.tab[(size_t)q * K + j], b = out.tab[(size_t)q * K + j + 1];
+            nu += (a + t * (b - a)) * v;
+        }
+        return nu;
+    };
+    const double emax = xs->emax;
+    I.emax = emax;
+    double numax = 0.0;
+    for (int b = 0; b < 16; ++b) {
+        const double lo = b == 0 ? 0.0 : std::ldexp(emax, b - 15);
+        const double hi = b == 15 ? emax : std::ldexp(emax, b - 14);
+        double mx = std::max(nu_at(lo), nu_at(hi));
+        const int j0 = std::max(0, (int)std::floor(lo / de)), j1 = std::min(K - 2, (int)std::floor(hi / de));
+        for (int j = j0; j <= j1; ++j)
+            for (int k = 0; k <= 16; ++k) {
+                const double E = (j + k / 16.0) * de;
+                if (E >= lo && E <= hi) mx = std::max(mx, nu_at(E));
+            }
+        I.band_nu_max[b] = mx * 1.001;
+        I.band_p_null[b] = -std::expm1(-I.band_nu_max[b] * dt);
+        numax = std::max(numax, I.band_nu_max[b]);
+    }
+    I.nu_max = numax;
     I.p_null = -std::expm1(-I.nu_max * dt);
     return 0;
 }
@@ -1040,6 +1062,11 @@ static int setup_t(pic_sim* S) {
                 D.inv_de = (T)mb.info.inv_de;
                 D.nu_max = (T)mb.info.nu_max;
                 D.p_null = (T)mb.info.p_null;
+                D.inv_emax = (T)(1.0 / mb.info.emax);
+                for (int b = 0; b < 16; ++b) {
+                    D.band_numax[b] = (T)mb.info.band_nu_max[b];
+                    D.band_pnull[b] = (T)mb.info.band_p_null[b];
+                }
                 D.gas_vth = mb.info.gas_vth;
                 D.other = mb.info.product_species;
             }
@@ -1456,6 +1483,11 @@ static int elastic_t(pic_particles* p, int species, double density, double dt, i
     D.inv_de = (T)mb.info.inv_de;
     D.nu_max = (T)mb.info.nu_max;
     D.p_null = (T)mb.info.p_null;
+    D.inv_emax = (T)(1.0 / mb.info.emax);
+    for (int b = 0; b < 16; ++b) {
+        D.band_numax[b] = (T)mb.info.band_nu_max[b];
+        D.band_pnull[b] = (T)mb.info.band_p_null[b];
+    }
     D.efac = (T)mb.info.efac;
     D.other = -1;
     C.seed = seed;

@@ -27,6 +27,9 @@ struct SpeciesDev {
     T p_null;                  // 1 - exp(-nu_max dt)
     T nu_max;
     T inv_de;                  // 1 / table energy step
+    T inv_emax;                // 1 / table end: energy bands of the majorant
+    T band_pnull[16];          // candidate probability per energy band
+    T band_numax[16];          // majorant per energy band
     int K;                     // table points
     int n_proc;
     int ptype[PIC_MAX_PROC];

@@ -204,17 +204,35 @@ struct Born {
     uint32_t eid, iid;
 };
 
-// Null-collision MCC for one particle (replaces elasticCollisions,
-// mcc.cu:114-150, and its kernels kernels.cu:196-263, mcc.cu:56-85).
+__device__ inline float dfrexp(float v, int* e) { return frexpf(v, e); }
+__device__ inline double dfrexp(double v, int* e) { return frexp(v, e); }
+
+// energy band of the null-collision majorant: 0 below emax*2^-14 (and NaN),
+// b = 15 + floor(log2(E/emax)) above, 15 from emax/2 on
 template <typename T>
-__device__ inline bool mcc_collide(const SpeciesDev<T>& S, uint32_t id, uint32_t step, uint32_t case_id,
-                                   uint32_t seed, T x, T y, T& vx, T& vy, unsigned* lcnt, Born<T>& b) {
+__device__ inline int mcc_band(T E, T inv_emax) {
+    const T r = E * inv_emax;
+    if (!(r >= (T)0x1p-14)) return 0;
+    int e = 0;
+    (void)dfrexp(r, &e);
+    return min(14 + e, 15);
+}
+
+// Null-collision MCC for one particle (replaces elasticCollisions,
+// mcc.cu:114-150, and its kernels kernels.cu:196-263, mcc.cu:56-85).  The
+// majorant is that of the particle's energy band (band values in LDS).
+template <typename T>
+__device__ inline bool mcc_collide(const SpeciesDev<T>& S, const T* band_lds, uint32_t id, uint32_t step,
+                                   uint32_t case_id, uint32_t seed, T x, T y, T& vx, T& vy, unsigned* lcnt,
+                                   Born<T>& b) {
     Stream rs(id, step, case_id, seed, S.key);
     const T u0 = rs.uni<T>(), u1 = rs.uni<T>();
-    if (!(u0 < S.p_null)) return false;
-    atomicAdd(lcnt + PIC_C_NULL_CAND, 1u);
     const T v2 = dfma(vx, vx, vy * vy);
     const T E = S.efac * v2;
+    const int band = mcc_band(E, S.inv_emax);
+    if (!(u0 < band_lds[band])) return false;
+    atomicAdd(lcnt + PIC_C_NULL_CAND, 1u);
+    const T nu_max = band_lds[16 + band];
     const T v = dsqrt(v2);
     const T s = E * S.inv_de;
     const int K = S.K;
@@ -222,7 +240,7 @@ __device__ inline bool mcc_collide(const SpeciesDev<T>& S, uint32_t id, uint32_t
     T t;
     if (s < (T)(K - 1)) { j = (int)s; t = s - (T)j; }   // false for NaN as well
     else { j = K - 2; t = (T)1; }
-    const T R = u1 * S.nu_max;
+    const T R = u1 * nu_max;
     T cum = (T)0;
     int sel = -1;
     for (int q = 0; q < S.n_proc; ++q) {
@@ -232,7 +250,7 @@ __device__ inline bool mcc_collide(const SpeciesDev<T>& S, uint32_t id, uint32_t
         cum = dfma(sig, v, cum);
         if (sel < 0 && R < cum) sel = q;
     }
-    if (cum > S.nu_max) atomicAdd(lcnt + PIC_C_NUMAX_EXCEEDED, 1u);
+    if (cum > nu_max) atomicAdd(lcnt + PIC_C_NUMAX_EXCEEDED, 1u);
     if (sel < 0) return false;
     atomicAdd(lcnt + PIC_C_PROC0 + sel, 1u);
     const int ty = S.ptype[sel];
@@ -315,10 +333,21 @@ struct Stage {
     int* key;
 };
 
+// LDS accesses of one wavefront complete in order; the fences keep the
+// compiler from moving LDS accesses across this point
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// write a wavefront's staged particles to their destination bins (the whole
+// wavefront calls it; nm is uniform)
 template <typename T>
 __device__ inline void flush_stage(const SpeciesDev<T>& S, int out, const Stage<T>& sg, int nm, unsigned* lcnt) {
-    const int wave = threadIdx.x >> 6, lane = __lane_id();
-    for (int base = wave * 64; base < nm; base += blockDim.x) {
+    const int lane = __lane_id();
+    wave_lds_sync();
+    for (int base = 0; base < nm; base += 64) {
         const int e = base + lane;
         const bool want = e < nm;
         const int key = want ? sg.key[e] : 0;
@@ -336,18 +365,17 @@ __device__ inline void flush_stage(const SpeciesDev<T>& S, int out, const Stage<
             }
         }
     }
+    wave_lds_sync();
 }
 
-// append the lanes with `want` to the stage (wave-aggregated LDS atomic)
+// append the lanes with `want` to the wavefront's stage; nm is the
+// wavefront's (uniform) fill
 template <typename T>
-__device__ inline void stage_put(const Stage<T>& sg, int* nm, bool want, T x, T y, T vx, T vy, uint32_t id, int key) {
+__device__ inline void stage_put(const Stage<T>& sg, int& nm, bool want, T x, T y, T vx, T vy, uint32_t id, int key) {
     const uint64_t m = __ballot(want);
     if (!m) return;
-    const int lane = __lane_id();
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(nm, __popcll(m));
-    base = __shfl(base, leader);
+    const int base = nm;
+    nm += __popcll(m);
     if (want) {
         const int e = base + __popcll(m & lanemask_lt());
         sg.x[e] = x;
@@ -369,7 +397,7 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
                                                    int flags_arg, long long mcc_key) {
     using T2 = typename Vec2<T>::type;
     extern __shared__ unsigned long long smem_u64[];
-    __shared__ int sh_stay, sh_nm;
+    __shared__ int sh_stay;
     const WorkItem it = items[blockIdx.x];
     const CaseDev<T>& C = cases[it.case_idx];
     const CaseState st = states[it.case_idx];
@@ -388,18 +416,25 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
     const int ew = C.bw + 1, eh = C.bh + 1;                  // E window nodes
     const int aw = C.bw + 1 + 2 * H, ah = C.bh + 1 + 2 * H;  // deposit window nodes
     const int wx0 = cx0 - H, wy0 = cy0 - H;
-    // LDS: deposit window (u64), E window (T2), stage (4 T + id + key), counters
+    // LDS: deposit window (u64), E window (T2), stage (4 T + id + key), counters,
+    // energy-band constants of the collisions
     unsigned long long* accw = smem_u64;
     T2* ew_lds = reinterpret_cast<T2*>(smem_u64 + (is_bin ? aw * ah : 0));
     T* stage_base = reinterpret_cast<T*>(ew_lds + (is_bin ? ew * eh : 0));
+    // the stage is split into one part per wavefront (PIC_MSTAGE / 4 entries)
+    constexpr int WST = PIC_MSTAGE / 4;
+    const int wv = threadIdx.x >> 6;
     Stage<T> sg;
-    sg.x = stage_base;
-    sg.y = sg.x + PIC_MSTAGE;
-    sg.vx = sg.y + PIC_MSTAGE;
-    sg.vy = sg.vx + PIC_MSTAGE;
-    sg.id = reinterpret_cast<uint32_t*>(sg.vy + PIC_MSTAGE);
-    sg.key = reinterpret_cast<int*>(sg.id + PIC_MSTAGE);
-    unsigned* lcnt = reinterpret_cast<unsigned*>(sg.key + PIC_MSTAGE);
+    sg.x = stage_base + wv * WST;
+    sg.y = stage_base + PIC_MSTAGE + wv * WST;
+    sg.vx = stage_base + 2 * PIC_MSTAGE + wv * WST;
+    sg.vy = stage_base + 3 * PIC_MSTAGE + wv * WST;
+    sg.id = reinterpret_cast<uint32_t*>(stage_base + 4 * PIC_MSTAGE) + wv * WST;
+    sg.key = reinterpret_cast<int*>(stage_base + 4 * PIC_MSTAGE) + PIC_MSTAGE + wv * WST;
+    unsigned* lcnt = reinterpret_cast<unsigned*>(reinterpret_cast<int*>(stage_base + 4 * PIC_MSTAGE) + 2 * PIC_MSTAGE);
+    int nmw = 0;   // this wavefront's stage fill
+    // candidate probability and majorant of each energy band (32 values)
+    T* band_lds = reinterpret_cast<T*>(lcnt + PIC_NCOUNT);
     const int flags = flags_arg >= 0 ? flags_arg : st.flags;
     const bool kick = flags & PIC_F_KICK, mcc = (flags & PIC_F_MCC) && S.n_proc > 0, drift = flags & PIC_F_DRIFT,
                dep = flags & PIC_F_DEPOSIT;
@@ -408,10 +443,8 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
     const uint32_t mcc_step = mcc_key >= 0 ? (uint32_t)mcc_key : st.step - 1u;
 
     for (int k = threadIdx.x; k < PIC_NCOUNT; k += blockDim.x) lcnt[k] = 0u;
-    if (threadIdx.x == 0) {
-        sh_stay = 0;
-        sh_nm = 0;
-    }
+    if (threadIdx.x < 32) band_lds[threadIdx.x] = threadIdx.x < 16 ? S.band_pnull[threadIdx.x] : S.band_numax[threadIdx.x - 16];
+    if (threadIdx.x == 0) sh_stay = 0;
     if (is_bin) {
         for (int k = threadIdx.x; k < aw * ah; k += blockDim.x) accw[k] = 0ull;
         if (kick) {
@@ -484,7 +517,7 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
                     vy = dfma(kq, Ey, vy);
                 }
             }
-            if (mcc) born = mcc_collide(S, id, mcc_step, C.case_id, C.seed, x, y, vx, vy, lcnt, b);
+            if (mcc) born = mcc_collide(S, band_lds, id, mcc_step, C.case_id, C.seed, x, y, vx, vy, lcnt, b);
             const T x0 = x, y0 = y;
             if (drift) {
                 x = dfma(vx, dtdx, x);
@@ -546,8 +579,8 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
             }
         }
         // movers and born particles of this species: staged
-        stage_put(sg, &sh_nm, alive && !stay, x, y, vx, vy, id, key);
-        stage_put(sg, &sh_nm, be_alive, b.ex, b.ey, b.evx, b.evy, b.eid, key_e);
+        stage_put(sg, nmw, alive && !stay, x, y, vx, vy, id, key);
+        stage_put(sg, nmw, be_alive, b.ex, b.ey, b.evx, b.evy, b.eid, key_e);
         // born ions of the other species (rare): written directly
         if (O && __ballot(bi_alive)) {
             const int di = alloc_slot(*O, out, bi_alive, key_i, true);
@@ -563,19 +596,17 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
                 }
             }
         }
-        __syncthreads();
-        const int nm = sh_nm;
-        if (nm > PIC_MSTAGE - 512) {
-            flush_stage(S, out, sg, nm, lcnt);
-            __syncthreads();
-            if (threadIdx.x == 0) sh_nm = 0;
-            __syncthreads();
+        // a pass adds at most 128 entries per wavefront (a mover and a born
+        // electron per lane)
+        if (nmw > WST - 128) {
+            flush_stage(S, out, sg, nmw, lcnt);
+            nmw = 0;
         }
     }
-    const int nm = sh_nm;
-    if (nm > 0) flush_stage(S, out, sg, nm, lcnt);
-    if (is_bin && threadIdx.x == 0) G(S.count[out])[it.bin] = min(sh_stay, home_front);
+    if (nmw > 0) flush_stage(S, out, sg, nmw, lcnt);
+    // every wavefront's stay count, deposits and counters are in LDS now
     __syncthreads();
+    if (is_bin && threadIdx.x == 0) G(S.count[out])[it.bin] = min(sh_stay, home_front);
     if (is_bin) {
         for (int k = threadIdx.x; k < aw * ah; k += blockDim.x) {
             const unsigned long long v = accw[k];

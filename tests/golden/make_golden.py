@@ -1,12 +1,12 @@
 """Generate tests/golden/tiny_step_*.npz: a tiny full PIC run of the oracle.
 
 Case: 17x17 nodes, 300 e- + 300 Ar+ from the seeded initial state, Te = 8 eV,
-box walls, null-collision MCC with tables computed here in numpy from the
-analytic shapes (elastic 1e-19, excitation 11.5 eV / 2e-21, ionization
-15.8 eV / 3e-20 Lotz-like, Ar+ charge exchange 5e-19, gas 1e22 m^-3), FP64
-multigrid 2+2 to 1e-8, 8 steps and the final half kick.  Stored: the MCC
-inputs, the per-species final particles sorted by (id, x, y, vx, vy), the
-counters, the last charge rhs and phi.
+box walls, null-collision MCC with tables and energy-band majorants computed
+here in numpy from the analytic shapes (elastic 1e-19, excitation 11.5 eV /
+2e-21, ionization 15.8 eV / 3e-20 Lotz-like, Ar+ charge exchange 5e-19, gas
+1e22 m^-3), FP64 multigrid 2+2 to 1e-8, 8 steps and the final half kick.
+Stored: the MCC inputs, the per-species final particles sorted by (id, x, y,
+vx, vy), the counters, the last charge rhs and phi.
 
 The fixture pins the oracle against later edits (tests/test_golden.py) and
 feeds the GPU test that runs the same inputs through the HIP path.
@@ -45,20 +45,38 @@ def case_inputs(prec):
         return s
 
     def mcc(m, tabs, types, thr, partner_mass, product):
+        # majorant of the interpolated collision frequency per energy band:
+        # band 0 below emax*2^-14, band b in [emax*2^(b-15), emax*2^(b-14)),
+        # band 15 up to emax; sampled 16 times per table interval and at the
+        # band edges, x1.001
         tab = np.stack(tabs) * gas
         efac = 0.5 * m / QE
-        numax = 0.0
-        for j in range(K - 1):
-            for k in range(17):
-                t = k / 16.0
-                e = (j + t) * de
-                v = math.sqrt(2 * e * QE / m)
-                nu = sum((tab[q, j] + t * (tab[q, j + 1] - tab[q, j])) * v for q in range(len(types)) if e >= thr[q])
-                numax = max(numax, nu)
-        numax *= 1.001
+
+        def nu(e):
+            x = e / de
+            j = int(math.floor(x))
+            t = x - j
+            if j >= K - 1:
+                j, t = K - 2, 1.0
+            v = math.sqrt(2 * e * QE / m)
+            return sum((tab[q, j] + t * (tab[q, j + 1] - tab[q, j])) * v for q in range(len(types)) if e >= thr[q])
+
+        band_nu = []
+        for b in range(16):
+            lo = 0.0 if b == 0 else math.ldexp(emax, b - 15)
+            hi = emax if b == 15 else math.ldexp(emax, b - 14)
+            mx = max(nu(lo), nu(hi))
+            for j in range(max(0, int(math.floor(lo / de))), min(K - 2, int(math.floor(hi / de))) + 1):
+                for k in range(17):
+                    e = (j + k / 16.0) * de
+                    if lo <= e <= hi:
+                        mx = max(mx, nu(e))
+            band_nu.append(mx * 1.001)
+        numax = max(band_nu)
         return dict(n_proc=len(types), type=list(types), threshold=list(thr), table=tab.ravel(), K=K,
                     inv_de=1.0 / de, nu_max=numax, p_null=-math.expm1(-numax * dt), efac=efac,
-                    gas_vth=math.sqrt(tgas * QE / partner_mass), product_species=product)
+                    gas_vth=math.sqrt(tgas * QE / partner_mass), product_species=product, emax=emax,
+                    band_nu_max=band_nu, band_p_null=[-math.expm1(-v * dt) for v in band_nu])
 
     m_e = mcc(ME, [np.full(K, 1e-19), lotz(2e-21, 11.5), lotz(3e-20, 15.8)], (0, 1, 2), (0.0, 11.5, 15.8), M_AR, 1)
     m_i = mcc(M_AR, [np.full(K, 5e-19)], (3,), (0.0,), M_AR, -1)
@@ -96,6 +114,8 @@ def pack(o, rhs, inp):
         out["s%d_table" % s] = inp["mcc"][s]["table"]
         for k in ("inv_de", "nu_max", "p_null", "efac", "gas_vth"):
             out["s%d_%s" % (s, k)] = np.float64(inp["mcc"][s][k])
+        for k in ("band_nu_max", "band_p_null"):
+            out["s%d_%s" % (s, k)] = np.asarray(inp["mcc"][s][k], dtype=np.float64)
     out["rhs"] = rhs
     out["phi"] = o.phi
     return out

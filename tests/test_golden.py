@@ -73,6 +73,8 @@ def test_gpu_reproduces_golden(pic, prec, bin_w):
         info = sim.mcc_info(s)
         assert info["nu_max"] == float(ref["s%d_nu_max" % s])
         assert info["p_null"] == float(ref["s%d_p_null" % s])
+        assert np.array_equal(np.array(info["band_nu_max"]), ref["s%d_band_nu_max" % s])
+        assert np.array_equal(np.array(info["band_p_null"]), ref["s%d_band_p_null" % s])
     sim.step(8)
     assert np.array_equal(grid.download(pic.FIELD_RHS), ref["rhs"])
     assert np.array_equal(grid.download(pic.FIELD_PHI), ref["phi"])

@@ -483,3 +483,80 @@ def test_openmp_oracle_pass_equals_serial(orc):
     assert np.array_equal(a.counters, b.counters) and a.counters[orc.C_PROC0 + 1] > 0
     for u, w in zip(born_a, born_b):
         assert np.array_equal(u, w)
+
+
+def _bands_const(m, n_gas, sigma, emax, dt):
+    """band majorants of a constant cross section: n sigma v at each band's top"""
+    QE = 1.602176634e-19
+    nu = []
+    for b in range(16):
+        hi = emax if b == 15 else math.ldexp(emax, b - 14)
+        nu.append(n_gas * sigma * math.sqrt(2 * hi * QE / m) * 1.001)
+    return nu, [-math.expm1(-v * dt) for v in nu]
+
+
+def test_mcc_energy_band_edges(orc):
+    """Band b holds [emax 2^(b-15), emax 2^(b-14)); band 0 everything below
+    emax 2^-14, band 15 everything from emax/2 on.  With only band k made a
+    certain candidate (and a majorant no real collision reaches), the
+    null-candidate counter says which band a particle fell in."""
+    QE = 1.602176634e-19
+    m_unit = QE                      # efac = 1/2: E = v^2 / 2 exactly for v = 2^a
+    emax = 256.0
+    m = _mcc_const(orc, 1.0, 1e-30, m_unit, 1e-9, K=257, emax=emax)
+    m["emax"] = emax
+
+    def band_of(vx):
+        found = []
+        for k in range(16):
+            mk = dict(m)
+            mk["band_p_null"] = [1.0 if b == k else 0.0 for b in range(16)]
+            mk["band_nu_max"] = [1e300] * 16
+            sp = orc.Species(8, np.array([8.0]), np.array([8.0]), np.array([vx]), np.array([0.0]),
+                             np.array([7], dtype=np.uint32), 0.0, mcc=mk)
+            orc.push(sp, np.zeros((17, 17)), 17, 17, 1.0, 1.0, 0.0, 0.0, dict(mcc=True), 0, 1, 0, 0)
+            if sp.counters[orc.C_NULL_CAND]:
+                found.append(k)
+        assert len(found) == 1
+        return found[0]
+
+    for a in range(-4, 6):
+        e = 0.5 * 4.0 ** a           # energy of vx = 2^a
+        r = e / emax
+        want = 0 if r < 2.0 ** -14 else min(15, 15 + int(math.floor(math.log2(r))))
+        assert band_of(2.0 ** a) == want
+        # just below a band edge: the band underneath (r is a power of two)
+        below = np.nextafter(2.0 ** a, 0.0)
+        if r <= 2.0 ** -14:
+            want_below = 0
+        elif r > 0.5:
+            want_below = 15
+        else:
+            want_below = 14 + int(math.log2(r))
+        assert band_of(below) == want_below
+    assert band_of(0.0) == 0
+    assert band_of(float(np.sqrt(2 * 1000.0))) == 15     # above the table end
+
+
+def test_mcc_banded_rate(orc):
+    """10 eV electrons, constant elastic sigma: collisions occur at
+    p_null(band) * nu(E) / nu_max(band), the null-collision rate of the
+    particle's own band."""
+    QE, ME = 1.602176634e-19, 9.1093837015e-31
+    n, dt, emax = 200000, 2e-9, 100.0
+    m = _mcc_const(orc, 1e21, 1e-19, ME, dt, emax=emax)
+    nu_b, p_b = _bands_const(ME, 1e21, 1e-19, emax, dt)
+    m.update(emax=emax, band_nu_max=nu_b, band_p_null=p_b)
+    e0 = 10.0
+    v = math.sqrt(2 * e0 * QE / ME)
+    sp = orc.Species(8, np.full(n, 8.0), np.full(n, 8.0), np.full(n, v), np.zeros(n),
+                     np.arange(n, dtype=np.uint32), 0.0, mcc=m)
+    orc.push(sp, np.zeros((17, 17)), 17, 17, 1.0, 1.0, 0.0, 0.0, dict(mcc=True), 3, 9, 0, 0)
+    b = 15 + int(math.floor(math.log2(e0 / emax)))
+    cand = int(sp.counters[orc.C_NULL_CAND])
+    assert abs(cand - n * p_b[b]) < 5 * math.sqrt(n * p_b[b])
+    hits = int(sp.counters[orc.C_PROC0])
+    p = p_b[b] * (1e21 * 1e-19 * v) / nu_b[b]
+    assert abs(hits - n * p) < 5 * math.sqrt(n * p * (1 - p))
+    # the band's majorant is that of its top energy, 12.5 eV, not of 100 eV
+    assert abs(nu_b[b] / m["nu_max"] - math.sqrt(12.5 / 100.0)) < 0.01
