@@ -275,6 +275,17 @@ int pic_sim_autotune(pic_sim* s, int what, pic_tune_report* out);
 int pic_sim_set_level_sweeps(pic_sim* s, int n, const int* pre, const int* post);
 int pic_mg_set_level_sweeps(pic_mg* m, int n, const int* pre, const int* post);
 
+/* ---- embedded electrodes (Multigrid_embedded.cu:7-190) -------------------
+   mask: n*n host bytes, 0 FLUID (solved), 1 DIRICHLET (held at phi_bc),
+   2 SOLID (held at its value); phi_bc: n*n host doubles (may be NULL without
+   Dirichlet nodes).  A NULL mask removes the electrodes.  The mask of a coarse
+   level is that of the fine node it sits on; fixed nodes are not smoothed,
+   have zero residual and take no correction.  The Dirichlet values are
+   written into phi at the start of every solve.  Particles are not absorbed by
+   electrodes: give them wall segments (pic_walls_add) for that. */
+int pic_mg_set_electrodes(pic_mg* m, const uint8_t* mask, const double* phi_bc);
+int pic_sim_set_electrodes(pic_sim* s, const uint8_t* mask, const double* phi_bc);
+
 /* ---- ensembles of independent plasmas (no counterpart in the reference: it
    steps one plasma).  Members step together, one launch per stage for all of
    them; each member computes exactly what it computes alone. ------------- */

@@ -78,6 +78,8 @@ def _load(omp=False):
     lib.orc_charge_to_rhs.argtypes = [vp, vp, i, i64, vp]
     lib.orc_mg_solve.restype = i
     lib.orc_mg_solve.argtypes = [vp, vp, i, i, d, ctypes.POINTER(MGParams), ctypes.POINTER(MGStats)]
+    lib.orc_mg_solve_masked.restype = i
+    lib.orc_mg_solve_masked.argtypes = [vp, vp, i, i, d, ctypes.POINTER(MGParams), vp, vp, ctypes.POINTER(MGStats)]
     lib.orc_mg_smooth.argtypes = [vp, vp, i, d, d, i]
     lib.orc_mg_residual.argtypes = [vp, vp, vp, i, d]
     lib.orc_mg_restrict.argtypes = [vp, vp, i]
@@ -147,11 +149,12 @@ def charge_to_rhs(accs, cfacs):
 
 
 def mg_solve(phi, rhs, h, levels=0, pre=2, post=2, coarse_sweeps=1, omega=1.0, tol=1e-8, max_cycles=100, omp=False,
-             pre_levels=None, post_levels=None):
+             pre_levels=None, post_levels=None, mask=None, phi_bc=None):
     """solve_multigrid restated (Multigrid.cu:573-642); phi updated in place.
     omp: the OpenMP build (colour sweeps split over threads; same values).
     pre_levels/post_levels: sweeps of each level above the coarsest (finest
-    first) instead of pre/post."""
+    first) instead of pre/post.  mask/phi_bc: embedded electrodes (uint8 mask,
+    0 solved, 1 Dirichlet at phi_bc, 2 solid)."""
     assert phi.dtype == np.float64 and phi.flags.c_contiguous
     rhs = np.ascontiguousarray(rhs, dtype=np.float64)
     n = phi.shape[0]
@@ -162,7 +165,13 @@ def mg_solve(phi, rhs, h, levels=0, pre=2, post=2, coarse_sweeps=1, omega=1.0, t
             p.pre_level[l] = pre_levels[l] if pre_levels is not None and l < len(pre_levels) else pre
             p.post_level[l] = post_levels[l] if post_levels is not None and l < len(post_levels) else post
     st = MGStats()
-    rc = lib(omp).orc_mg_solve(_p(phi), _p(rhs), n, n, h, ctypes.byref(p), ctypes.byref(st))
+    if mask is not None:
+        mask = np.ascontiguousarray(mask, dtype=np.uint8)
+        phi_bc = np.ascontiguousarray(np.zeros_like(phi) if phi_bc is None else phi_bc, dtype=np.float64)
+        rc = lib(omp).orc_mg_solve_masked(_p(phi), _p(rhs), n, n, h, ctypes.byref(p), _p(mask), _p(phi_bc),
+                                          ctypes.byref(st))
+    else:
+        rc = lib(omp).orc_mg_solve(_p(phi), _p(rhs), n, n, h, ctypes.byref(p), ctypes.byref(st))
     if rc != 0:
         raise RuntimeError("orc_mg_solve failed: %d" % rc)
     return st

@@ -138,14 +138,19 @@ extern "C" void orc_charge_to_rhs(const uint64_t* const* acc, const double* cfac
 /* ------------------------------------------------------------------------ */
 /* Multigrid, Multigrid.cu.                                                   */
 /* ------------------------------------------------------------------------ */
-/* One colour of red-black SOR, Multigrid.cu:192-227 (same expression form). */
-static void rb_colour(double* phi, const double* rhs, int n, double h2, double omega, int colour) {
+/* One colour of red-black SOR, Multigrid.cu:192-227 (same expression form).
+   mask (may be null): nodes with a nonzero mask are held fixed, as the
+   embedded-boundary smoother of Multigrid_embedded.cu:14-33 holds its
+   Dirichlet and solid nodes. */
+static void rb_colour(double* phi, const double* rhs, int n, double h2, double omega, int colour,
+                      const uint8_t* mask = nullptr) {
     /* the nodes of one colour are independent: any order gives the same values */
 #pragma omp parallel for schedule(static) if (n > 256)
     for (int j = 1; j < n - 1; ++j)
         for (int i = 1; i < n - 1; ++i) {
             if (((i + j) & 1) != colour) continue;
             size_t idx = (size_t)j * n + i;
+            if (mask && mask[idx]) continue;
             double c = phi[idx];
             double s = phi[idx + n] + phi[idx - n] + phi[idx + 1] + phi[idx - 1];
             double nv = 0.25 * (s - h2 * rhs[idx]);
@@ -154,25 +159,34 @@ static void rb_colour(double* phi, const double* rhs, int n, double h2, double o
 }
 
 /* smooth(): odd colour first, then even (Multigrid.cu:501-504). */
-extern "C" void orc_mg_smooth(double* phi, const double* rhs, int n, double h, double omega, int iters) {
+static void smooth_m(double* phi, const double* rhs, int n, double h, double omega, int iters, const uint8_t* mask) {
     double h2 = h * h;
     for (int it = 0; it < iters; ++it) {
-        rb_colour(phi, rhs, n, h2, omega, 1);
-        rb_colour(phi, rhs, n, h2, omega, 0);
+        rb_colour(phi, rhs, n, h2, omega, 1, mask);
+        rb_colour(phi, rhs, n, h2, omega, 0, mask);
     }
 }
 
-/* residual_2d, Multigrid.cu:269-297. */
-extern "C" void orc_mg_residual(double* res, const double* phi, const double* rhs, int n, double h) {
+extern "C" void orc_mg_smooth(double* phi, const double* rhs, int n, double h, double omega, int iters) {
+    smooth_m(phi, rhs, n, h, omega, iters, nullptr);
+}
+
+/* residual_2d, Multigrid.cu:269-297 (zero at fixed nodes of a mask, as
+   Multigrid_embedded.cu:36-54). */
+static void residual_m(double* res, const double* phi, const double* rhs, int n, double h, const uint8_t* mask) {
     double inv_h2 = 1.0 / (h * h);
     for (int j = 0; j < n; ++j)
         for (int i = 0; i < n; ++i) {
             size_t idx = (size_t)j * n + i;
-            if (i == 0 || i == n - 1 || j == 0 || j == n - 1) { res[idx] = 0.0; continue; }
+            if (i == 0 || i == n - 1 || j == 0 || j == n - 1 || (mask && mask[idx])) { res[idx] = 0.0; continue; }
             double c = phi[idx];
             double lap = (phi[idx - n] + phi[idx + n] + phi[idx - 1] + phi[idx + 1] - 4.0 * c) * inv_h2;
             res[idx] = rhs[idx] - lap;
         }
+}
+
+extern "C" void orc_mg_residual(double* res, const double* phi, const double* rhs, int n, double h) {
+    residual_m(res, phi, rhs, n, h, nullptr);
 }
 
 /* restrict_2d, Multigrid.cu:299-349: full weighting, out-of-range terms dropped. */
@@ -196,11 +210,13 @@ extern "C" void orc_mg_restrict(double* coarse, const double* fine, int nf) {
         }
 }
 
-/* prolong_2d, Multigrid.cu:351-407: fine += bilinear(coarse), clamped at the far edge. */
-extern "C" void orc_mg_prolong(double* fine, const double* coarse, int nc) {
+/* prolong_2d, Multigrid.cu:351-407: fine += bilinear(coarse), clamped at the
+   far edge; fixed nodes of a mask take no correction (Multigrid_embedded.cu:89-104). */
+static void prolong_m(double* fine, const double* coarse, int nc, const uint8_t* mask) {
     int nf = 2 * (nc - 1) + 1;
     for (int j = 0; j < nf; ++j)
         for (int i = 0; i < nf; ++i) {
+            if (mask && mask[(size_t)j * nf + i]) continue;
             int ic = i / 2, jc = j / 2;
             size_t c = (size_t)jc * nc + ic;
             bool io = i & 1, jo = j & 1;
@@ -214,6 +230,8 @@ extern "C" void orc_mg_prolong(double* fine, const double* coarse, int nc) {
             fine[(size_t)j * nf + i] += v;
         }
 }
+
+extern "C" void orc_mg_prolong(double* fine, const double* coarse, int nc) { prolong_m(fine, coarse, nc, nullptr); }
 
 static double sumsq(const double* v, size_t n) {
     double s = 0.0;
@@ -236,6 +254,19 @@ static int mg_levels_for(int n, int requested) {
    costs no V-cycle).  phi keeps its boundary values (Dirichlet). */
 extern "C" int orc_mg_solve(double* phi, const double* rhs, int nx, int ny, double h,
                             const orc_mg_params* p, orc_mg_stats* st) {
+    return orc_mg_solve_masked(phi, rhs, nx, ny, h, p, nullptr, nullptr, st);
+}
+
+/* Embedded electrodes (Multigrid_embedded.cu:7-190): mask 0 = solved node,
+   1 = Dirichlet node held at phi_bc, 2 = solid node held at its value.  The
+   mask of a coarse level is that of the fine node it sits on (injection,
+   :56-87); fixed nodes are skipped by the smoother, have zero residual and
+   take no prolongation.  Restriction and prolongation otherwise keep the
+   full-weighting / bilinear forms of Multigrid.cu (the embedded file's
+   injection prolongation and its red-black precedence defect, D11, are not
+   reproduced). */
+extern "C" int orc_mg_solve_masked(double* phi, const double* rhs, int nx, int ny, double h, const orc_mg_params* p,
+                                   const uint8_t* mask, const double* phi_bc, orc_mg_stats* st) {
     if (nx != ny || nx < 3) return -1;
     int L = mg_levels_for(nx, p->levels);
     std::vector<int> ns(L);
@@ -252,9 +283,23 @@ extern "C" int orc_mg_solve(double* phi, const double* rhs, int nx, int ny, doub
         if (l) { ph[l].assign(sz, 0.0); rh[l].assign(sz, 0.0); }
     }
     size_t nn = (size_t)nx * nx;
+    /* mask pyramid by injection; Dirichlet values applied to phi */
+    std::vector<std::vector<uint8_t>> mk(L);
+    if (mask) {
+        mk[0].assign(mask, mask + nn);
+        for (int l = 1; l < L; ++l) {
+            mk[l].assign((size_t)ns[l] * ns[l], 0);
+            for (int jc = 0; jc < ns[l]; ++jc)
+                for (int ic = 0; ic < ns[l]; ++ic)
+                    mk[l][(size_t)jc * ns[l] + ic] = mk[l - 1][(size_t)(2 * jc) * ns[l - 1] + 2 * ic];
+        }
+        for (size_t k = 0; k < nn; ++k)
+            if (mask[k] == 1) phi[k] = phi_bc[k];
+    }
+    auto M = [&](int l) -> const uint8_t* { return mask ? mk[l].data() : nullptr; };
     double rhs_norm = std::sqrt(sumsq(rhs, nn));
     auto rel_res = [&]() {
-        orc_mg_residual(rs[0].data(), phi, rhs, nx, h);
+        residual_m(rs[0].data(), phi, rhs, nx, h, M(0));
         double r = std::sqrt(sumsq(rs[0].data(), nn));
         return rhs_norm > 0.0 ? r / rhs_norm : r;
     };
@@ -265,8 +310,8 @@ extern "C" int orc_mg_solve(double* phi, const double* rhs, int nx, int ny, doub
         for (int l = 0; l < L - 1; ++l) {
             double* P = l ? ph[l].data() : phi;
             const double* R = l ? rh[l].data() : rhs;
-            orc_mg_smooth(P, R, ns[l], hh, p->omega, p->per_level ? p->pre_level[l] : p->pre);
-            orc_mg_residual(rs[l].data(), P, R, ns[l], hh);
+            smooth_m(P, R, ns[l], hh, p->omega, p->per_level ? p->pre_level[l] : p->pre, M(l));
+            residual_m(rs[l].data(), P, R, ns[l], hh, M(l));
             orc_mg_restrict(rh[l + 1].data(), rs[l].data(), ns[l]);
             std::fill(ph[l + 1].begin(), ph[l + 1].end(), 0.0);
             hh *= 2.0;
@@ -276,14 +321,14 @@ extern "C" int orc_mg_solve(double* phi, const double* rhs, int nx, int ny, doub
             const double* R = (L - 1) ? rh[L - 1].data() : rhs;
             /* a 4x4 coarsest level of an automatic ladder gets at least 32 sweeps */
             const int cs = (p->levels <= 0 && ns[L - 1] == 4 && p->coarse_sweeps < 32) ? 32 : p->coarse_sweeps;
-            orc_mg_smooth(P, R, ns[L - 1], hh, p->omega, cs);
+            smooth_m(P, R, ns[L - 1], hh, p->omega, cs, M(L - 1));
         }
         for (int l = L - 2; l >= 0; --l) {
             hh /= 2.0;
             double* P = l ? ph[l].data() : phi;
             const double* R = l ? rh[l].data() : rhs;
-            orc_mg_prolong(P, ph[l + 1].data(), ns[l + 1]);
-            orc_mg_smooth(P, R, ns[l], hh, p->omega, p->per_level ? p->post_level[l] : p->post);
+            prolong_m(P, ph[l + 1].data(), ns[l + 1], M(l));
+            smooth_m(P, R, ns[l], hh, p->omega, p->per_level ? p->post_level[l] : p->post, M(l));
         }
         ++cyc;
         rel = rel_res();

@@ -79,6 +79,9 @@ typedef struct {
 
 int orc_mg_solve(double* phi, const double* rhs, int nx, int ny, double h,
                  const orc_mg_params* p, orc_mg_stats* st);
+/* with embedded electrodes: mask[nx*ny] 0 solved, 1 Dirichlet at phi_bc, 2 solid */
+int orc_mg_solve_masked(double* phi, const double* rhs, int nx, int ny, double h, const orc_mg_params* p,
+                        const uint8_t* mask, const double* phi_bc, orc_mg_stats* st);
 /* one red-black SOR smoothing call on a level (Multigrid.cu:192-227, :476-506) */
 void orc_mg_smooth(double* phi, const double* rhs, int n, double h, double omega, int iters);
 void orc_mg_residual(double* res, const double* phi, const double* rhs, int n, double h);

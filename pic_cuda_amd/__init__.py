@@ -157,6 +157,8 @@ SIGNATURES = {
     "pic_sim_autotune": (_i, [_vp, _i, ctypes.POINTER(TuneReport)]),
     "pic_sim_set_level_sweeps": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "pic_mg_set_level_sweeps": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    "pic_mg_set_electrodes": (_i, [_vp, _vp, _vp]),
+    "pic_sim_set_electrodes": (_i, [_vp, _vp, _vp]),
     "pic_sim_destroy": (_i, [_vp]),
     "pic_ensemble_create": (_i, [_P(_vp)]),
     "pic_ensemble_add": (_i, [_vp, _vp]),
@@ -454,6 +456,24 @@ class Multigrid:
         b = (ctypes.c_int * max(1, n))(*post)
         _check(load().pic_mg_set_level_sweeps(self._h, n, a, b))
 
+    def set_electrodes(self, mask, phi_bc=None):
+        """Embedded electrodes: mask (n x n) 0 solved, 1 Dirichlet at phi_bc,
+        2 solid; None removes them."""
+        _check(load().pic_mg_set_electrodes(self._h, *_electrode_args(mask, phi_bc, self.n)))
+
+
+def _electrode_args(mask, phi_bc, n):
+    if mask is None:
+        return (None, None)
+    m = np.ascontiguousarray(mask, dtype=np.uint8)
+    if m.size != n * n:
+        raise PicError("electrode mask must have n*n entries")
+    b = None if phi_bc is None else np.ascontiguousarray(phi_bc, dtype=np.float64)
+    if b is not None and b.size != n * n:
+        raise PicError("phi_bc must have n*n entries")
+    # the arrays are copied by the library before the call returns
+    return (m.ctypes.data_as(ctypes.c_void_p), None if b is None else b.ctypes.data_as(ctypes.c_void_p))
+
 
 def solve_multigrid(phi_d, rho_d, nx, ny, levels, h, omega=1.25):
     """solve_multigrid(phi, rho, nx, ny, levels, h, omega) -- Multigrid.hpp:3-11 (device tensors)."""
@@ -593,6 +613,11 @@ class Simulation:
         return {"bin": (r.bin_w, r.bin_h), "push_ms": r.push_ms, "omega": r.omega, "levels": r.levels,
                 "pre": list(r.pre)[:nl], "post": list(r.post)[:nl], "solve_ms": r.solve_ms,
                 "solve_cycles": r.solve_cycles, "trials": r.trials}
+
+    def set_electrodes(self, mask, phi_bc=None):
+        """Embedded electrodes in the field solve (see Multigrid.set_electrodes)."""
+        self._electrodes = _electrode_args(mask, phi_bc, self.grid.m)
+        _check(load().pic_sim_set_electrodes(self._h, *self._electrodes))
 
     def set_level_sweeps(self, pre, post):
         """Sweeps of each multigrid level, finest first."""

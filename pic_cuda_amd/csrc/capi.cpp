@@ -1214,6 +1214,19 @@ extern "C" int pic_sim_set_level_sweeps(pic_sim* S, int n, const int* pre, const
     return sim_set_mg(S, p);
 }
 
+extern "C" int pic_mg_set_electrodes(pic_mg* m, const uint8_t* mask, const double* phi_bc) {
+    PIC_REQUIRE(m, "solver is NULL");
+    return m->solver->set_mask(mask, phi_bc);
+}
+
+extern "C" int pic_sim_set_electrodes(pic_sim* S, const uint8_t* mask, const double* phi_bc) {
+    PIC_REQUIRE(S, "sim is NULL");
+    drop_graph(S);
+    if (int rc = S->mg->set_mask(mask, phi_bc)) return rc;
+    ensemble_invalidate(S);
+    return 0;
+}
+
 extern "C" int pic_mg_set_level_sweeps(pic_mg* m, int n, const int* pre, const int* post) {
     PIC_REQUIRE(m, "solver is NULL");
     MGParams p = m->solver->params();

@@ -320,6 +320,10 @@ extern "C" int pic_ensemble_destroy(pic_ensemble* e) {
     return 0;
 }
 
+void ensemble_invalidate(pic_sim* S) {
+    if (S->ens) S->ens->dirty = true;
+}
+
 // a simulation leaving its ensemble when destroyed
 int ensemble_forget(pic_sim* S) {
     if (!S->ens) return 0;

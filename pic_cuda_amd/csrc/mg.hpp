@@ -67,6 +67,7 @@ struct MGLevelTask {
     int pad;
     T h2, inv_h2, omega;
     const MGNorm* gate;
+    const uint8_t* mask;   // embedded electrodes of this level (null: none); nonzero = fixed node
 };
 
 // one case of the single-workgroup coarse cycle
@@ -76,6 +77,7 @@ struct MGCoarseTask {
     const T* rhs;
     T* e_out;
     int n_top, Lc, csweeps, pad;
+    const uint8_t* mask;   // electrodes of the top level (coarser levels: injection)
     // sweeps of the kernel's level l (0 = its top) in bits 2l..2l+1 (packed so
     // that no array is indexed at run time)
     unsigned pre_bits, post_bits;
@@ -98,6 +100,8 @@ struct MGFineTask {
     int nblocks;       // residual partials of this case
     int nchunks;       // 256-node chunks of this case
     double inv_h2;
+    const uint8_t* mask;   // electrodes of the fine level (null: none)
+    const double* phi_bc;  // Dirichlet values of the mask's 1-nodes
 };
 
 struct MGTile {
@@ -119,6 +123,10 @@ public:
     int enqueue_fixed(int cycles);
     int complete_fixed(MGStats* st);
     int last_cycles() const;
+    // embedded electrodes (host arrays of n*n): mask 0 solved, 1 Dirichlet at
+    // phi_bc, 2 solid (held at its value); null mask removes them
+    int set_mask(const uint8_t* mask, const double* phi_bc);
+    bool masked() const { return !mask_.empty(); }
     const MGParams& params() const { return p_; }
     int n() const { return n_; }
     int levels() const { return L_; }
@@ -144,6 +152,8 @@ private:
     std::vector<double*> ed_[2], rd_;
     float* r32_ = nullptr;
     double* phi_tmp_ = nullptr;
+    std::vector<uint8_t*> mask_;     // per level (injection), empty: no electrodes
+    double* phi_bc_ = nullptr;
     double* partials_ = nullptr;
     int n_partials_ = 0;
     MGNorm* d_norm_ = nullptr;
@@ -179,6 +189,7 @@ private:
         int kind = 0;      // 0 smooth, 1 prolong, 2 coarse, 3 copy
         int nu = 0;        // sweeps of a tiled stage
         bool restr = false;   // smooth stage also restricts the residual
+        bool masked = false;  // some task has electrodes
         bool fp64 = false;
         void* tasks = nullptr;
         MGTile* tiles = nullptr;
@@ -211,6 +222,7 @@ private:
     MGNorm* d_mirror_ = nullptr;
     MGNorm* h_mirror_ = nullptr;        // pinned
     int last_cycles_ = 1;
+    bool any_mask_ = false;             // some member has electrodes
     int fixed_cycles_ = 0;
     int launches_ = 0;
 };

@@ -103,7 +103,8 @@ __device__ inline T sor_node(const T* e, const T* r, int idx, int stride, T h2, 
 // s half-sweeps the outer s rings are stale, so half-sweep s uses lo = s + 1.
 // Colour c updates the nodes with (gi + gj) odd for c = 1, even for c = 0.
 template <typename T, int W>
-__device__ inline void tile_colour(T* e, const T* r, int gi0, int gj0, int n, T h2, T omega, int colour, int lo) {
+__device__ inline void tile_colour(T* e, const T* r, const uint8_t* mk, int gi0, int gj0, int n, T h2, T omega,
+                                   int colour, int lo) {
     const int span = W - 2 * lo;
     const int halfc = (span + 1) / 2;
     const int items = span * halfc;
@@ -116,6 +117,7 @@ __device__ inline void tile_colour(T* e, const T* r, int gi0, int gj0, int n, T 
         const int gi = gi0 + li;
         if (gi <= 0 || gi >= n - 1 || gj <= 0 || gj >= n - 1) continue;
         const int idx = lj * W + li;
+        if (mk && mk[idx]) continue;   // electrode node: held fixed
         e[idx] = sor_node(e, r, idx, W, h2, omega);
     }
 }
@@ -123,8 +125,9 @@ __device__ inline void tile_colour(T* e, const T* r, int gi0, int gj0, int n, T 
 // Down-leg stage: load tile + halo, NU sweeps, store the tile; with RESTRICT
 // also the residual on tile + 1 ring and the restriction to the coarse nodes
 // (2ic, 2jc) inside the tile.  Without RESTRICT (tiled sweeps of a large
-// coarsest level) the halo is 2*NU.
-template <typename T, int NU, bool RESTRICT>
+// coarsest level) the halo is 2*NU.  With electrodes the mask tile is staged
+// in LDS too: fixed nodes are not smoothed and have zero residual.
+template <typename T, int NU, bool RESTRICT, bool MASKED>
 __global__ __launch_bounds__(256) void mg_smooth_stage(const StageRef<MGLevelTask<T>> a) {
     constexpr int H = RESTRICT ? 2 * NU + 2 : 2 * NU;
     constexpr int W = MG_TILE + 2 * H;
@@ -145,10 +148,15 @@ __global__ __launch_bounds__(256) void mg_smooth_stage(const StageRef<MGLevelTas
     extern __shared__ unsigned char smem_mg[];
     T* e = reinterpret_cast<T*>(smem_mg);
     T* r = e + WW;
+    // electrodes: only the MASKED instantiation carries the mask code
+    const gptr<const uint8_t> gmask = MASKED ? G(tk.mask) : nullptr;
+    uint8_t* mk = (MASKED && gmask) ? reinterpret_cast<uint8_t*>(r + WW + (RESTRICT ? (MG_TILE + 2) * (MG_TILE + 2) : 0))
+                                    : nullptr;
     const int tid = threadIdx.x;
     const int i0 = tile.tx * MG_TILE, j0 = tile.ty * MG_TILE;
     const int gi0 = i0 - H, gj0 = j0 - H;
     T ev[NL], rv[NL];
+    uint8_t mv[NL];
 #pragma unroll
     for (int q = 0; q < NL; ++q) {
         const int k = tid + 256 * q;
@@ -156,10 +164,12 @@ __global__ __launch_bounds__(256) void mg_smooth_stage(const StageRef<MGLevelTas
         const int gi = gi0 + li, gj = gj0 + lj;
         ev[q] = (T)0;
         rv[q] = (T)0;
+        mv[q] = 0;
         if (k < WW && gi >= 0 && gi < n && gj >= 0 && gj < n) {
             const size_t g = (size_t)gj * n + gi;
             if (e_in) ev[q] = e_in[g];
             rv[q] = rhs[g];
+            if (gmask) mv[q] = gmask[g];
         }
     }
     if (stop) return;
@@ -169,12 +179,13 @@ __global__ __launch_bounds__(256) void mg_smooth_stage(const StageRef<MGLevelTas
         if (k < WW) {
             e[k] = ev[q];
             r[k] = rv[q];
+            if (mk) mk[k] = mv[q];
         }
     }
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < 2 * NU; ++s) {
-        tile_colour<T, W>(e, r, gi0, gj0, n, h2, omega, 1 - (s & 1), s + 1);
+        tile_colour<T, W>(e, r, mk, gi0, gj0, n, h2, omega, 1 - (s & 1), s + 1);
         __syncthreads();
     }
 #pragma unroll
@@ -191,8 +202,8 @@ __global__ __launch_bounds__(256) void mg_smooth_stage(const StageRef<MGLevelTas
             const int q = k / RW;
             const int gi = i0 - 1 + (k - q * RW), gj = j0 - 1 + q;
             T v = (T)0;
-            if (gi > 0 && gi < n - 1 && gj > 0 && gj < n - 1) {
-                const int idx = (gj - gj0) * W + (gi - gi0);
+            const int idx = (gj - gj0) * W + (gi - gi0);
+            if (gi > 0 && gi < n - 1 && gj > 0 && gj < n - 1 && !(mk && mk[idx])) {
                 const T c = e[idx];
                 const T lap = (e[idx - W] + e[idx + W] + e[idx - 1] + e[idx + 1] - (T)4 * c) * inv_h2;
                 v = r[idx] - lap;
@@ -237,7 +248,7 @@ __device__ inline T prolong_lds(const T* C, int CW, int ic0, int jc0, int nc, in
 }
 
 // Up-leg stage: e = e_in + P(coarse) on tile + halo 2*NU, NU sweeps, store the tile.
-template <typename T, int NU>
+template <typename T, int NU, bool MASKED>
 __global__ __launch_bounds__(256) void mg_prolong_stage(const StageRef<MGLevelTask<T>> a) {
     constexpr int H = 2 * NU;
     constexpr int W = MG_TILE + 2 * H;
@@ -260,12 +271,15 @@ __global__ __launch_bounds__(256) void mg_prolong_stage(const StageRef<MGLevelTa
     T* e = reinterpret_cast<T*>(smem_mg);
     T* r = e + WW;
     T* C = r + WW;
+    const gptr<const uint8_t> gmask = MASKED ? G(tk.mask) : nullptr;
+    uint8_t* mk = (MASKED && gmask) ? reinterpret_cast<uint8_t*>(C + CC) : nullptr;
     const int tid = threadIdx.x;
     const int i0 = tile.tx * MG_TILE, j0 = tile.ty * MG_TILE;
     const int gi0 = i0 - H, gj0 = j0 - H;   // even: H and i0 are
     const int nc = (n + 1) / 2;
     const int ic0 = gi0 >> 1, jc0 = gj0 >> 1;
     T ev[NL], rv[NL], cv[NCL];
+    uint8_t mv[NL];
 #pragma unroll
     for (int q = 0; q < NCL; ++q) {
         const int k = tid + 256 * q;
@@ -281,10 +295,12 @@ __global__ __launch_bounds__(256) void mg_prolong_stage(const StageRef<MGLevelTa
         const int gi = gi0 + li, gj = gj0 + lj;
         ev[q] = (T)0;
         rv[q] = (T)0;
+        mv[q] = 0;
         if (k < WW && gi >= 0 && gi < n && gj >= 0 && gj < n) {
             const size_t g = (size_t)gj * n + gi;
             ev[q] = e_in[g];
             rv[q] = rhs[g];
+            if (gmask) mv[q] = gmask[g];
         }
     }
     if (stop) return;
@@ -301,16 +317,18 @@ __global__ __launch_bounds__(256) void mg_prolong_stage(const StageRef<MGLevelTa
             const int lj = k / W, li = k - lj * W;
             const int gi = gi0 + li, gj = gj0 + lj;
             T v = ev[q];
-            if (gi >= 0 && gi < n && gj >= 0 && gj < n && (gi >> 1) < nc && (gj >> 1) < nc)
+            // electrode nodes take no correction
+            if (gi >= 0 && gi < n && gj >= 0 && gj < n && (gi >> 1) < nc && (gj >> 1) < nc && !mv[q])
                 v = v + prolong_lds(C, CW, ic0, jc0, nc, gi, gj);
             e[k] = v;
             r[k] = rv[q];
+            if (mk) mk[k] = mv[q];
         }
     }
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < 2 * NU; ++s) {
-        tile_colour<T, W>(e, r, gi0, gj0, n, h2, omega, 1 - (s & 1), s + 1);
+        tile_colour<T, W>(e, r, mk, gi0, gj0, n, h2, omega, 1 - (s & 1), s + 1);
         __syncthreads();
     }
 #pragma unroll
@@ -324,8 +342,22 @@ __global__ __launch_bounds__(256) void mg_prolong_stage(const StageRef<MGLevelTa
 // ---- the single-workgroup sub-cycle for the small levels ---------------------
 // 1024 threads as 32 x 32: thread (tx, ty) takes rows ty, ty+32, ... and
 // columns tx, tx+32, ... (or every other column for one colour).
+//
+// Electrodes: a bitmap of the top level's mask in LDS; node (i, j) of level l
+// is fixed when the top-level node (i 2^l, j 2^l) is (the mask of a coarse
+// level is that of the fine node it sits on).
+struct CoarseMask {
+    const unsigned* bits;   // null: no electrodes
+    int n_top;
+    __device__ bool fixed(int l, int i, int j) const {
+        if (!bits) return false;
+        const int k = (j << l) * n_top + (i << l);
+        return (bits[k >> 5] >> (k & 31)) & 1u;
+    }
+};
+
 template <typename T>
-__device__ inline void lds_sweeps(T* e, const T* r, int n, T h, T omega, int iters) {
+__device__ inline void lds_sweeps(T* e, const T* r, int n, T h, T omega, int iters, const CoarseMask& cm, int l) {
     const T h2 = h * h;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     for (int it = 0; it < iters; ++it)
@@ -333,6 +365,7 @@ __device__ inline void lds_sweeps(T* e, const T* r, int n, T h, T omega, int ite
             for (int j = 1 + ty; j < n - 1; j += 32) {
                 const int ib = 1 + ((colour + j + 1) & 1);   // (i + j) & 1 == colour
                 for (int i = ib + 2 * tx; i < n - 1; i += 64) {
+                    if (cm.fixed(l, i, j)) continue;
                     const int idx = j * n + i;
                     e[idx] = sor_node(e, r, idx, n, h2, omega);
                 }
@@ -341,7 +374,7 @@ __device__ inline void lds_sweeps(T* e, const T* r, int n, T h, T omega, int ite
         }
 }
 
-template <typename T>
+template <typename T, bool MASKED>
 __global__ __launch_bounds__(1024) void mg_coarse_stage(const StageRef<MGCoarseTask<T>> a) {
     MGCoarseTask<T> tk;
     stage_case(a, tk);
@@ -374,11 +407,26 @@ __global__ __launch_bounds__(1024) void mg_coarse_stage(const StageRef<MGCoarseT
             base[k] = e_in ? e_in[k] : (T)0;
             base[n_top * n_top + k] = rhs_top[k];
         }
+    // electrode bitmap of the top level (n_top <= 65)
+    __shared__ unsigned s_mb[MASKED ? (65 * 65 + 31) / 32 : 1];
+    const gptr<const uint8_t> gmask = MASKED ? G(tk.mask) : nullptr;
+    if (MASKED && gmask) {
+        const int nw = (n_top * n_top + 31) / 32;
+        for (int w = threadIdx.x; w < nw; w += blockDim.x) {
+            unsigned v = 0u;
+            for (int b = 0; b < 32; ++b) {
+                const int k = w * 32 + b;
+                if (k < n_top * n_top && gmask[k]) v |= 1u << b;
+            }
+            s_mb[w] = v;
+        }
+    }
+    const CoarseMask cm{(MASKED && gmask) ? s_mb : nullptr, n_top};
     __syncthreads();
     T h = h_top;
     for (int l = 0; l < Lc - 1; ++l) {
         const int nf = s_n[l], ncs = s_n[l + 1];
-        lds_sweeps(E(l), R(l), nf, h, omega, (int)((tk.pre_bits >> (2 * l)) & 3u));
+        lds_sweeps(E(l), R(l), nf, h, omega, (int)((tk.pre_bits >> (2 * l)) & 3u), cm, l);
         const T inv_h2 = (T)1 / (h * h);
         const T* e = E(l);
         const T* rl = R(l);
@@ -387,7 +435,7 @@ __global__ __launch_bounds__(1024) void mg_coarse_stage(const StageRef<MGCoarseT
             for (int i = tx; i < nf; i += 32) {
                 const int k = j * nf + i;
                 T v = (T)0;
-                if (!(i == 0 || i == nf - 1 || j == 0 || j == nf - 1)) {
+                if (!(i == 0 || i == nf - 1 || j == 0 || j == nf - 1) && !cm.fixed(l, i, j)) {
                     const T lap = (e[k - nf] + e[k + nf] + e[k - 1] + e[k + 1] - (T)4 * e[k]) * inv_h2;
                     v = rl[k] - lap;
                 }
@@ -418,7 +466,7 @@ __global__ __launch_bounds__(1024) void mg_coarse_stage(const StageRef<MGCoarseT
         __syncthreads();
         h = h * (T)2;
     }
-    lds_sweeps(E(Lc - 1), R(Lc - 1), s_n[Lc - 1], h, omega, csweeps);
+    lds_sweeps(E(Lc - 1), R(Lc - 1), s_n[Lc - 1], h, omega, csweeps, cm, Lc - 1);
     for (int l = Lc - 2; l >= 0; --l) {
         h = h / (T)2;
         const int nf = s_n[l], ncs = s_n[l + 1];
@@ -426,12 +474,12 @@ __global__ __launch_bounds__(1024) void mg_coarse_stage(const StageRef<MGCoarseT
         T* el = E(l);
         for (int j = ty; j < nf; j += 32)
             for (int i = tx; i < nf; i += 32) {
-                if ((i >> 1) >= ncs || (j >> 1) >= ncs) continue;
+                if ((i >> 1) >= ncs || (j >> 1) >= ncs || cm.fixed(l, i, j)) continue;
                 const int k = j * nf + i;
                 el[k] = el[k] + prolong_lds(C, ncs, 0, 0, ncs, i, j);
             }
         __syncthreads();
-        lds_sweeps(el, R(l), nf, h, omega, (int)((tk.post_bits >> (2 * l)) & 3u));
+        lds_sweeps(el, R(l), nf, h, omega, (int)((tk.post_bits >> (2 * l)) & 3u), cm, l);
     }
     for (int j = ty; j < n_top; j += 32)
         for (int i = tx; i < n_top; i += 32) e_out[j * n_top + i] = base[j * n_top + i];
@@ -471,6 +519,7 @@ __global__ __launch_bounds__(256) void ir_residual_stage(const StageRef<MGFineTa
     const gptr<const float> e32 = G(tk.e32);
     const gptr<const double> rhs = G(tk.rhs);
     const gptr<float> r32 = G(tk.r32);
+    const gptr<const uint8_t> mask = G(tk.mask);   // electrodes: zero residual
     const double inv_h2 = tk.inv_h2;
     const int i = tile.tx * MG_RX + (threadIdx.x & (MG_RX - 1));
     const int jb = tile.ty * MG_RY + (threadIdx.x / MG_RX) * 4;
@@ -503,7 +552,7 @@ __global__ __launch_bounds__(256) void ir_residual_stage(const StageRef<MGFineTa
             const size_t g = (size_t)j * n + i;
             const double c = col[q + 1];
             double r = 0.0;
-            if (!(i == 0 || i == n - 1 || j == 0 || j == n - 1)) {
+            if (!(i == 0 || i == n - 1 || j == 0 || j == n - 1) && !(mask && mask[g])) {
                 const double lap = (col[q] + col[q + 2] + lf[q] + rt[q] - 4.0 * c) * inv_h2;
                 r = rh[q] - lap;
             }
@@ -538,6 +587,18 @@ __global__ __launch_bounds__(256) void ir_finish_stage(const StageRef<MGFineTask
     const size_t nn = (size_t)tk.n * tk.n;
     const size_t i = (size_t)tile.tx * 256 + threadIdx.x;
     if (i < nn) G(tk.phi)[i] = G(tk.phi_tmp)[i];
+}
+
+// Electrodes: the Dirichlet nodes of the mask take their values (solid nodes
+// keep theirs), at the start of every solve.
+__global__ __launch_bounds__(256) void dirichlet_stage(const StageRef<MGFineTask> a) {
+    MGTile tile;
+    MGFineTask tk;
+    stage_item(a, tk, tile);
+    if (!tk.mask) return;
+    const size_t nn = (size_t)tk.n * tk.n;
+    const size_t i = (size_t)tile.tx * 256 + threadIdx.x;
+    if (i < nn && G(tk.mask)[i] == 1) G(tk.phi)[i] = G(tk.phi_bc)[i];
 }
 
 // One workgroup per case: fixed-order sum of the block partials.  which = 0:
@@ -604,6 +665,44 @@ MGSolver::~MGSolver() {
     if (phi_tmp_) (void)hipFree(phi_tmp_);
     if (partials_) (void)hipFree(partials_);
     if (d_norm_) (void)hipFree(d_norm_);
+    for (auto* q : mask_) (void)hipFree(q);
+    if (phi_bc_) (void)hipFree(phi_bc_);
+}
+
+int MGSolver::set_mask(const uint8_t* mask, const double* phi_bc) {
+    PIC_REQUIRE(ok_, "multigrid: solver not initialised");
+    PIC_HIP(hipStreamSynchronize(stream_));
+    for (auto* q : mask_) (void)hipFree(q);
+    mask_.clear();
+    if (phi_bc_) (void)hipFree(phi_bc_);
+    phi_bc_ = nullptr;
+    delete solo_;   // its launch tables hold the old mask pointers
+    solo_ = nullptr;
+    if (!mask) return 0;
+    const size_t nn = (size_t)n_ * n_;
+    std::vector<uint8_t> cur(mask, mask + nn);
+    for (size_t k = 0; k < nn; ++k) PIC_REQUIRE(cur[k] <= 2, "electrode mask values are 0, 1 or 2");
+    bool dirichlet = false;
+    for (size_t k = 0; k < nn; ++k) dirichlet = dirichlet || cur[k] == 1;
+    PIC_REQUIRE(!dirichlet || phi_bc, "Dirichlet nodes need phi_bc");
+    mask_.assign(L_, nullptr);
+    for (int l = 0; l < L_; ++l) {
+        const int n = ns_[l];
+        if (l > 0) {
+            // coarse node (ic, jc) takes the mask of the fine node (2ic, 2jc)
+            const int nf = ns_[l - 1];
+            std::vector<uint8_t> c((size_t)n * n);
+            for (int jc = 0; jc < n; ++jc)
+                for (int ic = 0; ic < n; ++ic) c[(size_t)jc * n + ic] = cur[(size_t)(2 * jc) * nf + 2 * ic];
+            cur.swap(c);
+        }
+        PIC_HIP(hipMalloc(&mask_[l], (size_t)n * n));
+        PIC_HIP(hipMemcpy(mask_[l], cur.data(), (size_t)n * n, hipMemcpyHostToDevice));
+    }
+    PIC_HIP(hipMalloc(&phi_bc_, nn * sizeof(double)));
+    if (phi_bc) PIC_HIP(hipMemcpy(phi_bc_, phi_bc, nn * sizeof(double), hipMemcpyHostToDevice));
+    else PIC_HIP(hipMemset(phi_bc_, 0, nn * sizeof(double)));
+    return 0;
 }
 
 int MGSolver::init() {
@@ -726,13 +825,13 @@ static size_t coarse_lds_bytes(int n_top, int Lc) {
 
 // LDS of a tiled stage: e and rhs over tile + halo, plus the residual patch
 // (smooth with restriction) or the coarse patch (prolongation)
-static size_t tiled_lds(int kind, int nu, bool restr, size_t es) {
+static size_t tiled_lds(int kind, int nu, bool restr, size_t es, bool masked) {
     const size_t H = (kind == 0 && restr) ? 2 * nu + 2 : 2 * nu;
     const size_t W = MG_TILE + 2 * H;
     size_t tot = 2 * W * W;
     if (kind == 0 && restr) tot += (MG_TILE + 2) * (MG_TILE + 2);
     if (kind == 1) tot += (W / 2 + 2) * (W / 2 + 2);
-    return tot * es;
+    return tot * es + (masked ? W * W : 0);   // + the electrode mask tile (bytes)
 }
 
 MGBatch::~MGBatch() { free_all(); }
@@ -782,9 +881,10 @@ int MGBatch::add_level_stage(int kind, const std::vector<MGLevelTask<T>>& tasks,
     st.nu = nu;
     st.restr = restr;
     st.fp64 = sizeof(T) == 8;
-    st.lds = kind == 3 ? 0 : tiled_lds(kind, nu, restr, sizeof(T));
     st.n_items = (int)tiles.size();
     st.n_tasks = (int)tasks.size();
+    for (const auto& t : tasks) st.masked = st.masked || t.mask != nullptr;
+    st.lds = kind == 3 ? 0 : tiled_lds(kind, nu, restr, sizeof(T), st.masked);
     st.gx = kind == 3 ? 0 : div_up(tasks[0].n, MG_TILE);
     static_assert(sizeof(MGLevelTask<T>) <= sizeof(st.t0), "task does not fit the by-value slot");
     std::memcpy(st.t0, &tasks[0], sizeof(MGLevelTask<T>));
@@ -827,6 +927,7 @@ int MGBatch::build_cycle(const std::vector<double>& h) {
             t.inv_h2 = (T)(1.0 / (hh * hh));
             t.omega = omega;
             t.gate = s->d_norm_;
+            t.mask = s->masked() ? s->mask_[d] : nullptr;
             tasks.push_back(t);
         }
         if (int rc = add_level_stage<T>(0, tasks, p_.pre_at(d), true)) return rc;
@@ -856,6 +957,7 @@ int MGBatch::build_cycle(const std::vector<double>& h) {
             t.h_top = (T)hl(k, lc);
             t.omega = omega;
             t.gate = s->d_norm_;
+            t.mask = s->masked() ? s->mask_[lc] : nullptr;
             lds = std::max(lds, coarse_lds_bytes<T>(t.n_top, t.Lc));
             ct.push_back(t);
         }
@@ -866,6 +968,7 @@ int MGBatch::build_cycle(const std::vector<double>& h) {
             st.lds = lds;
             st.n_items = (int)ct.size();
             st.n_tasks = (int)ct.size();
+            for (const auto& t : ct) st.masked = st.masked || t.mask != nullptr;
             static_assert(sizeof(MGCoarseTask<T>) <= sizeof(st.t0), "task does not fit the by-value slot");
             std::memcpy(st.t0, &ct[0], sizeof(MGCoarseTask<T>));
             if (int rc = upload_vec(&st.tasks, ct)) return rc;
@@ -907,6 +1010,7 @@ int MGBatch::build_cycle(const std::vector<double>& h) {
                     t.inv_h2 = (T)(1.0 / (hh * hh));
                     t.omega = omega;
                     t.gate = s->d_norm_;
+                    t.mask = s->masked() ? s->mask_[lc] : nullptr;
                     tasks.push_back(t);
                     src[k] = t.e_out;
                     which[k] ^= 1;
@@ -947,6 +1051,7 @@ int MGBatch::build_cycle(const std::vector<double>& h) {
             t.inv_h2 = (T)(1.0 / (hh * hh));
             t.omega = omega;
             t.gate = s->d_norm_;
+            t.mask = s->masked() ? s->mask_[d] : nullptr;
             tasks.push_back(t);
         }
         if (int rc = add_level_stage<T>(1, tasks, p_.post_at(d), false)) return rc;
@@ -980,6 +1085,7 @@ int MGBatch::build(const std::vector<MGSolver*>& members, const std::vector<doub
     PIC_HIP(hipHostMalloc(&h_mirror_, M * sizeof(MGNorm), hipHostMallocDefault));
     std::memset(h_mirror_, 0, M * sizeof(MGNorm));
     std::vector<MGFineTask> fine(M);
+    any_mask_ = false;
     std::vector<MGTile> chunks, res;
     for (int k = 0; k < M; ++k) {
         MGSolver* s = m_[k];
@@ -1000,6 +1106,9 @@ int MGBatch::build(const std::vector<MGSolver*>& members, const std::vector<doub
         f.nblocks = f.gx * gy;
         f.nchunks = (int)((nn + 255) / 256);
         f.inv_h2 = 1.0 / (h[k] * h[k]);
+        f.mask = s->masked() ? s->mask_[0] : nullptr;
+        f.phi_bc = s->phi_bc_;
+        any_mask_ = any_mask_ || s->masked();
         for (int c = 0; c < f.nchunks; ++c) chunks.push_back(MGTile{k, c, 0, 0});
         for (int ty = 0; ty < gy; ++ty)
             for (int tx = 0; tx < f.gx; ++tx) res.push_back(MGTile{k, tx, ty, 0});
@@ -1015,7 +1124,7 @@ int MGBatch::build(const std::vector<MGSolver*>& members, const std::vector<doub
 
 struct MGBatchStageView {
     int kind, nu;
-    bool restr;
+    bool restr, masked;
     const void* tasks;
     const MGTile* tiles;
     int n_items;
@@ -1035,21 +1144,28 @@ static StageRef<Task> make_ref(const void* t0, const void* tasks, const MGTile* 
     return r;
 }
 
-template <typename T, int NU>
+template <typename T, int NU, bool M>
 static void launch_tiled_nu(int kind, bool restr, dim3 g, size_t lds, hipStream_t s, const StageRef<MGLevelTask<T>>& r) {
-    if (kind == 1) hipLaunchKernelGGL((mg_prolong_stage<T, NU>), g, dim3(256), lds, s, r);
-    else if (restr) hipLaunchKernelGGL((mg_smooth_stage<T, NU, true>), g, dim3(256), lds, s, r);
-    else hipLaunchKernelGGL((mg_smooth_stage<T, NU, false>), g, dim3(256), lds, s, r);
+    if (kind == 1) hipLaunchKernelGGL((mg_prolong_stage<T, NU, M>), g, dim3(256), lds, s, r);
+    else if (restr) hipLaunchKernelGGL((mg_smooth_stage<T, NU, true, M>), g, dim3(256), lds, s, r);
+    else hipLaunchKernelGGL((mg_smooth_stage<T, NU, false, M>), g, dim3(256), lds, s, r);
+}
+
+template <typename T, int NU>
+static void launch_tiled_m(int kind, bool restr, bool masked, dim3 g, size_t lds, hipStream_t s,
+                           const StageRef<MGLevelTask<T>>& r) {
+    if (masked) launch_tiled_nu<T, NU, true>(kind, restr, g, lds, s, r);
+    else launch_tiled_nu<T, NU, false>(kind, restr, g, lds, s, r);
 }
 
 template <typename T>
-static void launch_tiled(int kind, int nu, bool restr, dim3 g, size_t lds, hipStream_t s,
+static void launch_tiled(int kind, int nu, bool restr, bool masked, dim3 g, size_t lds, hipStream_t s,
                          const StageRef<MGLevelTask<T>>& r) {
     switch (nu) {
-    case 0: launch_tiled_nu<T, 0>(kind, restr, g, lds, s, r); break;
-    case 1: launch_tiled_nu<T, 1>(kind, restr, g, lds, s, r); break;
-    case 2: launch_tiled_nu<T, 2>(kind, restr, g, lds, s, r); break;
-    default: launch_tiled_nu<T, 3>(kind, restr, g, lds, s, r); break;
+    case 0: launch_tiled_m<T, 0>(kind, restr, masked, g, lds, s, r); break;
+    case 1: launch_tiled_m<T, 1>(kind, restr, masked, g, lds, s, r); break;
+    case 2: launch_tiled_m<T, 2>(kind, restr, masked, g, lds, s, r); break;
+    default: launch_tiled_m<T, 3>(kind, restr, masked, g, lds, s, r); break;
     }
 }
 
@@ -1058,10 +1174,11 @@ static void launch_stage_t(const MGBatchStageView& st, hipStream_t s) {
     const dim3 g(st.n_items);
     if (st.kind == 0 || st.kind == 1) {
         const auto r = make_ref<MGLevelTask<T>>(st.t0, st.tasks, st.tiles, st.n_tasks, st.gx);
-        launch_tiled<T>(st.kind, st.nu, st.restr, g, st.lds, s, r);
+        launch_tiled<T>(st.kind, st.nu, st.restr, st.masked, g, st.lds, s, r);
     } else if (st.kind == 2) {
         const auto r = make_ref<MGCoarseTask<T>>(st.t0, st.tasks, nullptr, st.n_tasks, 0);
-        hipLaunchKernelGGL(mg_coarse_stage<T>, g, dim3(1024), st.lds, s, r);
+        if (st.masked) hipLaunchKernelGGL((mg_coarse_stage<T, true>), g, dim3(1024), st.lds, s, r);
+        else hipLaunchKernelGGL((mg_coarse_stage<T, false>), g, dim3(1024), st.lds, s, r);
     } else {
         const auto r = make_ref<MGLevelTask<T>>(st.t0, st.tasks, st.tiles, st.n_tasks, 0);
         hipLaunchKernelGGL(mg_copy_stage<T>, g, dim3(256), 0, s, r);
@@ -1069,22 +1186,27 @@ static void launch_stage_t(const MGBatchStageView& st, hipStream_t s) {
 }
 
 int MGBatch::launch_stage(const Stage& st) {
-    const MGBatchStageView v{st.kind, st.nu, st.restr, st.tasks, st.tiles, st.n_items, st.lds, st.n_tasks, st.gx,
-                             st.t0};
+    const MGBatchStageView v{st.kind, st.nu, st.restr, st.masked, st.tasks, st.tiles, st.n_items, st.lds,
+                             st.n_tasks, st.gx, st.t0};
     if (st.fp64) launch_stage_t<double>(v, stream_);
     else launch_stage_t<float>(v, stream_);
     ++launches_;
     return 0;
 }
 
-// what: 0 rhs norm; 1 residual and its norm (which 1: before the first cycle,
-// 2: after a cycle, whose index has the given parity); 3 mixed mode: the
-// solution back into phi after an odd number of cycles
+// what: 0 rhs norm (and the Dirichlet values of electrodes into phi); 1
+// residual and its norm (which 1: before the first cycle, 2: after a cycle,
+// whose index has the given parity); 3 mixed mode: the solution back into phi
+// after an odd number of cycles
 int MGBatch::enqueue_fine(int what, int which, int parity) {
     const int M = (int)m_.size();
     const auto chunks = make_ref<MGFineTask>(&h_fine_[0], d_fine_, d_chunk_tiles_, M, 0);
     const auto cases = make_ref<MGFineTask>(&h_fine_[0], d_fine_, nullptr, M, 0);
     if (what == 0) {
+        if (any_mask_) {
+            hipLaunchKernelGGL(dirichlet_stage, dim3(n_chunk_), dim3(256), 0, stream_, chunks);
+            ++launches_;
+        }
         hipLaunchKernelGGL(sumsq_stage, dim3(n_chunk_), dim3(256), 0, stream_, chunks);
         hipLaunchKernelGGL(norm_final_stage, dim3(M), dim3(256), 0, stream_, cases, 0, p_.tol, p_.max_cycles);
         launches_ += 2;

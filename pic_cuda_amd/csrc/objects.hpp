@@ -145,5 +145,7 @@ size_t cur_lds(pic_sim* S);
 int mcc_active(pic_sim* S);
 int sim_rebin(pic_sim* S, int bw, int bh);
 void sim_drop_graph(pic_sim* S);
+// the simulation's ensemble (if any) rebuilds its batch before the next step
+void ensemble_invalidate(pic_sim* S);
 // replace the simulation's multigrid solver (new parameters)
 int sim_set_mg(pic_sim* S, const pic::MGParams& prm);

@@ -1,0 +1,79 @@
+"""Embedded electrodes (Multigrid_embedded.cu): the masked FP64 solve is
+bit-identical to the oracle's, the mixed-precision one reaches the FP64
+solution, and a simulation keeps its electrode at its potential."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _electrode_case(n, seed):
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:n, 0:n]
+    mask = np.zeros((n, n), np.uint8)
+    disc = (x - 0.35 * n) ** 2 + (y - 0.5 * n) ** 2 < (0.12 * n) ** 2
+    mask[disc] = 1
+    block = (x > 0.6 * n) & (x < 0.75 * n) & (y > 0.3 * n) & (y < 0.45 * n)
+    mask[block] = 2
+    phi_bc = np.where(disc, 1.5, 0.0)
+    phi0 = np.zeros((n, n))
+    phi0[block] = -0.5
+    phi0[0, :] = rng.normal(size=n)           # Dirichlet box values kept
+    rhs = rng.normal(size=(n, n))
+    return mask, phi_bc, phi0, rhs, disc, block
+
+
+def _solve_gpu(pic, n, rhs, phi0, h, params, mask, phi_bc):
+    g = pic.Grid2D(n, n, h * (n - 1), h * (n - 1))
+    g.upload(pic.FIELD_PHI, phi0)
+    g.upload(pic.FIELD_RHO, rhs)
+    rho_p, phi_p, _ = g.device_ptrs()
+    mg = pic.Multigrid(n, params)
+    mg.set_electrodes(mask, phi_bc)
+    st = mg.solve(phi_p, rho_p, g.dx)
+    return g.download(pic.FIELD_PHI), st
+
+
+@pytest.mark.parametrize("n", [33, 65, 129, 257])
+def test_masked_fp64_bit_identical(pic, orc, n):
+    mask, phi_bc, phi0, rhs, disc, block = _electrode_case(n, n)
+    h = 1.0 / (n - 1)
+    params = pic.mg_params(pic.PRESET_STEP, mode=pic.MG_FP64, tol=0.0, max_cycles=3, omega=1.2)
+    phi_g, st = _solve_gpu(pic, n, rhs, phi0, h, params, mask, phi_bc)
+    phi_o = phi0.copy()
+    ost = orc.mg_solve(phi_o, rhs, h, pre=2, post=2, coarse_sweeps=1, omega=1.2, tol=0.0, max_cycles=3,
+                       mask=mask, phi_bc=phi_bc)
+    assert st.cycles == ost.cycles == 3
+    assert np.array_equal(phi_g, phi_o)
+    assert np.all(phi_g[disc] == 1.5) and np.all(phi_g[block] == -0.5)
+
+
+def test_masked_mixed_precision_accuracy(pic, orc):
+    n = 129
+    mask, phi_bc, phi0, rhs, disc, block = _electrode_case(n, 5)
+    h = 1.0 / (n - 1)
+    phi_g, st = _solve_gpu(pic, n, rhs, phi0, h, pic.mg_params(pic.PRESET_POISSON), mask, phi_bc)
+    assert st.rel_residual < 1e-10
+    phi_o = phi0.copy()
+    orc.mg_solve(phi_o, rhs, h, omega=1.2, tol=1e-12, max_cycles=200, mask=mask, phi_bc=phi_bc)
+    assert np.linalg.norm(phi_g - phi_o) / np.linalg.norm(phi_o) < 1e-9
+    assert np.all(phi_g[disc] == 1.5) and np.all(phi_g[block] == -0.5)
+
+
+def test_simulation_with_electrode(pic):
+    from pic_cuda_amd import configs
+    case = configs.build(64, 40000, pic.PREC_FP32, mcc=True, seed=3)
+    sim, grid = case["sim"], case["grid"]
+    n = grid.m
+    y, x = np.mgrid[0:n, 0:n]
+    disc = (x - n / 2) ** 2 + (y - n / 2) ** 2 < (0.1 * n) ** 2
+    mask = disc.astype(np.uint8)
+    sim.set_electrodes(mask, np.where(disc, 5.0, 0.0))
+    sim.step(10)
+    phi = grid.download(pic.FIELD_PHI)
+    assert np.all(phi[disc] == 5.0)
+    assert sim.stats().last_mg_rel < 1e-8
+    # the potential falls off from the electrode towards the grounded box
+    assert phi[n // 2, n // 2 + int(0.2 * n)] > phi[n // 2, n // 2 + int(0.4 * n)] > 0.0
+    sim.set_electrodes(None)
+    sim.step(2)

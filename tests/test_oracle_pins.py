@@ -560,3 +560,62 @@ def test_mcc_banded_rate(orc):
     assert abs(hits - n * p) < 5 * math.sqrt(n * p * (1 - p))
     # the band's majorant is that of its top energy, 12.5 eV, not of 100 eV
     assert abs(nu_b[b] / m["nu_max"] - math.sqrt(12.5 / 100.0)) < 0.01
+
+
+def _masked_dense(rhs, mask, phi_fixed, h):
+    """direct solve of the 5-point system on the solved nodes, fixed nodes
+    (boundary and mask != 0) moved to the right-hand side"""
+    n = rhs.shape[0]
+    free = np.zeros((n, n), dtype=bool)
+    free[1:-1, 1:-1] = mask[1:-1, 1:-1] == 0
+    idx = -np.ones((n, n), dtype=int)
+    idx[free] = np.arange(free.sum())
+    A = np.zeros((free.sum(), free.sum()))
+    b = np.zeros(free.sum())
+    for j, i in zip(*np.nonzero(free)):
+        k = idx[j, i]
+        A[k, k] = -4.0 / h**2
+        b[k] = rhs[j, i]
+        for dj, di in ((1, 0), (-1, 0), (0, 1), (0, -1)):
+            jj, ii = j + dj, i + di
+            if free[jj, ii]:
+                A[k, idx[jj, ii]] = 1.0 / h**2
+            else:
+                b[k] -= phi_fixed[jj, ii] / h**2
+    out = phi_fixed.copy()
+    out[free] = np.linalg.solve(A, b)
+    return out
+
+
+def test_mg_masked_all_fluid_equals_unmasked(orc):
+    n = 65
+    rng = np.random.default_rng(41)
+    rhs = rng.normal(size=(n, n))
+    a, b = np.zeros((n, n)), np.zeros((n, n))
+    orc.mg_solve(a, rhs, 1.0 / (n - 1), tol=0.0, max_cycles=3, omega=1.2)
+    orc.mg_solve(b, rhs, 1.0 / (n - 1), tol=0.0, max_cycles=3, omega=1.2, mask=np.zeros((n, n), np.uint8))
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("n", [33, 49])
+def test_mg_embedded_electrode_matches_direct_solve(orc, n):
+    """A Dirichlet disc at +1 V and a solid block kept at -0.5 V inside a
+    grounded box with a charge blob: the masked solve reaches the direct solve
+    of the masked 5-point system; fixed nodes keep their values."""
+    h = 1.0 / (n - 1)
+    y, x = np.mgrid[0:n, 0:n]
+    mask = np.zeros((n, n), np.uint8)
+    disc = (x - 0.35 * n) ** 2 + (y - 0.5 * n) ** 2 < (0.12 * n) ** 2
+    mask[disc] = 1
+    block = (x > 0.6 * n) & (x < 0.75 * n) & (y > 0.3 * n) & (y < 0.45 * n)
+    mask[block] = 2
+    phi_bc = np.where(disc, 1.0, 0.0)
+    rhs = -50.0 * np.exp(-((x - 0.6 * n) ** 2 + (y - 0.7 * n) ** 2) / (0.05 * n) ** 2)
+    phi = np.zeros((n, n))
+    phi[block] = -0.5
+    st = orc.mg_solve(phi, rhs, h, tol=1e-12, max_cycles=200, omega=1.0, mask=mask, phi_bc=phi_bc)
+    assert st.rel_residual < 1e-12
+    assert np.all(phi[disc] == 1.0) and np.all(phi[block] == -0.5)
+    fixed = np.where(disc, 1.0, np.where(block, -0.5, 0.0))
+    ref = _masked_dense(rhs, mask, fixed, h)
+    assert np.max(np.abs(phi - ref)) < 1e-9 * max(1.0, np.max(np.abs(ref)))
