@@ -279,6 +279,38 @@ def tune(steps=40):
             "speedup": round(before["ms_per_step"] / after["ms_per_step"], 3)}
 
 
+def walls(n_walls=256, steps=20):
+    """The full step at scale with internal wall segments (random centres and
+    directions, lengths 4-64 cells) on top of the absorbing box: per-bin
+    candidate lists against every particle testing every wall, and the box
+    alone."""
+    import pic_cuda_amd as pic
+    from pic_cuda_amd import configs
+    segs = configs.random_walls(1024, n_walls, seed=3, min_len=4.0, max_len=64.0)
+    out = {"n_walls": n_walls}
+    for name, w, lists in (("box_only", None, True), ("lists", segs, True), ("full_scan", segs, False)):
+        c = configs.full_step(walls=w)
+        sim = c["sim"]
+        if not lists:
+            sim.set_wall_lists(False)
+        sim.step(3)
+        pic.synchronize()
+        st0 = sim.stats()
+        t0 = time.perf_counter()
+        sim.step(steps)
+        pic.synchronize()
+        el = time.perf_counter() - t0
+        st1 = sim.stats()
+        nb, ne, mx = sim.wall_list_stats()
+        out[name] = {"ms_per_step": round(1e3 * el / steps, 4),
+                     "push_ms": round((st1.ms_push - st0.ms_push) / steps, 4),
+                     "particle_steps_per_s": steps * 0.5 * (st0.alive + st1.alive) / el,
+                     "list_entries_per_bin": round(ne / nb, 2) if nb else None, "longest_list": mx}
+        del c, sim
+    out["push_speedup_lists_over_full_scan"] = round(out["full_scan"]["push_ms"] / out["lists"]["push_ms"], 2)
+    return out
+
+
 def serve(n_jobs=24, rate=20.0, budget=200e9, seed=9, max_steps=2000):
     """Serving: Poisson arrivals (rate jobs/s), cells log-uniform 64..1024
     (rounded to GRID_SIZES), 100 particles per cell and species, steps U(200, max),
@@ -326,15 +358,16 @@ def main():
     ap.add_argument("--bin", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="all",
-                    choices=["all", "step", "poisson", "ensemble", "serve", "graph", "tune"])
+                    choices=["all", "step", "poisson", "ensemble", "serve", "graph", "tune", "walls"])
     ap.add_argument("--ens-steps", type=int, default=500)
     ap.add_argument("--serve-jobs", type=int, default=24)
     a = ap.parse_args()
-    if a.workload in ("poisson", "ensemble", "serve", "graph", "tune"):
+    if a.workload in ("poisson", "ensemble", "serve", "graph", "tune", "walls"):
         import pic_cuda_amd as pic
         pic.load()
         fn = {"poisson": lambda: poisson(cpu=not a.no_cpu), "ensemble": lambda: ensemble(steps=a.ens_steps),
-              "serve": lambda: serve(n_jobs=a.serve_jobs), "graph": lambda: graph(), "tune": lambda: tune()}[a.workload]
+              "serve": lambda: serve(n_jobs=a.serve_jobs), "graph": lambda: graph(), "tune": lambda: tune(),
+              "walls": lambda: walls()}[a.workload]
         print(json.dumps({"workload": a.workload, **fn()}))
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

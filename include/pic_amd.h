@@ -248,6 +248,16 @@ int pic_sim_rebin(pic_sim* s, int bin_w, int bin_h);
    host synchronization; a solve that needs more cycles gets them eagerly.
    Results equal the eager path's.  Off by default. */
 int pic_sim_use_graph(pic_sim* s, int on);
+/* per-bin candidate walls (detect_intersections, collisions.cu:7-54, tests
+   every wall for every particle): each bin lists the walls within margin + 1
+   cells of it; a path that stays within `margin` cells (0..64, default 2) of
+   its bin tests that list, a longer one the lists of the bins its bounding
+   box meets (up to 16), a path beyond that every wall; same results in every
+   case.  On by default; rebuilt whenever the bin tile changes.
+   wall_list_stats reports the bins, the list entries and the longest list
+   (all 0 without internal walls or with the lists off). */
+int pic_sim_set_wall_lists(pic_sim* s, int on, int margin);
+int pic_sim_wall_list_stats(pic_sim* s, int* n_bins, long long* n_entries, int* max_per_bin);
 int pic_sim_destroy(pic_sim* s);
 
 /* ---- tuning (no counterpart in the reference, whose tiles and sweeps are

@@ -154,6 +154,8 @@ SIGNATURES = {
     "pic_sim_mcc_info": (_i, [_vp, _i, _P(MccInfo), _vp, _i]),
     "pic_sim_rebin": (_i, [_vp, _i, _i]),
     "pic_sim_use_graph": (_i, [_vp, _i]),
+    "pic_sim_set_wall_lists": (_i, [_vp, _i, _i]),
+    "pic_sim_wall_list_stats": (_i, [_vp, _P(_i), _P(ctypes.c_longlong), _P(_i)]),
     "pic_sim_autotune": (_i, [_vp, _i, ctypes.POINTER(TuneReport)]),
     "pic_sim_set_level_sweeps": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "pic_mg_set_level_sweeps": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
@@ -602,6 +604,17 @@ class Simulation:
     def use_graph(self, on=True):
         """Run each step after the first as one graph launch."""
         _check(load().pic_sim_use_graph(self._h, 1 if on else 0))
+
+    def set_wall_lists(self, on=True, margin=2):
+        """Per-bin candidate walls for the wall test (on by default): each bin
+        lists the walls within margin + 1 cells of it."""
+        _check(load().pic_sim_set_wall_lists(self._h, 1 if on else 0, int(margin)))
+
+    def wall_list_stats(self):
+        """(bins, list entries, longest list) of the candidate walls."""
+        nb, ne, mx = _i(), ctypes.c_longlong(), _i()
+        _check(load().pic_sim_wall_list_stats(self._h, ctypes.byref(nb), ctypes.byref(ne), ctypes.byref(mx)))
+        return nb.value, ne.value, mx.value
 
     def autotune(self, bins=True, multigrid=True):
         """Time candidate bin tiles and multigrid settings on the device and

@@ -47,8 +47,10 @@ def add_argon_processes(xs, electron=0, ion=1):
 
 
 def build(cells, n_per_species, precision, mcc, seed=1, case_id=0, mg=None, bin_w=16, bin_h=16, halo=2,
-          gas_density=1e21, gas_t=0.03, te=2.0, capacity=None):
-    """Create particles, grid, walls, cross sections and the step driver."""
+          gas_density=1e21, gas_t=0.03, te=2.0, capacity=None, walls=None):
+    """Create particles, grid, walls, cross sections and the step driver.
+    `walls`: internal wall segments (x1, y1, x2, y2) in cell units, added to
+    the absorbing box."""
     pl = Plasma(te=te, cells=cells)
     nn = pl.nodes
     grid = Grid2D(nn, nn, pl.length, pl.length)
@@ -61,18 +63,20 @@ def build(cells, n_per_species, precision, mcc, seed=1, case_id=0, mg=None, bin_
     parts.set_weight(ion, w)
     parts.init_maxwellian(0, n_per_species, nn, nn, pl.vte, seed, case_id, 0)
     parts.init_maxwellian(ion, n_per_species, nn, nn, pl.vti, seed, case_id, 0)
-    walls = Boundaries2D(pl.length, pl.length, nn, nn)
-    walls.add_box()
-    walls.compile_walls()
+    bnd = Boundaries2D(pl.length, pl.length, nn, nn)
+    bnd.add_box()
+    for (x1, y1, x2, y2) in (walls if walls is not None else ()):
+        bnd.add_wall(x1 * grid.dx, y1 * grid.dy, x2 * grid.dx, y2 * grid.dy)
+    bnd.compile_walls()
     xs = None
     if mcc:
         xs = CrossSections(gas_density, gas_t)
         add_argon_processes(xs, 0, ion)
     if mg is None:
         mg = mg_params(PRESET_STEP_SOR, mode=MG_MIXED if precision == PREC_FP32 else MG_FP64)
-    sim = Simulation(parts, grid, walls, xs, sim_params(dt=pl.dt, seed=seed, case_id=case_id, bin_w=bin_w,
+    sim = Simulation(parts, grid, bnd, xs, sim_params(dt=pl.dt, seed=seed, case_id=case_id, bin_w=bin_w,
                                                         bin_h=bin_h, halo=halo, mcc=1 if mcc else 0, mg=mg))
-    return dict(plasma=pl, grid=grid, particles=parts, walls=walls, xs=xs, sim=sim, mg=mg)
+    return dict(plasma=pl, grid=grid, particles=parts, walls=bnd, xs=xs, sim=sim, mg=mg)
 
 
 def plain_small(scale=1.0, **kw):
@@ -87,6 +91,18 @@ def full_step(scale=1.0, **kw):
     n = max(1000, int(2e7 * scale))
     cells = kw.pop("cells", 1024)
     return build(cells, n, PREC_FP32, mcc=True, **kw)
+
+
+def random_walls(cells, n, seed=1, min_len=1.0, max_len=24.0):
+    """`n` internal wall segments in cell units: centres uniform over the
+    interior, directions uniform, lengths uniform in [min_len, max_len]."""
+    rng = np.random.default_rng(seed)
+    cx = rng.uniform(1, cells - 1, n)
+    cy = rng.uniform(1, cells - 1, n)
+    th = rng.uniform(0, np.pi, n)
+    ln = rng.uniform(min_len, max_len, n)
+    hx, hy = 0.5 * ln * np.cos(th), 0.5 * ln * np.sin(th)
+    return np.stack([cx - hx, cy - hy, cx + hx, cy + hy], axis=1)
 
 
 def poisson_rhs(n=2049, blobs=200, seed=1, noise=1e-3):

@@ -856,6 +856,80 @@ static int rebuild_push_items(pic_sim* S) {
     return 0;
 }
 
+// Per-bin candidate walls for the bin tile in C (cell units). A wall is listed
+// for a bin when it meets the bin's rectangle grown by S->wall_margin + 1 cells
+// (Liang-Barsky clip of the segment); the push kernel uses one bin's list for a
+// path whose two ends lie in that bin's rectangle grown by S->wall_margin
+// cells, so a wall left out is at least one cell away from every such path.
+
+static bool segment_meets_rect(const double* w, double x0, double y0, double x1, double y1) {
+    const double ax = w[0], ay = w[1], dx = w[2] - w[0], dy = w[3] - w[1];
+    const double p[4] = {-dx, dx, -dy, dy};
+    const double q[4] = {ax - x0, x1 - ax, ay - y0, y1 - ay};
+    double t0 = 0.0, t1 = 1.0;
+    for (int i = 0; i < 4; ++i) {
+        if (p[i] == 0.0) {
+            if (q[i] < 0.0) return false;
+            continue;
+        }
+        const double t = q[i] / p[i];
+        if (p[i] < 0.0) t0 = std::max(t0, t);
+        else t1 = std::min(t1, t);
+        if (t0 > t1) return false;
+    }
+    return true;
+}
+
+template <typename T>
+static int build_wall_lists(pic_sim* S, CaseDev<T>& C) {
+    (void)hipFree(S->d_wall_lists);
+    S->d_wall_lists = nullptr;
+    S->wall_entries = 0;
+    S->wall_max_bin = 0;
+    C.wall_off = nullptr;
+    C.wall_idx = nullptr;
+    C.wall_margin = S->wall_margin;
+    if (!S->wall_lists || !S->w || S->w->walls.empty()) return 0;
+    const int nw = (int)(S->w->walls.size() / 4);
+    const int nb = C.nbx * C.nby;
+    std::vector<double> wc(4 * (size_t)nw);
+    for (int j = 0; j < nw; ++j) {
+        wc[4 * j] = S->w->walls[4 * j] / S->g->dx;
+        wc[4 * j + 1] = S->w->walls[4 * j + 1] / S->g->dy;
+        wc[4 * j + 2] = S->w->walls[4 * j + 2] / S->g->dx;
+        wc[4 * j + 3] = S->w->walls[4 * j + 3] / S->g->dy;
+    }
+    const int G = S->wall_margin + 1;
+    std::vector<std::vector<int>> per(nb);
+    for (int j = 0; j < nw; ++j) {
+        const double* w = &wc[4 * j];
+        const double xmn = std::min(w[0], w[2]), xmx = std::max(w[0], w[2]);
+        const double ymn = std::min(w[1], w[3]), ymx = std::max(w[1], w[3]);
+        const double lo_x = std::max(-1.0, std::floor((xmn - G) / C.bw) - 1);
+        const double hi_x = std::min((double)C.nbx, std::floor((xmx + G) / C.bw) + 1);
+        const double lo_y = std::max(-1.0, std::floor((ymn - G) / C.bh) - 1);
+        const double hi_y = std::min((double)C.nby, std::floor((ymx + G) / C.bh) + 1);
+        for (int by = std::max(0, (int)lo_y); by <= std::min(C.nby - 1, (int)hi_y); ++by)
+            for (int bx = std::max(0, (int)lo_x); bx <= std::min(C.nbx - 1, (int)hi_x); ++bx) {
+                const double rx0 = bx * C.bw - G, ry0 = by * C.bh - G;
+                if (segment_meets_rect(w, rx0, ry0, rx0 + C.bw + 2 * G, ry0 + C.bh + 2 * G))
+                    per[by * C.nbx + bx].push_back(j);
+            }
+    }
+    std::vector<int> lists(nb + 1);
+    for (int b = 0; b < nb; ++b) {
+        lists[b + 1] = lists[b] + (int)per[b].size();
+        S->wall_max_bin = std::max(S->wall_max_bin, (int)per[b].size());
+    }
+    for (int b = 0; b < nb; ++b) lists.insert(lists.end(), per[b].begin(), per[b].end());
+    S->wall_entries = lists[nb];
+    PIC_HIP(hipMalloc(&S->d_wall_lists, lists.size() * sizeof(int)));
+    PIC_HIP(hipMemcpy(S->d_wall_lists, lists.data(), lists.size() * sizeof(int), hipMemcpyHostToDevice));
+    C.wall_off = S->d_wall_lists;
+    C.wall_idx = S->d_wall_lists + nb + 1;
+    return 0;
+}
+
 template <typename T>
 static int rebin_t(pic_sim* S, int bw, int bh) {
     pic_particles* p = S->p;
@@ -968,6 +1042,7 @@ static int rebin_t(pic_sim* S, int bw, int bh) {
     C.nby = nby;
     S->bin_w = bw;
     S->bin_h = bh;
+    if (int rc = build_wall_lists<T>(S, C)) return rc;
     if (int rc = upload_case(p)) return rc;
     if (int rc = rebuild_push_items(S)) return rc;
     S->rebins++;
@@ -1019,6 +1094,7 @@ static int setup_t(pic_sim* S) {
         C.walls = S->d_walls_cell;
         C.n_walls = (int)(wc.size() / 4);
     }
+    if (int rc = build_wall_lists<T>(S, C)) return rc;
     const size_t nn = (size_t)g->m * g->n;
     S->acc.assign(ns, nullptr);
     S->tabs.assign(ns, nullptr);
@@ -1432,6 +1508,27 @@ extern "C" int pic_sim_use_graph(pic_sim* S, int on) {
     return 0;
 }
 
+extern "C" int pic_sim_set_wall_lists(pic_sim* S, int on, int margin) {
+    PIC_REQUIRE(S, "sim is NULL");
+    PIC_REQUIRE(margin >= 0 && margin <= 64, "margin must be 0..64 cells");
+    PIC_HIP(hipStreamSynchronize(lib_stream()));
+    S->wall_lists = on != 0;
+    S->wall_margin = margin;
+    int rc = S->p->prec == 4 ? build_wall_lists<float>(S, hcase<float>(S->p))
+                             : build_wall_lists<double>(S, hcase<double>(S->p));
+    if (rc) return rc;
+    return upload_case(S->p);
+}
+
+extern "C" int pic_sim_wall_list_stats(pic_sim* S, int* n_bins, long long* n_entries, int* max_per_bin) {
+    PIC_REQUIRE(S && n_bins && n_entries && max_per_bin, "NULL argument");
+    const bool on = S->d_wall_lists != nullptr;
+    *n_bins = on ? div_up(S->g->m - 1, S->bin_w) * div_up(S->g->n - 1, S->bin_h) : 0;
+    *n_entries = on ? S->wall_entries : 0;
+    *max_per_bin = on ? S->wall_max_bin : 0;
+    return 0;
+}
+
 extern "C" int pic_sim_destroy(pic_sim* S) {
     if (!S) return 0;
     (void)ensemble_forget(S);
@@ -1444,6 +1541,7 @@ extern "C" int pic_sim_destroy(pic_sim* S) {
     for (auto* a : S->acc) (void)hipFree(a);
     for (auto* t : S->tabs) (void)hipFree(t);
     (void)hipFree(S->d_walls_cell);
+    (void)hipFree(S->d_wall_lists);
     (void)hipFree(S->d_items);
     (void)hipFree(S->d_status);
     (void)hipFree(S->phi_prev);
@@ -1458,6 +1556,10 @@ extern "C" int pic_sim_destroy(pic_sim* S) {
             S->p->hf.sp[s].n_proc = 0;
             S->p->hd.sp[s].n_proc = 0;
         }
+        S->p->hf.n_walls = S->p->hd.n_walls = 0;
+        S->p->hf.walls = S->p->hd.walls = nullptr;
+        S->p->hf.wall_off = S->p->hd.wall_off = nullptr;
+        S->p->hf.wall_idx = S->p->hd.wall_idx = nullptr;
         (void)upload_case(S->p);
     }
     delete S;

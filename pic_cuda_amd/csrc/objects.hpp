@@ -105,6 +105,13 @@ struct pic_sim {
     std::vector<std::vector<double>> tab_host;
     std::vector<pic_mcc_info> info;
     double* d_walls_cell = nullptr;
+    // per-bin candidate walls of the current bin tile: offsets (bins + 1)
+    // then wall indices, one allocation; rebuilt on every re-binning
+    int* d_wall_lists = nullptr;
+    bool wall_lists = true;
+    int wall_margin = 2;
+    long long wall_entries = 0;
+    int wall_max_bin = 0;
     pic::WorkItem* d_items = nullptr;
     int n_items = 0;
     int spill_chunk = 4096;
@@ -121,12 +128,12 @@ struct pic_sim {
     pic::ChargeArgs charge{};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     struct pic_ensemble* ens = nullptr;   // batch this simulation belongs to
-    // graph mode: the instantiated step and the launch parameters it was
-    // captured with (a change of any of them captures it again)
     // potential before the last step's solve: with the charge, the last
     // warm-started solve, which the tuner times
     double* phi_prev = nullptr;
     bool have_prev = false;
+    // graph mode: the instantiated step and the launch parameters it was
+    // captured with (a change of any of them captures it again)
     bool use_graph = false;
     hipGraphExec_t gexec = nullptr;
     const void* g_items = nullptr;

@@ -55,6 +55,13 @@ struct CaseDev {
     T dtdx, dtdy;
     const double* phi;
     const double* walls;       // cell units x1,y1,x2,y2
+    // per-bin candidate walls (CSR over the bins; null: every particle tests
+    // every wall): the walls whose bounding box comes within wall_margin + 1
+    // cells of the bin, for particles whose path stays within wall_margin cells
+    const int* wall_off;       // nbx*nby + 1
+    const int* wall_idx;
+    int wall_margin;
+    int wall_pad;
     uint32_t seed, case_id;
     SpeciesDev<T> sp[PIC_MAX_SPECIES];
 };

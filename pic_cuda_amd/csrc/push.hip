@@ -113,22 +113,62 @@ __device__ inline int bin_key(const CaseDev<T>& C, int nb, T x, T y) {
     return (iy / C.bh) * C.nbx + (ix / C.bw);
 }
 
+// segment-segment intersection of collisions.cu:34-41
+__device__ inline bool segment_hits(const double* w, double X1, double Y1, double X2, double Y2, double eps) {
+    const double x3 = w[0], y3 = w[1], x4 = w[2], y4 = w[3];
+    const double den = (y4 - y3) * (X2 - X1) - (x4 - x3) * (Y2 - Y1);
+    if (fabs(den) < eps) return false;
+    const double ua = ((x4 - x3) * (Y1 - y3) - (y4 - y3) * (X1 - x3)) / den;
+    const double ub = ((X2 - X1) * (Y1 - y3) - (Y2 - Y1) * (X1 - x3)) / den;
+    return ua >= 0.0 && ua <= 1.0 && ub >= 0.0 && ub <= 1.0;
+}
+
+// Absorption test of one path (box edges, then the segment test of
+// detect_intersections, collisions.cu:7-54). The walls come from the per-bin
+// candidate lists, each holding every wall that meets its bin grown by
+// wall_margin + 1 cells:
+//  * a path with both ends within wall_margin cells of bin `bin` (cell origin
+//    cx0, cy0; the common case, uniform over the workgroup) tests that bin's
+//    list;
+//  * any other path tests the lists of every bin its bounding box meets
+//    (every point of the path lies in one of them), at most 16 bins;
+//  * a longer path, one outside the binned area, or no lists: every wall.
+// All three give the full scan's answer.
 template <typename T>
-__device__ inline bool hits_walls(const CaseDev<T>& C, T x0, T y0, T x1, T y1) {
+__device__ inline bool hits_walls(const CaseDev<T>& C, int bin, int cx0, int cy0, T x0, T y0, T x1, T y1) {
     if (C.box) {
         if (x1 <= (T)0 || x1 >= (T)(C.nx - 1) || y1 <= (T)0 || y1 >= (T)(C.ny - 1)) return true;
     }
     if (C.n_walls > 0) {
         const double X1 = x0, Y1 = y0, X2 = x1, Y2 = y1;
-        for (int j = 0; j < C.n_walls; ++j) {
-            const double* w = C.walls + 4 * j;
-            const double x3 = w[0], y3 = w[1], x4 = w[2], y4 = w[3];
-            const double den = (y4 - y3) * (X2 - X1) - (x4 - x3) * (Y2 - Y1);
-            if (fabs(den) < C.wall_eps) continue;
-            const double ua = ((x4 - x3) * (Y1 - y3) - (y4 - y3) * (X1 - x3)) / den;
-            const double ub = ((X2 - X1) * (Y1 - y3) - (Y2 - Y1) * (X1 - x3)) / den;
-            if (ua >= 0.0 && ua <= 1.0 && ub >= 0.0 && ub <= 1.0) return true;
+        const double rx0 = cx0 - C.wall_margin, ry0 = cy0 - C.wall_margin;
+        const double rx1 = cx0 + C.bw + C.wall_margin, ry1 = cy0 + C.bh + C.wall_margin;
+        const bool inside = bin >= 0 && C.wall_off && X1 >= rx0 && X1 <= rx1 && Y1 >= ry0 && Y1 <= ry1 &&
+                            X2 >= rx0 && X2 <= rx1 && Y2 >= ry0 && Y2 <= ry1;
+        bool hit = false;
+        if (inside) {
+            const int o0 = G(C.wall_off)[bin], o1 = G(C.wall_off)[bin + 1];
+            for (int k = o0; k < o1; ++k)
+                hit |= segment_hits(C.walls + 4 * G(C.wall_idx)[k], X1, Y1, X2, Y2, C.wall_eps);
+            return hit;
         }
+        const double lx = fmin(X1, X2), hx = fmax(X1, X2), ly = fmin(Y1, Y2), hy = fmax(Y1, Y2);
+        if (C.wall_off && lx >= 0.0 && ly >= 0.0 && hx < (double)(C.nbx * C.bw) && hy < (double)(C.nby * C.bh)) {
+            const int bx0 = (int)(lx / C.bw), bx1 = (int)(hx / C.bw);
+            const int by0 = (int)(ly / C.bh), by1 = (int)(hy / C.bh);
+            if ((bx1 - bx0 + 1) * (by1 - by0 + 1) <= 16) {
+                for (int by = by0; by <= by1; ++by)
+                    for (int bx = bx0; bx <= bx1; ++bx) {
+                        const int b = by * C.nbx + bx;
+                        const int o0 = G(C.wall_off)[b], o1 = G(C.wall_off)[b + 1];
+                        for (int k = o0; k < o1; ++k)
+                            hit |= segment_hits(C.walls + 4 * G(C.wall_idx)[k], X1, Y1, X2, Y2, C.wall_eps);
+                    }
+                return hit;
+            }
+        }
+        for (int j = 0; j < C.n_walls; ++j) hit |= segment_hits(C.walls + 4 * j, X1, Y1, X2, Y2, C.wall_eps);
+        return hit;
     }
     return false;
 }
@@ -466,6 +506,7 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
 
     const T kq = S.kq, dtdx = C.dtdx, dtdy = C.dtdy;
     const SpeciesDev<T>* O = (S.other >= 0) ? &C.sp[S.other] : nullptr;
+    const int wall_bin = is_bin ? it.bin : -1;
     const int home_start = is_bin ? G(S.start[out])[it.bin] : 0;
     const int home_front = is_bin ? front_cap(G(S.cap[out])[it.bin]) : 0;
     const int lane = __lane_id();
@@ -523,7 +564,7 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
                 x = dfma(vx, dtdx, x);
                 y = dfma(vy, dtdy, y);
             }
-            if (hits_walls(C, x0, y0, x, y)) {
+            if (hits_walls(C, wall_bin, cx0, cy0, x0, y0, x, y)) {
                 atomicAdd(lcnt + PIC_C_ABSORBED, 1u);
             } else {
                 alive = true;
@@ -533,7 +574,7 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
             if (born) {
                 // born particles drift from the collision point when the pass drifts
                 const T ex1 = drift ? dfma(b.evx, dtdx, b.ex) : b.ex, ey1 = drift ? dfma(b.evy, dtdy, b.ey) : b.ey;
-                if (hits_walls(C, b.ex, b.ey, ex1, ey1)) {
+                if (hits_walls(C, wall_bin, cx0, cy0, b.ex, b.ey, ex1, ey1)) {
                     atomicAdd(lcnt + PIC_C_BORN_ABSORBED, 1u);
                 } else {
                     be_alive = true;
@@ -544,7 +585,7 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
                     atomicAdd(lcnt + PIC_C_BORN_SELF, 1u);
                 }
                 const T ix1 = drift ? dfma(b.ivx, dtdx, b.ix) : b.ix, iy1 = drift ? dfma(b.ivy, dtdy, b.iy) : b.iy;
-                if (hits_walls(C, b.ix, b.iy, ix1, iy1)) {
+                if (hits_walls(C, wall_bin, cx0, cy0, b.ix, b.iy, ix1, iy1)) {
                     atomicAdd(lcnt + PIC_C_BORN_ABSORBED, 1u);
                 } else if (O) {
                     bi_alive = true;
