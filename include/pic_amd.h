@@ -177,6 +177,29 @@ int pic_xs_add_analytic(pic_xs* xs, int species, int type, double threshold_eV, 
 /* uniform energy table the processes are sampled on (default 2048 points, 200 eV) */
 int pic_xs_set_table(pic_xs* xs, int points, double emax_eV);
 int pic_xs_destroy(pic_xs* xs);
+/* ---- cross-section files beyond mcc.cu:17-41 (no counterpart in the
+   reference, which reads one two-column table per process) ---------------- */
+/* as pic_read_cross_section, but numbers may also be written m*10^{e} or
+   m*10^e (csf/Ar_conv.txt, whose rows read_cross_section drops); both
+   spellings of a value give the same double */
+int pic_read_table(const char* path, int cap, double* energies, double* sigmas, int* n);
+/* every block of an LXCat file as a tabulated process of `species`:
+   ELASTIC / EFFECTIVE / ISOTROPIC -> elastic, BACKSCAT -> charge exchange,
+   EXCITATION / IONIZATION -> inelastic with the block's threshold (ionization
+   products go to product_species); ATTACHMENT blocks are skipped and counted.
+   All or nothing: a malformed block, or more than 4 processes for the
+   species, adds none. */
+int pic_xs_load_lxcat(pic_xs* xs, const char* path, int species, int product_species, int* n_added,
+                      int* n_skipped);
+typedef struct {
+    int species, type, product_species;
+    int n_points;          /* table rows; 0 for an analytic process */
+    double threshold_eV;
+    double peak_m2;        /* analytic processes */
+} pic_xs_process;
+int pic_xs_process_count(const pic_xs* xs, int* n);
+/* process k; energies/sigmas (capacity cap) may be NULL */
+int pic_xs_get_process(const pic_xs* xs, int k, pic_xs_process* out, int cap, double* energies, double* sigmas);
 /* elasticCollisions(particles, density, dt, xs) mcc.cu:114-150: one null-collision
    elastic pass over a species with a tabulated sigma, keyed on (id, step) */
 int pic_elastic_collisions(pic_particles* p, int species, double density, double dt, int n,
@@ -240,6 +263,11 @@ int pic_sim_sync(pic_sim* s);
 int pic_sim_counters(pic_sim* s, int species, uint64_t out[16]);
 int pic_sim_stats(pic_sim* s, pic_sim_report* out);
 int pic_sim_mcc_info(pic_sim* s, int species, pic_mcc_info* out, double* table, int cap);
+/* the same constants and table (n_proc x K rows of n_gas sigma on the uniform
+   energy grid) that pic_sim_create builds for `species` of a cross-section
+   set, from the species masses (kg) and dt; host only, no device needed */
+int pic_xs_build(const pic_xs* xs, int species, int n_species, const double* masses, double dt, pic_mcc_info* out,
+                 int cap, double* table);
 /* re-bin now with a new tile (0 keeps the current one) */
 int pic_sim_rebin(pic_sim* s, int bin_w, int bin_h);
 /* graph mode (no counterpart in the reference, which launches every kernel

@@ -95,6 +95,11 @@ class JobDesc(ctypes.Structure):
                 ("arrival_s", ctypes.c_double), ("mcc", ctypes.c_int)]
 
 
+class XsProcess(ctypes.Structure):
+    _fields_ = [("species", ctypes.c_int), ("type", ctypes.c_int), ("product_species", ctypes.c_int),
+                ("n_points", ctypes.c_int), ("threshold_eV", ctypes.c_double), ("peak_m2", ctypes.c_double)]
+
+
 class JobResult(ctypes.Structure):
     _fields_ = [("state", ctypes.c_int), ("arrival_s", ctypes.c_double), ("admit_s", ctypes.c_double),
                 ("done_s", ctypes.c_double), ("steps_done", ctypes.c_int), ("step_ms_mean", ctypes.c_double),
@@ -144,6 +149,11 @@ SIGNATURES = {
     "pic_xs_add_analytic": (_i, [_vp, _i, _i, _d, _d, _i]),
     "pic_xs_set_table": (_i, [_vp, _i, _d]),
     "pic_xs_destroy": (_i, [_vp]),
+    "pic_read_table": (_i, [ctypes.c_char_p, _i, _vp, _vp, _P(_i)]),
+    "pic_xs_load_lxcat": (_i, [_vp, ctypes.c_char_p, _i, _i, _P(_i), _P(_i)]),
+    "pic_xs_process_count": (_i, [_vp, _P(_i)]),
+    "pic_xs_build": (_i, [_vp, _i, _i, _vp, _d, _P(MccInfo), _i, _vp]),
+    "pic_xs_get_process": (_i, [_vp, _i, _P(XsProcess), _i, _vp, _vp]),
     "pic_elastic_collisions": (_i, [_vp, _i, _d, _d, _i, _vp, _vp, _u32, _u32]),
     "pic_sim_default_params": (_i, [_P(SimParams)]),
     "pic_sim_create": (_i, [_vp, _vp, _vp, _vp, _P(SimParams), _P(_vp)]),
@@ -493,6 +503,25 @@ def read_cross_section(path):
     return e, s
 
 
+def read_table(path):
+    """Two-column table like read_cross_section, numbers also as m*10^{e};
+    returns (energies, sigmas)."""
+    n = ctypes.c_int()
+    _check(load().pic_read_table(path.encode(), 0, None, None, ctypes.byref(n)))
+    e = np.zeros(n.value)
+    s = np.zeros(n.value)
+    _check(load().pic_read_table(path.encode(), n.value, _ptr(e), _ptr(s), ctypes.byref(n)))
+    return e, s
+
+
+def _mcc_dict(info, tab):
+    return dict(n_proc=info.n_proc, K=info.K, type=list(info.type)[:info.n_proc],
+                threshold=list(info.threshold)[:info.n_proc], inv_de=info.inv_de, nu_max=info.nu_max,
+                p_null=info.p_null, efac=info.efac, gas_vth=info.gas_vth,
+                product_species=info.product_species, table=tab[:info.n_proc * info.K], emax=info.emax,
+                band_nu_max=list(info.band_nu_max), band_p_null=list(info.band_p_null))
+
+
 class CrossSections:
     """Background gas and collision processes (cross_sections, mcc.hpp:6-16)."""
 
@@ -517,6 +546,41 @@ class CrossSections:
     def add_analytic(self, species, ptype, threshold_eV, peak_m2, product_species=-1):
         _check(load().pic_xs_add_analytic(self._h, int(species), int(ptype), float(threshold_eV), float(peak_m2),
                                           int(product_species)))
+
+    def load_lxcat(self, path, species, product_species=-1):
+        """Every block of an LXCat file as a process of `species`; returns
+        (added, skipped attachment blocks)."""
+        a, k = _i(), _i()
+        _check(load().pic_xs_load_lxcat(self._h, path.encode(), int(species), int(product_species), ctypes.byref(a),
+                                        ctypes.byref(k)))
+        return a.value, k.value
+
+    def build(self, species, masses, dt):
+        """The null-collision constants and sampled table a simulation builds
+        for `species` (masses of every species in kg); host only."""
+        m = np.ascontiguousarray(masses, dtype=np.float64)
+        info = MccInfo()
+        _check(load().pic_xs_build(self._h, int(species), len(m), _ptr(m), float(dt), ctypes.byref(info), 0, None))
+        tab = np.zeros(max(1, info.n_proc * info.K))
+        _check(load().pic_xs_build(self._h, int(species), len(m), _ptr(m), float(dt), ctypes.byref(info), tab.size,
+                                   _ptr(tab)))
+        return _mcc_dict(info, tab)
+
+    def processes(self):
+        """The processes held, in order: dicts with species, type,
+        product_species, threshold_eV, peak_m2, energies, sigmas."""
+        n = _i()
+        _check(load().pic_xs_process_count(self._h, ctypes.byref(n)))
+        out = []
+        for k in range(n.value):
+            info = XsProcess()
+            _check(load().pic_xs_get_process(self._h, k, ctypes.byref(info), 0, None, None))
+            e = np.zeros(info.n_points)
+            s = np.zeros(info.n_points)
+            _check(load().pic_xs_get_process(self._h, k, ctypes.byref(info), info.n_points, _ptr(e), _ptr(s)))
+            out.append(dict(species=info.species, type=info.type, product_species=info.product_species,
+                            threshold_eV=info.threshold_eV, peak_m2=info.peak_m2, energies=e, sigmas=s))
+        return out
 
 
 def elasticCollisions(particles, species, density, dt, energies, sigmas, seed=1, step=0):
@@ -592,11 +656,7 @@ class Simulation:
         _check(load().pic_sim_mcc_info(self._h, int(species), ctypes.byref(info), None, 0))
         tab = np.zeros(max(1, info.n_proc * info.K))
         _check(load().pic_sim_mcc_info(self._h, int(species), ctypes.byref(info), _ptr(tab), tab.size))
-        return dict(n_proc=info.n_proc, K=info.K, type=list(info.type)[:info.n_proc],
-                    threshold=list(info.threshold)[:info.n_proc], inv_de=info.inv_de, nu_max=info.nu_max,
-                    p_null=info.p_null, efac=info.efac, gas_vth=info.gas_vth,
-                    product_species=info.product_species, table=tab[:info.n_proc * info.K], emax=info.emax,
-                    band_nu_max=list(info.band_nu_max), band_p_null=list(info.band_p_null))
+        return _mcc_dict(info, tab)
 
     def rebin(self, bin_w=0, bin_h=0):
         _check(load().pic_sim_rebin(self._h, int(bin_w), int(bin_h)))

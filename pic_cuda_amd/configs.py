@@ -47,10 +47,11 @@ def add_argon_processes(xs, electron=0, ion=1):
 
 
 def build(cells, n_per_species, precision, mcc, seed=1, case_id=0, mg=None, bin_w=16, bin_h=16, halo=2,
-          gas_density=1e21, gas_t=0.03, te=2.0, capacity=None, walls=None):
+          gas_density=1e21, gas_t=0.03, te=2.0, capacity=None, walls=None, processes=None):
     """Create particles, grid, walls, cross sections and the step driver.
     `walls`: internal wall segments (x1, y1, x2, y2) in cell units, added to
-    the absorbing box."""
+    the absorbing box. `processes(xs, electron, ion)` fills the cross-section
+    set (default: the synthetic e-Ar / Ar+-Ar set)."""
     pl = Plasma(te=te, cells=cells)
     nn = pl.nodes
     grid = Grid2D(nn, nn, pl.length, pl.length)
@@ -71,7 +72,7 @@ def build(cells, n_per_species, precision, mcc, seed=1, case_id=0, mg=None, bin_
     xs = None
     if mcc:
         xs = CrossSections(gas_density, gas_t)
-        add_argon_processes(xs, 0, ion)
+        (processes or add_argon_processes)(xs, 0, ion)
     if mg is None:
         mg = mg_params(PRESET_STEP_SOR, mode=MG_MIXED if precision == PREC_FP32 else MG_FP64)
     sim = Simulation(parts, grid, bnd, xs, sim_params(dt=pl.dt, seed=seed, case_id=case_id, bin_w=bin_w,

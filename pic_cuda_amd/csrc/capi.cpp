@@ -762,8 +762,8 @@ struct MccBuild {
 };
 
 // null-collision constants of one species: uniform table, and the majorant of
-// each energy band (16 samples per table interval of the interpolant inside
-// the band, and the band edges, x1.001), p_null = 1-exp(-nu_max dt)
+// each energy band (the exact supremum of the interpolated collision
+// frequency over the band, x1.001), p_null = 1-exp(-nu_max dt)
 static int build_mcc(const pic_xs* xs, const std::vector<SpeciesStore>& sp, int s, double dt, MccBuild& out) {
     std::vector<const pic_xs::Proc*> ps;
     for (auto& p : xs->procs)
@@ -812,25 +812,71 @@ static int build_mcc(const pic_xs* xs, const std::vector<SpeciesStore>& sp, int 
         }
         return nu;
     };
+    // Supremum of nu over [lo, hi]. Cut at the table nodes and thresholds in
+    // between; on each piece the active processes are fixed and their summed
+    // cross section is linear, A + B E, so nu = (A + B E) sqrt(2 E e / m), whose
+    // derivative vanishes only at E = -A / (3B) (a maximum when A > 0 > B).
+    auto band_max = [&](double lo, double hi) {
+        std::vector<double> cut = {lo, hi};
+        for (int j = (int)std::ceil(lo / de); j * de < hi && j <= K - 1; ++j)
+            if (j * de > lo) cut.push_back(j * de);
+        for (int q = 0; q < I.n_proc; ++q)
+            if (I.threshold[q] > lo && I.threshold[q] < hi) cut.push_back(I.threshold[q]);
+        std::sort(cut.begin(), cut.end());
+        double mx = 0.0;
+        for (double E : cut) mx = std::max(mx, nu_at(E));
+        for (size_t c = 0; c + 1 < cut.size(); ++c) {
+            const double a = cut[c], b = cut[c + 1], mid = 0.5 * (a + b);
+            if (!(b > a) || mid / de >= K - 1) continue;   // beyond the table nu grows with E: ends suffice
+            const int j = (int)std::floor(mid / de);
+            double A = 0.0, B = 0.0;
+            for (int q = 0; q < I.n_proc; ++q) {
+                if (mid < I.threshold[q]) continue;
+                const double s0 = out.tab[(size_t)q * K + j], s1 = out.tab[(size_t)q * K + j + 1];
+                A += s0 - j * (s1 - s0);
+                B += (s1 - s0) / de;
+            }
+            if (A > 0.0 && B < 0.0) {
+                const double Es = -A / (3.0 * B);
+                if (Es > a && Es < b) mx = std::max(mx, nu_at(Es));
+            }
+        }
+        return mx;
+    };
     const double emax = xs->emax;
     I.emax = emax;
     double numax = 0.0;
     for (int b = 0; b < 16; ++b) {
         const double lo = b == 0 ? 0.0 : std::ldexp(emax, b - 15);
         const double hi = b == 15 ? emax : std::ldexp(emax, b - 14);
-        double mx = std::max(nu_at(lo), nu_at(hi));
-        const int j0 = std::max(0, (int)std::floor(lo / de)), j1 = std::min(K - 2, (int)std::floor(hi / de));
-        for (int j = j0; j <= j1; ++j)
-            for (int k = 0; k <= 16; ++k) {
-                const double E = (j + k / 16.0) * de;
-                if (E >= lo && E <= hi) mx = std::max(mx, nu_at(E));
-            }
-        I.band_nu_max[b] = mx * 1.001;
+        // x1.001 covers the rounding of the kernel's interpolation
+        I.band_nu_max[b] = band_max(lo, hi) * 1.001;
         I.band_p_null[b] = -std::expm1(-I.band_nu_max[b] * dt);
         numax = std::max(numax, I.band_nu_max[b]);
     }
     I.nu_max = numax;
     I.p_null = -std::expm1(-I.nu_max * dt);
+    return 0;
+}
+
+extern "C" int pic_xs_build(const pic_xs* xs, int species, int n_species, const double* masses, double dt,
+                            pic_mcc_info* out, int cap, double* table) {
+    PIC_REQUIRE(xs && masses && out, "NULL argument");
+    PIC_REQUIRE(n_species >= 1 && n_species <= PIC_MAX_SPECIES && species >= 0 && species < n_species,
+                "bad species");
+    PIC_REQUIRE(dt > 0, "dt must be positive");
+    std::vector<SpeciesStore> sp(n_species);
+    for (int i = 0; i < n_species; ++i) {
+        PIC_REQUIRE(masses[i] > 0, "masses must be positive");
+        sp[i].m = masses[i];
+    }
+    MccBuild mb;
+    if (int rc = build_mcc(xs, sp, species, dt, mb)) return rc;
+    *out = mb.info;
+    if (table) {
+        PIC_REQUIRE(cap >= (int)mb.tab.size(), "table capacity too small");
+        std::copy(mb.tab.begin(), mb.tab.end(), table);
+    }
     return 0;
 }
 

@@ -47,8 +47,7 @@ def case_inputs(prec):
     def mcc(m, tabs, types, thr, partner_mass, product):
         # majorant of the interpolated collision frequency per energy band:
         # band 0 below emax*2^-14, band b in [emax*2^(b-15), emax*2^(b-14)),
-        # band 15 up to emax; sampled 16 times per table interval and at the
-        # band edges, x1.001
+        # band 15 up to emax; the supremum over the band, x1.001
         tab = np.stack(tabs) * gas
         efac = 0.5 * m / QE
 
@@ -61,17 +60,33 @@ def case_inputs(prec):
             v = math.sqrt(2 * e * QE / m)
             return sum((tab[q, j] + t * (tab[q, j + 1] - tab[q, j])) * v for q in range(len(types)) if e >= thr[q])
 
+        def band_max(lo, hi):
+            # supremum over [lo, hi]: cut at table nodes and thresholds; on a
+            # piece sigma_tot = A + B E, so nu = (A + B E) sqrt(E) peaks inside
+            # only at E = -A / (3B) when A > 0 > B
+            cut = [lo, hi] + [j * de for j in range(int(math.ceil(lo / de)), K) if lo < j * de < hi]
+            cut += [t for t in thr if lo < t < hi]
+            cut.sort()
+            mx = max(nu(e) for e in cut)
+            for a, b in zip(cut[:-1], cut[1:]):
+                mid = 0.5 * (a + b)
+                if not b > a or mid / de >= K - 1:
+                    continue
+                j = int(math.floor(mid / de))
+                A = B = 0.0
+                for q in range(len(types)):
+                    if mid >= thr[q]:
+                        A += tab[q, j] - j * (tab[q, j + 1] - tab[q, j])
+                        B += (tab[q, j + 1] - tab[q, j]) / de
+                if A > 0.0 and B < 0.0 and a < -A / (3.0 * B) < b:
+                    mx = max(mx, nu(-A / (3.0 * B)))
+            return mx
+
         band_nu = []
         for b in range(16):
             lo = 0.0 if b == 0 else math.ldexp(emax, b - 15)
             hi = emax if b == 15 else math.ldexp(emax, b - 14)
-            mx = max(nu(lo), nu(hi))
-            for j in range(max(0, int(math.floor(lo / de))), min(K - 2, int(math.floor(hi / de))) + 1):
-                for k in range(17):
-                    e = (j + k / 16.0) * de
-                    if lo <= e <= hi:
-                        mx = max(mx, nu(e))
-            band_nu.append(mx * 1.001)
+            band_nu.append(band_max(lo, hi) * 1.001)
         numax = max(band_nu)
         return dict(n_proc=len(types), type=list(types), threshold=list(thr), table=tab.ravel(), K=K,
                     inv_de=1.0 / de, nu_max=numax, p_null=-math.expm1(-numax * dt), efac=efac,
