@@ -311,6 +311,35 @@ def walls(n_walls=256, steps=20):
     return out
 
 
+def export(reps=20):
+    """Position export of the full step at scale (2e7 electrons): float pairs
+    into a preallocated device buffer (8 B read + 8 B written per particle),
+    and one frame file of both species under /dev/shm."""
+    import numpy as np
+    import pic_cuda_amd as pic
+    from pic_cuda_amd import configs
+    c = configs.full_step()
+    sim, parts = c["sim"], c["particles"]
+    sim.step(3)
+    pic.synchronize()
+    dst, n = parts.export_positions(0)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        parts.export_positions(0, dst)
+        ts.append(time.perf_counter() - t0)
+    t = float(np.median(ts))
+    path = "/dev/shm/pic_bench_frame.bin"
+    t0 = time.perf_counter()
+    sim.write_frame(path)
+    frame_s = time.perf_counter() - t0
+    size = os.path.getsize(path)
+    os.remove(path)
+    return {"particles": n, "export_ms": round(1e3 * t, 4), "export_GBps": round(16.0 * n / t / 1e9, 1),
+            "export_frac_hbm": round(16.0 * n / t / 1e9 / HBM_PEAK_GBS, 3),
+            "frame_ms": round(1e3 * frame_s, 2), "frame_MB": round(size / 1e6, 1)}
+
+
 def serve(n_jobs=24, rate=20.0, budget=200e9, seed=9, max_steps=2000):
     """Serving: Poisson arrivals (rate jobs/s), cells log-uniform 64..1024
     (rounded to GRID_SIZES), 100 particles per cell and species, steps U(200, max),
@@ -358,16 +387,16 @@ def main():
     ap.add_argument("--bin", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="all",
-                    choices=["all", "step", "poisson", "ensemble", "serve", "graph", "tune", "walls"])
+                    choices=["all", "step", "poisson", "ensemble", "serve", "graph", "tune", "walls", "export"])
     ap.add_argument("--ens-steps", type=int, default=500)
     ap.add_argument("--serve-jobs", type=int, default=24)
     a = ap.parse_args()
-    if a.workload in ("poisson", "ensemble", "serve", "graph", "tune", "walls"):
+    if a.workload in ("poisson", "ensemble", "serve", "graph", "tune", "walls", "export"):
         import pic_cuda_amd as pic
         pic.load()
         fn = {"poisson": lambda: poisson(cpu=not a.no_cpu), "ensemble": lambda: ensemble(steps=a.ens_steps),
               "serve": lambda: serve(n_jobs=a.serve_jobs), "graph": lambda: graph(), "tune": lambda: tune(),
-              "walls": lambda: walls()}[a.workload]
+              "walls": lambda: walls(), "export": lambda: export()}[a.workload]
         print(json.dumps({"workload": a.workload, **fn()}))
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

@@ -76,6 +76,12 @@ typedef struct pic_sim pic_sim;
 const char* pic_last_error(void);
 int pic_version(int* major, int* minor);
 int pic_device_synchronize(void);
+/* device memory for callers without an allocator of their own (the float
+   pair stream of pic_particles_export_positions, say); copies to the host
+   wait for the library's work first */
+int pic_device_alloc(int64_t bytes, void** out);
+int pic_device_free(void* ptr);
+int pic_device_to_host(void* host, const void* dev, int64_t bytes);
 
 /* ---- Particles2D: particles.cuh:3-39, particles.cu:112-261 --------------- */
 /* Particles2D(int num_particles) particles.cu:112-150.  Species 0 exists with
@@ -99,6 +105,15 @@ int pic_particles_upload(pic_particles* p, int species, int64_t n, const double*
 /* device -> host copy of a species (order unspecified; sort by id to compare) */
 int pic_particles_download(pic_particles* p, int species, int64_t cap, double* x, double* y, double* vx,
                            double* vy, uint32_t* id, int64_t* n);
+/* convertPositionsToFloat (kernels.cu:268-273) as used by updateVBO
+   (render.h:384-405): the positions of every live particle of `species` as
+   float (x, y) pairs packed into the device buffer dst_device (room for cap
+   pairs; NULL to count), in cell units (units 0) or domain units (1, needs the
+   particles attached to a simulation); the order is unspecified. *n = live
+   particles. Reads the store only: a simulation being shown is not synced or
+   changed. Ordered on the library stream and finished before return. */
+int pic_particles_export_positions(pic_particles* p, int species, float* dst_device, int64_t cap, int units,
+                                   int64_t* n);
 /* seeded initial state on the device (replaces initializeParticles, main.cu:17-32):
    uniform in the open box [0,nx-1]x[0,ny-1], Maxwellian sigma = vth per component */
 int pic_particles_init_maxwellian(pic_particles* p, int species, int64_t n, int nx, int ny, double vth,
@@ -285,6 +300,12 @@ int pic_sim_use_graph(pic_sim* s, int on);
    wall_list_stats reports the bins, the list entries and the longest list
    (all 0 without internal walls or with the lists off). */
 int pic_sim_set_wall_lists(pic_sim* s, int on, int margin);
+/* one frame of every species' positions (domain units) into a binary file,
+   replaced atomically (a path under /dev/shm serves a shared-memory reader):
+   "PICF", u32 version 1, u64 step, u32 species, u32 units 1, f64 width,
+   f64 height, then per species u64 count and count float pairs; little-endian.
+   No counterpart in the reference, which draws the pairs with OpenGL. */
+int pic_sim_write_frame(pic_sim* s, const char* path);
 int pic_sim_wall_list_stats(pic_sim* s, int* n_bins, long long* n_entries, int* max_per_bin);
 int pic_sim_destroy(pic_sim* s);
 

@@ -114,6 +114,9 @@ SIGNATURES = {
     "pic_last_error": (ctypes.c_char_p, []),
     "pic_version": (_i, [_P(_i), _P(_i)]),
     "pic_device_synchronize": (_i, []),
+    "pic_device_alloc": (_i, [_i64, _P(_vp)]),
+    "pic_device_free": (_i, [_vp]),
+    "pic_device_to_host": (_i, [_vp, _vp, _i64]),
     "pic_particles_create": (_i, [_i64, _i, _P(_vp)]),
     "pic_particles_destroy": (_i, [_vp]),
     "pic_particles_add_species": (_i, [_vp, _d, _d, _P(_i)]),
@@ -165,6 +168,8 @@ SIGNATURES = {
     "pic_sim_rebin": (_i, [_vp, _i, _i]),
     "pic_sim_use_graph": (_i, [_vp, _i]),
     "pic_sim_set_wall_lists": (_i, [_vp, _i, _i]),
+    "pic_sim_write_frame": (_i, [_vp, ctypes.c_char_p]),
+    "pic_particles_export_positions": (_i, [_vp, _i, _vp, _i64, _i, _P(_i64)]),
     "pic_sim_wall_list_stats": (_i, [_vp, _P(_i), _P(ctypes.c_longlong), _P(_i)]),
     "pic_sim_autotune": (_i, [_vp, _i, ctypes.POINTER(TuneReport)]),
     "pic_sim_set_level_sweeps": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
@@ -234,6 +239,41 @@ def _devptr(t):
 
 def synchronize():
     _check(load().pic_device_synchronize())
+
+
+class DeviceBuffer:
+    """Device memory from the library (no torch needed): `ptr` for the entry
+    points that take device pointers, to_numpy() to read it back."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self._p = ctypes.c_void_p()
+        _check(load().pic_device_alloc(self.nbytes, ctypes.byref(self._p)))
+
+    @property
+    def ptr(self):
+        return self._p.value
+
+    def data_ptr(self):
+        return self._p.value
+
+    def to_numpy(self, dtype=np.float32, count=None):
+        dt = np.dtype(dtype)
+        n = self.nbytes // dt.itemsize if count is None else int(count)
+        out = np.empty(n, dtype=dt)
+        _check(load().pic_device_to_host(out.ctypes.data_as(ctypes.c_void_p), self._p, n * dt.itemsize))
+        return out
+
+    def close(self):
+        if self._p.value and _lib is not None:
+            _lib.pic_device_free(self._p)
+            self._p = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------------------
@@ -334,6 +374,20 @@ class Particles2D:
         k = n.value
         sx, sy = self._scale(units)
         return x[:k] * sx, y[:k] * sy, vx[:k], vy[:k], ids[:k]
+
+    def export_positions(self, species, dst=None, units="cell"):
+        """Live positions as float32 (x, y) pairs on the device
+        (convertPositionsToFloat, kernels.cu:268-273): into `dst` (a
+        DeviceBuffer) or, with dst None, a new DeviceBuffer of the right size.
+        Returns (buffer, count); the order is unspecified."""
+        u = {"cell": 0, "domain": 1}[units]
+        n = _i64()
+        if dst is None:
+            _check(load().pic_particles_export_positions(self._h, int(species), None, 0, u, ctypes.byref(n)))
+            dst = DeviceBuffer(8 * max(1, n.value))
+        _check(load().pic_particles_export_positions(self._h, int(species), _devptr(dst.ptr), dst.nbytes // 8, u,
+                                                     ctypes.byref(n)))
+        return dst, n.value
 
     def init_maxwellian(self, species, n, nx, ny, vth, seed=1, case_id=0, id0=0):
         _check(load().pic_particles_init_maxwellian(self._h, int(species), int(n), int(nx), int(ny), float(vth),
@@ -514,6 +568,25 @@ def read_table(path):
     return e, s
 
 
+def read_frame(path):
+    """A frame file of Simulation.write_frame: dict(step, width, height,
+    positions=[float32 array (n, 2) per species])."""
+    raw = open(path, "rb").read()
+    if raw[:4] != b"PICF":
+        raise ValueError("not a frame file: %s" % path)
+    version, step, ns, units = np.frombuffer(raw, np.uint32, 1, 4)[0], np.frombuffer(raw, np.uint64, 1, 8)[0], \
+        np.frombuffer(raw, np.uint32, 1, 16)[0], np.frombuffer(raw, np.uint32, 1, 20)[0]
+    if version != 1 or units != 1:
+        raise ValueError("unsupported frame version or units")
+    width, height = np.frombuffer(raw, np.float64, 2, 24)
+    off, pos = 40, []
+    for _ in range(int(ns)):
+        c = int(np.frombuffer(raw, np.uint64, 1, off)[0])
+        pos.append(np.frombuffer(raw, np.float32, 2 * c, off + 8).reshape(c, 2).copy())
+        off += 8 + 8 * c
+    return dict(step=int(step), width=float(width), height=float(height), positions=pos)
+
+
 def _mcc_dict(info, tab):
     return dict(n_proc=info.n_proc, K=info.K, type=list(info.type)[:info.n_proc],
                 threshold=list(info.threshold)[:info.n_proc], inv_de=info.inv_de, nu_max=info.nu_max,
@@ -664,6 +737,11 @@ class Simulation:
     def use_graph(self, on=True):
         """Run each step after the first as one graph launch."""
         _check(load().pic_sim_use_graph(self._h, 1 if on else 0))
+
+    def write_frame(self, path):
+        """Every species' positions (domain units) into a frame file, replaced
+        atomically; read it back with read_frame."""
+        _check(load().pic_sim_write_frame(self._h, str(path).encode()))
 
     def set_wall_lists(self, on=True, margin=2):
         """Per-bin candidate walls for the wall test (on by default): each bin

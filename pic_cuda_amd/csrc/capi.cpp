@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <map>
@@ -179,6 +180,26 @@ extern "C" int pic_version(int* major, int* minor) {
 
 extern "C" int pic_device_synchronize(void) {
     PIC_HIP(hipDeviceSynchronize());
+    return 0;
+}
+
+extern "C" int pic_device_alloc(int64_t bytes, void** out) {
+    PIC_REQUIRE(out && bytes > 0, "bad size or NULL argument");
+    *out = nullptr;
+    PIC_HIP(hipMalloc(out, (size_t)bytes));
+    return 0;
+}
+
+extern "C" int pic_device_free(void* ptr) {
+    PIC_HIP(hipStreamSynchronize(lib_stream()));
+    PIC_HIP(hipFree(ptr));
+    return 0;
+}
+
+extern "C" int pic_device_to_host(void* host, const void* dev, int64_t bytes) {
+    PIC_REQUIRE(host && dev && bytes >= 0, "NULL argument or negative size");
+    PIC_HIP(hipStreamSynchronize(lib_stream()));
+    if (bytes) PIC_HIP(hipMemcpy(host, dev, (size_t)bytes, hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -382,6 +403,43 @@ extern "C" int pic_particles_download(pic_particles* p, int species, int64_t cap
     }
     return p->prec == 4 ? download_t<float>(p, species, cap, x, y, vx, vy, id, n)
                         : download_t<double>(p, species, cap, x, y, vx, vy, id, n);
+}
+
+template <typename T>
+static int export_t(pic_particles* p, int s, float* dst, int64_t cap, double sx, double sy, int64_t* nout) {
+    std::vector<WorkItem> items;
+    items_for_species(p, s, 4096, items);
+    if (int rc = upload_items(&p->d_items, &p->items_alloc, items)) return rc;
+    if (int rc = ensure_scratch(p, 1)) return rc;
+    hipStream_t sm = lib_stream();
+    PIC_HIP(hipMemsetAsync(p->d_scratch, 0, sizeof(int32_t), sm));
+    if (int rc = launch_export<T>((CaseDev<T>*)p->d_case, p->d_state, p->d_items, (int)items.size(), 4096,
+                                  p->d_scratch, (float2*)dst, dst ? cap : 0, sx, sy, sm))
+        return rc;
+    int32_t n = 0;
+    PIC_HIP(hipMemcpyAsync(&n, p->d_scratch, sizeof(n), hipMemcpyDeviceToHost, sm));
+    PIC_HIP(hipStreamSynchronize(sm));
+    *nout = n;
+    if (dst && n > cap) {
+        set_error("pic_particles_export_positions: output capacity too small");
+        return PIC_ERR_CAPACITY;
+    }
+    return 0;
+}
+
+extern "C" int pic_particles_export_positions(pic_particles* p, int species, float* dst_device, int64_t cap,
+                                              int units, int64_t* n) {
+    PIC_REQUIRE(p && n && species >= 0 && species < (int)p->sp.size(), "bad species or NULL argument");
+    PIC_REQUIRE(units == 0 || units == 1, "units: 0 cells, 1 domain");
+    PIC_REQUIRE(cap >= 0, "negative capacity");
+    double sx = 1.0, sy = 1.0;
+    if (units == 1) {
+        PIC_REQUIRE(p->attached, "domain units need the particles attached to a simulation (its grid spacing)");
+        sx = p->attached->g->dx;
+        sy = p->attached->g->dy;
+    }
+    return p->prec == 4 ? export_t<float>(p, species, dst_device, cap, sx, sy, n)
+                        : export_t<double>(p, species, dst_device, cap, sx, sy, n);
 }
 
 template <typename T>
@@ -1554,6 +1612,55 @@ extern "C" int pic_sim_use_graph(pic_sim* S, int on) {
     return 0;
 }
 
+// frame file: "PICF", u32 version 1, u64 step, u32 species, u32 units (1:
+// domain), f64 width, f64 height, then per species u64 count and count float
+// pairs (x, y); little-endian. Written to path + ".tmp" and renamed, so a
+// reader never sees a partial frame.
+extern "C" int pic_sim_write_frame(pic_sim* S, const char* path) {
+    PIC_REQUIRE(S && path, "NULL argument");
+    pic_particles* p = S->p;
+    const int ns = (int)p->sp.size();
+    std::vector<std::vector<float>> xy(ns);
+    for (int s = 0; s < ns; ++s) {
+        int64_t n = 0;
+        if (int rc = pic_particles_export_positions(p, s, nullptr, 0, 1, &n)) return rc;
+        if (n > S->frame_cap) {
+            (void)hipFree(S->d_frame);
+            S->d_frame = nullptr;
+            S->frame_cap = 0;
+            PIC_HIP(hipMalloc(&S->d_frame, (size_t)n * 2 * sizeof(float)));
+            S->frame_cap = n;
+        }
+        int64_t n2 = 0;
+        if (int rc = pic_particles_export_positions(p, s, S->d_frame, S->frame_cap, 1, &n2)) return rc;
+        xy[s].resize((size_t)n2 * 2);
+        if (n2) PIC_HIP(hipMemcpy(xy[s].data(), S->d_frame, (size_t)n2 * 2 * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE* f = std::fopen(tmp.c_str(), "wb");
+    if (!f) {
+        set_error(std::string("cannot write frame file: ") + tmp);
+        return PIC_ERR_IO;
+    }
+    const uint32_t version = 1, nsp = (uint32_t)ns, units = 1;
+    const uint64_t step = (uint64_t)S->steps;
+    const double wh[2] = {S->g->dx * (S->g->m - 1), S->g->dy * (S->g->n - 1)};
+    bool ok = std::fwrite("PICF", 1, 4, f) == 4 && std::fwrite(&version, 4, 1, f) == 1 &&
+              std::fwrite(&step, 8, 1, f) == 1 && std::fwrite(&nsp, 4, 1, f) == 1 && std::fwrite(&units, 4, 1, f) == 1 &&
+              std::fwrite(wh, 8, 2, f) == 2;
+    for (int s = 0; s < ns && ok; ++s) {
+        const uint64_t c = xy[s].size() / 2;
+        ok = std::fwrite(&c, 8, 1, f) == 1 && (c == 0 || std::fwrite(xy[s].data(), 8, c, f) == c);
+    }
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok || std::rename(tmp.c_str(), path) != 0) {
+        std::remove(tmp.c_str());
+        set_error(std::string("cannot write frame file: ") + path);
+        return PIC_ERR_IO;
+    }
+    return 0;
+}
+
 extern "C" int pic_sim_set_wall_lists(pic_sim* S, int on, int margin) {
     PIC_REQUIRE(S, "sim is NULL");
     PIC_REQUIRE(margin >= 0 && margin <= 64, "margin must be 0..64 cells");
@@ -1588,6 +1695,7 @@ extern "C" int pic_sim_destroy(pic_sim* S) {
     for (auto* t : S->tabs) (void)hipFree(t);
     (void)hipFree(S->d_walls_cell);
     (void)hipFree(S->d_wall_lists);
+    (void)hipFree(S->d_frame);
     (void)hipFree(S->d_items);
     (void)hipFree(S->d_status);
     (void)hipFree(S->phi_prev);

@@ -112,6 +112,9 @@ struct pic_sim {
     int wall_margin = 2;
     long long wall_entries = 0;
     int wall_max_bin = 0;
+    // frame export staging (float pairs), grown on demand
+    float* d_frame = nullptr;
+    int64_t frame_cap = 0;
     pic::WorkItem* d_items = nullptr;
     int n_items = 0;
     int spill_chunk = 4096;

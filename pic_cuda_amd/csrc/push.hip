@@ -12,6 +12,8 @@
 //  init_kernel    seeded uniform/Maxwellian particles (replaces main.cu:17-32).
 //  hist/scatter   re-binning of a species store (counting sort by tile).
 //  gather_kernel  flattening of a store for download.
+//  export_kernel  live positions as a packed float2 stream (replaces
+//                 convertPositionsToFloat, kernels.cu:268-273).
 //  end_step       buffer-parity flip and step counter, one block per case.
 //
 // Bin layout.  Bin b of a buffer is [start, start+cap): survivors that stay in
@@ -779,6 +781,7 @@ __global__ __launch_bounds__(256) void gather_kernel(const CaseDev<T>* __restric
                                                      const CaseState* __restrict__ states,
                                                      const WorkItem* __restrict__ items, int chunk,
                                                      int32_t* counter) {
+    __shared__ int sh_base;
     const WorkItem it = items[blockIdx.x];
     const CaseDev<T>& C = cases[it.case_idx];
     const SpeciesDev<T>& S = C.sp[it.species];
@@ -786,24 +789,16 @@ __global__ __launch_bounds__(256) void gather_kernel(const CaseDev<T>* __restric
     const Range rg = item_range(S, in, it, chunk);
     const int n = rg.n();
     if (n <= 0) return;
-    const int lane = __lane_id();
-    for (int base = 0; base < n; base += blockDim.x) {
-        const int t = base + threadIdx.x;
-        const bool want = t < n;
-        const uint64_t m = __ballot(want);
-        if (!m) continue;
-        const int leader = __ffsll((unsigned long long)m) - 1;
-        int b0 = 0;
-        if (lane == leader) b0 = atomicAdd(counter, __popcll(m));
-        b0 = __shfl(b0, leader);
-        if (want) {
-            const int s = rg.at(t), d = b0 + __popcll(m & lanemask_lt());
-            S.x[out][d] = S.x[in][s];
-            S.y[out][d] = S.y[in][s];
-            S.vx[out][d] = S.vx[in][s];
-            S.vy[out][d] = S.vy[in][s];
-            S.id[out][d] = S.id[in][s];
-        }
+    // one atomic per item: its particles stay contiguous
+    if (threadIdx.x == 0) sh_base = atomicAdd(counter, n);
+    __syncthreads();
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        const int s = rg.at(t), d = sh_base + t;
+        S.x[out][d] = S.x[in][s];
+        S.y[out][d] = S.y[in][s];
+        S.vx[out][d] = S.vx[in][s];
+        S.vy[out][d] = S.vy[in][s];
+        S.id[out][d] = S.id[in][s];
     }
 }
 
@@ -938,6 +933,50 @@ int launch_scatter(const CaseDev<T>* cases, const CaseState* states, const WorkI
     return 0;
 }
 
+// Positions of a store's live particles as (x*sx, y*sy) float pairs, packed
+// from dst[0]: each block takes its item's range with one atomic (so items
+// land in an unspecified order, each item's particles contiguous); pairs past
+// cap are counted, not written. The store is only read.
+template <typename T>
+__global__ __launch_bounds__(256) void export_kernel(const CaseDev<T>* __restrict__ cases,
+                                                     const CaseState* __restrict__ states,
+                                                     const WorkItem* __restrict__ items, int chunk,
+                                                     int32_t* counter, float2* __restrict__ dst, int64_t cap,
+                                                     double sx, double sy) {
+    __shared__ int sh_base;
+    const WorkItem it = items[blockIdx.x];
+    const CaseDev<T>& C = cases[it.case_idx];
+    const SpeciesDev<T>& S = C.sp[it.species];
+    const int in = states[it.case_idx].cur;
+    const Range rg = item_range(S, in, it, chunk);
+    const int n = rg.n();
+    if (n <= 0) return;
+    if (threadIdx.x == 0) sh_base = atomicAdd(counter, n);
+    __syncthreads();
+    const int64_t base = sh_base;
+    const gptr<const T> X = G((const T*)S.x[in]), Y = G((const T*)S.y[in]);
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        const int64_t d = base + t;
+        if (d < cap) {
+            const int s = rg.at(t);
+            float2 v;
+            v.x = (float)((double)X[s] * sx);
+            v.y = (float)((double)Y[s] * sy);
+            dst[d] = v;
+        }
+    }
+}
+
+template <typename T>
+int launch_export(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items, int chunk,
+                  int32_t* counter, float2* dst, int64_t cap, double sx, double sy, hipStream_t s) {
+    if (n_items <= 0) return 0;
+    hipLaunchKernelGGL(export_kernel<T>, dim3(n_items), dim3(256), 0, s, cases, states, items, chunk, counter, dst,
+                       cap, sx, sy);
+    PIC_HIP(hipGetLastError());
+    return 0;
+}
+
 template <typename T>
 int launch_gather(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items, int chunk,
                   int32_t* counter, hipStream_t s) {
@@ -988,6 +1027,8 @@ int launch_counts(const CaseDev<T>* cases, const CaseState* states, int c, int s
                                    unsigned long long*, hipStream_t);                                            \
     template int launch_gather<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int, int, int32_t*,       \
                                   hipStream_t);                                                                  \
+    template int launch_export<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int, int, int32_t*,       \
+                                  float2*, int64_t, double, double, hipStream_t);                                \
     template int launch_end_step<T>(const CaseDev<T>*, CaseState*, int, int, hipStream_t);                       \
     template int launch_clear_out<T>(const CaseDev<T>*, const CaseState*, int, hipStream_t);
 PIC_LAUNCH_INST(float)

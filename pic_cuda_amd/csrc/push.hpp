@@ -24,6 +24,9 @@ template <typename T>
 int launch_gather(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items, int chunk,
                   int32_t* counter, hipStream_t s);
 template <typename T>
+int launch_export(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items, int chunk,
+                  int32_t* counter, float2* dst, int64_t cap, double sx, double sy, hipStream_t s);
+template <typename T>
 int launch_end_step(const CaseDev<T>* cases, CaseState* states, int n_cases, int advance, hipStream_t s);
 // clear the bin counts of the output parity (after a trial push)
 template <typename T>
