@@ -127,6 +127,16 @@ public:
     void init_maxwellian(int species, int64_t n, const Grid2D& g, double vth, uint32_t seed, uint32_t id0 = 0) {
         check(pic_particles_init_maxwellian(h_, species, n, g.m, g.n, vth, seed, 0, id0), "init_maxwellian");
     }
+    // updateVBO / convertPositionsToFloat (render.h:384-405, kernels.cu:268-273):
+    // live positions as float (x, y) pairs into a device buffer (a mapped
+    // vertex buffer, say) of room for `cap` pairs; domain units need the
+    // particles attached to a Simulation. Returns the live count.
+    int64_t export_positions(int species, float* dst_device, int64_t cap, bool domain_units = true) {
+        int64_t n = 0;
+        check(pic_particles_export_positions(h_, species, dst_device, cap, domain_units ? 1 : 0, &n),
+              "export_positions");
+        return n;
+    }
     pic_particles* handle() const { return h_; }
     int64_t max_particles = 0;
 
@@ -172,6 +182,13 @@ public:
     void add_analytic(int species, int type, double threshold, double peak, int product = -1) {
         check(pic_xs_add_analytic(h_, species, type, threshold, peak, product), "add_analytic");
     }
+    // every block of an LXCat file as a process of `species`; returns the
+    // processes added (attachment blocks are skipped)
+    int load_lxcat(const std::string& path, int species, int product = -1) {
+        int added = 0, skipped = 0;
+        check(pic_xs_load_lxcat(h_, path.c_str(), species, product, &added, &skipped), "load_lxcat");
+        return added;
+    }
     pic_xs* handle() const { return h_; }
 
 private:
@@ -183,6 +200,15 @@ inline std::pair<std::vector<double>, std::vector<double>> read_cross_section(co
     check(pic_read_cross_section(filename.c_str(), 0, nullptr, nullptr, &n), "read_cross_section");
     std::vector<double> e(n), s(n);
     check(pic_read_cross_section(filename.c_str(), n, e.data(), s.data(), &n), "read_cross_section");
+    return {e, s};
+}
+
+// two columns like read_cross_section, numbers also written m*10^{e}
+inline std::pair<std::vector<double>, std::vector<double>> read_table(const std::string& filename) {
+    int n = 0;
+    check(pic_read_table(filename.c_str(), 0, nullptr, nullptr, &n), "read_table");
+    std::vector<double> e(n), s(n);
+    check(pic_read_table(filename.c_str(), n, e.data(), s.data(), &n), "read_table");
     return {e, s};
 }
 
@@ -218,6 +244,13 @@ public:
         std::vector<uint64_t> c(16);
         check(pic_sim_counters(h_, species, c.data()), "counters");
         return c;
+    }
+    void write_frame(const std::string& path) { check(pic_sim_write_frame(h_, path.c_str()), "write_frame"); }
+    void set_wall_lists(bool on, int margin = 2) { check(pic_sim_set_wall_lists(h_, on ? 1 : 0, margin), "wall_lists"); }
+    // embedded electrodes: n*n mask (0 fluid, 1 Dirichlet at phi_bc, 2 solid)
+    void set_electrodes(const std::vector<uint8_t>& mask, const std::vector<double>& phi_bc) {
+        check(pic_sim_set_electrodes(h_, mask.empty() ? nullptr : mask.data(), phi_bc.empty() ? nullptr : phi_bc.data()),
+              "set_electrodes");
     }
     pic_sim* handle() const { return h_; }
 

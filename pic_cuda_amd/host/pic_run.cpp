@@ -4,8 +4,14 @@
 //
 //   pic_run [--cells 128] [--particles 100000] [--steps 100] [--mcc 0|1]
 //           [--fp64] [--seed 1] [--te 2.0] [--gas 1e21]
+//           [--frames PATH --every K]
+//
+// With --frames, every K steps the positions are written to the frame file
+// PATH (replaced in place), where main.cu hands them to updateVBO every
+// UPDATE_STEPS_PER_FRAME steps.
 #include "pic2d.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -17,6 +23,8 @@ int main(int argc, char** argv) {
     long long n = 100000;
     unsigned seed = 1;
     double te = 2.0, gas = 1e21;
+    const char* frames = nullptr;
+    int every = 0;
     for (int i = 1; i < argc; ++i) {
         auto arg = [&](const char* k) { return std::strcmp(argv[i], k) == 0 && i + 1 < argc; };
         if (arg("--cells")) cells = std::atoi(argv[++i]);
@@ -26,6 +34,8 @@ int main(int argc, char** argv) {
         else if (arg("--seed")) seed = (unsigned)std::atol(argv[++i]);
         else if (arg("--te")) te = std::atof(argv[++i]);
         else if (arg("--gas")) gas = std::atof(argv[++i]);
+        else if (arg("--frames")) frames = argv[++i];
+        else if (arg("--every")) every = std::atoi(argv[++i]);
         else if (std::strcmp(argv[i], "--fp64") == 0) prec = PIC_PREC_FP64;
         else {
             std::fprintf(stderr, "unknown argument %s\n", argv[i]);
@@ -61,19 +71,29 @@ int main(int argc, char** argv) {
         prm.mcc = mcc;
         if (prec == PIC_PREC_FP64) prm.mg.mode = PIC_MG_FP64;
         pic2d::Simulation sim(particles, grid, &walls, mcc ? &xs : nullptr, prm);
+        if (frames && every <= 0) every = steps;
+        int n_frames = 0;
         auto t0 = std::chrono::steady_clock::now();
-        sim.step(steps);
+        if (frames) {
+            for (int done = 0; done < steps; done += every) {
+                sim.step(std::min(every, steps - done));
+                sim.write_frame(frames);
+                ++n_frames;
+            }
+        } else {
+            sim.step(steps);
+        }
         sim.sync();
         const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         const pic_sim_report r = sim.report();
         const auto ce = sim.counters(0), ci = sim.counters(1);
         std::printf("{\"cells\": %d, \"particles\": %lld, \"steps\": %d, \"seconds\": %.6f, \"alive\": %lld, "
                     "\"absorbed_e\": %llu, \"absorbed_i\": %llu, \"collisions_e\": %llu, \"mg_cycles\": %lld, "
-                    "\"particle_steps_per_s\": %.6g}\n",
+                    "\"particle_steps_per_s\": %.6g, \"frames\": %d}\n",
                     cells, 2 * n, steps, s, (long long)r.alive, (unsigned long long)ce[PIC_CNT_ABSORBED],
                     (unsigned long long)ci[PIC_CNT_ABSORBED],
                     (unsigned long long)(ce[2] + ce[3] + ce[4] + ce[5]), (long long)r.total_mg_cycles,
-                    2.0 * n * steps / s);
+                    2.0 * n * steps / s, n_frames);
     } catch (const pic2d::Error& e) {
         std::fprintf(stderr, "pic_run: %s\n", e.what());
         return 1;
