@@ -1034,6 +1034,14 @@ static int build_wall_lists(pic_sim* S, CaseDev<T>& C) {
     return 0;
 }
 
+// log2 of a power of two, -1 otherwise
+static int log2_exact(int v) {
+    if (v <= 0 || (v & (v - 1))) return -1;
+    int k = 0;
+    while ((1 << k) < v) ++k;
+    return k;
+}
+
 template <typename T>
 static int rebin_t(pic_sim* S, int bw, int bh) {
     pic_particles* p = S->p;
@@ -1142,6 +1150,8 @@ static int rebin_t(pic_sim* S, int bw, int bh) {
     }
     C.bw = bw;
     C.bh = bh;
+    C.bw_log2 = log2_exact(bw);
+    C.bh_log2 = log2_exact(bh);
     C.nbx = nbx;
     C.nby = nby;
     S->bin_w = bw;
@@ -1172,6 +1182,8 @@ static int setup_t(pic_sim* S) {
     C.ny = g->n;
     C.bw = S->bin_w;
     C.bh = S->bin_h;
+    C.bw_log2 = log2_exact(S->bin_w);
+    C.bh_log2 = log2_exact(S->bin_h);
     C.nbx = div_up(g->m - 1, S->bin_w);
     C.nby = div_up(g->n - 1, S->bin_h);
     C.halo = S->halo;
@@ -1346,7 +1358,7 @@ static int enqueue_step_body(pic_sim* S, int flags, size_t lds, int cycles) {
     hipStream_t sm = lib_stream();
     const size_t nn = (size_t)S->g->m * S->g->n;
     if (int rc = launch_push<T>((CaseDev<T>*)p->d_case, p->d_state, S->d_items, S->n_items, S->spill_chunk, flags, -1,
-                                lds, sm))
+                                lds, sm, S->d_walls_cell != nullptr))
         return rc;
     if (int rc = launch_end_step<T>((CaseDev<T>*)p->d_case, p->d_state, 1, 1, sm)) return rc;
     if (int rc = launch_status<T>((CaseDev<T>*)p->d_case, p->d_state, 1, S->d_status, sm)) return rc;
@@ -1488,7 +1500,7 @@ static int step_t(pic_sim* S, int n) {
         }
         PIC_HIP(hipEventRecord(S->ev[0], sm));
         if (int rc = launch_push<T>((CaseDev<T>*)p->d_case, p->d_state, S->d_items, S->n_items, S->spill_chunk, flags,
-                                    -1, lds, sm))
+                                    -1, lds, sm, S->d_walls_cell != nullptr))
             return rc;
         PIC_HIP(hipEventRecord(S->ev[1], sm));
         if (int rc = launch_end_step<T>((CaseDev<T>*)p->d_case, p->d_state, 1, 1, sm)) return rc;
@@ -1535,7 +1547,7 @@ static int sync_t(pic_sim* S) {
     hipStream_t sm = lib_stream();
     const int flags = PIC_F_KICK | (mcc_active(S) ? PIC_F_MCC : 0);
     if (int rc = launch_push<T>((CaseDev<T>*)p->d_case, p->d_state, S->d_items, S->n_items, S->spill_chunk, flags, -1,
-                                cur_lds(S), sm))
+                                cur_lds(S), sm, S->d_walls_cell != nullptr))
         return rc;
     if (int rc = launch_end_step<T>((CaseDev<T>*)p->d_case, p->d_state, 1, 0, sm)) return rc;
     p->h_cur ^= 1;
@@ -1767,7 +1779,7 @@ static int elastic_t(pic_particles* p, int species, double density, double dt, i
     const size_t lds = push_lds_bytes<T>(C.bw > 0 ? C.bw : 2, C.bh > 0 ? C.bh : 2, C.halo);
     if (!rc)
         rc = launch_push<T>((CaseDev<T>*)p->d_case, p->d_state, p->d_items, (int)items.size(), 4096, PIC_F_MCC,
-                            (long long)step, lds, lib_stream());
+                            (long long)step, lds, lib_stream(), C.n_walls > 0);
     if (!rc) rc = launch_end_step<T>((CaseDev<T>*)p->d_case, p->d_state, 1, 0, lib_stream());
     if (!rc) p->h_cur ^= 1;
     (void)hipStreamSynchronize(lib_stream());

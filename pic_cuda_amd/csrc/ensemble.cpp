@@ -207,8 +207,10 @@ template <typename T>
 static int enqueue_body(pic_ensemble* e, int cycles) {
     hipStream_t sm = lib_stream();
     const int M = (int)e->m.size();
+    bool walls = false;
+    for (const pic_sim* S : e->m) walls = walls || S->d_walls_cell != nullptr;
     if (int rc = launch_push<T>((CaseDev<T>*)e->d_cases, e->d_states, e->d_items, e->n_items, e->m[0]->spill_chunk,
-                                -1, -1, e->lds, sm))
+                                -1, -1, e->lds, sm, walls))
         return rc;
     if (int rc = launch_end_step<T>((CaseDev<T>*)e->d_cases, e->d_states, M, 1, sm)) return rc;
     if (int rc = launch_status<T>((CaseDev<T>*)e->d_cases, e->d_states, M, e->d_status, sm)) return rc;

@@ -46,6 +46,7 @@ template <typename T>
 struct CaseDev {
     int nx, ny;                // nodes
     int bw, bh, nbx, nby;      // bin tile (cells) and bins per axis
+    int bw_log2, bh_log2;      // log2 of the tile sides, -1 when not a power of two
     int halo;                  // deposit window halo (nodes)
     int box;                   // absorbing box walls on the domain edges
     int n_walls;

@@ -112,7 +112,10 @@ __device__ inline int bin_key(const CaseDev<T>& C, int nb, T x, T y) {
     int ix = (int)dfloor(x), iy = (int)dfloor(y);
     ix = min(max(ix, 0), C.nx - 2);
     iy = min(max(iy, 0), C.ny - 2);
-    return (iy / C.bh) * C.nbx + (ix / C.bw);
+    // ix, iy >= 0: a shift is the division for power-of-two tiles
+    const int bx = C.bw_log2 >= 0 ? ix >> C.bw_log2 : ix / C.bw;
+    const int by = C.bh_log2 >= 0 ? iy >> C.bh_log2 : iy / C.bh;
+    return by * C.nbx + bx;
 }
 
 // segment-segment intersection of collisions.cu:34-41
@@ -135,13 +138,15 @@ __device__ inline bool segment_hits(const double* w, double X1, double Y1, doubl
 //  * any other path tests the lists of every bin its bounding box meets
 //    (every point of the path lies in one of them), at most 16 bins;
 //  * a longer path, one outside the binned area, or no lists: every wall.
-// All three give the full scan's answer.
-template <typename T>
+// All three give the full scan's answer. WALLS false compiles the box test
+// alone, for launches in which no case has internal walls (the segment code
+// costs the particle loop registers even when it does not run).
+template <typename T, bool WALLS>
 __device__ inline bool hits_walls(const CaseDev<T>& C, int bin, int cx0, int cy0, T x0, T y0, T x1, T y1) {
     if (C.box) {
         if (x1 <= (T)0 || x1 >= (T)(C.nx - 1) || y1 <= (T)0 || y1 >= (T)(C.ny - 1)) return true;
     }
-    if (C.n_walls > 0) {
+    if (WALLS && C.n_walls > 0) {
         const double X1 = x0, Y1 = y0, X2 = x1, Y2 = y1;
         const double rx0 = cx0 - C.wall_margin, ry0 = cy0 - C.wall_margin;
         const double rx1 = cx0 + C.bw + C.wall_margin, ry1 = cy0 + C.bh + C.wall_margin;
@@ -432,7 +437,7 @@ __device__ inline void stage_put(const Stage<T>& sg, int& nm, bool want, T x, T 
 // ---------------------------------------------------------------------------
 // the fused particle kernel
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, bool WALLS>
 __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict__ cases,
                                                    const CaseState* __restrict__ states,
                                                    const WorkItem* __restrict__ items, int spill_chunk,
@@ -566,7 +571,7 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
                 x = dfma(vx, dtdx, x);
                 y = dfma(vy, dtdy, y);
             }
-            if (hits_walls(C, wall_bin, cx0, cy0, x0, y0, x, y)) {
+            if (hits_walls<T, WALLS>(C, wall_bin, cx0, cy0, x0, y0, x, y)) {
                 atomicAdd(lcnt + PIC_C_ABSORBED, 1u);
             } else {
                 alive = true;
@@ -576,7 +581,7 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
             if (born) {
                 // born particles drift from the collision point when the pass drifts
                 const T ex1 = drift ? dfma(b.evx, dtdx, b.ex) : b.ex, ey1 = drift ? dfma(b.evy, dtdy, b.ey) : b.ey;
-                if (hits_walls(C, wall_bin, cx0, cy0, b.ex, b.ey, ex1, ey1)) {
+                if (hits_walls<T, WALLS>(C, wall_bin, cx0, cy0, b.ex, b.ey, ex1, ey1)) {
                     atomicAdd(lcnt + PIC_C_BORN_ABSORBED, 1u);
                 } else {
                     be_alive = true;
@@ -587,7 +592,7 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
                     atomicAdd(lcnt + PIC_C_BORN_SELF, 1u);
                 }
                 const T ix1 = drift ? dfma(b.ivx, dtdx, b.ix) : b.ix, iy1 = drift ? dfma(b.ivy, dtdy, b.iy) : b.iy;
-                if (hits_walls(C, wall_bin, cx0, cy0, b.ix, b.iy, ix1, iy1)) {
+                if (hits_walls<T, WALLS>(C, wall_bin, cx0, cy0, b.ix, b.iy, ix1, iy1)) {
                     atomicAdd(lcnt + PIC_C_BORN_ABSORBED, 1u);
                 } else if (O) {
                     bi_alive = true;
@@ -894,10 +899,14 @@ __global__ __launch_bounds__(256) void clear_out_kernel(const CaseDev<T>* __rest
 // ---------------------------------------------------------------------------
 template <typename T>
 int launch_push(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items,
-                int spill_chunk, int flags, long long mcc_key, size_t lds, hipStream_t s) {
+                int spill_chunk, int flags, long long mcc_key, size_t lds, hipStream_t s, bool walls) {
     if (n_items <= 0) return 0;
-    hipLaunchKernelGGL(push_kernel<T>, dim3(n_items), dim3(256), lds, s, cases, states, items, spill_chunk, flags,
-                       mcc_key);
+    if (walls)
+        hipLaunchKernelGGL((push_kernel<T, true>), dim3(n_items), dim3(256), lds, s, cases, states, items,
+                           spill_chunk, flags, mcc_key);
+    else
+        hipLaunchKernelGGL((push_kernel<T, false>), dim3(n_items), dim3(256), lds, s, cases, states, items,
+                           spill_chunk, flags, mcc_key);
     PIC_HIP(hipGetLastError());
     return 0;
 }
@@ -1016,7 +1025,7 @@ int launch_counts(const CaseDev<T>* cases, const CaseState* states, int c, int s
 
 #define PIC_LAUNCH_INST(T)                                                                                       \
     template int launch_push<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int, int, int, long long,  \
-                                size_t, hipStream_t);                                                            \
+                                size_t, hipStream_t, bool);                                                      \
     template int launch_status<T>(const CaseDev<T>*, const CaseState*, int, int*, hipStream_t);                  \
     template int launch_counts<T>(const CaseDev<T>*, const CaseState*, int, int, long long*, hipStream_t);       \
     template int launch_init<T>(T*, T*, T*, T*, uint32_t*, int32_t*, int, int, int, double, uint32_t, uint32_t,  \

@@ -4,9 +4,11 @@
 
 namespace pic {
 
+// walls: some case of the launch has internal wall segments (false launches
+// the instantiation without the segment test)
 template <typename T>
 int launch_push(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items,
-                int spill_chunk, int flags, long long mcc_key, size_t lds, hipStream_t s);
+                int spill_chunk, int flags, long long mcc_key, size_t lds, hipStream_t s, bool walls);
 template <typename T>
 int launch_status(const CaseDev<T>* cases, const CaseState* states, int n_cases, int* out, hipStream_t s);
 template <typename T>

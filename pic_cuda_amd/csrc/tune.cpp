@@ -53,7 +53,7 @@ int trial_push(pic_sim* S, int reps, double* best) {
     for (int r = 0; r < reps; ++r) {
         PIC_HIP(hipEventRecord(S->ev[0], sm));
         if (int rc = launch_push<T>((CaseDev<T>*)p->d_case, p->d_state, S->d_items, S->n_items, S->spill_chunk, flags,
-                                    -1, cur_lds(S), sm))
+                                    -1, cur_lds(S), sm, S->d_walls_cell != nullptr))
             return rc;
         PIC_HIP(hipEventRecord(S->ev[1], sm));
         if (int rc = launch_clear_out<T>((CaseDev<T>*)p->d_case, p->d_state, 1, sm)) return rc;
