@@ -27,7 +27,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-steps/s at 1024^2+4e7 ptcls; MG-solve ms; ensemble jobs/s, p99 lat"
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_PARTICLE = 40.0   # x, y, vx, vy (FP32) and id (u32), read from one buffer, written to the other
+# algorithmic bytes per particle-step of the push: x, y, vx, vy (FP32) read and
+# written (32 B) and the id (u32) read (4 B). The id is written too (it moves
+# with the particle), but that 4 B is traffic the counting rule does not credit.
+BYTES_PER_PARTICLE = 36.0
 
 
 def read_traffic():
@@ -87,10 +90,18 @@ def cpu_baseline(gpu_case, budget_s=12.0):
     threads = orc.lib(True).orc_num_threads()
     v_omp, s_omp = run(True)
     v_ser, s_ser = run(False)
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": v_omp, "unit": "particle-steps/s", "cores": threads, "kind": "port",
             "sample": "oracle (OpenMP, %d threads), %d full steps of a 256^2-cell, 1.25e6 e- + 1.25e6 Ar+ sample "
                       "(same ppc, MCC and 1e-8 FP64 multigrid)" % (threads, s_omp),
-            "serial_value": v_ser, "serial_steps": s_ser}
+            "serial_value": v_ser, "serial_steps": s_ser, "nproc": os.cpu_count(), "cpu_model": model}
 
 
 def poisson(n=2049, cpu=True):
@@ -380,7 +391,7 @@ def serve(n_jobs=24, rate=20.0, budget=200e9, seed=9, max_steps=2000):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--scale", type=float, default=1.0, help="particle count scale (1.0 = 2e7 per species)")
     ap.add_argument("--cells", type=int, default=1024)
