@@ -153,8 +153,10 @@ typedef struct {
     int coarse_sweeps;   /* sweeps on the coarsest level */
     double omega;        /* SOR factor */
     double tol;          /* relative residual ||r||/||rhs|| */
-    int max_cycles;
+    int max_cycles;      /* V-cycles (mixed mode: rounded up to whole refinement steps) */
     int mode;            /* PIC_MG_FP64 or PIC_MG_MIXED */
+    int inner;           /* mixed mode: FP32 V-cycles per FP64 refinement step, 1..4 (0: 1);
+                            the convergence test runs after each refinement step */
 } pic_mg_params;
 
 typedef struct {
@@ -312,7 +314,7 @@ int pic_sim_destroy(pic_sim* s);
 /* ---- tuning (no counterpart in the reference, whose tiles and sweeps are
    fixed in the source) --------------------------------------------------- */
 #define PIC_TUNE_BINS 1    /* particle bin tile (the push kernel's LDS tiles) */
-#define PIC_TUNE_MG 2      /* SOR factor and sweeps per multigrid level */
+#define PIC_TUNE_MG 2      /* SOR factor, V-cycles per refinement step, sweeps per level */
 typedef struct {
     int bin_w, bin_h;        /* bin tile chosen */
     double push_ms;          /* its trial push time */
@@ -324,6 +326,7 @@ typedef struct {
                                 interior with them */
     int solve_cycles;
     int trials;              /* configurations timed */
+    int inner;               /* FP32 V-cycles per refinement step chosen (mixed mode) */
 } pic_tune_report;
 /* Times candidate bin tiles with trial pushes that leave the simulation's
    state unchanged, and multigrid settings with solves of the current charge

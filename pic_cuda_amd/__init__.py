@@ -52,7 +52,7 @@ EPS0 = 8.8541878128e-12
 class MGParams(ctypes.Structure):
     _fields_ = [("levels", ctypes.c_int), ("pre", ctypes.c_int), ("post", ctypes.c_int),
                 ("coarse_sweeps", ctypes.c_int), ("omega", ctypes.c_double), ("tol", ctypes.c_double),
-                ("max_cycles", ctypes.c_int), ("mode", ctypes.c_int)]
+                ("max_cycles", ctypes.c_int), ("mode", ctypes.c_int), ("inner", ctypes.c_int)]
 
 
 class MGStats(ctypes.Structure):
@@ -78,7 +78,7 @@ class TuneReport(ctypes.Structure):
     _fields_ = [("bin_w", ctypes.c_int), ("bin_h", ctypes.c_int), ("push_ms", ctypes.c_double),
                 ("omega", ctypes.c_double), ("levels", ctypes.c_int), ("pre", ctypes.c_int * 16),
                 ("post", ctypes.c_int * 16), ("solve_ms", ctypes.c_double), ("solve_cycles", ctypes.c_int),
-                ("trials", ctypes.c_int)]
+                ("trials", ctypes.c_int), ("inner", ctypes.c_int)]
 
 
 class MccInfo(ctypes.Structure):
@@ -763,7 +763,7 @@ class Simulation:
         nl = max(0, r.levels - 1)
         return {"bin": (r.bin_w, r.bin_h), "push_ms": r.push_ms, "omega": r.omega, "levels": r.levels,
                 "pre": list(r.pre)[:nl], "post": list(r.post)[:nl], "solve_ms": r.solve_ms,
-                "solve_cycles": r.solve_cycles, "trials": r.trials}
+                "solve_cycles": r.solve_cycles, "trials": r.trials, "inner": r.inner}
 
     def set_electrodes(self, mask, phi_bc=None):
         """Embedded electrodes in the field solve (see Multigrid.set_electrodes)."""

@@ -16,6 +16,7 @@
 // device flag.  A single solve is a batch of one.
 #pragma once
 #include "pic_common.hpp"
+#include <algorithm>
 #include <vector>
 
 namespace pic {
@@ -30,10 +31,12 @@ struct MGParams {
     double tol;         // ||r|| / ||rhs||
     int max_cycles;
     int mode;
+    int inner = 1;      // mixed mode: FP32 V-cycles per FP64 refinement step
     int per_level = 0;  // 1: pre_l/post_l give each level's sweeps (index 0 = finest)
     int pre_l[16] = {}, post_l[16] = {};
     int pre_at(int l) const { return per_level ? pre_l[l] : pre; }
     int post_at(int l) const { return per_level ? post_l[l] : post; }
+    int inner_eff() const { return mode == PIC_MG_MIXED ? std::max(1, std::min(inner, 4)) : 1; }
 };
 
 struct MGStats {
@@ -48,9 +51,10 @@ struct MGNorm {
     double rhs_sq;
     double rel;
     int converged;     // stop flag: tolerance met or max_cycles done
-    int cycles;        // cycles executed in this solve
+    int cycles;        // V-cycles executed in this solve
     int max_cycles;
     int hit_tol;       // tolerance met
+    int steps;         // refinement steps executed (mixed mode: the phi buffer parity)
 };
 
 // one (case, level) entry of a tiled stage
@@ -200,10 +204,12 @@ private:
         alignas(16) unsigned char t0[128];   // single task: passed by value
     };
     template <typename T>
-    int build_cycle(const std::vector<double>& h);
+    int build_cycle(const std::vector<double>& h, bool warm);
     template <typename T>
-    int add_level_stage(int kind, const std::vector<MGLevelTask<T>>& tasks, int nu, bool restr);
+    int add_level_stage(int kind, const std::vector<MGLevelTask<T>>& tasks, int nu, bool restr,
+                        std::vector<Stage>* dst = nullptr);
     int launch_stage(const Stage& st);
+    int launch_refinement();   // the inner V-cycles of one refinement step
     int enqueue_fine(int what, int which, int parity);
     void free_all();
 
@@ -214,6 +220,11 @@ private:
     MGParams p_{};
     bool mixed_ = false;
     std::vector<Stage> cycle_;          // stages of one V-cycle
+    // mixed mode with inner > 1: the V-cycle that starts from the correction
+    // so far (the later V-cycles of a refinement step)
+    std::vector<Stage> warm_;
+    int inner_ = 1;                     // V-cycles per refinement step
+    int max_outer_ = 1;                 // refinement steps allowed (max_cycles / inner, rounded up)
     MGFineTask* d_fine_ = nullptr;      // per-case refinement tasks
     std::vector<MGFineTask> h_fine_;
     MGTile* d_chunk_tiles_ = nullptr;   // (case, chunk of 256 nodes)

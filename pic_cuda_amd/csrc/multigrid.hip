@@ -583,7 +583,7 @@ __global__ __launch_bounds__(256) void ir_finish_stage(const StageRef<MGFineTask
     MGTile tile;
     MGFineTask tk;
     stage_item(a, tk, tile);
-    if (!(*G(&tk.norm->cycles) & 1)) return;
+    if (!(*G(&tk.norm->steps) & 1)) return;
     const size_t nn = (size_t)tk.n * tk.n;
     const size_t i = (size_t)tile.tx * 256 + threadIdx.x;
     if (i < nn) G(tk.phi)[i] = G(tk.phi_tmp)[i];
@@ -608,7 +608,7 @@ __global__ __launch_bounds__(256) void dirichlet_stage(const StageRef<MGFineTask
 // Relative residual ||r||/||rhs|| (the reference divides by sum rhs^2,
 // Multigrid.cu:466).
 __global__ __launch_bounds__(256) void norm_final_stage(const StageRef<MGFineTask> a, int which, double tol,
-                                                        int max_cycles) {
+                                                        int max_cycles, int inner) {
     MGFineTask tk;
     stage_case(a, tk);
     MGNorm* out = tk.norm;
@@ -623,13 +623,17 @@ __global__ __launch_bounds__(256) void norm_final_stage(const StageRef<MGFineTas
             out->rhs_sq = tot;
             out->converged = 0;
             out->cycles = 0;
+            out->steps = 0;
             out->max_cycles = max_cycles;
         } else {
             out->res_sq = tot;
             const double rn = sqrt(tot), fn = sqrt(out->rhs_sq);
             const double rel = fn > 0.0 ? rn / fn : rn;
             out->rel = rel;
-            if (which == 2) out->cycles += 1;
+            if (which == 2) {
+                out->cycles += inner;   // V-cycles of the refinement step just done
+                out->steps += 1;
+            }
             out->converged = (rel < tol || out->cycles >= out->max_cycles) ? 1 : 0;
             out->hit_tol = rel < tol ? 1 : 0;
         }
@@ -838,11 +842,13 @@ MGBatch::~MGBatch() { free_all(); }
 
 void MGBatch::free_all() {
     (void)hipStreamSynchronize(stream_);
-    for (auto& st : cycle_) {
-        if (st.tasks) (void)hipFree(st.tasks);
-        if (st.tiles) (void)hipFree(st.tiles);
-    }
+    for (auto* list : {&cycle_, &warm_})
+        for (auto& st : *list) {
+            if (st.tasks) (void)hipFree(st.tasks);
+            if (st.tiles) (void)hipFree(st.tiles);
+        }
     cycle_.clear();
+    warm_.clear();
     if (d_fine_) (void)hipFree(d_fine_);
     if (d_chunk_tiles_) (void)hipFree(d_chunk_tiles_);
     if (d_res_tiles_) (void)hipFree(d_res_tiles_);
@@ -862,7 +868,8 @@ static int upload_vec(void** dst, const std::vector<T>& v) {
 }
 
 template <typename T>
-int MGBatch::add_level_stage(int kind, const std::vector<MGLevelTask<T>>& tasks, int nu, bool restr) {
+int MGBatch::add_level_stage(int kind, const std::vector<MGLevelTask<T>>& tasks, int nu, bool restr,
+                             std::vector<Stage>* dst) {
     if (tasks.empty()) return 0;
     std::vector<MGTile> tiles;
     for (int k = 0; k < (int)tasks.size(); ++k) {
@@ -890,19 +897,24 @@ int MGBatch::add_level_stage(int kind, const std::vector<MGLevelTask<T>>& tasks,
     std::memcpy(st.t0, &tasks[0], sizeof(MGLevelTask<T>));
     if (int rc = upload_vec(&st.tasks, tasks)) return rc;
     if (int rc = upload_vec((void**)&st.tiles, tiles)) return rc;
-    cycle_.push_back(st);
+    (dst ? dst : &cycle_)->push_back(st);
     return 0;
 }
 
+// warm: the V-cycle of mixed mode that starts from the correction of the
+// previous V-cycle instead of zero (the later V-cycles of a refinement step)
 template <typename T>
-int MGBatch::build_cycle(const std::vector<double>& h) {
+int MGBatch::build_cycle(const std::vector<double>& h, bool warm) {
+    std::vector<Stage>* dst = warm ? &warm_ : &cycle_;
     const int M = (int)m_.size();
     const T omega = (T)p_.omega;
     // per member: level-0 input, output, scratch and rhs
     auto E = [&](MGSolver* s, int b, int l) -> T* { return (T*)(mixed_ ? (void*)s->ef_[b][l] : (void*)s->ed_[b][l]); };
     auto R = [&](MGSolver* s, int l) -> T* { return (T*)(mixed_ ? (void*)s->rf_[l] : (void*)s->rd_[l]); };
-    auto in0 = [&](int k) -> const T* { return mixed_ ? nullptr : (const T*)(void*)phi_[k]; };
     auto out0 = [&](int k) -> T* { return mixed_ ? (T*)(void*)m_[k]->ef_[0][0] : (T*)(void*)phi_[k]; };
+    auto in0 = [&](int k) -> const T* {
+        return mixed_ ? (warm ? (const T*)out0(k) : nullptr) : (const T*)(void*)phi_[k];
+    };
     auto tmp0 = [&](int k) -> T* { return mixed_ ? (T*)(void*)m_[k]->ef_[1][0] : (T*)(void*)m_[k]->phi_tmp_; };
     auto rhs0 = [&](int k) -> const T* { return mixed_ ? (const T*)(void*)m_[k]->r32_ : (const T*)(void*)rhs_[k]; };
     auto hl = [&](int k, int l) { return h[k] * std::ldexp(1.0, l); };
@@ -930,7 +942,7 @@ int MGBatch::build_cycle(const std::vector<double>& h) {
             t.mask = s->masked() ? s->mask_[d] : nullptr;
             tasks.push_back(t);
         }
-        if (int rc = add_level_stage<T>(0, tasks, p_.pre_at(d), true)) return rc;
+        if (int rc = add_level_stage<T>(0, tasks, p_.pre_at(d), true, dst)) return rc;
     }
     // coarse part: the single-workgroup cycle, or tiled sweeps on a large coarsest level
     {
@@ -972,7 +984,7 @@ int MGBatch::build_cycle(const std::vector<double>& h) {
             static_assert(sizeof(MGCoarseTask<T>) <= sizeof(st.t0), "task does not fit the by-value slot");
             std::memcpy(st.t0, &ct[0], sizeof(MGCoarseTask<T>));
             if (int rc = upload_vec(&st.tasks, ct)) return rc;
-            cycle_.push_back(st);
+            dst->push_back(st);
         }
         // large coarsest levels: chunks of up to 3 sweeps, ping-pong, then a copy
         std::vector<int> big;
@@ -1015,7 +1027,7 @@ int MGBatch::build_cycle(const std::vector<double>& h) {
                     src[k] = t.e_out;
                     which[k] ^= 1;
                 }
-                if (int rc = add_level_stage<T>(0, tasks, nu, false)) return rc;
+                if (int rc = add_level_stage<T>(0, tasks, nu, false, dst)) return rc;
                 left -= nu;
                 wrote = true;
             }
@@ -1029,7 +1041,7 @@ int MGBatch::build_cycle(const std::vector<double>& h) {
                 t.gate = m_[k]->d_norm_;
                 copies.push_back(t);
             }
-            if (int rc = add_level_stage<T>(3, copies, 0, false)) return rc;
+            if (int rc = add_level_stage<T>(3, copies, 0, false, dst)) return rc;
         }
     }
     // up leg
@@ -1054,7 +1066,7 @@ int MGBatch::build_cycle(const std::vector<double>& h) {
             t.mask = s->masked() ? s->mask_[d] : nullptr;
             tasks.push_back(t);
         }
-        if (int rc = add_level_stage<T>(1, tasks, p_.post_at(d), false)) return rc;
+        if (int rc = add_level_stage<T>(1, tasks, p_.post_at(d), false, dst)) return rc;
     }
     return 0;
 }
@@ -1075,10 +1087,13 @@ int MGBatch::build(const std::vector<MGSolver*>& members, const std::vector<doub
         for (int l = 0; l < 16 && same_levels && q.per_level; ++l)
             same_levels = q.pre_l[l] == p_.pre_l[l] && q.post_l[l] == p_.post_l[l];
         PIC_REQUIRE(q.mode == p_.mode && q.pre == p_.pre && q.post == p_.post && q.coarse_sweeps == p_.coarse_sweeps &&
-                        q.omega == p_.omega && q.tol == p_.tol && q.max_cycles == p_.max_cycles && same_levels,
+                        q.omega == p_.omega && q.tol == p_.tol && q.max_cycles == p_.max_cycles && same_levels &&
+                        q.inner_eff() == p_.inner_eff(),
                     "multigrid batch: members need identical cycle parameters");
     }
     mixed_ = p_.mode == PIC_MG_MIXED;
+    inner_ = p_.inner_eff();
+    max_outer_ = std::max(1, (p_.max_cycles + inner_ - 1) / inner_);
     const int M = (int)m_.size();
     PIC_HIP(hipMalloc(&d_mirror_, M * sizeof(MGNorm)));
     PIC_HIP(hipMemset(d_mirror_, 0, M * sizeof(MGNorm)));
@@ -1119,7 +1134,10 @@ int MGBatch::build(const std::vector<MGSolver*>& members, const std::vector<doub
     if (int rc = upload_vec((void**)&d_res_tiles_, res)) return rc;
     n_chunk_ = (int)chunks.size();
     n_res_ = (int)res.size();
-    return mixed_ ? build_cycle<float>(h) : build_cycle<double>(h);
+    if (int rc = mixed_ ? build_cycle<float>(h, false) : build_cycle<double>(h, false)) return rc;
+    if (mixed_ && inner_ > 1)
+        if (int rc = build_cycle<float>(h, true)) return rc;
+    return 0;
 }
 
 struct MGBatchStageView {
@@ -1208,7 +1226,8 @@ int MGBatch::enqueue_fine(int what, int which, int parity) {
             ++launches_;
         }
         hipLaunchKernelGGL(sumsq_stage, dim3(n_chunk_), dim3(256), 0, stream_, chunks);
-        hipLaunchKernelGGL(norm_final_stage, dim3(M), dim3(256), 0, stream_, cases, 0, p_.tol, p_.max_cycles);
+        hipLaunchKernelGGL(norm_final_stage, dim3(M), dim3(256), 0, stream_, cases, 0, p_.tol, max_outer_ * inner_,
+                           inner_);
         launches_ += 2;
     } else if (what == 1) {
         const auto res = make_ref<MGFineTask>(&h_fine_[0], d_fine_, d_res_tiles_, M, h_fine_[0].gx);
@@ -1216,12 +1235,24 @@ int MGBatch::enqueue_fine(int what, int which, int parity) {
         if (!mixed_) hipLaunchKernelGGL(ir_residual_stage<0>, g, b, 0, stream_, res, which, 0);
         else if (which == 1) hipLaunchKernelGGL(ir_residual_stage<1>, g, b, 0, stream_, res, which, 0);
         else hipLaunchKernelGGL(ir_residual_stage<2>, g, b, 0, stream_, res, which, parity);
-        hipLaunchKernelGGL(norm_final_stage, dim3(M), dim3(256), 0, stream_, cases, which, p_.tol, p_.max_cycles);
+        hipLaunchKernelGGL(norm_final_stage, dim3(M), dim3(256), 0, stream_, cases, which, p_.tol,
+                           max_outer_ * inner_, inner_);
         launches_ += 2;
     } else {
         hipLaunchKernelGGL(ir_finish_stage, dim3(n_chunk_), dim3(256), 0, stream_, chunks);
         ++launches_;
     }
+    return 0;
+}
+
+// One refinement step (FP64 modes: one V-cycle): the V-cycle, then inner - 1
+// more whose first stage starts from the correction so far.
+int MGBatch::launch_refinement() {
+    for (const auto& st : cycle_)
+        if (int rc = launch_stage(st)) return rc;
+    for (int j = 1; j < inner_; ++j)
+        for (const auto& st : warm_)
+            if (int rc = launch_stage(st)) return rc;
     return 0;
 }
 
@@ -1237,13 +1268,12 @@ int MGBatch::solve() {
     const int M = (int)m_.size();
     enqueue_fine(0, 0, 0);
     enqueue_fine(1, 1, 0);
-    int enqueued = 0;
-    int batch = std::max(1, last_cycles_);
+    int enqueued = 0;   // refinement steps
+    int batch = std::max(1, (last_cycles_ + inner_ - 1) / inner_);
     for (;;) {
-        batch = std::min(batch, p_.max_cycles - enqueued);
+        batch = std::min(batch, max_outer_ - enqueued);
         for (int b = 0; b < batch; ++b) {
-            for (const auto& st : cycle_)
-                if (int rc = launch_stage(st)) return rc;
+            if (int rc = launch_refinement()) return rc;
             enqueue_fine(1, 2, (enqueued + b) & 1);
         }
         enqueued += std::max(batch, 0);
@@ -1251,7 +1281,7 @@ int MGBatch::solve() {
         PIC_HIP(hipStreamSynchronize(stream_));
         bool all = true;
         for (int k = 0; k < M; ++k) all = all && h_mirror_[k].converged;
-        if (all || enqueued >= p_.max_cycles) break;
+        if (all || enqueued >= max_outer_) break;
         batch = 1;
     }
     if (mixed_) enqueue_fine(3, 0, 0);
@@ -1266,12 +1296,12 @@ int MGBatch::enqueue_fixed(int cycles) {
     PIC_REQUIRE(!m_.empty(), "multigrid batch: not built");
     launches_ = 0;
     const int M = (int)m_.size();
-    const int K = std::max(1, std::min(cycles, p_.max_cycles));
+    // cycles: V-cycles, run as whole refinement steps
+    const int K = std::max(1, std::min((cycles + inner_ - 1) / inner_, max_outer_));
     enqueue_fine(0, 0, 0);
     enqueue_fine(1, 1, 0);
     for (int b = 0; b < K; ++b) {
-        for (const auto& st : cycle_)
-            if (int rc = launch_stage(st)) return rc;
+        if (int rc = launch_refinement()) return rc;
         enqueue_fine(1, 2, b & 1);
     }
     if (mixed_) enqueue_fine(3, 0, 0);
@@ -1289,9 +1319,8 @@ int MGBatch::complete_fixed() {
     for (;;) {
         bool all = true;
         for (int k = 0; k < M; ++k) all = all && h_mirror_[k].converged;
-        if (all || enqueued >= p_.max_cycles) break;
-        for (const auto& st : cycle_)
-            if (int rc = launch_stage(st)) return rc;
+        if (all || enqueued >= max_outer_) break;
+        if (int rc = launch_refinement()) return rc;
         enqueue_fine(1, 2, enqueued & 1);
         ++enqueued;
         PIC_HIP(hipMemcpyAsync(h_mirror_, d_mirror_, M * sizeof(MGNorm), hipMemcpyDeviceToHost, stream_));

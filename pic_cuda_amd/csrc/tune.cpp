@@ -164,6 +164,14 @@ int tune_mg(pic_sim* S, pic_tune_report* out) {
         p.omega = om;
         if (int rc = trial(p, &b)) return rc;
     }
+    // FP32 V-cycles per refinement step (mixed mode)
+    if (best.mode == PIC_MG_MIXED)
+        for (int in : {1, 2, 3}) {
+            MGParams p = best;
+            if (p.inner_eff() == in) continue;
+            p.inner = in;
+            if (int rc = trial(p, &b)) return rc;
+        }
     // uniform sweep counts
     static const int uni[][2] = {{1, 1}, {2, 1}, {1, 2}, {2, 2}, {3, 2}, {2, 3}, {3, 3}};
     for (const auto& u : uni) {
@@ -194,6 +202,7 @@ int tune_mg(pic_sim* S, pic_tune_report* out) {
             }
     if (int rc = sim_set_mg(S, best)) return rc;
     out->omega = best.omega;
+    out->inner = best.inner_eff();
     out->levels = L;
     for (int l = 0; l < 16; ++l) {
         out->pre[l] = best.pre_at(l);

@@ -69,8 +69,10 @@ def test_mg_presets(pic):
     st = pic.mg_params(pic.PRESET_STEP)
     assert (st.pre, st.post, st.omega, st.tol, st.mode) == (2, 2, 1.0, 1e-8, pic.MG_MIXED)
     sor = pic.mg_params(pic.PRESET_STEP_SOR)
-    assert (sor.pre, sor.post, sor.omega, sor.tol, sor.mode) == (2, 2, 1.2, 1e-8, pic.MG_MIXED)
-    assert pic.mg_params(pic.PRESET_POISSON).omega == 1.2
+    assert (sor.pre, sor.post, sor.omega, sor.tol, sor.mode, sor.inner) == (2, 2, 1.2, 1e-8, pic.MG_MIXED, 2)
+    poi = pic.mg_params(pic.PRESET_POISSON)
+    assert (poi.omega, poi.inner) == (1.2, 1)
+    assert ref.inner == 1 and st.inner == 2
     with pytest.raises(pic.PicError):
         pic.mg_params(7)
 

@@ -105,6 +105,29 @@ def test_mg_mixed_precision_accuracy(pic, orc, n):
     assert err < 1e-9, err
 
 
+@pytest.mark.parametrize("n", [65, 129, 513, 1025])
+def test_mg_mixed_inner_cycles(pic, orc, n):
+    """1, 2 and 3 FP32 V-cycles per FP64 refinement step (65: the whole grid
+    inside the single-workgroup coarse cycle; 1025: a large coarsest tiled
+    level is not involved, four tiled levels are) all reach the tolerance and
+    the FP64 solution, and take whole refinement steps."""
+    rng = np.random.default_rng(7 * n)
+    rhs = rng.normal(size=(n, n))
+    phi0 = np.zeros((n, n))
+    phi0[:, 0] = rng.normal(size=n)          # Dirichlet values kept
+    h = 1.0 / (n - 1)
+    phi_o = phi0.copy()
+    orc.mg_solve(phi_o, rhs, h, tol=1e-13, max_cycles=100)
+    for inner in (1, 2, 3):
+        params = pic.mg_params(pic.PRESET_POISSON, inner=inner)
+        phi_g, st = _mg_gpu(pic, n, rhs, phi0, h, params)
+        assert st.rel_residual < 1e-10, (inner, st.rel_residual)
+        assert st.cycles % inner == 0 and st.cycles <= 12, (inner, st.cycles)
+        err = np.linalg.norm(phi_g - phi_o) / np.linalg.norm(phi_o)
+        assert err < 1e-9, (inner, err)
+        assert np.array_equal(phi_g[:, 0], phi0[:, 0])
+
+
 def test_solve_multigrid_reference_signature(pic, orc):
     n = 65
     rng = np.random.default_rng(5)

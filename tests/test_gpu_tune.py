@@ -74,6 +74,7 @@ def test_multigrid_tuning_keeps_the_tolerance(pic):
     rep = a["sim"].autotune(bins=False, multigrid=True)
     assert rep["trials"] > 10 and rep["solve_cycles"] >= 1
     assert len(rep["pre"]) == rep["levels"] - 1
+    assert rep["inner"] in (1, 2, 3)
     # same charge, different solver settings: potentials agree to the tolerance
     a["sim"].step(1)
     b["sim"].step(1)
