@@ -118,6 +118,12 @@ int pic_particles_export_positions(pic_particles* p, int species, float* dst_dev
    uniform in the open box [0,nx-1]x[0,ny-1], Maxwellian sigma = vth per component */
 int pic_particles_init_maxwellian(pic_particles* p, int species, int64_t n, int nx, int ny, double vth,
                                   uint32_t seed, uint32_t case_id, uint32_t id0);
+/* initializeParticles (main.cu:17-32) with its distributions, made
+   reproducible: positions uniform in [x_lo,x_hi)x[y_lo,y_hi) (cell units),
+   each velocity component uniform in [v_lo,v_hi) (m/s); keyed on (seed,
+   species, id, case) like init_maxwellian, on a stream of its own */
+int pic_particles_init_uniform(pic_particles* p, int species, int64_t n, double x_lo, double x_hi, double y_lo,
+                               double y_hi, double v_lo, double v_hi, uint32_t seed, uint32_t case_id, uint32_t id0);
 /* removeCollidedParticles() particles.cu:217-261: compaction is fused into
    pic_sim_step; kept for the interface and returns immediately */
 int pic_particles_remove_collided(pic_particles* p);

@@ -90,6 +90,7 @@ def _load(omp=False):
     lib.orc_segment_hits.restype = i
     lib.orc_segment_hits.argtypes = [d, d, d, d, vp, i, d]
     lib.orc_init_particles.argtypes = [i, i64, i, i, d, u32, u32, u32, u32, vp, vp, vp, vp, vp]
+    lib.orc_init_uniform.argtypes = [i, i64, d, d, d, d, d, d, u32, u32, u32, u32, vp, vp, vp, vp, vp]
     lib.orc_num_threads.restype = i
     if omp:
         _LIB_OMP = lib
@@ -183,6 +184,17 @@ def init_particles(prec, n, nx, ny, vth, seed, case_id, species, id0=0):
     ids = np.zeros(n, dtype=np.uint32)
     lib().orc_init_particles(prec, n, nx, ny, vth, seed, case_id, species, id0, _p(x), _p(y), _p(vx), _p(vy),
                              _p(ids))
+    return x, y, vx, vy, ids
+
+
+def init_uniform(prec, n, x0, x1, y0, y1, v0, v1, seed, case_id, species, id0=0):
+    """initializeParticles (main.cu:17-32) made reproducible: uniform box
+    positions (cell units) and uniform velocities."""
+    dt = np.float32 if prec == 4 else np.float64
+    x, y, vx, vy = (np.zeros(n, dtype=dt) for _ in range(4))
+    ids = np.zeros(n, dtype=np.uint32)
+    lib().orc_init_uniform(prec, n, x0, x1, y0, y1, v0, v1, seed, case_id, species, id0, _p(x), _p(y), _p(vx),
+                           _p(vy), _p(ids))
     return x, y, vx, vy, ids
 
 

@@ -695,6 +695,38 @@ static void init_t(int64_t n, int nx, int ny, double vth, uint32_t seed, uint32_
     }
 }
 
+/* Uniform positions in a box and uniform velocities in a range, the
+   distributions of initializeParticles (main.cu:17-32) made reproducible:
+   Philox counter {i, 0xFFFFFFFE, c, case_id}, key {seed, species}; the four
+   words of the first block give x, y, vx, vy. */
+template <typename T>
+static void init_uniform_t(int64_t n, double x0, double x1, double y0, double y1, double v0, double v1,
+                           uint32_t seed, uint32_t case_id, uint32_t species, uint32_t id0, T* x, T* y, T* vx,
+                           T* vy, uint32_t* id) {
+    for (int64_t i = 0; i < n; ++i) {
+        const uint32_t pid = id0 + (uint32_t)i;
+        Stream rs(pid, 0xFFFFFFFEu, case_id, seed, species);
+        const double ux = ((double)rs.u32() + 0.5) * 0x1p-32, uy = ((double)rs.u32() + 0.5) * 0x1p-32;
+        const double uvx = ((double)rs.u32() + 0.5) * 0x1p-32, uvy = ((double)rs.u32() + 0.5) * 0x1p-32;
+        x[i] = (T)(x0 + ux * (x1 - x0));
+        y[i] = (T)(y0 + uy * (y1 - y0));
+        vx[i] = (T)(v0 + uvx * (v1 - v0));
+        vy[i] = (T)(v0 + uvy * (v1 - v0));
+        id[i] = pid;
+    }
+}
+
+extern "C" void orc_init_uniform(int prec, int64_t n, double x0, double x1, double y0, double y1, double v0,
+                                 double v1, uint32_t seed, uint32_t case_id, uint32_t species, uint32_t id0,
+                                 void* x, void* y, void* vx, void* vy, uint32_t* id) {
+    if (prec == 4)
+        init_uniform_t<float>(n, x0, x1, y0, y1, v0, v1, seed, case_id, species, id0, (float*)x, (float*)y,
+                              (float*)vx, (float*)vy, id);
+    else
+        init_uniform_t<double>(n, x0, x1, y0, y1, v0, v1, seed, case_id, species, id0, (double*)x, (double*)y,
+                               (double*)vx, (double*)vy, id);
+}
+
 extern "C" void orc_init_particles(int prec, int64_t n, int nx, int ny, double vth,
                                    uint32_t seed, uint32_t case_id, uint32_t species, uint32_t id0,
                                    void* x, void* y, void* vx, void* vy, uint32_t* id) {

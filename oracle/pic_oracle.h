@@ -152,6 +152,13 @@ void orc_init_particles(int prec, int64_t n, int nx, int ny, double vth,
                         uint32_t seed, uint32_t case_id, uint32_t species, uint32_t id0,
                         void* x, void* y, void* vx, void* vy, uint32_t* id);
 
+/* Uniform positions in [x0,x1)x[y0,y1) (cell units) and uniform velocities in
+   [v0,v1) per component (initializeParticles, main.cu:17-32, made
+   reproducible): Philox counter {i, 0xFFFFFFFE, 0, case_id}, key {seed, species}. */
+void orc_init_uniform(int prec, int64_t n, double x0, double x1, double y0, double y1, double v0, double v1,
+                      uint32_t seed, uint32_t case_id, uint32_t species, uint32_t id0,
+                      void* x, void* y, void* vx, void* vy, uint32_t* id);
+
 /* Wall-clock helper for the CPU baseline (OpenMP build only uses >1 thread). */
 int orc_num_threads(void);
 

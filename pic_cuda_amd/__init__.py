@@ -127,6 +127,7 @@ SIGNATURES = {
     "pic_particles_upload": (_i, [_vp, _i, _i64, _vp, _vp, _vp, _vp, _vp]),
     "pic_particles_download": (_i, [_vp, _i, _i64, _vp, _vp, _vp, _vp, _vp, _P(_i64)]),
     "pic_particles_init_maxwellian": (_i, [_vp, _i, _i64, _i, _i, _d, _u32, _u32, _u32]),
+    "pic_particles_init_uniform": (_i, [_vp, _i, _i64, _d, _d, _d, _d, _d, _d, _u32, _u32, _u32]),
     "pic_particles_remove_collided": (_i, [_vp]),
     "pic_grid_create": (_i, [_i, _i, _d, _d, _P(_vp)]),
     "pic_grid_destroy": (_i, [_vp]),
@@ -388,6 +389,13 @@ class Particles2D:
         _check(load().pic_particles_export_positions(self._h, int(species), _devptr(dst.ptr), dst.nbytes // 8, u,
                                                      ctypes.byref(n)))
         return dst, n.value
+
+    def init_uniform(self, species, n, x_range, y_range, v_range, seed=1, case_id=0, id0=0):
+        """initializeParticles (main.cu:17-32), reproducible: positions uniform
+        in x_range x y_range (cell units), velocity components uniform in v_range."""
+        _check(load().pic_particles_init_uniform(self._h, int(species), int(n), float(x_range[0]), float(x_range[1]),
+                                                 float(y_range[0]), float(y_range[1]), float(v_range[0]),
+                                                 float(v_range[1]), int(seed), int(case_id), int(id0)))
 
     def init_maxwellian(self, species, n, nx, ny, vth, seed=1, case_id=0, id0=0):
         _check(load().pic_particles_init_maxwellian(self._h, int(species), int(n), int(nx), int(ny), float(vth),

@@ -460,6 +460,41 @@ static int init_t(pic_particles* p, int s, int64_t n, int nx, int ny, double vth
                           id0, lib_stream());
 }
 
+template <typename T>
+static int init_uniform_t(pic_particles* p, int s, int64_t n, double x0, double x1, double y0, double y1, double v0,
+                          double v1, uint32_t seed, uint32_t case_id, uint32_t id0) {
+    SpeciesStore& st = p->sp[s];
+    const int par = p->h_cur, b = st.nb;
+    int32_t cnt = 0;
+    if (int rc = read_count(p, s, par, b, &cnt)) return rc;
+    if (cnt < 0 || (int64_t)cnt + n > st.h_cap[b]) {
+        set_error("pic_particles_init_uniform: not enough free slots (capacity per species exceeded)");
+        return PIC_ERR_CAPACITY;
+    }
+    const size_t off = (size_t)st.h_start[b];
+    return launch_init_uniform<T>((T*)st.x[par] + off, (T*)st.y[par] + off, (T*)st.vx[par] + off,
+                                  (T*)st.vy[par] + off, st.id[par] + off, st.d_count[par] + b, (int)n, x0, x1, y0, y1,
+                                  v0, v1, seed, case_id, (uint32_t)s, id0, lib_stream());
+}
+
+extern "C" int pic_particles_init_uniform(pic_particles* p, int species, int64_t n, double x_lo, double x_hi,
+                                          double y_lo, double y_hi, double v_lo, double v_hi, uint32_t seed,
+                                          uint32_t case_id, uint32_t id0) {
+    PIC_REQUIRE(p && species >= 0 && species < (int)p->sp.size(), "bad species");
+    PIC_REQUIRE(n >= 0 && n < INT32_MAX, "bad particle count");
+    PIC_REQUIRE(x_hi >= x_lo && y_hi >= y_lo && v_hi >= v_lo, "empty range");
+    if (n == 0) return 0;
+    if (p->attached) {
+        if (int rc = sim_sync_impl(p->attached)) return rc;
+    }
+    int rc = p->prec == 4 ? init_uniform_t<float>(p, species, n, x_lo, x_hi, y_lo, y_hi, v_lo, v_hi, seed, case_id, id0)
+                          : init_uniform_t<double>(p, species, n, x_lo, x_hi, y_lo, y_hi, v_lo, v_hi, seed, case_id,
+                                                   id0);
+    if (rc) return rc;
+    PIC_HIP(hipStreamSynchronize(lib_stream()));
+    return 0;
+}
+
 extern "C" int pic_particles_init_maxwellian(pic_particles* p, int species, int64_t n, int nx, int ny, double vth,
                                              uint32_t seed, uint32_t case_id, uint32_t id0) {
     PIC_REQUIRE(p && species >= 0 && species < (int)p->sp.size(), "bad species");

@@ -9,7 +9,8 @@
 //                 atomics of kernels.cu:26-69) and wave-ballot compaction of the
 //                 survivors into the bin of their new cell (replaces
 //                 removeCollidedParticles, particles.cu:217-261).
-//  init_kernel    seeded uniform/Maxwellian particles (replaces main.cu:17-32).
+//  init_kernel    seeded particles: uniform positions with Maxwellian velocities,
+//  init_uniform_kernel  or uniform in a box and velocity range (replace main.cu:17-32).
 //  hist/scatter   re-binning of a species store (counting sort by tile).
 //  gather_kernel  flattening of a store for download.
 //  export_kernel  live positions as a packed float2 stream (replaces
@@ -690,6 +691,27 @@ __global__ void init_kernel(T* x, T* y, T* vx, T* vy, uint32_t* idv, const int32
     idv[dst] = pid;
 }
 
+// uniform positions in a box and uniform velocities (initializeParticles,
+// main.cu:17-32, made reproducible): the four words of the first Philox block
+// of counter {id, 0xFFFFFFFE, 0, case} give x, y, vx, vy
+template <typename T>
+__global__ void init_uniform_kernel(T* x, T* y, T* vx, T* vy, uint32_t* idv, const int32_t* base_ptr, int n,
+                                    double x0, double x1, double y0, double y1, double v0, double v1, uint32_t seed,
+                                    uint32_t case_id, uint32_t species, uint32_t id0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int dst = *base_ptr + i;
+    const uint32_t pid = id0 + (uint32_t)i;
+    Stream rs(pid, 0xFFFFFFFEu, case_id, seed, species);
+    const double ux = ((double)rs.u32() + 0.5) * 0x1p-32, uy = ((double)rs.u32() + 0.5) * 0x1p-32;
+    const double uvx = ((double)rs.u32() + 0.5) * 0x1p-32, uvy = ((double)rs.u32() + 0.5) * 0x1p-32;
+    x[dst] = (T)(x0 + ux * (x1 - x0));
+    y[dst] = (T)(y0 + uy * (y1 - y0));
+    vx[dst] = (T)(v0 + uvx * (v1 - v0));
+    vy[dst] = (T)(v0 + uvy * (v1 - v0));
+    idv[dst] = pid;
+}
+
 __global__ void add_count_kernel(int32_t* c, int n) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *c += n;
 }
@@ -923,6 +945,18 @@ int launch_init(T* x, T* y, T* vx, T* vy, uint32_t* id, int32_t* count_ptr, int 
 }
 
 template <typename T>
+int launch_init_uniform(T* x, T* y, T* vx, T* vy, uint32_t* id, int32_t* count_ptr, int n, double x0, double x1,
+                        double y0, double y1, double v0, double v1, uint32_t seed, uint32_t case_id,
+                        uint32_t species, uint32_t id0, hipStream_t s) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(init_uniform_kernel<T>, dim3(div_up(n, 256)), dim3(256), 0, s, x, y, vx, vy, id, count_ptr, n,
+                       x0, x1, y0, y1, v0, v1, seed, case_id, species, id0);
+    hipLaunchKernelGGL(add_count_kernel, dim3(1), dim3(64), 0, s, count_ptr, n);
+    PIC_HIP(hipGetLastError());
+    return 0;
+}
+
+template <typename T>
 int launch_hist(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items, int chunk,
                 int bw, int bh, int nbx, int nb_new, int32_t* hist, hipStream_t s) {
     if (n_items <= 0) return 0;
@@ -1030,6 +1064,8 @@ int launch_counts(const CaseDev<T>* cases, const CaseState* states, int c, int s
     template int launch_counts<T>(const CaseDev<T>*, const CaseState*, int, int, long long*, hipStream_t);       \
     template int launch_init<T>(T*, T*, T*, T*, uint32_t*, int32_t*, int, int, int, double, uint32_t, uint32_t,  \
                                 uint32_t, uint32_t, hipStream_t);                                                \
+    template int launch_init_uniform<T>(T*, T*, T*, T*, uint32_t*, int32_t*, int, double, double, double, double, \
+                                        double, double, uint32_t, uint32_t, uint32_t, uint32_t, hipStream_t);     \
     template int launch_hist<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int, int, int, int, int,   \
                                 int, int32_t*, hipStream_t);                                                     \
     template int launch_scatter<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int, int, int, int, int, \

@@ -17,6 +17,10 @@ template <typename T>
 int launch_init(T* x, T* y, T* vx, T* vy, uint32_t* id, int32_t* count_ptr, int n, int nx, int ny, double vth,
                 uint32_t seed, uint32_t case_id, uint32_t species, uint32_t id0, hipStream_t s);
 template <typename T>
+int launch_init_uniform(T* x, T* y, T* vx, T* vy, uint32_t* id, int32_t* count_ptr, int n, double x0, double x1,
+                        double y0, double y1, double v0, double v1, uint32_t seed, uint32_t case_id,
+                        uint32_t species, uint32_t id0, hipStream_t s);
+template <typename T>
 int launch_hist(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items, int chunk,
                 int bw, int bh, int nbx, int nb_new, int32_t* hist, hipStream_t s);
 template <typename T>

@@ -127,6 +127,14 @@ public:
     void init_maxwellian(int species, int64_t n, const Grid2D& g, double vth, uint32_t seed, uint32_t id0 = 0) {
         check(pic_particles_init_maxwellian(h_, species, n, g.m, g.n, vth, seed, 0, id0), "init_maxwellian");
     }
+    // uniform positions in [x_lo,x_hi)x[y_lo,y_hi) (domain units) and velocity
+    // components uniform in [v_lo,v_hi)
+    void init_uniform(int species, int64_t n, const Grid2D& g, double x_lo, double x_hi, double y_lo, double y_hi,
+                      double v_lo, double v_hi, uint32_t seed, uint32_t id0 = 0) {
+        check(pic_particles_init_uniform(h_, species, n, x_lo / g.dx, x_hi / g.dx, y_lo / g.dy, y_hi / g.dy, v_lo,
+                                         v_hi, seed, 0, id0),
+              "init_uniform");
+    }
     // updateVBO / convertPositionsToFloat (render.h:384-405, kernels.cu:268-273):
     // live positions as float (x, y) pairs into a device buffer (a mapped
     // vertex buffer, say) of room for `cap` pairs; domain units need the
@@ -201,6 +209,13 @@ inline std::pair<std::vector<double>, std::vector<double>> read_cross_section(co
     std::vector<double> e(n), s(n);
     check(pic_read_cross_section(filename.c_str(), n, e.data(), s.data(), &n), "read_cross_section");
     return {e, s};
+}
+
+// initializeParticles(h_positions, h_velocities, num_particles) main.cu:17-32:
+// positions uniform in [0.2, 0.8] (domain units) on both axes, velocities
+// uniform in [-0.01, 0.01], here seeded and on the device
+inline void initializeParticles(Particles2D& p, int species, int64_t n, const Grid2D& g, uint32_t seed) {
+    p.init_uniform(species, n, g, 0.2, 0.8, 0.2, 0.8, -0.01, 0.01, seed);
 }
 
 // two columns like read_cross_section, numbers also written m*10^{e}

@@ -44,6 +44,20 @@ def test_init_matches_oracle(pic, orc, prec):
     p.close()
 
 
+@pytest.mark.parametrize("prec", [4, 8])
+def test_init_uniform_matches_oracle(pic, orc, prec):
+    p = pic.Particles2D(50000, prec)
+    p.init_uniform(0, 30000, (12.8, 51.2), (6.4, 25.6), (-0.01, 0.01), seed=3, case_id=1, id0=7)
+    x, y, vx, vy, ids = p.download(0)
+    ox, oy, ovx, ovy, oid = orc.init_uniform(prec, 30000, 12.8, 51.2, 6.4, 25.6, -0.01, 0.01, 3, 1, 0, 7)
+    o = np.argsort(ids)
+    dt = np.float32 if prec == 4 else np.float64
+    for a, b in ((x, ox), (y, oy), (vx, ovx), (vy, ovy)):
+        assert np.array_equal(a[o].astype(dt), b)
+    assert np.array_equal(ids[o], oid)
+    p.close()
+
+
 def test_efield_matches_oracle(pic, orc):
     g = pic.Grid2D(97, 97, 0.3, 0.3)
     rng = np.random.default_rng(1)

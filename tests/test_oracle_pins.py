@@ -444,6 +444,29 @@ def test_init_statistics(orc):
     assert abs(k - 3.0) < 0.05
 
 
+def test_init_uniform_statistics(orc):
+    """initializeParticles' distributions (main.cu:17-32): uniform box and
+    uniform velocity components; moments of U(a, b), independence of the
+    components, keyed on the particle id."""
+    n = 200000
+    x, y, vx, vy, ids = orc.init_uniform(8, n, 12.8, 51.2, 6.4, 25.6, -0.01, 0.01, 3, 0, 0)
+    assert x.min() >= 12.8 and x.max() < 51.2 and y.min() >= 6.4 and y.max() < 25.6
+    assert vx.min() >= -0.01 and vx.max() < 0.01
+    for a, lo, hi in ((x, 12.8, 51.2), (y, 6.4, 25.6), (vx, -0.01, 0.01), (vy, -0.01, 0.01)):
+        w = hi - lo
+        assert abs(a.mean() - 0.5 * (lo + hi)) < 4 * w / np.sqrt(12 * n)
+        assert abs(a.var() / (w * w / 12.0) - 1.0) < 0.02
+    for a, b in ((x, y), (vx, vy), (x, vx)):
+        assert abs(np.corrcoef(a, b)[0, 1]) < 0.01
+    # keyed on id: a later slice with id0 reproduces the same particles
+    x2, y2, vx2, vy2, ids2 = orc.init_uniform(8, 1000, 12.8, 51.2, 6.4, 25.6, -0.01, 0.01, 3, 0, 0, id0=5000)
+    assert np.array_equal(x2, x[5000:6000]) and np.array_equal(vy2, vy[5000:6000])
+    assert np.array_equal(ids2, np.arange(5000, 6000))
+    # a separate stream from the Maxwellian init of the same ids
+    xm = orc.init_particles(8, 1000, 65, 65, 1.0, 3, 0, 0)[0]
+    assert abs(np.corrcoef(xm, x[:1000])[0, 1]) < 0.15
+
+
 def test_openmp_oracle_pass_equals_serial(orc):
     """The OpenMP build of the oracle (used only for the CPU baseline) gives the
     serial result: same survivors in the same order, same born particles, same
