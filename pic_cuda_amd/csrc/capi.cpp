@@ -1080,10 +1080,30 @@ static int log2_exact(int v) {
     return k;
 }
 
+// dynamic LDS a workgroup may have on the current device
+static size_t max_lds_per_block() {
+    static size_t v = 0;
+    if (!v) {
+        int dev = 0, bytes = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && bytes > 0)
+            v = (size_t)bytes;
+        else
+            v = 65536;
+    }
+    return v;
+}
+
 template <typename T>
 static int rebin_t(pic_sim* S, int bw, int bh) {
     pic_particles* p = S->p;
     CaseDev<T>& C = hcase<T>(p);
+    if (push_lds_bytes<T>(bw, bh, S->halo) > max_lds_per_block()) {
+        set_error("bin tile " + std::to_string(bw) + "x" + std::to_string(bh) + " needs " +
+                  std::to_string(push_lds_bytes<T>(bw, bh, S->halo)) + " B of LDS per workgroup, more than the " +
+                  std::to_string(max_lds_per_block()) + " B the device allows");
+        return PIC_ERR_ARG;
+    }
     hipStream_t sm = lib_stream();
     const int in = p->h_cur, out = in ^ 1;
     const int nbx = div_up(C.nx - 1, bw), nby = div_up(C.ny - 1, bh);

@@ -80,6 +80,8 @@ enum { PIC_F_KICK = 1, PIC_F_MCC = 2, PIC_F_DRIFT = 4, PIC_F_DEPOSIT = 8 };
 // entries of the LDS stage of particles leaving their bin (a pass of 256
 // particles adds at most 512: 256 movers and 256 born particles)
 #define PIC_MSTAGE 768
+// entries of each wavefront's queue of deferred collision candidates
+#define PIC_MQUEUE 64
 
 // work item of the particle kernels: a bin of a species of a case, or a chunk
 // of its spill region (bin == nb)

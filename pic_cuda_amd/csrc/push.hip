@@ -26,6 +26,7 @@
 // the path of a particle that stays.  Overflow of either part goes to the spill
 // region (index nb).
 #include "pic_device.hpp"
+#include <type_traits>
 
 namespace pic {
 
@@ -266,6 +267,18 @@ __device__ inline int mcc_band(T E, T inv_emax) {
     return min(14 + e, 15);
 }
 
+// The null-collision candidacy test of mcc_collide alone: the first uniform
+// of the particle's draw against the candidate probability of its energy band.
+template <typename T>
+__device__ inline bool mcc_candidate(const SpeciesDev<T>& S, const T* band_lds, uint32_t id, uint32_t step,
+                                     uint32_t case_id, uint32_t seed, T vx, T vy) {
+    Stream rs(id, step, case_id, seed, S.key);
+    const T u0 = rs.uni<T>();
+    const T v2 = dfma(vx, vx, vy * vy);
+    const T E = S.efac * v2;
+    return u0 < band_lds[mcc_band(E, S.inv_emax)];
+}
+
 // Null-collision MCC for one particle (replaces elasticCollisions,
 // mcc.cu:114-150, and its kernels kernels.cu:196-263, mcc.cu:56-85).  The
 // majorant is that of the particle's energy band (band values in LDS).
@@ -483,6 +496,15 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
     int nmw = 0;   // this wavefront's stage fill
     // candidate probability and majorant of each energy band (32 values)
     T* band_lds = reinterpret_cast<T*>(lcnt + PIC_NCOUNT);
+    // collision candidates deferred by each wavefront (PIC_MQUEUE entries per
+    // wavefront: x, y, vx, vy after the kick, and the id)
+    T* queue_base = band_lds + 32;
+    T* qx = queue_base + wv * PIC_MQUEUE;
+    T* qy = queue_base + 4 * PIC_MQUEUE + wv * PIC_MQUEUE;
+    T* qvx = queue_base + 8 * PIC_MQUEUE + wv * PIC_MQUEUE;
+    T* qvy = queue_base + 12 * PIC_MQUEUE + wv * PIC_MQUEUE;
+    uint32_t* qid = reinterpret_cast<uint32_t*>(queue_base + 16 * PIC_MQUEUE) + wv * PIC_MQUEUE;
+    int nq = 0;    // this wavefront's queue fill
     const int flags = flags_arg >= 0 ? flags_arg : st.flags;
     const bool kick = flags & PIC_F_KICK, mcc = (flags & PIC_F_MCC) && S.n_proc > 0, drift = flags & PIC_F_DRIFT,
                dep = flags & PIC_F_DEPOSIT;
@@ -519,54 +541,17 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
     const int home_front = is_bin ? front_cap(G(S.cap[out])[it.bin]) : 0;
     const int lane = __lane_id();
 
-    for (int base = 0; base < n; base += blockDim.x) {
-        const int t = base + threadIdx.x;
-        const bool valid = t < n;
-        T x = (T)0, y = (T)0, vx = (T)0, vy = (T)0;
-        uint32_t id = 0;
-        if (valid) {
-            const int src = rg.at(t);
-            x = G(S.x[in])[src];
-            y = G(S.y[in])[src];
-            vx = G(S.vx[in])[src];
-            vy = G(S.vy[in])[src];
-            id = G(S.id[in])[src];
-        }
+    // Drift, walls, deposit and compaction of one particle per lane after the
+    // kick. FULL: the particle is a collision candidate and collides first
+    // (born particles included); otherwise it does not collide. The whole
+    // wavefront calls it (ballots inside).
+    auto finish = [&](auto full_tag, bool valid, T x, T y, T vx, T vy, uint32_t id) {
+        constexpr bool FULL = decltype(full_tag)::value;
         bool alive = false, born = false, be_alive = false, bi_alive = false;
         int key = 0, key_e = 0, key_i = 0;
         Born<T> b;
         if (valid) {
-            if (kick) {
-                const int ix = (int)dfloor(x), iy = (int)dfloor(y);
-                if (!(ix < 0 || ix >= nx - 1 || iy < 0 || iy >= ny - 1)) {
-                    const T fx = x - (T)ix, fy = y - (T)iy;
-                    const T gx = (T)1 - fx, gy = (T)1 - fy;
-                    const T w00 = gx * gy, w10 = fx * gy, w01 = gx * fy, w11 = fx * fy;
-                    T2 e00, e10, e01, e11;
-                    const int lx = ix - cx0, ly = iy - cy0;
-                    if (is_bin && lx >= 0 && lx < C.bw && ly >= 0 && ly < C.bh) {
-                        const T2* p = ew_lds + ly * ew + lx;
-                        e00 = p[0];
-                        e10 = p[1];
-                        e01 = p[ew];
-                        e11 = p[ew + 1];
-                    } else {
-                        e00.x = (T)ex_node(C.phi, nx, ix, iy, C.dx);
-                        e00.y = (T)ey_node(C.phi, nx, ny, ix, iy, C.dy);
-                        e10.x = (T)ex_node(C.phi, nx, ix + 1, iy, C.dx);
-                        e10.y = (T)ey_node(C.phi, nx, ny, ix + 1, iy, C.dy);
-                        e01.x = (T)ex_node(C.phi, nx, ix, iy + 1, C.dx);
-                        e01.y = (T)ey_node(C.phi, nx, ny, ix, iy + 1, C.dy);
-                        e11.x = (T)ex_node(C.phi, nx, ix + 1, iy + 1, C.dx);
-                        e11.y = (T)ey_node(C.phi, nx, ny, ix + 1, iy + 1, C.dy);
-                    }
-                    const T Ex = dfma(w11, e11.x, dfma(w01, e01.x, dfma(w10, e10.x, w00 * e00.x)));
-                    const T Ey = dfma(w11, e11.y, dfma(w01, e01.y, dfma(w10, e10.y, w00 * e00.y)));
-                    vx = dfma(kq, Ex, vx);
-                    vy = dfma(kq, Ey, vy);
-                }
-            }
-            if (mcc) born = mcc_collide(S, band_lds, id, mcc_step, C.case_id, C.seed, x, y, vx, vy, lcnt, b);
+            if (FULL) born = mcc_collide(S, band_lds, id, mcc_step, C.case_id, C.seed, x, y, vx, vy, lcnt, b);
             const T x0 = x, y0 = y;
             if (drift) {
                 x = dfma(vx, dtdx, x);
@@ -579,7 +564,7 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
                 if (dep) deposit(x, y, nx, ny, is_bin, wx0, wy0, aw, ah, accw, S.acc, lcnt);
                 key = bin_key(C, S.nb[out], x, y);
             }
-            if (born) {
+            if (FULL && born) {
                 // born particles drift from the collision point when the pass drifts
                 const T ex1 = drift ? dfma(b.evx, dtdx, b.ex) : b.ex, ey1 = drift ? dfma(b.evy, dtdy, b.ey) : b.ey;
                 if (hits_walls<T, WALLS>(C, wall_bin, cx0, cy0, b.ex, b.ey, ex1, ey1)) {
@@ -629,29 +614,114 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
         }
         // movers and born particles of this species: staged
         stage_put(sg, nmw, alive && !stay, x, y, vx, vy, id, key);
-        stage_put(sg, nmw, be_alive, b.ex, b.ey, b.evx, b.evy, b.eid, key_e);
-        // born ions of the other species (rare): written directly
-        if (O && __ballot(bi_alive)) {
-            const int di = alloc_slot(*O, out, bi_alive, key_i, true);
-            if (bi_alive) {
-                if (di >= 0) {
-                    G(O->x[out])[di] = b.ix;
-                    G(O->y[out])[di] = b.iy;
-                    G(O->vx[out])[di] = b.ivx;
-                    G(O->vy[out])[di] = b.ivy;
-                    G(O->id[out])[di] = b.iid;
-                } else {
-                    atomicAdd(lcnt + PIC_C_LOST, 1u);
+        if (FULL) {
+            stage_put(sg, nmw, be_alive, b.ex, b.ey, b.evx, b.evy, b.eid, key_e);
+            // born ions of the other species (rare): written directly
+            if (O && __ballot(bi_alive)) {
+                const int di = alloc_slot(*O, out, bi_alive, key_i, true);
+                if (bi_alive) {
+                    if (di >= 0) {
+                        G(O->x[out])[di] = b.ix;
+                        G(O->y[out])[di] = b.iy;
+                        G(O->vx[out])[di] = b.ivx;
+                        G(O->vy[out])[di] = b.ivy;
+                        G(O->id[out])[di] = b.iid;
+                    } else {
+                        atomicAdd(lcnt + PIC_C_LOST, 1u);
+                    }
                 }
             }
         }
-        // a pass adds at most 128 entries per wavefront (a mover and a born
+        // a call adds at most 128 entries per wavefront (a mover and a born
         // electron per lane)
         if (nmw > WST - 128) {
             flush_stage(S, out, sg, nmw, lcnt);
             nmw = 0;
         }
+    };
+    // the deferred candidates of this wavefront, one per lane
+    auto flush_queue = [&]() {
+        wave_lds_sync();
+        const bool v = lane < nq;
+        T x = (T)0, y = (T)0, vx = (T)0, vy = (T)0;
+        uint32_t id = 0;
+        if (v) {
+            x = qx[lane];
+            y = qy[lane];
+            vx = qvx[lane];
+            vy = qvy[lane];
+            id = qid[lane];
+        }
+        wave_lds_sync();
+        finish(std::true_type{}, v, x, y, vx, vy, id);
+        nq = 0;
+    };
+
+    for (int base = 0; base < n; base += blockDim.x) {
+        const int t = base + threadIdx.x;
+        const bool valid = t < n;
+        T x = (T)0, y = (T)0, vx = (T)0, vy = (T)0;
+        uint32_t id = 0;
+        if (valid) {
+            const int src = rg.at(t);
+            x = G(S.x[in])[src];
+            y = G(S.y[in])[src];
+            vx = G(S.vx[in])[src];
+            vy = G(S.vy[in])[src];
+            id = G(S.id[in])[src];
+        }
+        if (valid && kick) {
+            const int ix = (int)dfloor(x), iy = (int)dfloor(y);
+            if (!(ix < 0 || ix >= nx - 1 || iy < 0 || iy >= ny - 1)) {
+                const T fx = x - (T)ix, fy = y - (T)iy;
+                const T gx = (T)1 - fx, gy = (T)1 - fy;
+                const T w00 = gx * gy, w10 = fx * gy, w01 = gx * fy, w11 = fx * fy;
+                T2 e00, e10, e01, e11;
+                const int lx = ix - cx0, ly = iy - cy0;
+                if (is_bin && lx >= 0 && lx < C.bw && ly >= 0 && ly < C.bh) {
+                    const T2* p = ew_lds + ly * ew + lx;
+                    e00 = p[0];
+                    e10 = p[1];
+                    e01 = p[ew];
+                    e11 = p[ew + 1];
+                } else {
+                    e00.x = (T)ex_node(C.phi, nx, ix, iy, C.dx);
+                    e00.y = (T)ey_node(C.phi, nx, ny, ix, iy, C.dy);
+                    e10.x = (T)ex_node(C.phi, nx, ix + 1, iy, C.dx);
+                    e10.y = (T)ey_node(C.phi, nx, ny, ix + 1, iy, C.dy);
+                    e01.x = (T)ex_node(C.phi, nx, ix, iy + 1, C.dx);
+                    e01.y = (T)ey_node(C.phi, nx, ny, ix, iy + 1, C.dy);
+                    e11.x = (T)ex_node(C.phi, nx, ix + 1, iy + 1, C.dx);
+                    e11.y = (T)ey_node(C.phi, nx, ny, ix + 1, iy + 1, C.dy);
+                }
+                const T Ex = dfma(w11, e11.x, dfma(w01, e01.x, dfma(w10, e10.x, w00 * e00.x)));
+                const T Ey = dfma(w11, e11.y, dfma(w01, e01.y, dfma(w10, e10.y, w00 * e00.y)));
+                vx = dfma(kq, Ex, vx);
+                vy = dfma(kq, Ey, vy);
+            }
+        }
+        // collision candidates leave the batch for this wavefront's queue,
+        // which is processed (one candidate per lane) whenever it would overflow
+        bool cand = false;
+        if (mcc) {
+            cand = valid && mcc_candidate(S, band_lds, id, mcc_step, C.case_id, C.seed, vx, vy);
+            const uint64_t cmask = __ballot(cand);
+            if (cmask) {
+                if (nq + __popcll(cmask) > PIC_MQUEUE) flush_queue();
+                if (cand) {
+                    const int q = nq + __popcll(cmask & lanemask_lt());
+                    qx[q] = x;
+                    qy[q] = y;
+                    qvx[q] = vx;
+                    qvy[q] = vy;
+                    qid[q] = id;
+                }
+                nq += __popcll(cmask);
+            }
+        }
+        finish(std::false_type{}, valid && !cand, x, y, vx, vy, id);
     }
+    if (nq > 0) flush_queue();
     if (nmw > 0) flush_stage(S, out, sg, nmw, lcnt);
     // every wavefront's stay count, deposits and counters are in LDS now
     __syncthreads();
