@@ -163,6 +163,10 @@ typedef struct {
     int mode;            /* PIC_MG_FP64 or PIC_MG_MIXED */
     int inner;           /* mixed mode: FP32 V-cycles per FP64 refinement step, 1..4 (0: 1);
                             the convergence test runs after each refinement step */
+    int coarse_direct;   /* mixed mode, no electrodes: 1 solves the first level with at most
+                            33 x 33 nodes exactly (FP64 sine transform) instead of recursing
+                            and sweeping; 0 keeps the sweeps.  FP64 mode ignores it (its
+                            cycle is the reference's, Multigrid.cu:539-569) */
 } pic_mg_params;
 
 typedef struct {

@@ -52,7 +52,8 @@ EPS0 = 8.8541878128e-12
 class MGParams(ctypes.Structure):
     _fields_ = [("levels", ctypes.c_int), ("pre", ctypes.c_int), ("post", ctypes.c_int),
                 ("coarse_sweeps", ctypes.c_int), ("omega", ctypes.c_double), ("tol", ctypes.c_double),
-                ("max_cycles", ctypes.c_int), ("mode", ctypes.c_int), ("inner", ctypes.c_int)]
+                ("max_cycles", ctypes.c_int), ("mode", ctypes.c_int), ("inner", ctypes.c_int),
+                ("coarse_direct", ctypes.c_int)]
 
 
 class MGStats(ctypes.Structure):

@@ -164,7 +164,7 @@ extern "C" int pic_ensemble_add(pic_ensemble* e, pic_sim* S) {
         const MGParams& b = S->mgp;
         PIC_REQUIRE(a.mode == b.mode && a.pre == b.pre && a.post == b.post && a.coarse_sweeps == b.coarse_sweeps &&
                         a.omega == b.omega && a.tol == b.tol && a.max_cycles == b.max_cycles &&
-                        a.inner_eff() == b.inner_eff(),
+                        a.inner_eff() == b.inner_eff() && a.coarse_direct == b.coarse_direct,
                     "ensemble members need identical multigrid cycle parameters");
         PIC_REQUIRE(S->spill_chunk == e->m[0]->spill_chunk, "ensemble members need the same spill chunk");
     }

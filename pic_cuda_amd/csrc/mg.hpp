@@ -32,6 +32,7 @@ struct MGParams {
     int max_cycles;
     int mode;
     int inner = 1;      // mixed mode: FP32 V-cycles per FP64 refinement step
+    int coarse_direct = 0;   // mixed mode: exact FP64 solve on the first level with n <= 33
     int per_level = 0;  // 1: pre_l/post_l give each level's sweeps (index 0 = finest)
     int pre_l[16] = {}, post_l[16] = {};
     int pre_at(int l) const { return per_level ? pre_l[l] : pre; }
@@ -80,8 +81,12 @@ struct MGCoarseTask {
     const T* e_in;
     const T* rhs;
     T* e_out;
-    int n_top, Lc, csweeps, pad;
+    int n_top, Lc, csweeps;
+    int Ld;                // level of the direct solve (with dtab)
     const uint8_t* mask;   // electrodes of the top level (coarser levels: injection)
+    // FP64 sine-transform tables of level Ld (S then W, 32 x 32 each, zero
+    // padded); null: the levels below are swept like the reference's
+    const double* dtab;
     // sweeps of the kernel's level l (0 = its top) in bits 2l..2l+1 (packed so
     // that no array is indexed at run time)
     unsigned pre_bits, post_bits;
@@ -205,6 +210,7 @@ private:
     };
     template <typename T>
     int build_cycle(const std::vector<double>& h, bool warm);
+    int build_direct_tables();
     template <typename T>
     int add_level_stage(int kind, const std::vector<MGLevelTask<T>>& tasks, int nu, bool restr,
                         std::vector<Stage>* dst = nullptr);
@@ -234,6 +240,10 @@ private:
     MGNorm* h_mirror_ = nullptr;        // pinned
     int last_cycles_ = 1;
     bool any_mask_ = false;             // some member has electrodes
+    // direct coarse solve: per member the kernel level and its tables (null: none)
+    std::vector<int> dlev_;
+    std::vector<const double*> dtab_;
+    double* d_dtab_ = nullptr;
     int fixed_cycles_ = 0;
     int launches_ = 0;
 };

@@ -20,7 +20,10 @@
 //   mg_prolong_stage  coarse patch + fine tile + halo 2*nu in LDS,
 //                     prolongation-add, nu sweeps.
 //   mg_coarse_stage   one workgroup runs the whole sub-cycle of the levels
-//                     with n <= 65 (FP32) / 33 (FP64) in LDS.
+//                     with n <= 65 (FP32) / 33 (FP64) in LDS.  In mixed mode
+//                     the first of them with n <= 33 can instead be solved
+//                     exactly in FP64 by a 2D sine transform (four 32x32
+//                     products), which replaces the recursion below it.
 //   ir_residual_stage FP64 residual on 64x16 tiles, per-tile sums of r^2; in
 //                     mixed mode fused with the correction phi += e of the
 //                     previous FP32 cycle (phi alternates between two buffers).
@@ -40,6 +43,11 @@ namespace pic {
 // below 64 KB: FP32 65^2, FP64 33^2)
 #define MG_NC_F32 65
 #define MG_NC_F64 33
+// direct coarse solve: levels up to 33 nodes (31 unknowns a side), tables and
+// LDS products padded to 32 with row stride MG_DP (doubles, conflict-free)
+#define MG_ND 33
+#define MG_DP 33
+#define MG_DSCRATCH (2 * 32 * MG_DP * 2)   // X and Y in floats
 // residual tiles: 64 columns x 16 rows, 256 threads with 4 rows each
 #define MG_RX 64
 #define MG_RY 16
@@ -145,7 +153,7 @@ __global__ __launch_bounds__(256) void mg_smooth_stage(const StageRef<MGLevelTas
     const gptr<T> rhs_c = G(tk.rhs_c);
     const int n = tk.n;
     const T h2 = tk.h2, inv_h2 = tk.inv_h2, omega = tk.omega;
-    extern __shared__ unsigned char smem_mg[];
+    extern __shared__ __align__(16) unsigned char smem_mg[];
     T* e = reinterpret_cast<T*>(smem_mg);
     T* r = e + WW;
     // electrodes: only the MASKED instantiation carries the mask code
@@ -267,7 +275,7 @@ __global__ __launch_bounds__(256) void mg_prolong_stage(const StageRef<MGLevelTa
     const gptr<T> e_out = G(tk.e_out);
     const int n = tk.n;
     const T h2 = tk.h2, omega = tk.omega;
-    extern __shared__ unsigned char smem_mg[];
+    extern __shared__ __align__(16) unsigned char smem_mg[];
     T* e = reinterpret_cast<T*>(smem_mg);
     T* r = e + WW;
     T* C = r + WW;
@@ -374,6 +382,55 @@ __device__ inline void lds_sweeps(T* e, const T* r, int n, T h, T omega, int ite
         }
 }
 
+// Exact solve of lap_h e = r (e = 0 on the boundary) on a level of n <= 33
+// nodes, m = n - 2 unknowns a side, in FP64 with the 2D discrete sine transform:
+//   e = S (h^2 W o (S R S)) S,   S[a][b] = sin(pi (a+1)(b+1) / (m+1)),
+//   W[p][q] = (2/(m+1))^2 / (-4 (sin^2(pi (p+1) / (2(m+1))) + sin^2(pi (q+1) / (2(m+1))))),
+// since S^2 = (m+1)/2 I and the columns of S are the eigenvectors of the 1D
+// second difference.  tab holds S then W, each 32 x 32 and zero outside
+// m x m, so every product runs over 32 terms with no bounds.  Thread (j, k)
+// holds column k of S in registers (S is symmetric, so it is also row k);
+// the products that would need row j of S are formed transposed instead, and
+// each term reads one LDS value that all lanes of a half-wavefront share.
+// X and Y are 32 x MG_DP FP64 LDS buffers.
+template <typename T>
+__device__ void direct_sine_solve(T* e, const T* r, int n, T h, const gptr<const double> tab, double* X, double* Y) {
+    const int k = threadIdx.x & 31, j = threadIdx.x >> 5;
+    const int m = n - 2;
+    double s[32];
+#pragma unroll
+    for (int l = 0; l < 32; ++l) s[l] = tab[l * 32 + k];
+    const double hd = (double)h;
+    const double w = tab[1024 + j * 32 + k] * (hd * hd);   // W[j][k] = W[k][j]
+    // R[j][k] = r at node (k+1, j+1), zero outside the m x m block
+    X[j * MG_DP + k] = (j < m && k < m) ? (double)r[(j + 1) * n + (k + 1)] : 0.0;
+    __syncthreads();
+    // T1 = R S:                Y[j][k] = sum_l R[j][l] S[l][k]
+    double acc = 0.0;
+#pragma unroll
+    for (int l = 0; l < 32; ++l) acc = fma(X[j * MG_DP + l], s[l], acc);
+    Y[j * MG_DP + k] = acc;
+    __syncthreads();
+    // T2 = S T1, scaled, stored transposed: X[j][k] = h^2 W[k][j] sum_l S[k][l] T1[l][j]
+    acc = 0.0;
+#pragma unroll
+    for (int l = 0; l < 32; ++l) acc = fma(s[l], Y[l * MG_DP + j], acc);
+    X[j * MG_DP + k] = acc * w;
+    __syncthreads();
+    // T3 = T2 S:               Y[j][k] = sum_l T2[j][l] S[l][k] = sum_l X[l][j] S[l][k]
+    acc = 0.0;
+#pragma unroll
+    for (int l = 0; l < 32; ++l) acc = fma(X[l * MG_DP + j], s[l], acc);
+    Y[j * MG_DP + k] = acc;
+    __syncthreads();
+    // E = S T3, transposed:    E[k][j] = sum_l S[k][l] T3[l][j], node (j+1, k+1)
+    acc = 0.0;
+#pragma unroll
+    for (int l = 0; l < 32; ++l) acc = fma(s[l], Y[l * MG_DP + j], acc);
+    if (j < m && k < m) e[(k + 1) * n + (j + 1)] = (T)acc;
+    __syncthreads();
+}
+
 template <typename T, bool MASKED>
 __global__ __launch_bounds__(1024) void mg_coarse_stage(const StageRef<MGCoarseTask<T>> a) {
     MGCoarseTask<T> tk;
@@ -384,7 +441,7 @@ __global__ __launch_bounds__(1024) void mg_coarse_stage(const StageRef<MGCoarseT
     const gptr<T> e_out = G(tk.e_out);
     const int n_top = tk.n_top, Lc = tk.Lc, csweeps = tk.csweeps;
     const T h_top = tk.h_top, omega = tk.omega;
-    extern __shared__ unsigned char smem_mg[];
+    extern __shared__ __align__(16) unsigned char smem_mg[];
     T* base = reinterpret_cast<T*>(smem_mg);
     // level l: n = s_n[l], e at base + s_off[l], rhs right after it; the
     // residual scratch follows the last level (s_off[Lc])
@@ -423,8 +480,12 @@ __global__ __launch_bounds__(1024) void mg_coarse_stage(const StageRef<MGCoarseT
     }
     const CoarseMask cm{(MASKED && gmask) ? s_mb : nullptr, n_top};
     __syncthreads();
+    // bottom of the recursion: the coarsest level, or the directly solved one
+    int Lb = Lc - 1;
+    if constexpr (sizeof(T) == 4)
+        if (tk.dtab) Lb = tk.Ld;
     T h = h_top;
-    for (int l = 0; l < Lc - 1; ++l) {
+    for (int l = 0; l < Lb; ++l) {
         const int nf = s_n[l], ncs = s_n[l + 1];
         lds_sweeps(E(l), R(l), nf, h, omega, (int)((tk.pre_bits >> (2 * l)) & 3u), cm, l);
         const T inv_h2 = (T)1 / (h * h);
@@ -466,8 +527,17 @@ __global__ __launch_bounds__(1024) void mg_coarse_stage(const StageRef<MGCoarseT
         __syncthreads();
         h = h * (T)2;
     }
-    lds_sweeps(E(Lc - 1), R(Lc - 1), s_n[Lc - 1], h, omega, csweeps, cm, Lc - 1);
-    for (int l = Lc - 2; l >= 0; --l) {
+    if constexpr (sizeof(T) == 4) {
+        if (tk.dtab) {
+            double* X = reinterpret_cast<double*>(base + s_off[Lc]);
+            direct_sine_solve(E(Lb), R(Lb), s_n[Lb], h, G(tk.dtab), X, X + 32 * MG_DP);
+        } else {
+            lds_sweeps(E(Lb), R(Lb), s_n[Lb], h, omega, csweeps, cm, Lb);
+        }
+    } else {
+        lds_sweeps(E(Lb), R(Lb), s_n[Lb], h, omega, csweeps, cm, Lb);
+    }
+    for (int l = Lb - 1; l >= 0; --l) {
         h = h / (T)2;
         const int nf = s_n[l], ncs = s_n[l + 1];
         const T* C = E(l + 1);
@@ -818,13 +888,15 @@ int MGSolver::last_cycles() const { return solo_ ? solo_->last_cycles() : 1; }
 // ---------------------------------------------------------------------------
 // MGBatch: one launch per stage for all member grids
 // ---------------------------------------------------------------------------
+// levels' e and rhs, then the residual scratch (with a direct solve also its
+// two FP64 product buffers)
 template <typename T>
-static size_t coarse_lds_bytes(int n_top, int Lc) {
+static size_t coarse_lds_bytes(int n_top, int Lc, bool direct) {
     size_t tot = 0;
     int n = n_top;
     for (int l = 0; l < Lc; ++l) { tot += 2 * (size_t)n * n; n = (n + 1) / 2; }
-    tot += (size_t)n_top * n_top;
-    return tot * sizeof(T);
+    const size_t scratch = (size_t)n_top * n_top * sizeof(T);
+    return tot * sizeof(T) + std::max(scratch, direct ? (size_t)MG_DSCRATCH * 4 : (size_t)0);
 }
 
 // LDS of a tiled stage: e and rhs over tile + halo, plus the residual patch
@@ -854,6 +926,8 @@ void MGBatch::free_all() {
     if (d_res_tiles_) (void)hipFree(d_res_tiles_);
     if (d_mirror_) (void)hipFree(d_mirror_);
     if (h_mirror_) (void)hipHostFree(h_mirror_);
+    if (d_dtab_) (void)hipFree(d_dtab_);
+    d_dtab_ = nullptr;
     d_fine_ = nullptr;
     d_chunk_tiles_ = d_res_tiles_ = nullptr;
     d_mirror_ = nullptr;
@@ -970,7 +1044,9 @@ int MGBatch::build_cycle(const std::vector<double>& h, bool warm) {
             t.omega = omega;
             t.gate = s->d_norm_;
             t.mask = s->masked() ? s->mask_[lc] : nullptr;
-            lds = std::max(lds, coarse_lds_bytes<T>(t.n_top, t.Lc));
+            t.dtab = sizeof(T) == 4 ? dtab_[k] : nullptr;
+            t.Ld = dlev_[k];
+            lds = std::max(lds, coarse_lds_bytes<T>(t.n_top, t.Lc, t.dtab != nullptr));
             ct.push_back(t);
         }
         if (!ct.empty()) {
@@ -1071,6 +1147,56 @@ int MGBatch::build_cycle(const std::vector<double>& h, bool warm) {
     return 0;
 }
 
+// Direct coarse solve (mixed mode, members without electrodes whose coarse
+// kernel reaches a level of at most MG_ND nodes): that level's index within
+// the kernel and the sine-transform tables of its size, one set per size.
+int MGBatch::build_direct_tables() {
+    const int M = (int)m_.size();
+    dlev_.assign(M, 0);
+    dtab_.assign(M, nullptr);
+    if (!mixed_ || !p_.coarse_direct) return 0;
+    std::vector<int> sizes, want(M, 0);
+    for (int k = 0; k < M; ++k) {
+        const MGSolver* s = m_[k];
+        if (s->masked() || s->ns_[s->lc_] > s->nc_) continue;
+        for (int l = s->lc_; l < s->L_; ++l)
+            if (s->ns_[l] <= MG_ND) {
+                if (s->ns_[l] >= 3) {
+                    dlev_[k] = l - s->lc_;
+                    want[k] = s->ns_[l];
+                    if (std::find(sizes.begin(), sizes.end(), want[k]) == sizes.end()) sizes.push_back(want[k]);
+                }
+                break;
+            }
+    }
+    if (sizes.empty()) return 0;
+    std::vector<double> tab(sizes.size() * 2048, 0.0);
+    for (size_t q = 0; q < sizes.size(); ++q) {
+        const int m = sizes[q] - 2;
+        double* S = &tab[q * 2048];
+        double* W = S + 1024;
+        const long double pi = 3.141592653589793238462643383279502884L;
+        std::vector<long double> sig(m);
+        for (int p = 0; p < m; ++p) {
+            const long double sp = std::sin(pi * (p + 1) / (2.0L * (m + 1)));
+            sig[p] = sp * sp;
+        }
+        const long double norm = (2.0L / (m + 1)) * (2.0L / (m + 1));
+        for (int a = 0; a < m; ++a)
+            for (int b = 0; b < m; ++b) {
+                S[a * 32 + b] = (double)std::sin(pi * (long double)((a + 1) * (b + 1)) / (long double)(m + 1));
+                W[a * 32 + b] = (double)(norm / (-4.0L * (sig[a] + sig[b])));
+            }
+    }
+    if (int rc = upload_vec((void**)&d_dtab_, tab)) return rc;
+    for (int k = 0; k < M; ++k)
+        if (want[k]) {
+            const size_t q = std::find(sizes.begin(), sizes.end(), want[k]) - sizes.begin();
+            dtab_[k] = d_dtab_ + q * 2048;
+        }
+    return 0;
+}
+
 int MGBatch::build(const std::vector<MGSolver*>& members, const std::vector<double*>& phi,
                    const std::vector<const double*>& rhs, const std::vector<double>& h) {
     free_all();
@@ -1088,7 +1214,7 @@ int MGBatch::build(const std::vector<MGSolver*>& members, const std::vector<doub
             same_levels = q.pre_l[l] == p_.pre_l[l] && q.post_l[l] == p_.post_l[l];
         PIC_REQUIRE(q.mode == p_.mode && q.pre == p_.pre && q.post == p_.post && q.coarse_sweeps == p_.coarse_sweeps &&
                         q.omega == p_.omega && q.tol == p_.tol && q.max_cycles == p_.max_cycles && same_levels &&
-                        q.inner_eff() == p_.inner_eff(),
+                        q.inner_eff() == p_.inner_eff() && q.coarse_direct == p_.coarse_direct,
                     "multigrid batch: members need identical cycle parameters");
     }
     mixed_ = p_.mode == PIC_MG_MIXED;
@@ -1134,6 +1260,7 @@ int MGBatch::build(const std::vector<MGSolver*>& members, const std::vector<doub
     if (int rc = upload_vec((void**)&d_res_tiles_, res)) return rc;
     n_chunk_ = (int)chunks.size();
     n_res_ = (int)res.size();
+    if (int rc = build_direct_tables()) return rc;
     if (int rc = mixed_ ? build_cycle<float>(h, false) : build_cycle<double>(h, false)) return rc;
     if (mixed_ && inner_ > 1)
         if (int rc = build_cycle<float>(h, true)) return rc;

@@ -105,18 +105,63 @@ def test_mg_reference_preset_bit_identical(pic, orc):
     assert np.array_equal(phi_g, phi_o)
 
 
+@pytest.mark.parametrize("direct", [1, 0])
 @pytest.mark.parametrize("n", [129, 513])
-def test_mg_mixed_precision_accuracy(pic, orc, n):
+def test_mg_mixed_precision_accuracy(pic, orc, n, direct):
     rng = np.random.default_rng(2 * n)
     rhs = rng.normal(size=(n, n))
     h = 1.0 / (n - 1)
-    params = pic.mg_params(pic.PRESET_POISSON)          # FP32 cycles in FP64 refinement, 1e-10
+    # FP32 cycles in FP64 refinement to 1e-10; the coarsest part solved
+    # directly (FP64 sine transform at 33^2) or swept down to 3^2
+    params = pic.mg_params(pic.PRESET_POISSON, coarse_direct=direct)
     phi_g, st = _mg_gpu(pic, n, rhs, np.zeros((n, n)), h, params)
     assert st.rel_residual < 1e-10
     phi_o = np.zeros((n, n))
     orc.mg_solve(phi_o, rhs, h, tol=1e-13, max_cycles=100)
     err = np.linalg.norm(phi_g - phi_o) / np.linalg.norm(phi_o)
     assert err < 1e-9, err
+
+
+def _laplacian_dense(n, h):
+    """The 5-point Laplacian on the (n-2)^2 interior nodes, Dirichlet zero."""
+    m = n - 2
+    A = np.zeros((m * m, m * m))
+    for y in range(m):
+        for x in range(m):
+            p = y * m + x
+            A[p, p] = -4.0 / h ** 2
+            for dx, dy in ((1, 0), (-1, 0), (0, 1), (0, -1)):
+                if 0 <= x + dx < m and 0 <= y + dy < m:
+                    A[p, (y + dy) * m + x + dx] = 1.0 / h ** 2
+    return A
+
+
+@pytest.mark.parametrize("n", [3, 4, 5, 9, 17, 33])
+def test_mg_direct_coarse_solve_is_exact(pic, n):
+    """A grid of at most 33 nodes a side is solved by the coarse kernel's FP64
+    sine transform alone, so one FP32 V-cycle from zero is the solution of the
+    discrete system up to FP32 rounding (against numpy's dense solve of the
+    5-point Laplacian).  The same V-cycle with the reference's sweeps instead
+    is far from it, so the check sees the transform."""
+    rng = np.random.default_rng(100 + n)
+    rhs = rng.normal(size=(n, n))
+    h = 0.37 / (n - 1)
+    exact = np.zeros((n, n))
+    exact[1:-1, 1:-1] = np.linalg.solve(_laplacian_dense(n, h), rhs[1:-1, 1:-1].reshape(-1)).reshape(n - 2, n - 2)
+    one = dict(tol=0.0, max_cycles=1, inner=1)
+    phi_d, st = _mg_gpu(pic, n, rhs, np.zeros((n, n)), h, pic.mg_params(pic.PRESET_POISSON, **one))
+    assert st.cycles == 1
+    err_d = np.linalg.norm(phi_d - exact) / np.linalg.norm(exact)
+    assert err_d < 1e-6, err_d
+    assert not np.any(phi_d[0, :]) and not np.any(phi_d[-1, :]) and not np.any(phi_d[:, 0])
+    phi_s, _ = _mg_gpu(pic, n, rhs, np.zeros((n, n)), h, pic.mg_params(pic.PRESET_POISSON, coarse_direct=0, **one))
+    err_s = np.linalg.norm(phi_s - exact) / np.linalg.norm(exact)
+    if n >= 9:
+        assert err_s > 1e-3, err_s
+    # and the whole solve converges in few refinement steps
+    phi_c, st = _mg_gpu(pic, n, rhs, np.zeros((n, n)), h, pic.mg_params(pic.PRESET_POISSON, tol=1e-12))
+    assert st.rel_residual < 1e-12 and st.cycles <= 4, (st.cycles, st.rel_residual)
+    assert np.linalg.norm(phi_c - exact) / np.linalg.norm(exact) < 1e-11
 
 
 @pytest.mark.parametrize("n", [65, 129, 513, 1025])
