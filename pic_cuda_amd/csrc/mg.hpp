@@ -32,7 +32,7 @@ struct MGParams {
     int max_cycles;
     int mode;
     int inner = 1;      // mixed mode: FP32 V-cycles per FP64 refinement step
-    int coarse_direct = 0;   // mixed mode: exact FP64 solve on the first level with n <= 33
+    int coarse_direct = 0;   // mixed mode: exact FP64 solve of the coarse kernel's level (n <= 65)
     int per_level = 0;  // 1: pre_l/post_l give each level's sweeps (index 0 = finest)
     int pre_l[16] = {}, post_l[16] = {};
     int pre_at(int l) const { return per_level ? pre_l[l] : pre; }
@@ -81,11 +81,10 @@ struct MGCoarseTask {
     const T* e_in;
     const T* rhs;
     T* e_out;
-    int n_top, Lc, csweeps;
-    int Ld;                // level of the direct solve (with dtab)
+    int n_top, Lc, csweeps, pad;
     const uint8_t* mask;   // electrodes of the top level (coarser levels: injection)
-    // FP64 sine-transform tables of level Ld (S then W, 32 x 32 each, zero
-    // padded); null: the levels below are swept like the reference's
+    // FP64 sine-transform tables of the top level (S then W, 64 x 64 each,
+    // zero padded): solve it exactly; null: the reference's V-cycle below it
     const double* dtab;
     // sweeps of the kernel's level l (0 = its top) in bits 2l..2l+1 (packed so
     // that no array is indexed at run time)
@@ -240,8 +239,7 @@ private:
     MGNorm* h_mirror_ = nullptr;        // pinned
     int last_cycles_ = 1;
     bool any_mask_ = false;             // some member has electrodes
-    // direct coarse solve: per member the kernel level and its tables (null: none)
-    std::vector<int> dlev_;
+    // direct coarse solve: per member its tables (null: none)
     std::vector<const double*> dtab_;
     double* d_dtab_ = nullptr;
     int fixed_cycles_ = 0;

@@ -20,10 +20,9 @@
 //   mg_prolong_stage  coarse patch + fine tile + halo 2*nu in LDS,
 //                     prolongation-add, nu sweeps.
 //   mg_coarse_stage   one workgroup runs the whole sub-cycle of the levels
-//                     with n <= 65 (FP32) / 33 (FP64) in LDS.  In mixed mode
-//                     the first of them with n <= 33 can instead be solved
-//                     exactly in FP64 by a 2D sine transform (four 32x32
-//                     products), which replaces the recursion below it.
+//                     with n <= 65 (FP32) / 33 (FP64) in LDS; in mixed mode
+//                     it can instead solve its top level exactly in FP64 with
+//                     a 2D sine transform on the matrix cores.
 //   ir_residual_stage FP64 residual on 64x16 tiles, per-tile sums of r^2; in
 //                     mixed mode fused with the correction phi += e of the
 //                     previous FP32 cycle (phi alternates between two buffers).
@@ -43,11 +42,12 @@ namespace pic {
 // below 64 KB: FP32 65^2, FP64 33^2)
 #define MG_NC_F32 65
 #define MG_NC_F64 33
-// direct coarse solve: levels up to 33 nodes (31 unknowns a side), tables and
-// LDS products padded to 32 with row stride MG_DP (doubles, conflict-free)
-#define MG_ND 33
-#define MG_DP 33
-#define MG_DSCRATCH (2 * 32 * MG_DP * 2)   // X and Y in floats
+// direct coarse solve: levels up to 65 nodes (63 unknowns a side); the two
+// FP64 LDS buffers have row stride MG_DS (63 x 63 each: 63.5 KB in all, and
+// 16 consecutive rows fall in distinct bank pairs)
+#define MG_ND 65
+#define MG_DS 63
+#define MG_DLDS (2 * MG_DS * MG_DS * 8)
 // residual tiles: 64 columns x 16 rows, 256 threads with 4 rows each
 #define MG_RX 64
 #define MG_RY 16
@@ -382,53 +382,132 @@ __device__ inline void lds_sweeps(T* e, const T* r, int n, T h, T omega, int ite
         }
 }
 
-// Exact solve of lap_h e = r (e = 0 on the boundary) on a level of n <= 33
-// nodes, m = n - 2 unknowns a side, in FP64 with the 2D discrete sine transform:
+// ---- mixed mode: exact FP64 solve of the coarse kernel's level ---------------
+// lap_h e = r with e = 0 on the boundary, on n <= 65 nodes (m = n - 2 <= 63
+// unknowns a side), by the 2D discrete sine transform in FP64:
 //   e = S (h^2 W o (S R S)) S,   S[a][b] = sin(pi (a+1)(b+1) / (m+1)),
 //   W[p][q] = (2/(m+1))^2 / (-4 (sin^2(pi (p+1) / (2(m+1))) + sin^2(pi (q+1) / (2(m+1))))),
 // since S^2 = (m+1)/2 I and the columns of S are the eigenvectors of the 1D
-// second difference.  tab holds S then W, each 32 x 32 and zero outside
-// m x m, so every product runs over 32 terms with no bounds.  Thread (j, k)
-// holds column k of S in registers (S is symmetric, so it is also row k);
-// the products that would need row j of S are formed transposed instead, and
-// each term reads one LDS value that all lanes of a half-wavefront share.
-// X and Y are 32 x MG_DP FP64 LDS buffers.
+// second difference.  tab holds S then W, each 64 x 64 and zero outside
+// m x m.  The four 64 x 64 products run on the FP64 matrix cores
+// (v_mfma_f64_16x16x4f64: A[i][k] in lane i + 16k, B[k][j] in lane j + 16k,
+// D[i][j] in lane j + 16 (i % 4), element i / 4).  Wavefront w = 4 rb + cb
+// keeps column block cb of S for all k in registers; S is symmetric, so the
+// same registers are the A operand of S's row block cb.  Products with S on
+// the right give tile (rb, cb), products with S on the left tile (cb, rb).
+// X and Y hold m x m FP64 values with row stride MG_DS in LDS.
+typedef double mg_v4d __attribute__((ext_vector_type(4)));
+
+// One 16 x 16 tile of a product over the full k range 0..63: the LDS operand
+// is the A side (LDS_LEFT: A[base + i][k]) or the B side (B[k][base + j]), the
+// register block F the other.  All 16 LDS values are loaded first (entries
+// beyond m read a clamped in-range address and are replaced by zero), then the
+// MFMAs of the k steps below m run on two accumulators.
+template <bool LDS_LEFT>
+__device__ inline mg_v4d tile_product(const double* A, int base, const double (&F)[16], int m, int li, int lk) {
+    double v[16];
+#pragma unroll
+    for (int kb = 0; kb < 16; ++kb) {
+        const int row = LDS_LEFT ? base + li : kb * 4 + lk;
+        const int col = LDS_LEFT ? kb * 4 + lk : base + li;
+        const double x = A[min(row, m - 1) * MG_DS + min(col, m - 1)];
+        v[kb] = (row < m && col < m) ? x : 0.0;
+    }
+    mg_v4d a0 = {0.0, 0.0, 0.0, 0.0}, a1 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kb = 0; kb < 16; kb += 2) {
+        if (kb * 4 >= m) break;   // the rest is zero padding
+        if (LDS_LEFT) {
+            a0 = __builtin_amdgcn_mfma_f64_16x16x4f64(v[kb], F[kb], a0, 0, 0, 0);
+            a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(v[kb + 1], F[kb + 1], a1, 0, 0, 0);
+        } else {
+            a0 = __builtin_amdgcn_mfma_f64_16x16x4f64(F[kb], v[kb], a0, 0, 0, 0);
+            a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(F[kb + 1], v[kb + 1], a1, 0, 0, 0);
+        }
+    }
+    return a0 + a1;
+}
+
 template <typename T>
-__device__ void direct_sine_solve(T* e, const T* r, int n, T h, const gptr<const double> tab, double* X, double* Y) {
-    const int k = threadIdx.x & 31, j = threadIdx.x >> 5;
-    const int m = n - 2;
-    double s[32];
+__device__ void coarse_direct_solve(const MGCoarseTask<T>& tk, double* X, double* Y) {
+    const int n = tk.n_top, m = n - 2;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rb = wave >> 2, cb = wave & 3;
+    const int li = lane & 15, lk = lane >> 4;
+    const bool live = rb * 16 < m && cb * 16 < m;   // tiles beyond m stay zero
+    const gptr<const double> tab = G(tk.dtab);
+    const gptr<const T> r = G(tk.rhs);
+    const gptr<T> e = G(tk.e_out);
+    // R into X (m*m <= 3969 < 4 * 1024): all loads first, from clamped indices
+    float rv[4];
 #pragma unroll
-    for (int l = 0; l < 32; ++l) s[l] = tab[l * 32 + k];
-    const double hd = (double)h;
-    const double w = tab[1024 + j * 32 + k] * (hd * hd);   // W[j][k] = W[k][j]
-    // R[j][k] = r at node (k+1, j+1), zero outside the m x m block
-    X[j * MG_DP + k] = (j < m && k < m) ? (double)r[(j + 1) * n + (k + 1)] : 0.0;
-    __syncthreads();
-    // T1 = R S:                Y[j][k] = sum_l R[j][l] S[l][k]
-    double acc = 0.0;
+    for (int u = 0; u < 4; ++u) {
+        const int t = min((int)threadIdx.x + 1024 * u, m * m - 1);
+        const int j = t / m, k = t - j * m;
+        rv[u] = (float)r[(j + 1) * n + (k + 1)];
+    }
+    double F[16];
 #pragma unroll
-    for (int l = 0; l < 32; ++l) acc = fma(X[j * MG_DP + l], s[l], acc);
-    Y[j * MG_DP + k] = acc;
-    __syncthreads();
-    // T2 = S T1, scaled, stored transposed: X[j][k] = h^2 W[k][j] sum_l S[k][l] T1[l][j]
-    acc = 0.0;
+    for (int kb = 0; kb < 16; ++kb) F[kb] = tab[(kb * 4 + lk) * 64 + cb * 16 + li];
+    // the scale of the transposed tile (cb, rb): h^2 W[cb*16 + i][rb*16 + j]
+    const double h2 = (double)tk.h_top * (double)tk.h_top;
+    double wsc[4];
 #pragma unroll
-    for (int l = 0; l < 32; ++l) acc = fma(s[l], Y[l * MG_DP + j], acc);
-    X[j * MG_DP + k] = acc * w;
-    __syncthreads();
-    // T3 = T2 S:               Y[j][k] = sum_l T2[j][l] S[l][k] = sum_l X[l][j] S[l][k]
-    acc = 0.0;
+    for (int q = 0; q < 4; ++q) wsc[q] = tab[4096 + (cb * 16 + 4 * q + lk) * 64 + rb * 16 + li] * h2;
 #pragma unroll
-    for (int l = 0; l < 32; ++l) acc = fma(X[l * MG_DP + j], s[l], acc);
-    Y[j * MG_DP + k] = acc;
+    for (int u = 0; u < 4; ++u) {
+        const int t = threadIdx.x + 1024 * u;
+        const int j = t / m, k = t - j * m;
+        if (t < m * m) X[j * MG_DS + k] = (double)rv[u];
+    }
     __syncthreads();
-    // E = S T3, transposed:    E[k][j] = sum_l S[k][l] T3[l][j], node (j+1, k+1)
-    acc = 0.0;
+    mg_v4d acc;
+    // T1 = R S, tile (rb, cb) -> Y
+    if (live) {
+        acc = tile_product<true>(X, rb * 16, F, m, li, lk);
 #pragma unroll
-    for (int l = 0; l < 32; ++l) acc = fma(s[l], Y[l * MG_DP + j], acc);
-    if (j < m && k < m) e[(k + 1) * n + (j + 1)] = (T)acc;
+        for (int q = 0; q < 4; ++q) {
+            const int i = rb * 16 + 4 * q + lk, j = cb * 16 + li;
+            if (i < m && j < m) Y[i * MG_DS + j] = acc[q];
+        }
+    }
     __syncthreads();
+    // T2 = S T1, tile (cb, rb), times h^2 W -> X
+    if (live) {
+        acc = tile_product<false>(Y, rb * 16, F, m, li, lk);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = cb * 16 + 4 * q + lk, j = rb * 16 + li;
+            if (i < m && j < m) X[i * MG_DS + j] = acc[q] * wsc[q];
+        }
+    }
+    __syncthreads();
+    // T3 = T2 S, tile (rb, cb) -> Y
+    if (live) {
+        acc = tile_product<true>(X, rb * 16, F, m, li, lk);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = rb * 16 + 4 * q + lk, j = cb * 16 + li;
+            if (i < m && j < m) Y[i * MG_DS + j] = acc[q];
+        }
+    }
+    __syncthreads();
+    // E = S T3, tile (cb, rb): E[i][j] is node (j + 1, i + 1)
+    if (live) {
+        acc = tile_product<false>(Y, rb * 16, F, m, li, lk);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = cb * 16 + 4 * q + lk, j = rb * 16 + li;
+            if (i < m && j < m) e[(i + 1) * n + (j + 1)] = (T)acc[q];
+        }
+    }
+    // the boundary of the correction is zero
+    for (int t = threadIdx.x; t < 4 * (n - 1); t += 1024) {
+        const int side = t / (n - 1), p = t - side * (n - 1);
+        const int idx = side == 0 ? p : side == 1 ? p * n + (n - 1) : side == 2 ? (n - 1) * n + (n - 1 - p)
+                                                                          : (n - 1 - p) * n;
+        e[idx] = (T)0;
+    }
 }
 
 template <typename T, bool MASKED>
@@ -436,12 +515,19 @@ __global__ __launch_bounds__(1024) void mg_coarse_stage(const StageRef<MGCoarseT
     MGCoarseTask<T> tk;
     stage_case(a, tk);
     if (tk.gate && *G(&tk.gate->converged)) return;
+    extern __shared__ __align__(16) unsigned char smem_mg[];
+    if constexpr (sizeof(T) == 4) {
+        if (tk.dtab) {
+            double* X = reinterpret_cast<double*>(smem_mg);
+            coarse_direct_solve(tk, X, X + MG_DS * MG_DS);
+            return;
+        }
+    }
     const gptr<const T> e_in = G(tk.e_in);
     const gptr<const T> rhs_top = G(tk.rhs);
     const gptr<T> e_out = G(tk.e_out);
     const int n_top = tk.n_top, Lc = tk.Lc, csweeps = tk.csweeps;
     const T h_top = tk.h_top, omega = tk.omega;
-    extern __shared__ __align__(16) unsigned char smem_mg[];
     T* base = reinterpret_cast<T*>(smem_mg);
     // level l: n = s_n[l], e at base + s_off[l], rhs right after it; the
     // residual scratch follows the last level (s_off[Lc])
@@ -480,12 +566,8 @@ __global__ __launch_bounds__(1024) void mg_coarse_stage(const StageRef<MGCoarseT
     }
     const CoarseMask cm{(MASKED && gmask) ? s_mb : nullptr, n_top};
     __syncthreads();
-    // bottom of the recursion: the coarsest level, or the directly solved one
-    int Lb = Lc - 1;
-    if constexpr (sizeof(T) == 4)
-        if (tk.dtab) Lb = tk.Ld;
     T h = h_top;
-    for (int l = 0; l < Lb; ++l) {
+    for (int l = 0; l < Lc - 1; ++l) {
         const int nf = s_n[l], ncs = s_n[l + 1];
         lds_sweeps(E(l), R(l), nf, h, omega, (int)((tk.pre_bits >> (2 * l)) & 3u), cm, l);
         const T inv_h2 = (T)1 / (h * h);
@@ -527,17 +609,8 @@ __global__ __launch_bounds__(1024) void mg_coarse_stage(const StageRef<MGCoarseT
         __syncthreads();
         h = h * (T)2;
     }
-    if constexpr (sizeof(T) == 4) {
-        if (tk.dtab) {
-            double* X = reinterpret_cast<double*>(base + s_off[Lc]);
-            direct_sine_solve(E(Lb), R(Lb), s_n[Lb], h, G(tk.dtab), X, X + 32 * MG_DP);
-        } else {
-            lds_sweeps(E(Lb), R(Lb), s_n[Lb], h, omega, csweeps, cm, Lb);
-        }
-    } else {
-        lds_sweeps(E(Lb), R(Lb), s_n[Lb], h, omega, csweeps, cm, Lb);
-    }
-    for (int l = Lb - 1; l >= 0; --l) {
+    lds_sweeps(E(Lc - 1), R(Lc - 1), s_n[Lc - 1], h, omega, csweeps, cm, Lc - 1);
+    for (int l = Lc - 2; l >= 0; --l) {
         h = h / (T)2;
         const int nf = s_n[l], ncs = s_n[l + 1];
         const T* C = E(l + 1);
@@ -888,15 +961,16 @@ int MGSolver::last_cycles() const { return solo_ ? solo_->last_cycles() : 1; }
 // ---------------------------------------------------------------------------
 // MGBatch: one launch per stage for all member grids
 // ---------------------------------------------------------------------------
-// levels' e and rhs, then the residual scratch (with a direct solve also its
-// two FP64 product buffers)
+// the levels' e and rhs and the residual scratch, or the two FP64 buffers of
+// the direct solve
 template <typename T>
 static size_t coarse_lds_bytes(int n_top, int Lc, bool direct) {
+    if (direct) return MG_DLDS;
     size_t tot = 0;
     int n = n_top;
     for (int l = 0; l < Lc; ++l) { tot += 2 * (size_t)n * n; n = (n + 1) / 2; }
-    const size_t scratch = (size_t)n_top * n_top * sizeof(T);
-    return tot * sizeof(T) + std::max(scratch, direct ? (size_t)MG_DSCRATCH * 4 : (size_t)0);
+    tot += (size_t)n_top * n_top;
+    return tot * sizeof(T);
 }
 
 // LDS of a tiled stage: e and rhs over tile + halo, plus the residual patch
@@ -1045,7 +1119,6 @@ int MGBatch::build_cycle(const std::vector<double>& h, bool warm) {
             t.gate = s->d_norm_;
             t.mask = s->masked() ? s->mask_[lc] : nullptr;
             t.dtab = sizeof(T) == 4 ? dtab_[k] : nullptr;
-            t.Ld = dlev_[k];
             lds = std::max(lds, coarse_lds_bytes<T>(t.n_top, t.Lc, t.dtab != nullptr));
             ct.push_back(t);
         }
@@ -1148,33 +1221,27 @@ int MGBatch::build_cycle(const std::vector<double>& h, bool warm) {
 }
 
 // Direct coarse solve (mixed mode, members without electrodes whose coarse
-// kernel reaches a level of at most MG_ND nodes): that level's index within
-// the kernel and the sine-transform tables of its size, one set per size.
+// kernel level has 3..65 nodes): the sine-transform tables of that size, one
+// set per size (S then W, 64 x 64 each, zero outside m x m).
 int MGBatch::build_direct_tables() {
     const int M = (int)m_.size();
-    dlev_.assign(M, 0);
     dtab_.assign(M, nullptr);
     if (!mixed_ || !p_.coarse_direct) return 0;
     std::vector<int> sizes, want(M, 0);
     for (int k = 0; k < M; ++k) {
         const MGSolver* s = m_[k];
-        if (s->masked() || s->ns_[s->lc_] > s->nc_) continue;
-        for (int l = s->lc_; l < s->L_; ++l)
-            if (s->ns_[l] <= MG_ND) {
-                if (s->ns_[l] >= 3) {
-                    dlev_[k] = l - s->lc_;
-                    want[k] = s->ns_[l];
-                    if (std::find(sizes.begin(), sizes.end(), want[k]) == sizes.end()) sizes.push_back(want[k]);
-                }
-                break;
-            }
+        const int n = s->ns_[s->lc_];
+        if (s->masked() || n > MG_ND || n < 3) continue;
+        want[k] = n;
+        if (std::find(sizes.begin(), sizes.end(), n) == sizes.end()) sizes.push_back(n);
     }
     if (sizes.empty()) return 0;
-    std::vector<double> tab(sizes.size() * 2048, 0.0);
+    const size_t per = 2 * 64 * 64;
+    std::vector<double> tab(sizes.size() * per, 0.0);
     for (size_t q = 0; q < sizes.size(); ++q) {
         const int m = sizes[q] - 2;
-        double* S = &tab[q * 2048];
-        double* W = S + 1024;
+        double* S = &tab[q * per];
+        double* W = S + 64 * 64;
         const long double pi = 3.141592653589793238462643383279502884L;
         std::vector<long double> sig(m);
         for (int p = 0; p < m; ++p) {
@@ -1184,15 +1251,15 @@ int MGBatch::build_direct_tables() {
         const long double norm = (2.0L / (m + 1)) * (2.0L / (m + 1));
         for (int a = 0; a < m; ++a)
             for (int b = 0; b < m; ++b) {
-                S[a * 32 + b] = (double)std::sin(pi * (long double)((a + 1) * (b + 1)) / (long double)(m + 1));
-                W[a * 32 + b] = (double)(norm / (-4.0L * (sig[a] + sig[b])));
+                S[a * 64 + b] = (double)std::sin(pi * (long double)((a + 1) * (b + 1)) / (long double)(m + 1));
+                W[a * 64 + b] = (double)(norm / (-4.0L * (sig[a] + sig[b])));
             }
     }
     if (int rc = upload_vec((void**)&d_dtab_, tab)) return rc;
     for (int k = 0; k < M; ++k)
         if (want[k]) {
             const size_t q = std::find(sizes.begin(), sizes.end(), want[k]) - sizes.begin();
-            dtab_[k] = d_dtab_ + q * 2048;
+            dtab_[k] = d_dtab_ + q * per;
         }
     return 0;
 }

@@ -136,9 +136,9 @@ def _laplacian_dense(n, h):
     return A
 
 
-@pytest.mark.parametrize("n", [3, 4, 5, 9, 17, 33])
+@pytest.mark.parametrize("n", [3, 4, 5, 9, 17, 33, 49, 65])
 def test_mg_direct_coarse_solve_is_exact(pic, n):
-    """A grid of at most 33 nodes a side is solved by the coarse kernel's FP64
+    """A grid of at most 65 nodes a side is solved by the coarse kernel's FP64
     sine transform alone, so one FP32 V-cycle from zero is the solution of the
     discrete system up to FP32 rounding (against numpy's dense solve of the
     5-point Laplacian).  The same V-cycle with the reference's sweeps instead
