@@ -1392,6 +1392,7 @@ extern "C" int pic_sim_create(pic_particles* p, pic_grid* g, pic_walls* w, pic_x
         set_error("status buffers");
         return fail(PIC_ERR_HIP);
     }
+    S->mg->set_phi_save(S->phi_prev);
     std::memset(S->h_status, 0, PIC_MAX_SPECIES * sizeof(int));
     for (int k = 0; k < 4; ++k)
         if (hipEventCreate(&S->ev[k]) != hipSuccess) return fail(PIC_ERR_HIP);
@@ -1423,11 +1424,10 @@ static int enqueue_step_body(pic_sim* S, int flags, size_t lds, int cycles) {
     if (int rc = launch_push<T>((CaseDev<T>*)p->d_case, p->d_state, S->d_items, S->n_items, S->spill_chunk, flags, -1,
                                 lds, sm, S->d_walls_cell != nullptr))
         return rc;
-    if (int rc = launch_end_step<T>((CaseDev<T>*)p->d_case, p->d_state, 1, 1, sm)) return rc;
-    if (int rc = launch_status<T>((CaseDev<T>*)p->d_case, p->d_state, 1, S->d_status, sm)) return rc;
+    if (int rc = launch_end_step<T>((CaseDev<T>*)p->d_case, p->d_state, 1, 1, sm, S->d_status)) return rc;
     PIC_HIP(hipMemcpyAsync(S->h_status, S->d_status, PIC_MAX_SPECIES * sizeof(int), hipMemcpyDeviceToHost, sm));
     if (int rc = launch_charge(S->charge, nn, sm)) return rc;
-    PIC_HIP(hipMemcpyAsync(S->phi_prev, S->g->phi, nn * sizeof(double), hipMemcpyDeviceToDevice, sm));
+    // the solve's first residual pass also copies phi (its warm start) to phi_prev
     return S->mg->enqueue_fixed(cycles);
 }
 
@@ -1442,6 +1442,7 @@ int sim_set_mg(pic_sim* S, const MGParams& prm) {
     PIC_HIP(hipStreamSynchronize(lib_stream()));
     std::unique_ptr<MGSolver> mg(new MGSolver(S->g->m, prm, lib_stream()));
     if (int rc = mg->init()) return rc;
+    mg->set_phi_save(S->phi_prev);
     drop_graph(S);
     S->mg = std::move(mg);
     S->mgp = prm;
@@ -1566,13 +1567,11 @@ static int step_t(pic_sim* S, int n) {
                                     -1, lds, sm, S->d_walls_cell != nullptr))
             return rc;
         PIC_HIP(hipEventRecord(S->ev[1], sm));
-        if (int rc = launch_end_step<T>((CaseDev<T>*)p->d_case, p->d_state, 1, 1, sm)) return rc;
+        if (int rc = launch_end_step<T>((CaseDev<T>*)p->d_case, p->d_state, 1, 1, sm, S->d_status)) return rc;
         p->h_cur ^= 1;
-        if (int rc = launch_status<T>((CaseDev<T>*)p->d_case, p->d_state, 1, S->d_status, sm)) return rc;
         PIC_HIP(hipMemcpyAsync(S->h_status, S->d_status, PIC_MAX_SPECIES * sizeof(int), hipMemcpyDeviceToHost, sm));
         if (int rc = launch_charge(S->charge, nn, sm)) return rc;
-        PIC_HIP(hipMemcpyAsync(S->phi_prev, S->g->phi, nn * sizeof(double), hipMemcpyDeviceToDevice, sm));
-        S->have_prev = true;
+        S->have_prev = true;   // the solve's first residual pass copies phi to phi_prev
         MGStats st{};
         if (int rc = S->mg->solve(S->g->phi, S->g->rhs, S->g->dx, &st)) return rc;
         PIC_HIP(hipEventRecord(S->ev[2], sm));

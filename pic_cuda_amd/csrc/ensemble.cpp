@@ -213,8 +213,7 @@ static int enqueue_body(pic_ensemble* e, int cycles) {
     if (int rc = launch_push<T>((CaseDev<T>*)e->d_cases, e->d_states, e->d_items, e->n_items, e->m[0]->spill_chunk,
                                 -1, -1, e->lds, sm, walls))
         return rc;
-    if (int rc = launch_end_step<T>((CaseDev<T>*)e->d_cases, e->d_states, M, 1, sm)) return rc;
-    if (int rc = launch_status<T>((CaseDev<T>*)e->d_cases, e->d_states, M, e->d_status, sm)) return rc;
+    if (int rc = launch_end_step<T>((CaseDev<T>*)e->d_cases, e->d_states, M, 1, sm, e->d_status)) return rc;
     PIC_HIP(hipMemcpyAsync(e->h_status, e->d_status, M * PIC_MAX_SPECIES * sizeof(int), hipMemcpyDeviceToHost, sm));
     if (int rc = launch_charge_batch(e->d_charge, e->d_nns, e->d_chunks, e->n_chunks, sm)) return rc;
     return cycles > 0 ? e->mg->enqueue_fixed(cycles) : e->mg->solve();

@@ -100,7 +100,8 @@ struct MGFineTask {
     const double* rhs;
     float* r32;        // mixed mode: FP32 residual (the cycle's rhs)
     const float* e32;  // mixed mode: the cycle's correction
-    double* partial;   // per-tile sums of squares
+    double* partial;   // per-tile sums of r^2, then (first residual) of rhs^2
+    double* phi_save;  // first residual pass: copy of phi (the warm start), or null
     MGNorm* norm;
     MGNorm* mirror;    // contiguous copy of all norms of the batch
     int n;
@@ -140,6 +141,9 @@ public:
     int levels() const { return L_; }
     int last_launches() const { return launches_; }
     void set_stream(hipStream_t s) { stream_ = s; }
+    // n*n device buffer that receives phi as each solve starts (its warm
+    // start, written by the first residual pass), or null
+    void set_phi_save(double* dst);
     // sweeps on the coarsest level: a 4x4 coarsest level of an automatic ladder
     // (2x2 unknowns, Gauss-Seidel factor 1/4 per sweep) gets at least 32, which
     // solves it to rounding; a 3x3 level is solved exactly by one sweep
@@ -162,6 +166,7 @@ private:
     double* phi_tmp_ = nullptr;
     std::vector<uint8_t*> mask_;     // per level (injection), empty: no electrodes
     double* phi_bc_ = nullptr;
+    double* phi_save_ = nullptr;
     double* partials_ = nullptr;
     int n_partials_ = 0;
     MGNorm* d_norm_ = nullptr;

@@ -25,7 +25,9 @@
 //                     a 2D sine transform on the matrix cores.
 //   ir_residual_stage FP64 residual on 64x16 tiles, per-tile sums of r^2; in
 //                     mixed mode fused with the correction phi += e of the
-//                     previous FP32 cycle (phi alternates between two buffers).
+//                     previous FP32 cycle (phi alternates between two buffers);
+//                     the first one of a solve also sums rhs^2 and can copy
+//                     phi (the warm start) for the caller.
 //   norm_final_stage  fixed-order reduction of the tile sums, relative
 //                     residual and the convergence flag (build rule
 //                     ||r||/||rhs||; the reference divides by sum rhs^2,
@@ -648,6 +650,9 @@ __device__ inline double block_sum256(double v, double* sh) {
 //           phi_new = phi_src + e (every node value, neighbours included, is
 //           formed the same way), written to the other phi buffer (parity 0:
 //           phi -> phi_tmp, 1: phi_tmp -> phi); r stored FP32.
+// which == 1: the residual before the first cycle; it also writes the tile's
+// sum of rhs^2 (every node, the boundary included) after the r^2 sums and,
+// with phi_save, copies phi.
 // which == 2: a residual at the end of a cycle, a no-op once the case stopped.
 template <int MODE>
 __global__ __launch_bounds__(256) void ir_residual_stage(const StageRef<MGFineTask> a, int which, int parity) {
@@ -672,7 +677,8 @@ __global__ __launch_bounds__(256) void ir_residual_stage(const StageRef<MGFineTa
         if (MODE == 2) v = v + (double)e32[g];
         return v;
     };
-    double acc = 0.0;
+    double acc = 0.0, acc_rhs = 0.0;
+    const gptr<double> save = G(which == 1 ? tk.phi_save : nullptr);
     if (i < n) {
         double col[6], lf[4], rt[4], rh[4];
 #pragma unroll
@@ -702,22 +708,21 @@ __global__ __launch_bounds__(256) void ir_residual_stage(const StageRef<MGFineTa
             if (MODE == 2) dst[g] = c;
             if (MODE >= 1) r32[g] = (float)r;
             acc += r * r;
+            if (which == 1) {
+                const double f = rhs[g];
+                acc_rhs += f * f;
+                if (save) save[g] = c;
+            }
         }
     }
     const double s = block_sum256(acc, sh);
-    if (threadIdx.x == 0) G(tk.partial)[tile.ty * tk.gx + tile.tx] = s;
-}
-
-__global__ __launch_bounds__(256) void sumsq_stage(const StageRef<MGFineTask> a) {
-    MGTile tile;
-    MGFineTask tk;
-    stage_item(a, tk, tile);
-    __shared__ double sh[256];
-    const size_t nn = (size_t)tk.n * tk.n;
-    const size_t i = (size_t)tile.tx * 256 + threadIdx.x;
-    const double x = (i < nn) ? G(tk.rhs)[i] : 0.0;
-    const double s = block_sum256(x * x, sh);
-    if (threadIdx.x == 0) G(tk.partial)[tile.tx] = s;
+    const int tix = tile.ty * tk.gx + tile.tx;
+    if (threadIdx.x == 0) G(tk.partial)[tix] = s;
+    if (which == 1) {
+        __syncthreads();
+        const double s2 = block_sum256(acc_rhs, sh);
+        if (threadIdx.x == 0) G(tk.partial)[tk.nblocks + tix] = s2;
+    }
 }
 
 // End of a mixed-mode solve: after an odd number of executed cycles the
@@ -757,29 +762,35 @@ __global__ __launch_bounds__(256) void norm_final_stage(const StageRef<MGFineTas
     MGNorm* out = tk.norm;
     if (which == 2 && out->converged) return;
     __shared__ double sh[256];
-    const int np = which == 0 ? tk.nchunks : tk.nblocks;
+    const int np = tk.nblocks;
     double acc = 0.0;
     for (int k = threadIdx.x; k < np; k += 256) acc += G(tk.partial)[k];
     const double tot = block_sum256(acc, sh);
+    double tot_rhs = 0.0;
+    if (which == 1) {   // a new solve: the rhs norm from the same pass, counters reset
+        __syncthreads();
+        acc = 0.0;
+        for (int k = threadIdx.x; k < np; k += 256) acc += G(tk.partial)[np + k];
+        tot_rhs = block_sum256(acc, sh);
+    }
     if (threadIdx.x == 0) {
-        if (which == 0) {
-            out->rhs_sq = tot;
+        if (which == 1) {
+            out->rhs_sq = tot_rhs;
             out->converged = 0;
             out->cycles = 0;
             out->steps = 0;
             out->max_cycles = max_cycles;
-        } else {
-            out->res_sq = tot;
-            const double rn = sqrt(tot), fn = sqrt(out->rhs_sq);
-            const double rel = fn > 0.0 ? rn / fn : rn;
-            out->rel = rel;
-            if (which == 2) {
-                out->cycles += inner;   // V-cycles of the refinement step just done
-                out->steps += 1;
-            }
-            out->converged = (rel < tol || out->cycles >= out->max_cycles) ? 1 : 0;
-            out->hit_tol = rel < tol ? 1 : 0;
         }
+        out->res_sq = tot;
+        const double rn = sqrt(tot), fn = sqrt(out->rhs_sq);
+        const double rel = fn > 0.0 ? rn / fn : rn;
+        out->rel = rel;
+        if (which == 2) {
+            out->cycles += inner;   // V-cycles of the refinement step just done
+            out->steps += 1;
+        }
+        out->converged = (rel < tol || out->cycles >= out->max_cycles) ? 1 : 0;
+        out->hit_tol = rel < tol ? 1 : 0;
         *tk.mirror = *out;
     }
 }
@@ -903,7 +914,7 @@ int MGSolver::init() {
     const size_t nn = (size_t)n_ * n_;
     if (mixed) PIC_HIP(hipMalloc(&r32_, nn * sizeof(float)));
     PIC_HIP(hipMalloc(&phi_tmp_, nn * sizeof(double)));
-    n_partials_ = std::max(div_up(n_, MG_RX) * div_up(n_, MG_RY), (int)((nn + 255) / 256));
+    n_partials_ = 2 * div_up(n_, MG_RX) * div_up(n_, MG_RY);   // r^2 and rhs^2 per residual tile
     PIC_HIP(hipMalloc(&partials_, (size_t)n_partials_ * sizeof(double)));
     PIC_HIP(hipMalloc(&d_norm_, sizeof(MGNorm)));
     PIC_HIP(hipMemset(d_norm_, 0, sizeof(MGNorm)));
@@ -957,6 +968,14 @@ int MGSolver::complete_fixed(MGStats* st) {
 }
 
 int MGSolver::last_cycles() const { return solo_ ? solo_->last_cycles() : 1; }
+
+void MGSolver::set_phi_save(double* dst) {
+    if (dst != phi_save_) {   // the launch tables of the batch of one hold the pointer
+        delete solo_;
+        solo_ = nullptr;
+    }
+    phi_save_ = dst;
+}
 
 // ---------------------------------------------------------------------------
 // MGBatch: one launch per stage for all member grids
@@ -1306,6 +1325,7 @@ int MGBatch::build(const std::vector<MGSolver*>& members, const std::vector<doub
         f.r32 = s->r32_;
         f.e32 = mixed_ ? s->ef_[0][0] : nullptr;
         f.partial = s->partials_;
+        f.phi_save = s->phi_save_;
         f.norm = s->d_norm_;
         f.mirror = d_mirror_ + k;
         f.n = n;
@@ -1406,10 +1426,11 @@ int MGBatch::launch_stage(const Stage& st) {
     return 0;
 }
 
-// what: 0 rhs norm (and the Dirichlet values of electrodes into phi); 1
-// residual and its norm (which 1: before the first cycle, 2: after a cycle,
-// whose index has the given parity); 3 mixed mode: the solution back into phi
-// after an odd number of cycles
+// what: 0 the Dirichlet values of electrodes into phi (a no-op without
+// electrodes); 1 residual and its norm (which 1: before the first cycle, with
+// the rhs norm and the counters of a new solve; 2: after a cycle, whose index
+// has the given parity); 3 mixed mode: the solution back into phi after an odd
+// number of cycles
 int MGBatch::enqueue_fine(int what, int which, int parity) {
     const int M = (int)m_.size();
     const auto chunks = make_ref<MGFineTask>(&h_fine_[0], d_fine_, d_chunk_tiles_, M, 0);
@@ -1419,10 +1440,6 @@ int MGBatch::enqueue_fine(int what, int which, int parity) {
             hipLaunchKernelGGL(dirichlet_stage, dim3(n_chunk_), dim3(256), 0, stream_, chunks);
             ++launches_;
         }
-        hipLaunchKernelGGL(sumsq_stage, dim3(n_chunk_), dim3(256), 0, stream_, chunks);
-        hipLaunchKernelGGL(norm_final_stage, dim3(M), dim3(256), 0, stream_, cases, 0, p_.tol, max_outer_ * inner_,
-                           inner_);
-        launches_ += 2;
     } else if (what == 1) {
         const auto res = make_ref<MGFineTask>(&h_fine_[0], d_fine_, d_res_tiles_, M, h_fine_[0].gx);
         const dim3 g(n_res_), b(256);

@@ -899,25 +899,6 @@ __global__ __launch_bounds__(256) void gather_kernel(const CaseDev<T>* __restric
     }
 }
 
-// spill fill of every species of every case (read back by the host after the
-// Poisson solve to decide on re-binning)
-template <typename T>
-__global__ void status_kernel(const CaseDev<T>* __restrict__ cases, const CaseState* __restrict__ states,
-                              int n_cases, int* out) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n_cases) return;
-    const CaseDev<T>& C = cases[c];
-    const int cur = states[c].cur;
-    for (int s = 0; s < PIC_MAX_SPECIES; ++s) {
-        int v = 0;
-        if (s < C.ns) {
-            const SpeciesDev<T>& S = C.sp[s];
-            v = S.count[cur][S.nb[cur]];
-        }
-        out[c * PIC_MAX_SPECIES + s] = v;
-    }
-}
-
 // number of particles held by one species of one case (both parts of every
 // bin plus the spill region), for last_alive
 template <typename T>
@@ -945,10 +926,12 @@ __global__ __launch_bounds__(256) void count_kernel(const CaseDev<T>* __restrict
 }
 
 // one block per case: clear the counts of the buffer that becomes the next
-// output, flip the parity and advance the step counter
+// output, flip the parity and advance the step counter; with status_out, also
+// the spill fill of every species of the new state (read back by the host
+// after the Poisson solve to decide on re-binning)
 template <typename T>
 __global__ __launch_bounds__(256) void end_step_kernel(const CaseDev<T>* __restrict__ cases, CaseState* states,
-                                                       int advance_step) {
+                                                       int advance_step, int* __restrict__ status_out) {
     const int c = blockIdx.x;
     const CaseDev<T>& C = cases[c];
     const int in = states[c].cur;
@@ -965,6 +948,15 @@ __global__ __launch_bounds__(256) void end_step_kernel(const CaseDev<T>* __restr
     if (threadIdx.x == 0) {
         states[c].cur = in ^ 1;
         if (advance_step) states[c].step += 1;
+        if (status_out)
+            for (int s = 0; s < PIC_MAX_SPECIES; ++s) {
+                int v = 0;
+                if (s < C.ns) {
+                    const SpeciesDev<T>& S = C.sp[s];
+                    v = S.count[in ^ 1][S.nb[in ^ 1]];
+                }
+                status_out[c * PIC_MAX_SPECIES + s] = v;
+            }
     }
 }
 
@@ -1100,8 +1092,9 @@ int launch_gather(const CaseDev<T>* cases, const CaseState* states, const WorkIt
 }
 
 template <typename T>
-int launch_end_step(const CaseDev<T>* cases, CaseState* states, int n_cases, int advance, hipStream_t s) {
-    hipLaunchKernelGGL(end_step_kernel<T>, dim3(n_cases), dim3(256), 0, s, cases, states, advance);
+int launch_end_step(const CaseDev<T>* cases, CaseState* states, int n_cases, int advance, hipStream_t s,
+                    int* status_out) {
+    hipLaunchKernelGGL(end_step_kernel<T>, dim3(n_cases), dim3(256), 0, s, cases, states, advance, status_out);
     PIC_HIP(hipGetLastError());
     return 0;
 }
@@ -1113,12 +1106,6 @@ int launch_clear_out(const CaseDev<T>* cases, const CaseState* states, int n_cas
     return 0;
 }
 
-template <typename T>
-int launch_status(const CaseDev<T>* cases, const CaseState* states, int n_cases, int* out, hipStream_t s) {
-    hipLaunchKernelGGL(status_kernel<T>, dim3(div_up(n_cases, 64)), dim3(64), 0, s, cases, states, n_cases, out);
-    PIC_HIP(hipGetLastError());
-    return 0;
-}
 
 template <typename T>
 int launch_counts(const CaseDev<T>* cases, const CaseState* states, int c, int sp, long long* out, hipStream_t s) {
@@ -1130,7 +1117,6 @@ int launch_counts(const CaseDev<T>* cases, const CaseState* states, int c, int s
 #define PIC_LAUNCH_INST(T)                                                                                       \
     template int launch_push<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int, int, int, long long,  \
                                 size_t, hipStream_t, bool);                                                      \
-    template int launch_status<T>(const CaseDev<T>*, const CaseState*, int, int*, hipStream_t);                  \
     template int launch_counts<T>(const CaseDev<T>*, const CaseState*, int, int, long long*, hipStream_t);       \
     template int launch_init<T>(T*, T*, T*, T*, uint32_t*, int32_t*, int, int, int, double, uint32_t, uint32_t,  \
                                 uint32_t, uint32_t, hipStream_t);                                                \
@@ -1144,7 +1130,7 @@ int launch_counts(const CaseDev<T>* cases, const CaseState* states, int c, int s
                                   hipStream_t);                                                                  \
     template int launch_export<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int, int, int32_t*,       \
                                   float2*, int64_t, double, double, hipStream_t);                                \
-    template int launch_end_step<T>(const CaseDev<T>*, CaseState*, int, int, hipStream_t);                       \
+    template int launch_end_step<T>(const CaseDev<T>*, CaseState*, int, int, hipStream_t, int*);                 \
     template int launch_clear_out<T>(const CaseDev<T>*, const CaseState*, int, hipStream_t);
 PIC_LAUNCH_INST(float)
 PIC_LAUNCH_INST(double)

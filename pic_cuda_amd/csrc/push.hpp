@@ -10,8 +10,6 @@ template <typename T>
 int launch_push(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items,
                 int spill_chunk, int flags, long long mcc_key, size_t lds, hipStream_t s, bool walls);
 template <typename T>
-int launch_status(const CaseDev<T>* cases, const CaseState* states, int n_cases, int* out, hipStream_t s);
-template <typename T>
 int launch_counts(const CaseDev<T>* cases, const CaseState* states, int c, int sp, long long* out, hipStream_t s);
 template <typename T>
 int launch_init(T* x, T* y, T* vx, T* vy, uint32_t* id, int32_t* count_ptr, int n, int nx, int ny, double vth,
@@ -32,8 +30,12 @@ int launch_gather(const CaseDev<T>* cases, const CaseState* states, const WorkIt
 template <typename T>
 int launch_export(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items, int chunk,
                   int32_t* counter, float2* dst, int64_t cap, double sx, double sy, hipStream_t s);
+// end of step: clear the next output's counts, flip the parity, advance the
+// step; status_out (PIC_MAX_SPECIES ints per case, may be null) receives the
+// spill fill of every species
 template <typename T>
-int launch_end_step(const CaseDev<T>* cases, CaseState* states, int n_cases, int advance, hipStream_t s);
+int launch_end_step(const CaseDev<T>* cases, CaseState* states, int n_cases, int advance, hipStream_t s,
+                    int* status_out = nullptr);
 // clear the bin counts of the output parity (after a trial push)
 template <typename T>
 int launch_clear_out(const CaseDev<T>* cases, const CaseState* states, int n_cases, hipStream_t s);
