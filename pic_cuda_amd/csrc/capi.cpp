@@ -1500,8 +1500,17 @@ template <typename T>
 static int graph_step(pic_sim* S, int flags, size_t lds, MGStats* st) {
     hipStream_t sm = lib_stream();
     const int cycles = std::max(1, S->last_cycles);
+    // A graph with one refinement step (mixed mode: inner V-cycles; FP64: one
+    // V-cycle) more than the last solve needed is kept unless the lower count
+    // holds for PIC_GRAPH_SHRINK steps: the device flag turns cycles beyond
+    // convergence into no-ops, so a count that moves between two values does
+    // not re-capture every step, while a count that settled lower re-captures
+    // (gated no-op launches are not free).
+    const int step = S->mgp.inner_eff();
+    S->g_shrink = (S->gexec && cycles < S->g_cycles) ? S->g_shrink + 1 : 0;
+    const bool cycles_fit = S->g_cycles >= cycles && S->g_cycles <= cycles + step && S->g_shrink < PIC_GRAPH_SHRINK;
     if (S->gexec && (S->g_items != S->d_items || S->g_n_items != S->n_items || S->g_lds != lds ||
-                     S->g_flags != flags || S->g_cycles != cycles))
+                     S->g_flags != flags || !cycles_fit))
         drop_graph(S);
     if (!S->gexec) {
         if (int rc = S->mg->prepare(S->g->phi, S->g->rhs, S->g->dx)) return rc;   // allocates: not in a capture

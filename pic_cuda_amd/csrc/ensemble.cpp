@@ -46,6 +46,7 @@ struct pic_ensemble {
     hipGraphExec_t gexec = nullptr;
     int64_t gen = 0, g_gen = -1;   // layout generation (bumped by every rebuild)
     int g_cycles = 0;
+    int g_shrink = 0;   // consecutive graph steps that needed fewer cycles than captured
     int64_t graph_steps = 0, graph_captures = 0;
 };
 
@@ -247,7 +248,13 @@ static int ens_step_t(pic_ensemble* e) {
         // one graph launch: the captured body below with a solve of the last
         // step's cycle count, then completion of members that need more
         const int cycles = e->mg->last_cycles();
-        if (e->gexec && (e->g_gen != e->gen || e->g_cycles != cycles)) drop_graph(e);
+        // kept with up to one refinement step of gated no-op cycles to spare
+        // until a lower count holds for PIC_GRAPH_SHRINK steps (the rule of
+        // the single-simulation graph)
+        const int step = e->m.empty() ? 1 : e->m[0]->mgp.inner_eff();
+        e->g_shrink = (e->gexec && cycles < e->g_cycles) ? e->g_shrink + 1 : 0;
+        const bool cycles_fit = e->g_cycles >= cycles && e->g_cycles <= cycles + step && e->g_shrink < PIC_GRAPH_SHRINK;
+        if (e->gexec && (e->g_gen != e->gen || !cycles_fit)) drop_graph(e);
         if (!e->gexec) {
             PIC_HIP(hipStreamBeginCapture(sm, hipStreamCaptureModeThreadLocal));
             const int rc = enqueue_body<T>(e, cycles);

@@ -11,6 +11,10 @@
 
 struct pic_sim;
 
+// graph mode: consecutive steps that need fewer multigrid cycles than the
+// captured graph runs before the graph is captured again with the lower count
+#define PIC_GRAPH_SHRINK 8
+
 namespace pic {
 
 hipStream_t lib_stream();          // the library's stream (default priority)
@@ -141,6 +145,7 @@ struct pic_sim {
     hipGraphExec_t gexec = nullptr;
     const void* g_items = nullptr;
     int g_n_items = 0, g_flags = 0, g_cycles = 0;
+    int g_shrink = 0;   // consecutive graph steps that needed fewer cycles than captured
     size_t g_lds = 0;
     int64_t graph_steps = 0, graph_captures = 0;
 };
