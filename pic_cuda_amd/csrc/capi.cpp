@@ -1386,9 +1386,12 @@ extern "C" int pic_sim_create(pic_particles* p, pic_grid* g, pic_walls* w, pic_x
     if (rc) return fail(rc);
     S->mg.reset(new MGSolver(g->m, S->mgp, lib_stream()));
     if ((rc = S->mg->init())) return fail(rc);
+    // the spill status is written by end_step_kernel straight into pinned,
+    // coherent host memory (read after the step's synchronization)
     if (hipMalloc(&S->phi_prev, (size_t)g->m * g->n * sizeof(double)) != hipSuccess ||
-        hipMalloc(&S->d_status, PIC_MAX_SPECIES * sizeof(int)) != hipSuccess ||
-        hipHostMalloc(&S->h_status, PIC_MAX_SPECIES * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc(&S->h_status, PIC_MAX_SPECIES * sizeof(int), hipHostMallocMapped | hipHostMallocCoherent) !=
+            hipSuccess ||
+        hipHostGetDevicePointer((void**)&S->d_status, S->h_status, 0) != hipSuccess) {
         set_error("status buffers");
         return fail(PIC_ERR_HIP);
     }
@@ -1425,7 +1428,6 @@ static int enqueue_step_body(pic_sim* S, int flags, size_t lds, int cycles) {
                                 lds, sm, S->d_walls_cell != nullptr))
         return rc;
     if (int rc = launch_end_step<T>((CaseDev<T>*)p->d_case, p->d_state, 1, 1, sm, S->d_status)) return rc;
-    PIC_HIP(hipMemcpyAsync(S->h_status, S->d_status, PIC_MAX_SPECIES * sizeof(int), hipMemcpyDeviceToHost, sm));
     if (int rc = launch_charge(S->charge, nn, sm)) return rc;
     // the solve's first residual pass also copies phi (its warm start) to phi_prev
     return S->mg->enqueue_fixed(cycles);
@@ -1578,7 +1580,6 @@ static int step_t(pic_sim* S, int n) {
         PIC_HIP(hipEventRecord(S->ev[1], sm));
         if (int rc = launch_end_step<T>((CaseDev<T>*)p->d_case, p->d_state, 1, 1, sm, S->d_status)) return rc;
         p->h_cur ^= 1;
-        PIC_HIP(hipMemcpyAsync(S->h_status, S->d_status, PIC_MAX_SPECIES * sizeof(int), hipMemcpyDeviceToHost, sm));
         if (int rc = launch_charge(S->charge, nn, sm)) return rc;
         S->have_prev = true;   // the solve's first residual pass copies phi to phi_prev
         MGStats st{};
@@ -1780,7 +1781,6 @@ extern "C" int pic_sim_destroy(pic_sim* S) {
     (void)hipFree(S->d_wall_lists);
     (void)hipFree(S->d_frame);
     (void)hipFree(S->d_items);
-    (void)hipFree(S->d_status);
     (void)hipFree(S->phi_prev);
     if (S->h_status) (void)hipHostFree(S->h_status);
     for (auto& e : S->ev)

@@ -33,8 +33,8 @@ struct pic_ensemble {
     long long* d_nns = nullptr;
     int4* d_chunks = nullptr;
     int n_chunks = 0;
-    int* d_status = nullptr;
-    int* h_status = nullptr;
+    int* h_status = nullptr;   // pinned, coherent host memory
+    int* d_status = nullptr;   // its device address
     std::unique_ptr<MGBatch> mg;
     std::vector<int> flags;        // flags currently stored in each case state
     bool dirty = true;
@@ -63,7 +63,6 @@ static void free_arrays(pic_ensemble* e) {
     (void)hipFree(e->d_charge);
     (void)hipFree(e->d_nns);
     (void)hipFree(e->d_chunks);
-    (void)hipFree(e->d_status);
     if (e->h_status) (void)hipHostFree(e->h_status);
     e->d_cases = nullptr;
     e->d_states = nullptr;
@@ -140,8 +139,8 @@ static int rebuild(pic_ensemble* e) {
     PIC_HIP(hipMalloc(&e->d_chunks, chunks.size() * sizeof(int4)));
     PIC_HIP(hipMemcpy(e->d_chunks, chunks.data(), chunks.size() * sizeof(int4), hipMemcpyHostToDevice));
     e->n_chunks = (int)chunks.size();
-    PIC_HIP(hipMalloc(&e->d_status, M * PIC_MAX_SPECIES * sizeof(int)));
-    PIC_HIP(hipHostMalloc(&e->h_status, M * PIC_MAX_SPECIES * sizeof(int), hipHostMallocDefault));
+    PIC_HIP(hipHostMalloc(&e->h_status, M * PIC_MAX_SPECIES * sizeof(int), hipHostMallocMapped | hipHostMallocCoherent));
+    PIC_HIP(hipHostGetDevicePointer((void**)&e->d_status, e->h_status, 0));
     std::memset(e->h_status, 0, M * PIC_MAX_SPECIES * sizeof(int));
     e->mg.reset(new MGBatch(sm));
     if (int rc = e->mg->build(solvers, phis, rhss, hs)) return rc;
@@ -215,7 +214,6 @@ static int enqueue_body(pic_ensemble* e, int cycles) {
                                 -1, -1, e->lds, sm, walls))
         return rc;
     if (int rc = launch_end_step<T>((CaseDev<T>*)e->d_cases, e->d_states, M, 1, sm, e->d_status)) return rc;
-    PIC_HIP(hipMemcpyAsync(e->h_status, e->d_status, M * PIC_MAX_SPECIES * sizeof(int), hipMemcpyDeviceToHost, sm));
     if (int rc = launch_charge_batch(e->d_charge, e->d_nns, e->d_chunks, e->n_chunks, sm)) return rc;
     return cycles > 0 ? e->mg->enqueue_fixed(cycles) : e->mg->solve();
 }

@@ -103,7 +103,7 @@ struct MGFineTask {
     double* partial;   // per-tile sums of r^2, then (first residual) of rhs^2
     double* phi_save;  // first residual pass: copy of phi (the warm start), or null
     MGNorm* norm;
-    MGNorm* mirror;    // contiguous copy of all norms of the batch
+    MGNorm* mirror;    // this case's entry of the batch's norms in pinned host memory
     int n;
     int gx;            // residual tiles per row
     int nblocks;       // residual partials of this case
@@ -240,8 +240,8 @@ private:
     MGTile* d_chunk_tiles_ = nullptr;   // (case, chunk of 256 nodes)
     MGTile* d_res_tiles_ = nullptr;     // (case, 64x16 residual tile)
     int n_chunk_ = 0, n_res_ = 0;
-    MGNorm* d_mirror_ = nullptr;
-    MGNorm* h_mirror_ = nullptr;        // pinned
+    MGNorm* h_mirror_ = nullptr;        // the norms of all members: pinned, coherent host memory
+    MGNorm* d_mirror_ = nullptr;        // its device address
     int last_cycles_ = 1;
     bool any_mask_ = false;             // some member has electrodes
     // direct coarse solve: per member its tables (null: none)

@@ -1017,7 +1017,6 @@ void MGBatch::free_all() {
     if (d_fine_) (void)hipFree(d_fine_);
     if (d_chunk_tiles_) (void)hipFree(d_chunk_tiles_);
     if (d_res_tiles_) (void)hipFree(d_res_tiles_);
-    if (d_mirror_) (void)hipFree(d_mirror_);
     if (h_mirror_) (void)hipHostFree(h_mirror_);
     if (d_dtab_) (void)hipFree(d_dtab_);
     d_dtab_ = nullptr;
@@ -1307,10 +1306,11 @@ int MGBatch::build(const std::vector<MGSolver*>& members, const std::vector<doub
     inner_ = p_.inner_eff();
     max_outer_ = std::max(1, (p_.max_cycles + inner_ - 1) / inner_);
     const int M = (int)m_.size();
-    PIC_HIP(hipMalloc(&d_mirror_, M * sizeof(MGNorm)));
-    PIC_HIP(hipMemset(d_mirror_, 0, M * sizeof(MGNorm)));
-    PIC_HIP(hipHostMalloc(&h_mirror_, M * sizeof(MGNorm), hipHostMallocDefault));
+    // the norm kernel writes the norms straight into pinned, coherent host
+    // memory, read after each synchronization
+    PIC_HIP(hipHostMalloc(&h_mirror_, M * sizeof(MGNorm), hipHostMallocMapped | hipHostMallocCoherent));
     std::memset(h_mirror_, 0, M * sizeof(MGNorm));
+    PIC_HIP(hipHostGetDevicePointer((void**)&d_mirror_, h_mirror_, 0));
     std::vector<MGFineTask> fine(M);
     any_mask_ = false;
     std::vector<MGTile> chunks, res;
@@ -1488,7 +1488,6 @@ int MGBatch::solve() {
             enqueue_fine(1, 2, (enqueued + b) & 1);
         }
         enqueued += std::max(batch, 0);
-        PIC_HIP(hipMemcpyAsync(h_mirror_, d_mirror_, M * sizeof(MGNorm), hipMemcpyDeviceToHost, stream_));
         PIC_HIP(hipStreamSynchronize(stream_));
         bool all = true;
         for (int k = 0; k < M; ++k) all = all && h_mirror_[k].converged;
@@ -1506,7 +1505,6 @@ int MGBatch::solve() {
 int MGBatch::enqueue_fixed(int cycles) {
     PIC_REQUIRE(!m_.empty(), "multigrid batch: not built");
     launches_ = 0;
-    const int M = (int)m_.size();
     // cycles: V-cycles, run as whole refinement steps
     const int K = std::max(1, std::min((cycles + inner_ - 1) / inner_, max_outer_));
     enqueue_fine(0, 0, 0);
@@ -1516,7 +1514,6 @@ int MGBatch::enqueue_fixed(int cycles) {
         enqueue_fine(1, 2, b & 1);
     }
     if (mixed_) enqueue_fine(3, 0, 0);
-    PIC_HIP(hipMemcpyAsync(h_mirror_, d_mirror_, M * sizeof(MGNorm), hipMemcpyDeviceToHost, stream_));
     fixed_cycles_ = K;
     return 0;
 }
@@ -1534,7 +1531,6 @@ int MGBatch::complete_fixed() {
         if (int rc = launch_refinement()) return rc;
         enqueue_fine(1, 2, enqueued & 1);
         ++enqueued;
-        PIC_HIP(hipMemcpyAsync(h_mirror_, d_mirror_, M * sizeof(MGNorm), hipMemcpyDeviceToHost, stream_));
         PIC_HIP(hipStreamSynchronize(stream_));
     }
     if (enqueued > fixed_cycles_ && mixed_) enqueue_fine(3, 0, 0);

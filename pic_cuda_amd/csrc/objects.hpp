@@ -130,8 +130,8 @@ struct pic_sim {
     double last_rel = 0;
     int rebins = 0;
     double ms_push = 0, ms_field = 0, ms_total = 0, ms_graph = 0;
-    int* d_status = nullptr;
-    int* h_status = nullptr;
+    int* h_status = nullptr;   // pinned, coherent host memory
+    int* d_status = nullptr;   // its device address
     pic::ChargeArgs charge{};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     struct pic_ensemble* ens = nullptr;   // batch this simulation belongs to
