@@ -199,7 +199,7 @@ public:
 
 private:
     struct Stage {
-        int kind = 0;      // 0 smooth, 1 prolong, 2 coarse, 3 copy
+        int kind = 0;      // 0 smooth, 1 prolong, 2 coarse, 3 copy, 4 prolong + next cycle's smooth
         int nu = 0;        // sweeps of a tiled stage
         bool restr = false;   // smooth stage also restricts the residual
         bool masked = false;  // some task has electrodes
@@ -213,7 +213,7 @@ private:
         alignas(16) unsigned char t0[128];   // single task: passed by value
     };
     template <typename T>
-    int build_cycle(const std::vector<double>& h, bool warm);
+    int build_step(const std::vector<double>& h);
     int build_direct_tables();
     template <typename T>
     int add_level_stage(int kind, const std::vector<MGLevelTask<T>>& tasks, int nu, bool restr,
@@ -229,10 +229,10 @@ private:
     std::vector<const double*> rhs_;
     MGParams p_{};
     bool mixed_ = false;
-    std::vector<Stage> cycle_;          // stages of one V-cycle
-    // mixed mode with inner > 1: the V-cycle that starts from the correction
-    // so far (the later V-cycles of a refinement step)
-    std::vector<Stage> warm_;
+    std::vector<Stage> cycle_;          // stages of one refinement step (inner_ V-cycles)
+    // per member: the level-0 buffer (ef_[x][0]) holding the correction at the
+    // end of a refinement step
+    std::vector<int> final_e0_;
     int inner_ = 1;                     // V-cycles per refinement step
     int max_outer_ = 1;                 // refinement steps allowed (max_cycles / inner, rounded up)
     MGFineTask* d_fine_ = nullptr;      // per-case refinement tasks

@@ -132,6 +132,50 @@ __device__ inline void tile_colour(T* e, const T* r, const uint8_t* mk, int gi0,
     }
 }
 
+// Residual on the tile + 1 ring (into the (MG_TILE+2)^2 patch res) and its
+// full-weighting restriction to the coarse nodes (2ic, 2jc) inside the tile;
+// e and r are the W-wide LDS tile whose element (0,0) is node (gi0, gj0).
+// Fixed (electrode) nodes have zero residual.
+template <typename T, int W>
+__device__ inline void restrict_tile(const T* e, const T* r, T* res, const uint8_t* mk, int i0, int j0, int gi0,
+                                     int gj0, int n, T inv_h2, const gptr<T> rhs_c) {
+    const int tid = threadIdx.x;
+    constexpr int RW = MG_TILE + 2;
+    for (int k = tid; k < RW * RW; k += 256) {
+        const int q = k / RW;
+        const int gi = i0 - 1 + (k - q * RW), gj = j0 - 1 + q;
+        T v = (T)0;
+        const int idx = (gj - gj0) * W + (gi - gi0);
+        if (gi > 0 && gi < n - 1 && gj > 0 && gj < n - 1 && !(mk && mk[idx])) {
+            const T c = e[idx];
+            const T lap = (e[idx - W] + e[idx + W] + e[idx - 1] + e[idx + 1] - (T)4 * c) * inv_h2;
+            v = r[idx] - lap;
+        }
+        res[k] = v;
+    }
+    __syncthreads();
+    const int nc = (n + 1) / 2;
+    constexpr int TC = MG_TILE / 2;
+    const int ic = i0 / 2 + tid % TC, jc = j0 / 2 + tid / TC;   // TC*TC == 256
+    if (ic >= nc || jc >= nc) return;
+    const int i = 2 * ic, j = 2 * jc;
+    if (i >= n || j >= n) {
+        rhs_c[(size_t)jc * nc + ic] = (T)0;
+        return;
+    }
+    const int f = (j - j0 + 1) * RW + (i - i0 + 1);
+    T s = (T)4 * res[f];
+    if (i > 0) s += (T)2 * res[f - 1];
+    if (i < n - 1) s += (T)2 * res[f + 1];
+    if (j > 0) s += (T)2 * res[f - RW];
+    if (j < n - 1) s += (T)2 * res[f + RW];
+    if (i > 0 && j > 0) s += res[f - RW - 1];
+    if (i < n - 1 && j > 0) s += res[f - RW + 1];
+    if (i > 0 && j < n - 1) s += res[f + RW - 1];
+    if (i < n - 1 && j < n - 1) s += res[f + RW + 1];
+    rhs_c[(size_t)jc * nc + ic] = s * (T)0.0625;
+}
+
 // Down-leg stage: load tile + halo, NU sweeps, store the tile; with RESTRICT
 // also the residual on tile + 1 ring and the restriction to the coarse nodes
 // (2ic, 2jc) inside the tile.  Without RESTRICT (tiled sweeps of a large
@@ -205,42 +249,7 @@ __global__ __launch_bounds__(256) void mg_smooth_stage(const StageRef<MGLevelTas
         if (gi < n && gj < n) e_out[(size_t)gj * n + gi] = e[(gj - gj0) * W + (gi - gi0)];
     }
     if constexpr (RESTRICT) {
-        if (!rhs_c) return;
-        constexpr int RW = MG_TILE + 2;
-        T* res = r + WW;
-        for (int k = tid; k < RW * RW; k += 256) {
-            const int q = k / RW;
-            const int gi = i0 - 1 + (k - q * RW), gj = j0 - 1 + q;
-            T v = (T)0;
-            const int idx = (gj - gj0) * W + (gi - gi0);
-            if (gi > 0 && gi < n - 1 && gj > 0 && gj < n - 1 && !(mk && mk[idx])) {
-                const T c = e[idx];
-                const T lap = (e[idx - W] + e[idx + W] + e[idx - 1] + e[idx + 1] - (T)4 * c) * inv_h2;
-                v = r[idx] - lap;
-            }
-            res[k] = v;
-        }
-        __syncthreads();
-        const int nc = (n + 1) / 2;
-        constexpr int TC = MG_TILE / 2;
-        const int ic = i0 / 2 + tid % TC, jc = j0 / 2 + tid / TC;   // TC*TC == 256
-        if (ic >= nc || jc >= nc) return;
-        const int i = 2 * ic, j = 2 * jc;
-        if (i >= n || j >= n) {
-            rhs_c[(size_t)jc * nc + ic] = (T)0;
-            return;
-        }
-        const int f = (j - j0 + 1) * RW + (i - i0 + 1);
-        T s = (T)4 * res[f];
-        if (i > 0) s += (T)2 * res[f - 1];
-        if (i < n - 1) s += (T)2 * res[f + 1];
-        if (j > 0) s += (T)2 * res[f - RW];
-        if (j < n - 1) s += (T)2 * res[f + RW];
-        if (i > 0 && j > 0) s += res[f - RW - 1];
-        if (i < n - 1 && j > 0) s += res[f - RW + 1];
-        if (i > 0 && j < n - 1) s += res[f + RW - 1];
-        if (i < n - 1 && j < n - 1) s += res[f + RW + 1];
-        rhs_c[(size_t)jc * nc + ic] = s * (T)0.0625;
+        if (rhs_c) restrict_tile<T, W>(e, r, r + WW, mk, i0, j0, gi0, gj0, n, inv_h2, rhs_c);
     }
 }
 
@@ -257,10 +266,14 @@ __device__ inline T prolong_lds(const T* C, int CW, int ic0, int jc0, int nc, in
     return (ic + 1 < nc && jc + 1 < nc) ? (T)0.25 * (C[c] + C[c + 1] + C[c + CW] + C[c + CW + 1]) : C[c];
 }
 
-// Up-leg stage: e = e_in + P(coarse) on tile + halo 2*NU, NU sweeps, store the tile.
-template <typename T, int NU, bool MASKED>
+// Up-leg stage: e = e_in + P(coarse) on tile + halo 2*NU, NU sweeps, store the
+// tile.  With RESTRICT (the finest level between two V-cycles of one
+// refinement step) NU is the post-sweeps of this cycle plus the pre-sweeps of
+// the next, the halo is 2*NU+2, and the residual is restricted into rhs_c as
+// the next cycle's down leg would: the same arithmetic in one launch fewer.
+template <typename T, int NU, bool RESTRICT, bool MASKED>
 __global__ __launch_bounds__(256) void mg_prolong_stage(const StageRef<MGLevelTask<T>> a) {
-    constexpr int H = 2 * NU;
+    constexpr int H = RESTRICT ? 2 * NU + 2 : 2 * NU;
     constexpr int W = MG_TILE + 2 * H;
     constexpr int WW = W * W;
     constexpr int NL = (WW + 255) / 256;
@@ -275,14 +288,17 @@ __global__ __launch_bounds__(256) void mg_prolong_stage(const StageRef<MGLevelTa
     const gptr<const T> coarse = G(tk.coarse);
     const gptr<const T> rhs = G(tk.rhs);
     const gptr<T> e_out = G(tk.e_out);
+    const gptr<T> rhs_c = G(tk.rhs_c);
     const int n = tk.n;
-    const T h2 = tk.h2, omega = tk.omega;
+    const T h2 = tk.h2, inv_h2 = tk.inv_h2, omega = tk.omega;
     extern __shared__ __align__(16) unsigned char smem_mg[];
     T* e = reinterpret_cast<T*>(smem_mg);
     T* r = e + WW;
     T* C = r + WW;
+    T* res = C + CC;   // RESTRICT: the residual patch
     const gptr<const uint8_t> gmask = MASKED ? G(tk.mask) : nullptr;
-    uint8_t* mk = (MASKED && gmask) ? reinterpret_cast<uint8_t*>(C + CC) : nullptr;
+    uint8_t* mk = (MASKED && gmask) ? reinterpret_cast<uint8_t*>(res + (RESTRICT ? (MG_TILE + 2) * (MG_TILE + 2) : 0))
+                                    : nullptr;
     const int tid = threadIdx.x;
     const int i0 = tile.tx * MG_TILE, j0 = tile.ty * MG_TILE;
     const int gi0 = i0 - H, gj0 = j0 - H;   // even: H and i0 are
@@ -346,6 +362,9 @@ __global__ __launch_bounds__(256) void mg_prolong_stage(const StageRef<MGLevelTa
         const int k = tid + 256 * q;
         const int gi = i0 + k % MG_TILE, gj = j0 + k / MG_TILE;
         if (gi < n && gj < n) e_out[(size_t)gj * n + gi] = e[(gj - gj0) * W + (gi - gi0)];
+    }
+    if constexpr (RESTRICT) {
+        if (rhs_c) restrict_tile<T, W>(e, r, res, mk, i0, j0, gi0, gj0, n, inv_h2, rhs_c);
     }
 }
 
@@ -993,13 +1012,15 @@ static size_t coarse_lds_bytes(int n_top, int Lc, bool direct) {
 }
 
 // LDS of a tiled stage: e and rhs over tile + halo, plus the residual patch
-// (smooth with restriction) or the coarse patch (prolongation)
+// (smooth with restriction) or the coarse patch (prolongation), or both (the
+// fused up/down stage, kind 4)
 static size_t tiled_lds(int kind, int nu, bool restr, size_t es, bool masked) {
-    const size_t H = (kind == 0 && restr) ? 2 * nu + 2 : 2 * nu;
+    const bool res = (kind == 0 && restr) || kind == 4;
+    const size_t H = res ? 2 * nu + 2 : 2 * nu;
     const size_t W = MG_TILE + 2 * H;
     size_t tot = 2 * W * W;
-    if (kind == 0 && restr) tot += (MG_TILE + 2) * (MG_TILE + 2);
-    if (kind == 1) tot += (W / 2 + 2) * (W / 2 + 2);
+    if (res) tot += (MG_TILE + 2) * (MG_TILE + 2);
+    if (kind == 1 || kind == 4) tot += (W / 2 + 2) * (W / 2 + 2);
     return tot * es + (masked ? W * W : 0);   // + the electrode mask tile (bytes)
 }
 
@@ -1007,13 +1028,11 @@ MGBatch::~MGBatch() { free_all(); }
 
 void MGBatch::free_all() {
     (void)hipStreamSynchronize(stream_);
-    for (auto* list : {&cycle_, &warm_})
-        for (auto& st : *list) {
-            if (st.tasks) (void)hipFree(st.tasks);
-            if (st.tiles) (void)hipFree(st.tiles);
-        }
+    for (auto& st : cycle_) {
+        if (st.tasks) (void)hipFree(st.tasks);
+        if (st.tiles) (void)hipFree(st.tiles);
+    }
     cycle_.clear();
-    warm_.clear();
     if (d_fine_) (void)hipFree(d_fine_);
     if (d_chunk_tiles_) (void)hipFree(d_chunk_tiles_);
     if (d_res_tiles_) (void)hipFree(d_res_tiles_);
@@ -1067,174 +1086,190 @@ int MGBatch::add_level_stage(int kind, const std::vector<MGLevelTask<T>>& tasks,
     return 0;
 }
 
-// warm: the V-cycle of mixed mode that starts from the correction of the
-// previous V-cycle instead of zero (the later V-cycles of a refinement step)
+// The stages of one refinement step (FP64 mode: one V-cycle): inner_ V-cycles,
+// the later ones starting from the correction so far.  Between two V-cycles
+// the finest up-leg stage of a member with tiled levels also runs the next
+// cycle's pre-sweeps and restriction (stage kind 4), so that cycle's down leg
+// starts one level lower; its level-0 correction then alternates between the
+// two level-0 buffers (final_e0 tells where it ends).
 template <typename T>
-int MGBatch::build_cycle(const std::vector<double>& h, bool warm) {
-    std::vector<Stage>* dst = warm ? &warm_ : &cycle_;
+int MGBatch::build_step(const std::vector<double>& h) {
+    std::vector<Stage>* dst = &cycle_;
     const int M = (int)m_.size();
     const T omega = (T)p_.omega;
-    // per member: level-0 input, output, scratch and rhs
     auto E = [&](MGSolver* s, int b, int l) -> T* { return (T*)(mixed_ ? (void*)s->ef_[b][l] : (void*)s->ed_[b][l]); };
     auto R = [&](MGSolver* s, int l) -> T* { return (T*)(mixed_ ? (void*)s->rf_[l] : (void*)s->rd_[l]); };
     auto out0 = [&](int k) -> T* { return mixed_ ? (T*)(void*)m_[k]->ef_[0][0] : (T*)(void*)phi_[k]; };
-    auto in0 = [&](int k) -> const T* {
-        return mixed_ ? (warm ? (const T*)out0(k) : nullptr) : (const T*)(void*)phi_[k];
-    };
     auto tmp0 = [&](int k) -> T* { return mixed_ ? (T*)(void*)m_[k]->ef_[1][0] : (T*)(void*)m_[k]->phi_tmp_; };
     auto rhs0 = [&](int k) -> const T* { return mixed_ ? (const T*)(void*)m_[k]->r32_ : (const T*)(void*)rhs_[k]; };
     auto hl = [&](int k, int l) { return h[k] * std::ldexp(1.0, l); };
     int D = 0;
     for (auto* s : m_) D = std::max(D, s->lc_);
-    // down leg
-    for (int d = 0; d < D; ++d) {
-        std::vector<MGLevelTask<T>> tasks;
-        for (int k = 0; k < M; ++k) {
-            MGSolver* s = m_[k];
-            if (s->lc_ <= d) continue;
-            const double hh = hl(k, d);
-            MGLevelTask<T> t{};
-            t.e_in = d == 0 ? in0(k) : nullptr;
-            t.rhs = d == 0 ? rhs0(k) : R(s, d);
-            t.e_out = d == 0 ? tmp0(k) : E(s, 1, d);
-            t.rhs_c = R(s, d + 1);
-            t.n = s->ns_[d];
-            t.nu = p_.pre_at(d);
-            t.H = 2 * t.nu + 2;
-            t.h2 = (T)(hh * hh);
-            t.inv_h2 = (T)(1.0 / (hh * hh));
-            t.omega = omega;
-            t.gate = s->d_norm_;
-            t.mask = s->masked() ? s->mask_[d] : nullptr;
-            tasks.push_back(t);
-        }
-        if (int rc = add_level_stage<T>(0, tasks, p_.pre_at(d), true, dst)) return rc;
+    auto level_task = [&](int k, int d) {
+        MGSolver* s = m_[k];
+        const double hh = hl(k, d);
+        MGLevelTask<T> t{};
+        t.rhs = d == 0 ? rhs0(k) : R(s, d);
+        t.n = s->ns_[d];
+        t.h2 = (T)(hh * hh);
+        t.inv_h2 = (T)(1.0 / (hh * hh));
+        t.omega = omega;
+        t.gate = s->d_norm_;
+        t.mask = s->masked() ? s->mask_[d] : nullptr;
+        return t;
+    };
+    // level-0 buffers of members with tiled levels: the pre-smoothed
+    // correction (input of the finest up stage) and the up stage's output
+    std::vector<T*> pre0(M), post0(M);
+    for (int k = 0; k < M; ++k) {
+        pre0[k] = tmp0(k);
+        post0[k] = out0(k);
     }
-    // coarse part: the single-workgroup cycle, or tiled sweeps on a large coarsest level
-    {
-        std::vector<MGCoarseTask<T>> ct;
-        size_t lds = 0;
-        for (int k = 0; k < M; ++k) {
-            MGSolver* s = m_[k];
-            const int lc = s->lc_;
-            if (s->ns_[lc] > s->nc_) continue;
-            MGCoarseTask<T> t{};
-            t.e_in = lc == 0 ? in0(k) : nullptr;
-            t.rhs = lc == 0 ? rhs0(k) : R(s, lc);
-            t.e_out = lc == 0 ? out0(k) : E(s, 0, lc);
-            t.n_top = s->ns_[lc];
-            t.Lc = s->L_ - lc;
-            t.pre_bits = t.post_bits = 0u;
-            for (int l = 0; l < 16; ++l) {
-                const unsigned a = (unsigned)(lc + l < 16 ? p_.pre_at(lc + l) : p_.pre);
-                const unsigned b = (unsigned)(lc + l < 16 ? p_.post_at(lc + l) : p_.post);
-                t.pre_bits |= (a & 3u) << (2 * l);
-                t.post_bits |= (b & 3u) << (2 * l);
+    final_e0_.assign(M, 0);
+    for (int c = 0; c < inner_; ++c) {
+        const bool first = c == 0, last = c == inner_ - 1;
+        // the level-0 input of a cycle: FP64 mode phi; mixed mode zero, or the
+        // correction so far in the later cycles
+        auto in0 = [&](int k) -> const T* {
+            return mixed_ ? (first ? nullptr : (const T*)out0(k)) : (const T*)(void*)phi_[k];
+        };
+        // down leg (after the first cycle, level 0 was done by the fused stage)
+        for (int d = first ? 0 : 1; d < D; ++d) {
+            std::vector<MGLevelTask<T>> tasks;
+            for (int k = 0; k < M; ++k) {
+                MGSolver* s = m_[k];
+                if (s->lc_ <= d) continue;
+                MGLevelTask<T> t = level_task(k, d);
+                t.e_in = d == 0 ? in0(k) : nullptr;
+                t.e_out = d == 0 ? pre0[k] : E(s, 1, d);
+                t.rhs_c = R(s, d + 1);
+                t.nu = p_.pre_at(d);
+                t.H = 2 * t.nu + 2;
+                tasks.push_back(t);
             }
-            t.csweeps = s->coarse_sweeps_eff();
-            t.h_top = (T)hl(k, lc);
-            t.omega = omega;
-            t.gate = s->d_norm_;
-            t.mask = s->masked() ? s->mask_[lc] : nullptr;
-            t.dtab = sizeof(T) == 4 ? dtab_[k] : nullptr;
-            lds = std::max(lds, coarse_lds_bytes<T>(t.n_top, t.Lc, t.dtab != nullptr));
-            ct.push_back(t);
+            if (int rc = add_level_stage<T>(0, tasks, p_.pre_at(d), true, dst)) return rc;
         }
-        if (!ct.empty()) {
-            Stage st;
-            st.kind = 2;
-            st.fp64 = sizeof(T) == 8;
-            st.lds = lds;
-            st.n_items = (int)ct.size();
-            st.n_tasks = (int)ct.size();
-            for (const auto& t : ct) st.masked = st.masked || t.mask != nullptr;
-            static_assert(sizeof(MGCoarseTask<T>) <= sizeof(st.t0), "task does not fit the by-value slot");
-            std::memcpy(st.t0, &ct[0], sizeof(MGCoarseTask<T>));
-            if (int rc = upload_vec(&st.tasks, ct)) return rc;
-            dst->push_back(st);
-        }
-        // large coarsest levels: chunks of up to 3 sweeps, ping-pong, then a copy
-        std::vector<int> big;
-        for (int k = 0; k < M; ++k)
-            if (m_[k]->ns_[m_[k]->lc_] > m_[k]->nc_) big.push_back(k);
-        if (!big.empty()) {
-            std::vector<const T*> src(M, nullptr);
-            std::vector<int> which(M, 0);
-            std::vector<T*> b0(M), b1(M);
-            for (int k : big) {
+        // coarse part: the single-workgroup cycle, or tiled sweeps on a large coarsest level
+        {
+            std::vector<MGCoarseTask<T>> ct;
+            size_t lds = 0;
+            for (int k = 0; k < M; ++k) {
                 MGSolver* s = m_[k];
                 const int lc = s->lc_;
-                b0[k] = lc == 0 ? out0(k) : E(s, 0, lc);
-                b1[k] = lc == 0 ? tmp0(k) : E(s, 1, lc);
-                src[k] = lc == 0 ? in0(k) : nullptr;
-                which[k] = (src[k] == b0[k]) ? 1 : 0;   // never write the buffer a launch reads
+                if (s->ns_[lc] > s->nc_) continue;
+                MGCoarseTask<T> t{};
+                t.e_in = lc == 0 ? in0(k) : nullptr;
+                t.rhs = lc == 0 ? rhs0(k) : R(s, lc);
+                t.e_out = lc == 0 ? out0(k) : E(s, 0, lc);
+                t.n_top = s->ns_[lc];
+                t.Lc = s->L_ - lc;
+                t.pre_bits = t.post_bits = 0u;
+                for (int l = 0; l < 16; ++l) {
+                    const unsigned a = (unsigned)(lc + l < 16 ? p_.pre_at(lc + l) : p_.pre);
+                    const unsigned b = (unsigned)(lc + l < 16 ? p_.post_at(lc + l) : p_.post);
+                    t.pre_bits |= (a & 3u) << (2 * l);
+                    t.post_bits |= (b & 3u) << (2 * l);
+                }
+                t.csweeps = s->coarse_sweeps_eff();
+                t.h_top = (T)hl(k, lc);
+                t.omega = omega;
+                t.gate = s->d_norm_;
+                t.mask = s->masked() ? s->mask_[lc] : nullptr;
+                t.dtab = sizeof(T) == 4 ? dtab_[k] : nullptr;
+                lds = std::max(lds, coarse_lds_bytes<T>(t.n_top, t.Lc, t.dtab != nullptr));
+                ct.push_back(t);
             }
-            int left = p_.coarse_sweeps;
-            bool wrote = false;
-            while (left > 0 || !wrote) {
-                const int nu = std::min(left, 3);
-                std::vector<MGLevelTask<T>> tasks;
+            if (!ct.empty()) {
+                Stage st;
+                st.kind = 2;
+                st.fp64 = sizeof(T) == 8;
+                st.lds = lds;
+                st.n_items = (int)ct.size();
+                st.n_tasks = (int)ct.size();
+                for (const auto& t : ct) st.masked = st.masked || t.mask != nullptr;
+                static_assert(sizeof(MGCoarseTask<T>) <= sizeof(st.t0), "task does not fit the by-value slot");
+                std::memcpy(st.t0, &ct[0], sizeof(MGCoarseTask<T>));
+                if (int rc = upload_vec(&st.tasks, ct)) return rc;
+                dst->push_back(st);
+            }
+            // large coarsest levels: chunks of up to 3 sweeps, ping-pong, then a copy
+            std::vector<int> big;
+            for (int k = 0; k < M; ++k)
+                if (m_[k]->ns_[m_[k]->lc_] > m_[k]->nc_) big.push_back(k);
+            if (!big.empty()) {
+                std::vector<const T*> src(M, nullptr);
+                std::vector<int> which(M, 0);
+                std::vector<T*> b0(M), b1(M);
                 for (int k : big) {
                     MGSolver* s = m_[k];
                     const int lc = s->lc_;
-                    const double hh = hl(k, lc);
+                    b0[k] = lc == 0 ? out0(k) : E(s, 0, lc);
+                    b1[k] = lc == 0 ? tmp0(k) : E(s, 1, lc);
+                    src[k] = lc == 0 ? in0(k) : nullptr;
+                    which[k] = (src[k] == b0[k]) ? 1 : 0;   // never write the buffer a launch reads
+                }
+                int left = p_.coarse_sweeps;
+                bool wrote = false;
+                while (left > 0 || !wrote) {
+                    const int nu = std::min(left, 3);
+                    std::vector<MGLevelTask<T>> tasks;
+                    for (int k : big) {
+                        MGSolver* s = m_[k];
+                        const int lc = s->lc_;
+                        MGLevelTask<T> t = level_task(k, lc);
+                        t.e_in = src[k];
+                        t.e_out = which[k] ? b1[k] : b0[k];
+                        t.nu = nu;
+                        t.H = 2 * nu;
+                        tasks.push_back(t);
+                        src[k] = t.e_out;
+                        which[k] ^= 1;
+                    }
+                    if (int rc = add_level_stage<T>(0, tasks, nu, false, dst)) return rc;
+                    left -= nu;
+                    wrote = true;
+                }
+                std::vector<MGLevelTask<T>> copies;
+                for (int k : big) {
+                    if (src[k] == b0[k]) continue;
                     MGLevelTask<T> t{};
                     t.e_in = src[k];
-                    t.rhs = lc == 0 ? rhs0(k) : R(s, lc);
-                    t.e_out = which[k] ? b1[k] : b0[k];
-                    t.n = s->ns_[lc];
-                    t.nu = nu;
-                    t.H = 2 * nu;
-                    t.h2 = (T)(hh * hh);
-                    t.inv_h2 = (T)(1.0 / (hh * hh));
-                    t.omega = omega;
-                    t.gate = s->d_norm_;
-                    t.mask = s->masked() ? s->mask_[lc] : nullptr;
-                    tasks.push_back(t);
-                    src[k] = t.e_out;
-                    which[k] ^= 1;
+                    t.e_out = b0[k];
+                    t.n = m_[k]->ns_[m_[k]->lc_];
+                    t.gate = m_[k]->d_norm_;
+                    copies.push_back(t);
                 }
-                if (int rc = add_level_stage<T>(0, tasks, nu, false, dst)) return rc;
-                left -= nu;
-                wrote = true;
+                if (int rc = add_level_stage<T>(3, copies, 0, false, dst)) return rc;
             }
-            std::vector<MGLevelTask<T>> copies;
-            for (int k : big) {
-                if (src[k] == b0[k]) continue;
-                MGLevelTask<T> t{};
-                t.e_in = src[k];
-                t.e_out = b0[k];
-                t.n = m_[k]->ns_[m_[k]->lc_];
-                t.gate = m_[k]->d_norm_;
-                copies.push_back(t);
+        }
+        // up leg; level 0 fused with the next cycle's down-leg stage unless last
+        for (int d = D - 1; d >= 0; --d) {
+            const bool fuse = d == 0 && !last;
+            std::vector<MGLevelTask<T>> tasks;
+            for (int k = 0; k < M; ++k) {
+                MGSolver* s = m_[k];
+                if (s->lc_ <= d) continue;
+                MGLevelTask<T> t = level_task(k, d);
+                t.e_in = d == 0 ? pre0[k] : E(s, 1, d);
+                t.coarse = E(s, 0, d + 1);
+                t.e_out = d == 0 ? post0[k] : E(s, 0, d);
+                t.nu = fuse ? p_.post_at(0) + p_.pre_at(0) : p_.post_at(d);
+                t.H = fuse ? 2 * t.nu + 2 : 2 * t.nu;
+                if (fuse) t.rhs_c = R(s, 1);
+                tasks.push_back(t);
             }
-            if (int rc = add_level_stage<T>(3, copies, 0, false, dst)) return rc;
+            if (tasks.empty()) continue;
+            if (fuse) {
+                if (int rc = add_level_stage<T>(4, tasks, tasks[0].nu, true, dst)) return rc;
+            } else if (int rc = add_level_stage<T>(1, tasks, p_.post_at(d), false, dst)) {
+                return rc;
+            }
         }
+        // the next cycle's finest up stage reads what this one wrote
+        for (int k = 0; k < M; ++k)
+            if (m_[k]->lc_ > 0 && !last) std::swap(pre0[k], post0[k]);
     }
-    // up leg
-    for (int d = D - 1; d >= 0; --d) {
-        std::vector<MGLevelTask<T>> tasks;
-        for (int k = 0; k < M; ++k) {
-            MGSolver* s = m_[k];
-            if (s->lc_ <= d) continue;
-            const double hh = hl(k, d);
-            MGLevelTask<T> t{};
-            t.e_in = d == 0 ? tmp0(k) : E(s, 1, d);
-            t.coarse = E(s, 0, d + 1);
-            t.rhs = d == 0 ? rhs0(k) : R(s, d);
-            t.e_out = d == 0 ? out0(k) : E(s, 0, d);
-            t.n = s->ns_[d];
-            t.nu = p_.post_at(d);
-            t.H = 2 * t.nu;
-            t.h2 = (T)(hh * hh);
-            t.inv_h2 = (T)(1.0 / (hh * hh));
-            t.omega = omega;
-            t.gate = s->d_norm_;
-            t.mask = s->masked() ? s->mask_[d] : nullptr;
-            tasks.push_back(t);
-        }
-        if (int rc = add_level_stage<T>(1, tasks, p_.post_at(d), false, dst)) return rc;
-    }
+    for (int k = 0; k < M; ++k) final_e0_[k] = (m_[k]->lc_ > 0 && post0[k] != out0(k)) ? 1 : 0;
     return 0;
 }
 
@@ -1311,6 +1346,10 @@ int MGBatch::build(const std::vector<MGSolver*>& members, const std::vector<doub
     PIC_HIP(hipHostMalloc(&h_mirror_, M * sizeof(MGNorm), hipHostMallocMapped | hipHostMallocCoherent));
     std::memset(h_mirror_, 0, M * sizeof(MGNorm));
     PIC_HIP(hipHostGetDevicePointer((void**)&d_mirror_, h_mirror_, 0));
+    // the stages of a refinement step first: they decide which level-0 buffer
+    // holds each member's correction at its end
+    if (int rc = build_direct_tables()) return rc;
+    if (int rc = mixed_ ? build_step<float>(h) : build_step<double>(h)) return rc;
     std::vector<MGFineTask> fine(M);
     any_mask_ = false;
     std::vector<MGTile> chunks, res;
@@ -1323,7 +1362,7 @@ int MGBatch::build(const std::vector<MGSolver*>& members, const std::vector<doub
         f.phi_tmp = s->phi_tmp_;
         f.rhs = rhs[k];
         f.r32 = s->r32_;
-        f.e32 = mixed_ ? s->ef_[0][0] : nullptr;
+        f.e32 = mixed_ ? s->ef_[final_e0_[k]][0] : nullptr;
         f.partial = s->partials_;
         f.phi_save = s->phi_save_;
         f.norm = s->d_norm_;
@@ -1347,10 +1386,6 @@ int MGBatch::build(const std::vector<MGSolver*>& members, const std::vector<doub
     if (int rc = upload_vec((void**)&d_res_tiles_, res)) return rc;
     n_chunk_ = (int)chunks.size();
     n_res_ = (int)res.size();
-    if (int rc = build_direct_tables()) return rc;
-    if (int rc = mixed_ ? build_cycle<float>(h, false) : build_cycle<double>(h, false)) return rc;
-    if (mixed_ && inner_ > 1)
-        if (int rc = build_cycle<float>(h, true)) return rc;
     return 0;
 }
 
@@ -1378,7 +1413,7 @@ static StageRef<Task> make_ref(const void* t0, const void* tasks, const MGTile* 
 
 template <typename T, int NU, bool M>
 static void launch_tiled_nu(int kind, bool restr, dim3 g, size_t lds, hipStream_t s, const StageRef<MGLevelTask<T>>& r) {
-    if (kind == 1) hipLaunchKernelGGL((mg_prolong_stage<T, NU, M>), g, dim3(256), lds, s, r);
+    if (kind == 1) hipLaunchKernelGGL((mg_prolong_stage<T, NU, false, M>), g, dim3(256), lds, s, r);
     else if (restr) hipLaunchKernelGGL((mg_smooth_stage<T, NU, true, M>), g, dim3(256), lds, s, r);
     else hipLaunchKernelGGL((mg_smooth_stage<T, NU, false, M>), g, dim3(256), lds, s, r);
 }
@@ -1388,6 +1423,27 @@ static void launch_tiled_m(int kind, bool restr, bool masked, dim3 g, size_t lds
                            const StageRef<MGLevelTask<T>>& r) {
     if (masked) launch_tiled_nu<T, NU, true>(kind, restr, g, lds, s, r);
     else launch_tiled_nu<T, NU, false>(kind, restr, g, lds, s, r);
+}
+
+// the finest up-leg stage fused with the next V-cycle's down-leg stage (mixed
+// mode only, so FP32): NU = post + pre sweeps, 0..6
+template <int NU>
+static void launch_fused_m(bool masked, dim3 g, size_t lds, hipStream_t s, const StageRef<MGLevelTask<float>>& r) {
+    if (masked) hipLaunchKernelGGL((mg_prolong_stage<float, NU, true, true>), g, dim3(256), lds, s, r);
+    else hipLaunchKernelGGL((mg_prolong_stage<float, NU, true, false>), g, dim3(256), lds, s, r);
+}
+
+static void launch_fused(int nu, bool masked, dim3 g, size_t lds, hipStream_t s,
+                         const StageRef<MGLevelTask<float>>& r) {
+    switch (nu) {
+    case 0: launch_fused_m<0>(masked, g, lds, s, r); break;
+    case 1: launch_fused_m<1>(masked, g, lds, s, r); break;
+    case 2: launch_fused_m<2>(masked, g, lds, s, r); break;
+    case 3: launch_fused_m<3>(masked, g, lds, s, r); break;
+    case 4: launch_fused_m<4>(masked, g, lds, s, r); break;
+    case 5: launch_fused_m<5>(masked, g, lds, s, r); break;
+    default: launch_fused_m<6>(masked, g, lds, s, r); break;
+    }
 }
 
 template <typename T>
@@ -1407,6 +1463,11 @@ static void launch_stage_t(const MGBatchStageView& st, hipStream_t s) {
     if (st.kind == 0 || st.kind == 1) {
         const auto r = make_ref<MGLevelTask<T>>(st.t0, st.tasks, st.tiles, st.n_tasks, st.gx);
         launch_tiled<T>(st.kind, st.nu, st.restr, st.masked, g, st.lds, s, r);
+    } else if (st.kind == 4) {
+        if constexpr (sizeof(T) == 4) {
+            const auto r = make_ref<MGLevelTask<float>>(st.t0, st.tasks, st.tiles, st.n_tasks, st.gx);
+            launch_fused(st.nu, st.masked, g, st.lds, s, r);
+        }
     } else if (st.kind == 2) {
         const auto r = make_ref<MGCoarseTask<T>>(st.t0, st.tasks, nullptr, st.n_tasks, 0);
         if (st.masked) hipLaunchKernelGGL((mg_coarse_stage<T, true>), g, dim3(1024), st.lds, s, r);
@@ -1456,14 +1517,10 @@ int MGBatch::enqueue_fine(int what, int which, int parity) {
     return 0;
 }
 
-// One refinement step (FP64 modes: one V-cycle): the V-cycle, then inner - 1
-// more whose first stage starts from the correction so far.
+// One refinement step (FP64 mode: one V-cycle): its inner_ V-cycles.
 int MGBatch::launch_refinement() {
     for (const auto& st : cycle_)
         if (int rc = launch_stage(st)) return rc;
-    for (int j = 1; j < inner_; ++j)
-        for (const auto& st : warm_)
-            if (int rc = launch_stage(st)) return rc;
     return 0;
 }
 

@@ -122,6 +122,32 @@ def test_mg_mixed_precision_accuracy(pic, orc, n, direct):
     assert err < 1e-9, err
 
 
+@pytest.mark.parametrize("inner", [2, 3])
+@pytest.mark.parametrize("n", [129, 513])
+def test_mg_refinement_step_equals_fp64_cycles(pic, orc, n, inner):
+    """One mixed-mode refinement step of `inner` FP32 V-cycles (the later ones
+    entered through the fused finest up/down stage) is the same linear
+    iteration as `inner` FP64 V-cycles, so with the same coarse treatment the
+    relative residuals agree to FP32 rounding; a cycle that lost or repeated
+    sweeps at the finest level would move it by a large factor.  V(1,1)
+    Gauss-Seidel keeps the residual after three cycles far above the FP32
+    floor of one refinement step (~1e-6)."""
+    rng = np.random.default_rng(31 * n + inner)
+    rhs = rng.normal(size=(n, n))
+    h = 1.0 / (n - 1)
+    zero = np.zeros((n, n))
+    cyc = dict(pre=1, post=1, omega=1.0, tol=0.0, max_cycles=inner)
+    _, st64 = _mg_gpu(pic, n, rhs, zero, h, pic.mg_params(pic.PRESET_POISSON, mode=pic.MG_FP64, **cyc))
+    _, stmx = _mg_gpu(pic, n, rhs, zero, h, pic.mg_params(pic.PRESET_POISSON, inner=inner, coarse_direct=0, **cyc))
+    assert st64.cycles == stmx.cycles == inner
+    assert 1e-4 < st64.rel_residual < 0.5, st64.rel_residual
+    assert abs(stmx.rel_residual / st64.rel_residual - 1.0) < 0.01, (stmx.rel_residual, st64.rel_residual)
+    # the oracle's FP64 cycles give the same residual (the GPU FP64 path is bitwise the oracle's)
+    phi_o = zero.copy()
+    ost = orc.mg_solve(phi_o, rhs, h, pre=1, post=1, coarse_sweeps=1, omega=1.0, tol=0.0, max_cycles=inner)
+    assert abs(ost.rel_residual / st64.rel_residual - 1.0) < 1e-9
+
+
 def _laplacian_dense(n, h):
     """The 5-point Laplacian on the (n-2)^2 interior nodes, Dirichlet zero."""
     m = n - 2
