@@ -388,6 +388,68 @@ def serve(n_jobs=24, rate=20.0, budget=200e9, seed=9, max_steps=2000):
             "completion_s_p50": float(np.percentile(comp, 50)), "completion_s_p99": float(np.percentile(comp, 99))}
 
 
+def fp64_check(cells, scale, bw, steps=20):
+    """Errors of the FP32 step against FP64 on the same workload.
+    field_solve: the step's FP32-in-FP64 multigrid potential against an FP64
+    multigrid solve of the same charge to 1e-13 (the solver's own error).
+    trajectories: the FP32 step (FP32 particles, mixed solve) against the FP64
+    step (FP64 particles, FP64 multigrid: the path that equals the FP64 oracle
+    bit for bit at test sizes) from the same initial state after `steps`
+    steps: the potential, energy and momentum diagnostics, live counts and
+    collision counts.  The particle runs drift apart as a few particles cross a
+    wall or a collision threshold in one precision and not the other; each
+    such particle moves the potential by about 1/sqrt(N) of its norm."""
+    import numpy as np
+    import pic_cuda_amd as pic
+    from pic_cuda_amd import configs
+    n = max(1000, int(2e7 * scale))
+    a = configs.full_step(scale=scale, cells=cells, bin_w=bw, bin_h=bw)
+    state = [a["particles"].download(s) for s in range(2)]
+    b = configs.build(cells, n, pic.PREC_FP64, mcc=True, bin_w=bw, bin_h=bw, state=state)
+    del state
+    a["sim"].step(steps)
+    b["sim"].step(steps)
+    da, db = a["sim"].diagnostics(), b["sim"].diagnostics()
+    pa, pb = a["grid"].download(pic.FIELD_PHI), b["grid"].download(pic.FIELD_PHI)
+
+    def rel(x, y):
+        return abs(x - y) / abs(y) if y else abs(x - y)
+
+    def collisions(c):
+        k = c["sim"].counters(0)
+        return int(sum(k[pic.CNT["process%d" % q]] for q in range(4)))
+    # momentum scale of a species: sum of w m |v| ~ sqrt(2 KE M), M = n w m its mass
+    pl = configs.Plasma(te=2.0, cells=cells)
+    w = pl.density * pl.length * pl.length / n
+    masses = (pic.ELECTRON_MASS, 39.948 * pic.AMU)
+    # the solver alone: the FP32 run's last charge solved in FP64 to 1e-13,
+    # warm-started from the FP32 run's potential
+    g = a["grid"]
+    ref = pic.Grid2D(g.m, g.n, g.dx * (g.m - 1), g.dy * (g.n - 1))
+    ref.upload(pic.FIELD_RHO, g.download(pic.FIELD_RHS))
+    ref.upload(pic.FIELD_PHI, pa)
+    rho_p, phi_p, _ = ref.device_ptrs()
+    mg64 = pic.Multigrid(g.m, pic.mg_params(pic.PRESET_POISSON, mode=pic.MG_FP64, tol=1e-13, max_cycles=100))
+    st64 = mg64.solve(phi_p, rho_p, g.dx)
+    p64 = ref.download(pic.FIELD_PHI)
+    solve = {"phi_rel_l2": float(np.linalg.norm(pa - p64) / np.linalg.norm(p64)), "fp64_cycles": st64.cycles,
+             "fp64_rel_residual": st64.rel_residual, "step_tol": 1e-8}
+    traj = {"steps": steps, "reference": "FP64 particles and FP64 multigrid from the same initial state",
+           "phi_rel_l2": float(np.linalg.norm(pa - pb) / np.linalg.norm(pb)),
+           "kinetic_rel": [rel(da["kinetic"][s], db["kinetic"][s]) for s in range(2)],
+           "momentum_diff_rel_to_scale": [
+               float(np.hypot(da["px"][s] - db["px"][s], da["py"][s] - db["py"][s]) /
+                     np.sqrt(2.0 * db["kinetic"][s] * db["count"][s] * w * masses[s])) for s in range(2)],
+           "field_energy_rel": rel(da["field_energy"], db["field_energy"]),
+           "alive_fp32": [int(c) for c in da["count"]], "alive_fp64": [int(c) for c in db["count"]],
+           "electron_collisions_fp32": collisions(a), "electron_collisions_fp64": collisions(b)}
+    out = {"field_solve": solve, "trajectories": traj}
+    for c in (a, b):
+        c["sim"].close()
+        c["particles"].close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -397,6 +459,7 @@ def main():
     ap.add_argument("--cells", type=int, default=1024)
     ap.add_argument("--bin", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64-check", action="store_true")
     ap.add_argument("--workload", default="all",
                     choices=["all", "step", "poisson", "ensemble", "serve", "graph", "tune", "walls", "export"])
     ap.add_argument("--ens-steps", type=int, default=500)
@@ -444,6 +507,10 @@ def main():
         print("bench: gpu done (%.3g particle-steps/s); timing the CPU baseline" % value, file=sys.stderr,
               flush=True)
         cpu = cpu_baseline(case)
+    acc = None
+    if not a.no_fp64_check:
+        print("bench: comparing 20 FP32 steps with the FP64 path", file=sys.stderr, flush=True)
+        acc = fp64_check(a.cells, a.scale, a.bin)
     cycles = (st1.total_mg_cycles - st0.total_mg_cycles) / a.steps
     line = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps,
@@ -456,6 +523,7 @@ def main():
         "roofline": roof, "cpu_baseline": cpu,
         "field_ms_per_step": round(field_ms, 4), "mg_cycles_per_step": cycles,
         "alive_start": int(st0.alive), "alive_end": int(st1.alive),
+        "fp64_error": acc,
     }
     print(json.dumps(line))
 

@@ -164,9 +164,10 @@ typedef struct {
     int inner;           /* mixed mode: FP32 V-cycles per FP64 refinement step, 1..4 (0: 1);
                             the convergence test runs after each refinement step */
     int coarse_direct;   /* mixed mode, no electrodes: 1 solves the first level with at most
-                            33 x 33 nodes exactly (FP64 sine transform) instead of recursing
-                            and sweeping; 0 keeps the sweeps.  FP64 mode ignores it (its
-                            cycle is the reference's, Multigrid.cu:539-569) */
+                            65 x 65 nodes exactly (FP64 sine transform on the matrix cores)
+                            instead of recursing and sweeping below it; 0 keeps the sweeps.
+                            FP64 mode ignores it (its cycle is the reference's,
+                            Multigrid.cu:539-569) */
 } pic_mg_params;
 
 typedef struct {
@@ -289,6 +290,19 @@ int pic_sim_step(pic_sim* s, int n_steps);
 int pic_sim_sync(pic_sim* s);
 int pic_sim_counters(pic_sim* s, int species, uint64_t out[16]);
 int pic_sim_stats(pic_sim* s, pic_sim_report* out);
+
+/* Energy and momentum diagnostics of the synchronized state (the owed half
+   step is completed first).  No counterpart in the reference, whose
+   diagnostics.h holds stage timings only; energies are per unit length in z. */
+typedef struct {
+    int n_species;
+    double count[4];      /* live particles of each species */
+    double kinetic[4];    /* sum of w m (vx^2 + vy^2) / 2, J/m */
+    double px[4], py[4];  /* sum of w m v, kg/s */
+    double field_energy;  /* eps0/2 sum over the nodes of |E|^2 dx dy, J/m (E as computeElectricField) */
+    double phi_l2;        /* sqrt of the sum over the nodes of phi^2 */
+} pic_diagnostics;
+int pic_sim_diagnostics(pic_sim* s, pic_diagnostics* out);
 int pic_sim_mcc_info(pic_sim* s, int species, pic_mcc_info* out, double* table, int cap);
 /* the same constants and table (n_proc x K rows of n_gas sigma on the uniform
    energy grid) that pic_sim_create builds for `species` of a cross-section

@@ -75,6 +75,12 @@ class SimReport(ctypes.Structure):
                 ("graph_steps", ctypes.c_int64), ("graph_captures", ctypes.c_int64), ("ms_graph", ctypes.c_double)]
 
 
+class Diagnostics(ctypes.Structure):
+    _fields_ = [("n_species", ctypes.c_int), ("count", ctypes.c_double * 4), ("kinetic", ctypes.c_double * 4),
+                ("px", ctypes.c_double * 4), ("py", ctypes.c_double * 4), ("field_energy", ctypes.c_double),
+                ("phi_l2", ctypes.c_double)]
+
+
 class TuneReport(ctypes.Structure):
     _fields_ = [("bin_w", ctypes.c_int), ("bin_h", ctypes.c_int), ("push_ms", ctypes.c_double),
                 ("omega", ctypes.c_double), ("levels", ctypes.c_int), ("pre", ctypes.c_int * 16),
@@ -166,6 +172,7 @@ SIGNATURES = {
     "pic_sim_sync": (_i, [_vp]),
     "pic_sim_counters": (_i, [_vp, _i, _vp]),
     "pic_sim_stats": (_i, [_vp, _P(SimReport)]),
+    "pic_sim_diagnostics": (_i, [_vp, _P(Diagnostics)]),
     "pic_sim_mcc_info": (_i, [_vp, _i, _P(MccInfo), _vp, _i]),
     "pic_sim_rebin": (_i, [_vp, _i, _i]),
     "pic_sim_use_graph": (_i, [_vp, _i]),
@@ -732,6 +739,15 @@ class Simulation:
         r = SimReport()
         _check(load().pic_sim_stats(self._h, ctypes.byref(r)))
         return r
+
+    def diagnostics(self):
+        """Energy and momentum of the synchronized state: per species count,
+        kinetic energy and momentum (lists), field energy and |phi|_2."""
+        d = Diagnostics()
+        _check(load().pic_sim_diagnostics(self._h, ctypes.byref(d)))
+        ns = d.n_species
+        return dict(count=list(d.count[:ns]), kinetic=list(d.kinetic[:ns]), px=list(d.px[:ns]),
+                    py=list(d.py[:ns]), field_energy=d.field_energy, phi_l2=d.phi_l2)
 
     def mcc_info(self, species):
         info = MccInfo()

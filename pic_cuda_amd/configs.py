@@ -47,11 +47,13 @@ def add_argon_processes(xs, electron=0, ion=1):
 
 
 def build(cells, n_per_species, precision, mcc, seed=1, case_id=0, mg=None, bin_w=16, bin_h=16, halo=2,
-          gas_density=1e21, gas_t=0.03, te=2.0, capacity=None, walls=None, processes=None):
+          gas_density=1e21, gas_t=0.03, te=2.0, capacity=None, walls=None, processes=None, state=None):
     """Create particles, grid, walls, cross sections and the step driver.
     `walls`: internal wall segments (x1, y1, x2, y2) in cell units, added to
     the absorbing box. `processes(xs, electron, ion)` fills the cross-section
-    set (default: the synthetic e-Ar / Ar+-Ar set)."""
+    set (default: the synthetic e-Ar / Ar+-Ar set). `state`: per species
+    (x, y, vx, vy, ids) in cell units, uploaded instead of the Maxwellian
+    initialization (the macro-particle weight still follows n_per_species)."""
     pl = Plasma(te=te, cells=cells)
     nn = pl.nodes
     grid = Grid2D(nn, nn, pl.length, pl.length)
@@ -62,8 +64,12 @@ def build(cells, n_per_species, precision, mcc, seed=1, case_id=0, mg=None, bin_
     w = pl.density * pl.length * pl.length / n_per_species
     parts.set_weight(0, w)
     parts.set_weight(ion, w)
-    parts.init_maxwellian(0, n_per_species, nn, nn, pl.vte, seed, case_id, 0)
-    parts.init_maxwellian(ion, n_per_species, nn, nn, pl.vti, seed, case_id, 0)
+    if state is None:
+        parts.init_maxwellian(0, n_per_species, nn, nn, pl.vte, seed, case_id, 0)
+        parts.init_maxwellian(ion, n_per_species, nn, nn, pl.vti, seed, case_id, 0)
+    else:
+        for s, arrays in enumerate(state):
+            parts.upload(s, *arrays)
     bnd = Boundaries2D(pl.length, pl.length, nn, nn)
     bnd.add_box()
     for (x1, y1, x2, y2) in (walls if walls is not None else ()):

@@ -1638,6 +1638,67 @@ extern "C" int pic_sim_sync(pic_sim* S) {
     return sim_sync_impl(S);
 }
 
+// velocity moments of every species (moments_kernel over the species' work
+// items, summed on the host in item order) and the field energy and potential
+// norm (sums of squares of E and phi over the nodes)
+template <typename T>
+static int diagnostics_t(pic_sim* S, pic_diagnostics* out) {
+    pic_particles* p = S->p;
+    pic_grid* g = S->g;
+    hipStream_t sm = lib_stream();
+    const size_t nn = (size_t)g->m * g->n;
+    size_t need = (2 * nn + 255) / 256;
+    std::vector<std::vector<WorkItem>> items(p->sp.size());
+    for (size_t s = 0; s < p->sp.size(); ++s) {
+        items_for_species(p, (int)s, 4096, items[s]);
+        need = std::max(need, 4 * items[s].size());
+    }
+    double* d = nullptr;
+    PIC_HIP(hipMalloc(&d, std::max<size_t>(need, 1) * sizeof(double)));
+    std::unique_ptr<double, void (*)(double*)> hold(d, [](double* q) { (void)hipFree(q); });
+    std::vector<double> h(need);
+    for (size_t s = 0; s < p->sp.size(); ++s) {
+        const int ni = (int)items[s].size();
+        if (int rc = upload_items(&p->d_items, &p->items_alloc, items[s])) return rc;
+        if (int rc = launch_moments<T>((CaseDev<T>*)p->d_case, p->d_state, p->d_items, ni, 4096, d, sm)) return rc;
+        PIC_HIP(hipMemcpyAsync(h.data(), d, 4 * (size_t)ni * sizeof(double), hipMemcpyDeviceToHost, sm));
+        PIC_HIP(hipStreamSynchronize(sm));
+        double a[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int b = 0; b < ni; ++b)
+            for (int k = 0; k < 4; ++k) a[k] += h[4 * (size_t)b + k];
+        const double mw = p->sp[s].m * p->sp[s].w;
+        out->count[s] = a[0];
+        out->px[s] = mw * a[1];
+        out->py[s] = mw * a[2];
+        out->kinetic[s] = 0.5 * mw * a[3];
+    }
+    auto sum_sq = [&](const double* src, size_t n, double* res) -> int {
+        const size_t nc = (n + 255) / 256;
+        if (int rc = launch_sumsq(src, n, d, sm)) return rc;
+        PIC_HIP(hipMemcpyAsync(h.data(), d, nc * sizeof(double), hipMemcpyDeviceToHost, sm));
+        PIC_HIP(hipStreamSynchronize(sm));
+        double t = 0.0;
+        for (size_t c = 0; c < nc; ++c) t += h[c];
+        *res = t;
+        return 0;
+    };
+    if (int rc = launch_efield(g->phi, (double2*)g->E, g->m, g->n, g->dx, g->dy, sm)) return rc;
+    double e2 = 0.0, p2 = 0.0;
+    if (int rc = sum_sq(g->E, 2 * nn, &e2)) return rc;
+    if (int rc = sum_sq(g->phi, nn, &p2)) return rc;
+    out->field_energy = 0.5 * S->eps0 * e2 * g->dx * g->dy;
+    out->phi_l2 = std::sqrt(p2);
+    return 0;
+}
+
+extern "C" int pic_sim_diagnostics(pic_sim* S, pic_diagnostics* out) {
+    PIC_REQUIRE(S && out, "NULL argument");
+    if (int rc = sim_sync_impl(S)) return rc;
+    *out = pic_diagnostics{};
+    out->n_species = (int)S->p->sp.size();
+    return S->p->prec == 4 ? diagnostics_t<float>(S, out) : diagnostics_t<double>(S, out);
+}
+
 extern "C" int pic_sim_counters(pic_sim* S, int species, uint64_t out[16]) {
     PIC_REQUIRE(S && out, "NULL argument");
     PIC_REQUIRE(species >= 0 && species < (int)S->p->sp.size(), "bad species");

@@ -114,6 +114,28 @@ int launch_zero_interior(double* f, int n, hipStream_t s) {
     return 0;
 }
 
+// sums of squares of 256-value chunks: partial[c] = sum over a[256 c ..] of a^2
+__global__ __launch_bounds__(256) void sumsq_kernel(const double* __restrict__ a, size_t n,
+                                                    double* __restrict__ partial) {
+    __shared__ double sh[256];
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const double v = i < n ? a[i] : 0.0;
+    sh[threadIdx.x] = v * v;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+int launch_sumsq(const double* a, size_t n, double* partial, hipStream_t s) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(sumsq_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, n, partial);
+    PIC_HIP(hipGetLastError());
+    return 0;
+}
+
 int launch_deposit(int prec, const void* x, const void* y, int n, int nx, int ny, unsigned long long* acc,
                    hipStream_t s) {
     if (n <= 0) return 0;

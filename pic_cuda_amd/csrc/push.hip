@@ -1072,6 +1072,54 @@ __global__ __launch_bounds__(256) void export_kernel(const CaseDev<T>* __restric
     }
 }
 
+// Velocity moments of the particles of each work item, in FP64: out[4 b + k]
+// for item b holds the sums of 1, vx, vy and vx^2 + vy^2 (fixed-order block
+// reduction; the host adds the items in order).
+template <typename T>
+__global__ __launch_bounds__(256) void moments_kernel(const CaseDev<T>* __restrict__ cases,
+                                                      const CaseState* __restrict__ states,
+                                                      const WorkItem* __restrict__ items, int chunk,
+                                                      double* __restrict__ out) {
+    __shared__ double sh[4][256];
+    const WorkItem it = items[blockIdx.x];
+    const CaseDev<T>& C = cases[it.case_idx];
+    const SpeciesDev<T>& S = C.sp[it.species];
+    const int in = states[it.case_idx].cur;
+    const Range rg = item_range(S, in, it, chunk);
+    const int n = rg.n();
+    const gptr<const T> VX = G((const T*)S.vx[in]), VY = G((const T*)S.vy[in]);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        const int s = rg.at(t);
+        const double vx = (double)VX[s], vy = (double)VY[s];
+        a0 += 1.0;
+        a1 += vx;
+        a2 += vy;
+        a3 += vx * vx + vy * vy;
+    }
+    const int tid = threadIdx.x;
+    sh[0][tid] = a0;
+    sh[1][tid] = a1;
+    sh[2][tid] = a2;
+    sh[3][tid] = a3;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (tid < k)
+            for (int q = 0; q < 4; ++q) sh[q][tid] = sh[q][tid] + sh[q][tid + k];
+        __syncthreads();
+    }
+    if (tid < 4) G(out)[4 * (size_t)blockIdx.x + tid] = sh[tid][0];
+}
+
+template <typename T>
+int launch_moments(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items, int chunk,
+                   double* out, hipStream_t s) {
+    if (n_items <= 0) return 0;
+    hipLaunchKernelGGL(moments_kernel<T>, dim3(n_items), dim3(256), 0, s, cases, states, items, chunk, out);
+    PIC_HIP(hipGetLastError());
+    return 0;
+}
+
 template <typename T>
 int launch_export(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items, int chunk,
                   int32_t* counter, float2* dst, int64_t cap, double sx, double sy, hipStream_t s) {
@@ -1128,6 +1176,8 @@ int launch_counts(const CaseDev<T>* cases, const CaseState* states, int c, int s
                                    unsigned long long*, hipStream_t);                                            \
     template int launch_gather<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int, int, int32_t*,       \
                                   hipStream_t);                                                                  \
+    template int launch_moments<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int, int, double*,       \
+                                   hipStream_t);                                                                 \
     template int launch_export<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int, int, int32_t*,       \
                                   float2*, int64_t, double, double, hipStream_t);                                \
     template int launch_end_step<T>(const CaseDev<T>*, CaseState*, int, int, hipStream_t, int*);                 \

@@ -27,6 +27,10 @@ int launch_scatter(const CaseDev<T>* cases, const CaseState* states, const WorkI
 template <typename T>
 int launch_gather(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items, int chunk,
                   int32_t* counter, hipStream_t s);
+// per work item: sums of 1, vx, vy, vx^2 + vy^2 (FP64) into out[4 * item + k]
+template <typename T>
+int launch_moments(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items, int chunk,
+                   double* out, hipStream_t s);
 template <typename T>
 int launch_export(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items, int chunk,
                   int32_t* counter, float2* dst, int64_t cap, double sx, double sy, hipStream_t s);

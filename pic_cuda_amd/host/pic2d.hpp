@@ -255,6 +255,12 @@ public:
         check(pic_sim_stats(h_, &r), "report");
         return r;
     }
+    // energy and momentum of the synchronized state
+    pic_diagnostics diagnostics() {
+        pic_diagnostics d{};
+        check(pic_sim_diagnostics(h_, &d), "diagnostics");
+        return d;
+    }
     std::vector<uint64_t> counters(int species) {
         std::vector<uint64_t> c(16);
         check(pic_sim_counters(h_, species, c.data()), "counters");

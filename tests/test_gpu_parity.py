@@ -285,6 +285,79 @@ def test_step_passes_bit_identical(pic, orc, prec, mcc):
         assert c0[2] > 0 and c0[3] > 0 and c0[4] > 0, c0    # elastic, excitation and ionization all happened
 
 
+def _species_mass(pic, s):
+    return pic.ELECTRON_MASS if s == 0 else 39.948 * pic.AMU
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_diagnostics_equal_oracle_sums(pic, orc, prec):
+    """Kinetic energy, momentum and counts of each species, the field energy
+    and |phi| from pic_sim_diagnostics equal the sums over the oracle's state
+    after the same steps (same precision, lockstep passes, MCC on)."""
+    case = _small_case(pic, prec, True)
+    sim, grid = case["sim"], case["grid"]
+    n0 = len(case["particles"].download(0)[0])
+    o = _oracle_from(pic, orc, case, prec)
+    for _ in range(4):
+        o.phi = grid.download(pic.FIELD_PHI)
+        sim.step(1)
+        o.step(1, solve=False)
+    o.phi = grid.download(pic.FIELD_PHI)
+    d = sim.diagnostics()       # completes the owed half step
+    o.sync()
+    pl = case["plasma"]
+    w = pl.density * pl.length * pl.length / n0    # the weight configs.build sets
+    for s in range(2):
+        vx, vy = o.sp[s].vx.astype(np.float64), o.sp[s].vy.astype(np.float64)
+        mw = _species_mass(pic, s) * w
+        assert d["count"][s] == len(vx)
+        ke = 0.5 * mw * np.sum(vx * vx + vy * vy)
+        assert abs(d["kinetic"][s] / ke - 1.0) < 1e-12, (s, d["kinetic"][s], ke)
+        scale = mw * np.sum(np.abs(vx) + np.abs(vy))
+        assert abs(d["px"][s] - mw * np.sum(vx)) < 1e-12 * scale
+        assert abs(d["py"][s] - mw * np.sum(vy)) < 1e-12 * scale
+    ex, ey = orc.efield(o.phi, grid.dx, grid.dy)
+    fe = 0.5 * pic.EPS0 * np.sum(ex * ex + ey * ey) * grid.dx * grid.dy
+    assert fe > 0 and abs(d["field_energy"] / fe - 1.0) < 1e-12, (d["field_energy"], fe)
+    assert abs(d["phi_l2"] / np.linalg.norm(o.phi) - 1.0) < 1e-12
+
+
+def test_fp32_step_against_fp64_oracle(pic, orc):
+    """FP32 particles with the mixed-precision solve against the FP64 oracle
+    (FP64 particles, FP64 multigrid to the same tolerance) from the same
+    initial state, 10 steps without collisions.  Stated tolerances: potential
+    relative L2 <= 1e-5, kinetic energy of each species <= 1e-5 relative, field
+    energy <= 1e-4 relative, momentum within 1e-5 of the momentum scale, live
+    counts within 1e-4 (particles near the absorbing walls may cross in one
+    precision and not the other)."""
+    case = _small_case(pic, 4, False, cells=64, n=40000)
+    sim, grid = case["sim"], case["grid"]
+    n0 = len(case["particles"].download(0)[0])
+    o = _oracle_from(pic, orc, case, 8)            # FP32 values are exact in FP64
+    for _ in range(10):
+        sim.step(1)
+        o.step(1)
+    d = sim.diagnostics()
+    o.sync()
+    phi_g = grid.download(pic.FIELD_PHI)
+    err_phi = np.linalg.norm(phi_g - o.phi) / np.linalg.norm(o.phi)
+    assert err_phi < 1e-5, err_phi
+    pl = case["plasma"]
+    w = pl.density * pl.length * pl.length / n0
+    for s in range(2):
+        vx, vy = o.sp[s].vx, o.sp[s].vy
+        mw = _species_mass(pic, s) * w
+        ke = 0.5 * mw * np.sum(vx * vx + vy * vy)
+        assert abs(d["kinetic"][s] / ke - 1.0) < 1e-5, (s, d["kinetic"][s], ke)
+        scale = mw * np.sum(np.abs(vx) + np.abs(vy))
+        assert abs(d["px"][s] - mw * np.sum(vx)) < 1e-5 * scale
+        assert abs(d["py"][s] - mw * np.sum(vy)) < 1e-5 * scale
+        assert abs(d["count"][s] - len(vx)) <= 1e-4 * n0
+    ex, ey = orc.efield(o.phi, grid.dx, grid.dy)
+    fe = 0.5 * pic.EPS0 * np.sum(ex * ex + ey * ey) * grid.dx * grid.dy
+    assert abs(d["field_energy"] / fe - 1.0) < 1e-4, (d["field_energy"], fe)
+
+
 def test_field_solve_in_step_matches_fp64(pic, orc):
     case = _small_case(pic, 4, True, cells=64, n=40000)
     sim, grid = case["sim"], case["grid"]
