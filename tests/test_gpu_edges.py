@@ -99,6 +99,9 @@ def test_empty_simulation_and_empty_species(pic, orc):
     for s in range(2):
         assert len(c["particles"].download(s)[0]) == 0
         assert c["particles"].export_positions(s)[1] == 0
+    d = c["sim"].diagnostics()
+    assert d["count"] == [0.0, 0.0] and d["kinetic"] == [0.0, 0.0] and d["px"] == [0.0, 0.0]
+    assert d["field_energy"] == 0.0 and d["phi_l2"] == 0.0
     # electrons only, ions empty: the oracle agrees pass by pass
     rng = np.random.default_rng(1)
     n = 3000

@@ -51,6 +51,36 @@ def test_ensemble_members_match_alone(pic, prec):
     ens.close()
 
 
+def test_ensemble_with_an_electrode_member_matches_alone(pic):
+    """Mixed-mode members without electrodes solve their coarse level exactly
+    (sine transform), a member with an electrode keeps the swept cycle; in one
+    batch (shared launches, masked kernel variants) each still equals its run
+    alone bit for bit."""
+    specs = [(32, 6000, 21, 3.0), (64, 20000, 22, 5.0), (64, 15000, 23, 4.0)]
+
+    def electrode(case):
+        n = case["grid"].m
+        y, x = np.mgrid[0:n, 0:n]
+        disc = (x - n / 3) ** 2 + (y - n / 2) ** 2 < (0.12 * n) ** 2
+        case["sim"].set_electrodes(disc.astype(np.uint8), np.where(disc, 2.0, 0.0))
+    alone = []
+    for i, (c, n, seed, te) in enumerate(specs):
+        k = _case(pic, c, n, seed, te)
+        if i == 1:
+            electrode(k)
+        k["sim"].step(4)
+        alone.append(_state(pic, k))
+    batch = [_case(pic, c, n, seed, te) for c, n, seed, te in specs]
+    electrode(batch[1])
+    ens = pic.Ensemble()
+    for k in batch:
+        ens.add(k["sim"])
+    ens.step(4)
+    for k, ref in zip(batch, alone):
+        _same(_state(pic, k), ref)
+    ens.close()
+
+
 def test_ensemble_late_join_and_leave(pic):
     a_ref = _case(pic, 32, 5000, 11)
     a_ref["sim"].step(6)

@@ -122,9 +122,9 @@ def test_mg_mixed_precision_accuracy(pic, orc, n, direct):
     assert err < 1e-9, err
 
 
-@pytest.mark.parametrize("inner", [2, 3])
+@pytest.mark.parametrize("inner,split", [(2, False), (3, False), (2, True)])
 @pytest.mark.parametrize("n", [129, 513])
-def test_mg_refinement_step_equals_fp64_cycles(pic, orc, n, inner):
+def test_mg_refinement_step_equals_fp64_cycles(pic, orc, n, inner, split):
     """One mixed-mode refinement step of `inner` FP32 V-cycles (the later ones
     entered through the fused finest up/down stage) is the same linear
     iteration as `inner` FP64 V-cycles, so with the same coarse treatment the
@@ -137,14 +137,29 @@ def test_mg_refinement_step_equals_fp64_cycles(pic, orc, n, inner):
     h = 1.0 / (n - 1)
     zero = np.zeros((n, n))
     cyc = dict(pre=1, post=1, omega=1.0, tol=0.0, max_cycles=inner)
-    _, st64 = _mg_gpu(pic, n, rhs, zero, h, pic.mg_params(pic.PRESET_POISSON, mode=pic.MG_FP64, **cyc))
-    _, stmx = _mg_gpu(pic, n, rhs, zero, h, pic.mg_params(pic.PRESET_POISSON, inner=inner, coarse_direct=0, **cyc))
+    # split: the finest level's post-sweeps 3 and pre-sweeps 1 (the fused
+    # stage then runs 3 + 1 sweeps in that order), 1 + 1 below
+    nl = int(np.log2(n - 1))                      # levels above the 3x3 coarsest
+    pre_l = [1] * nl
+    post_l = [3] + [1] * (nl - 1) if split else [1] * nl
+    st = []
+    for prm in (pic.mg_params(pic.PRESET_POISSON, mode=pic.MG_FP64, **cyc),
+                pic.mg_params(pic.PRESET_POISSON, inner=inner, coarse_direct=0, **cyc)):
+        g = pic.Grid2D(n, n, h * (n - 1), h * (n - 1))
+        g.upload(pic.FIELD_RHO, rhs)
+        rho_p, phi_p, _ = g.device_ptrs()
+        mg = pic.Multigrid(n, prm)
+        if split:
+            mg.set_level_sweeps(pre_l, post_l)
+        st.append(mg.solve(phi_p, rho_p, h))
+    st64, stmx = st
     assert st64.cycles == stmx.cycles == inner
     assert 1e-4 < st64.rel_residual < 0.5, st64.rel_residual
     assert abs(stmx.rel_residual / st64.rel_residual - 1.0) < 0.01, (stmx.rel_residual, st64.rel_residual)
     # the oracle's FP64 cycles give the same residual (the GPU FP64 path is bitwise the oracle's)
     phi_o = zero.copy()
-    ost = orc.mg_solve(phi_o, rhs, h, pre=1, post=1, coarse_sweeps=1, omega=1.0, tol=0.0, max_cycles=inner)
+    lv = dict(pre_levels=pre_l, post_levels=post_l) if split else {}
+    ost = orc.mg_solve(phi_o, rhs, h, pre=1, post=1, coarse_sweeps=1, omega=1.0, tol=0.0, max_cycles=inner, **lv)
     assert abs(ost.rel_residual / st64.rel_residual - 1.0) < 1e-9
 
 
