@@ -1427,8 +1427,8 @@ static int enqueue_step_body(pic_sim* S, int flags, size_t lds, int cycles) {
     if (int rc = launch_push<T>((CaseDev<T>*)p->d_case, p->d_state, S->d_items, S->n_items, S->spill_chunk, flags, -1,
                                 lds, sm, S->d_walls_cell != nullptr))
         return rc;
-    if (int rc = launch_end_step<T>((CaseDev<T>*)p->d_case, p->d_state, 1, 1, sm, S->d_status)) return rc;
-    if (int rc = launch_charge(S->charge, nn, sm)) return rc;
+    if (int rc = launch_charge_end_step<T>(S->charge, nn, (CaseDev<T>*)p->d_case, p->d_state, 1, sm, S->d_status))
+        return rc;
     // the solve's first residual pass also copies phi (its warm start) to phi_prev
     return S->mg->enqueue_fixed(cycles);
 }
@@ -1578,9 +1578,10 @@ static int step_t(pic_sim* S, int n) {
                                     -1, lds, sm, S->d_walls_cell != nullptr))
             return rc;
         PIC_HIP(hipEventRecord(S->ev[1], sm));
-        if (int rc = launch_end_step<T>((CaseDev<T>*)p->d_case, p->d_state, 1, 1, sm, S->d_status)) return rc;
+        if (int rc = launch_charge_end_step<T>(S->charge, nn, (CaseDev<T>*)p->d_case, p->d_state, 1, sm,
+                                               S->d_status))
+            return rc;
         p->h_cur ^= 1;
-        if (int rc = launch_charge(S->charge, nn, sm)) return rc;
         S->have_prev = true;   // the solve's first residual pass copies phi to phi_prev
         MGStats st{};
         if (int rc = S->mg->solve(S->g->phi, S->g->rhs, S->g->dx, &st)) return rc;

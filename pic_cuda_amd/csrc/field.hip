@@ -1,6 +1,7 @@
 // field.hip -- grid kernels of the step outside the Poisson solve.
 //
-//  charge_kernel   Q20 accumulators -> charge density rho and Poisson rhs
+//  charge_node (field.hpp), in charge_end_step_kernel and charge_batch_kernel:
+//                  Q20 accumulators -> charge density rho and Poisson rhs
 //                  (-rho/eps0); clears the accumulators for the next step.
 //                  Replaces cudaMemset(rho) + updateDensity's FP64 result
 //                  (main.cu:93-102, kernels.cu:26-69).
@@ -11,20 +12,6 @@
 #include "field.hpp"
 
 namespace pic {
-
-__global__ __launch_bounds__(256) void charge_kernel(ChargeArgs a, size_t nn) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nn) return;
-    double r = 0.0, q = 0.0;
-    for (int s = 0; s < a.ns; ++s) {
-        const double v = (double)a.acc[s][i];
-        r = fma(a.cfac[s], v, r);
-        q = fma(a.qfac[s], v, q);
-        if (a.zero) a.acc[s][i] = 0ull;
-    }
-    if (a.rho) a.rho[i] = q;
-    a.rhs[i] = r;
-}
 
 __global__ __launch_bounds__(256) void efield_kernel(const double* __restrict__ phi, double2* __restrict__ E,
                                                      int nx, int ny, double dx, double dy) {
@@ -58,13 +45,6 @@ __global__ __launch_bounds__(256) void deposit_kernel(const T* __restrict__ x, c
     atomicAdd(b + 1, qx * (W - qy));
     atomicAdd(b + nx, (W - qx) * qy);
     atomicAdd(b + nx + 1, qx * qy);
-}
-
-int launch_charge(const ChargeArgs& a, size_t nn, hipStream_t s) {
-    if (nn == 0) return 0;
-    hipLaunchKernelGGL(charge_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, a, nn);
-    PIC_HIP(hipGetLastError());
-    return 0;
 }
 
 __global__ __launch_bounds__(256) void charge_batch_kernel(const ChargeArgs* __restrict__ args,

@@ -14,7 +14,20 @@ struct ChargeArgs {
     double* rhs;
 };
 
-int launch_charge(const ChargeArgs& a, size_t nn, hipStream_t s);
+// node i of the charge conversion: rhs (and rho) from the Q20 accumulators,
+// which are cleared when a.zero is set
+__device__ inline void charge_node(const ChargeArgs& a, size_t i) {
+    double r = 0.0, q = 0.0;
+    for (int s = 0; s < a.ns; ++s) {
+        const double v = (double)a.acc[s][i];
+        r = fma(a.cfac[s], v, r);
+        q = fma(a.qfac[s], v, q);
+        if (a.zero) a.acc[s][i] = 0ull;
+    }
+    if (a.rho) a.rho[i] = q;
+    a.rhs[i] = r;
+}
+
 
 // batched: one ChargeArgs per case, items (case, 256-node chunk) in `chunks`
 // (int4 {case, chunk, 0, 0}); nn per case in `nns`

@@ -220,6 +220,7 @@ private:
                         std::vector<Stage>* dst = nullptr);
     int launch_stage(const Stage& st);
     int launch_refinement();   // the inner V-cycles of one refinement step
+    bool any_odd_steps() const;
     int enqueue_fine(int what, int which, int parity);
     void free_all();
 

@@ -1517,6 +1517,14 @@ int MGBatch::enqueue_fine(int what, int which, int parity) {
     return 0;
 }
 
+// Some member ended after an odd number of refinement steps, so its solution
+// is in phi_tmp and the gated copy back is needed (read after a synchronization).
+bool MGBatch::any_odd_steps() const {
+    for (int k = 0; k < (int)m_.size(); ++k)
+        if (h_mirror_[k].steps & 1) return true;
+    return false;
+}
+
 // One refinement step (FP64 mode: one V-cycle): its inner_ V-cycles.
 int MGBatch::launch_refinement() {
     for (const auto& st : cycle_)
@@ -1551,7 +1559,7 @@ int MGBatch::solve() {
         if (all || enqueued >= max_outer_) break;
         batch = 1;
     }
-    if (mixed_) enqueue_fine(3, 0, 0);
+    if (mixed_ && any_odd_steps()) enqueue_fine(3, 0, 0);
     PIC_HIP(hipGetLastError());
     int mx = 0;
     for (int k = 0; k < M; ++k) mx = std::max(mx, h_mirror_[k].cycles);
@@ -1590,7 +1598,7 @@ int MGBatch::complete_fixed() {
         ++enqueued;
         PIC_HIP(hipStreamSynchronize(stream_));
     }
-    if (enqueued > fixed_cycles_ && mixed_) enqueue_fine(3, 0, 0);
+    if (enqueued > fixed_cycles_ && mixed_ && any_odd_steps()) enqueue_fine(3, 0, 0);
     PIC_HIP(hipGetLastError());
     int mx = 0;
     for (int k = 0; k < M; ++k) mx = std::max(mx, h_mirror_[k].cycles);

@@ -26,6 +26,7 @@
 // the path of a particle that stays.  Overflow of either part goes to the spill
 // region (index nb).
 #include "pic_device.hpp"
+#include "field.hpp"
 #include <type_traits>
 
 namespace pic {
@@ -930,9 +931,8 @@ __global__ __launch_bounds__(256) void count_kernel(const CaseDev<T>* __restrict
 // the spill fill of every species of the new state (read back by the host
 // after the Poisson solve to decide on re-binning)
 template <typename T>
-__global__ __launch_bounds__(256) void end_step_kernel(const CaseDev<T>* __restrict__ cases, CaseState* states,
-                                                       int advance_step, int* __restrict__ status_out) {
-    const int c = blockIdx.x;
+__device__ inline void end_step_block(const CaseDev<T>* __restrict__ cases, CaseState* states, int c,
+                                      int advance_step, int* __restrict__ status_out) {
     const CaseDev<T>& C = cases[c];
     const int in = states[c].cur;
     __syncthreads();
@@ -957,6 +957,28 @@ __global__ __launch_bounds__(256) void end_step_kernel(const CaseDev<T>* __restr
                 }
                 status_out[c * PIC_MAX_SPECIES + s] = v;
             }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void end_step_kernel(const CaseDev<T>* __restrict__ cases, CaseState* states,
+                                                       int advance_step, int* __restrict__ status_out) {
+    end_step_block(cases, states, (int)blockIdx.x, advance_step, status_out);
+}
+
+// The end of a single-case step and the charge conversion in one launch (they
+// touch disjoint data): block 0 ends the step of case 0 (first, so that its
+// latency overlaps the conversion), the other blocks convert 256 nodes each.
+template <typename T>
+__global__ __launch_bounds__(256) void charge_end_step_kernel(ChargeArgs a, size_t nn,
+                                                              const CaseDev<T>* __restrict__ cases,
+                                                              CaseState* states, int advance_step,
+                                                              int* __restrict__ status_out) {
+    if (blockIdx.x == 0) {
+        end_step_block(cases, states, 0, advance_step, status_out);
+    } else {
+        const size_t i = (size_t)(blockIdx.x - 1) * blockDim.x + threadIdx.x;
+        if (i < nn) charge_node(a, i);
     }
 }
 
@@ -1140,6 +1162,16 @@ int launch_gather(const CaseDev<T>* cases, const CaseState* states, const WorkIt
 }
 
 template <typename T>
+int launch_charge_end_step(const ChargeArgs& a, size_t nn, const CaseDev<T>* cases, CaseState* states, int advance,
+                           hipStream_t s, int* status_out) {
+    const int n_charge = (int)((nn + 255) / 256);
+    hipLaunchKernelGGL(charge_end_step_kernel<T>, dim3(n_charge + 1), dim3(256), 0, s, a, nn, cases, states,
+                       advance, status_out);
+    PIC_HIP(hipGetLastError());
+    return 0;
+}
+
+template <typename T>
 int launch_end_step(const CaseDev<T>* cases, CaseState* states, int n_cases, int advance, hipStream_t s,
                     int* status_out) {
     hipLaunchKernelGGL(end_step_kernel<T>, dim3(n_cases), dim3(256), 0, s, cases, states, advance, status_out);
@@ -1181,6 +1213,8 @@ int launch_counts(const CaseDev<T>* cases, const CaseState* states, int c, int s
     template int launch_export<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int, int, int32_t*,       \
                                   float2*, int64_t, double, double, hipStream_t);                                \
     template int launch_end_step<T>(const CaseDev<T>*, CaseState*, int, int, hipStream_t, int*);                 \
+    template int launch_charge_end_step<T>(const ChargeArgs&, size_t, const CaseDev<T>*, CaseState*, int,         \
+                                           hipStream_t, int*);                                                    \
     template int launch_clear_out<T>(const CaseDev<T>*, const CaseState*, int, hipStream_t);
 PIC_LAUNCH_INST(float)
 PIC_LAUNCH_INST(double)

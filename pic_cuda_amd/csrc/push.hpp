@@ -1,6 +1,7 @@
 // push.hpp -- launchers of the particle kernels in push.hip.
 #pragma once
 #include "pic_device.hpp"
+#include "field.hpp"
 
 namespace pic {
 
@@ -40,6 +41,11 @@ int launch_export(const CaseDev<T>* cases, const CaseState* states, const WorkIt
 template <typename T>
 int launch_end_step(const CaseDev<T>* cases, CaseState* states, int n_cases, int advance, hipStream_t s,
                     int* status_out = nullptr);
+// the end of step of a single case and the charge conversion of nn nodes in
+// one launch (disjoint data)
+template <typename T>
+int launch_charge_end_step(const ChargeArgs& a, size_t nn, const CaseDev<T>* cases, CaseState* states, int advance,
+                           hipStream_t s, int* status_out);
 // clear the bin counts of the output parity (after a trial push)
 template <typename T>
 int launch_clear_out(const CaseDev<T>* cases, const CaseState* states, int n_cases, hipStream_t s);
