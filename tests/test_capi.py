@@ -71,7 +71,7 @@ def test_mg_presets(pic):
     sor = pic.mg_params(pic.PRESET_STEP_SOR)
     assert (sor.pre, sor.post, sor.omega, sor.tol, sor.mode, sor.inner) == (2, 2, 1.2, 1e-8, pic.MG_MIXED, 2)
     poi = pic.mg_params(pic.PRESET_POISSON)
-    assert (poi.omega, poi.inner) == (1.2, 1)
+    assert (poi.omega, poi.inner) == (1.2, 2)
     assert ref.inner == 1 and st.inner == 2
     assert ref.coarse_direct == 0 and st.coarse_direct == sor.coarse_direct == poi.coarse_direct == 1
     with pytest.raises(pic.PicError):
