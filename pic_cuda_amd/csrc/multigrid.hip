@@ -50,6 +50,9 @@ namespace pic {
 #define MG_ND 65
 #define MG_DS 63
 #define MG_DLDS (2 * MG_DS * MG_DS * 8)
+// largest finest level (nodes a side) of a batch that fuses the finest
+// up/down stages between V-cycles
+#define MG_FUSE_MAX 1025
 // residual tiles: 64 columns x 16 rows, 256 threads with 4 rows each
 #define MG_RX 64
 #define MG_RY 16
@@ -1091,7 +1094,10 @@ int MGBatch::add_level_stage(int kind, const std::vector<MGLevelTask<T>>& tasks,
 // the finest up-leg stage of a member with tiled levels also runs the next
 // cycle's pre-sweeps and restriction (stage kind 4), so that cycle's down leg
 // starts one level lower; its level-0 correction then alternates between the
-// two level-0 buffers (final_e0 tells where it ends).
+// two level-0 buffers (final_e0 tells where it ends).  Only batches whose
+// finest levels have at most MG_FUSE_MAX nodes a side fuse: on larger grids the
+// fused stage's bigger tile (fewer workgroups resident per CU) costs more than
+// the launch it saves (2049^2: 67.9 us against 27.9 + 27.9 us).
 template <typename T>
 int MGBatch::build_step(const std::vector<double>& h) {
     std::vector<Stage>* dst = &cycle_;
@@ -1126,6 +1132,8 @@ int MGBatch::build_step(const std::vector<double>& h) {
         post0[k] = out0(k);
     }
     final_e0_.assign(M, 0);
+    bool fuse_ok = mixed_;
+    for (auto* s : m_) fuse_ok = fuse_ok && s->ns_[0] <= MG_FUSE_MAX;
     for (int c = 0; c < inner_; ++c) {
         const bool first = c == 0, last = c == inner_ - 1;
         // the level-0 input of a cycle: FP64 mode phi; mixed mode zero, or the
@@ -1134,7 +1142,7 @@ int MGBatch::build_step(const std::vector<double>& h) {
             return mixed_ ? (first ? nullptr : (const T*)out0(k)) : (const T*)(void*)phi_[k];
         };
         // down leg (after the first cycle, level 0 was done by the fused stage)
-        for (int d = first ? 0 : 1; d < D; ++d) {
+        for (int d = (first || !fuse_ok) ? 0 : 1; d < D; ++d) {
             std::vector<MGLevelTask<T>> tasks;
             for (int k = 0; k < M; ++k) {
                 MGSolver* s = m_[k];
@@ -1244,7 +1252,7 @@ int MGBatch::build_step(const std::vector<double>& h) {
         }
         // up leg; level 0 fused with the next cycle's down-leg stage unless last
         for (int d = D - 1; d >= 0; --d) {
-            const bool fuse = d == 0 && !last;
+            const bool fuse = d == 0 && !last && fuse_ok;
             std::vector<MGLevelTask<T>> tasks;
             for (int k = 0; k < M; ++k) {
                 MGSolver* s = m_[k];
@@ -1265,9 +1273,9 @@ int MGBatch::build_step(const std::vector<double>& h) {
                 return rc;
             }
         }
-        // the next cycle's finest up stage reads what this one wrote
+        // after a fused stage the next cycle's finest up stage reads what it wrote
         for (int k = 0; k < M; ++k)
-            if (m_[k]->lc_ > 0 && !last) std::swap(pre0[k], post0[k]);
+            if (m_[k]->lc_ > 0 && !last && fuse_ok) std::swap(pre0[k], post0[k]);
     }
     for (int k = 0; k < M; ++k) final_e0_[k] = (m_[k]->lc_ > 0 && post0[k] != out0(k)) ? 1 : 0;
     return 0;

@@ -123,10 +123,11 @@ def test_mg_mixed_precision_accuracy(pic, orc, n, direct):
 
 
 @pytest.mark.parametrize("inner,split", [(2, False), (3, False), (2, True)])
-@pytest.mark.parametrize("n", [129, 513])
+@pytest.mark.parametrize("n", [129, 513, 2049])
 def test_mg_refinement_step_equals_fp64_cycles(pic, orc, n, inner, split):
     """One mixed-mode refinement step of `inner` FP32 V-cycles (the later ones
-    entered through the fused finest up/down stage) is the same linear
+    entered through the fused finest up/down stage up to 1025^2, through the
+    plain finest down stage above) is the same linear
     iteration as `inner` FP64 V-cycles, so with the same coarse treatment the
     relative residuals agree to FP32 rounding; a cycle that lost or repeated
     sweeps at the finest level would move it by a large factor.  V(1,1)
