@@ -73,16 +73,28 @@ struct StageRef {
     int gx;                // single task: tiles per row (0: one tile per block, tile.tx = blockIdx.x)
 };
 
+// The dispatcher deals workgroups to the 8 XCDs in turn (workgroup b to XCD
+// b % 8), and each XCD has its own L2.  Renumbered so that XCD x takes one
+// contiguous run of the grid's items (counts q or q + 1, in order), a tile's
+// halo rows were mostly loaded by tiles of the same XCD.  A bijection on
+// [0, gridDim.x) whatever the partition, so only the schedule depends on it.
+__device__ inline int xcd_block() {
+    const int b = blockIdx.x, nb = gridDim.x;
+    const int x = b & 7, q = nb >> 3, r = nb & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
+
 template <typename Task>
 __device__ inline void stage_item(const StageRef<Task>& a, Task& tk, MGTile& tile) {
+    const int b = xcd_block();
     if (a.tasks) {
-        tile = a.tiles[blockIdx.x];
+        tile = a.tiles[b];
         tk = a.tasks[tile.task];
     } else {
         tk = a.t0;
         tile.task = 0;
-        tile.tx = a.gx ? (int)blockIdx.x % a.gx : (int)blockIdx.x;
-        tile.ty = a.gx ? (int)blockIdx.x / a.gx : 0;
+        tile.tx = a.gx ? b % a.gx : b;
+        tile.ty = a.gx ? b / a.gx : 0;
     }
 }
 
