@@ -7,8 +7,6 @@
 //                  (main.cu:93-102, kernels.cu:26-69).
 //  efield_kernel   E = -grad(phi) into an interleaved double2 grid,
 //                  computeElectricField (kernels.cu:134-191).
-//  deposit_kernel  stand-alone Q20 CIC deposit of a flat particle array
-//                  (updateDensity without the step around it).
 #include "field.hpp"
 
 namespace pic {
@@ -27,24 +25,6 @@ __global__ __launch_bounds__(256) void efield_kernel(const double* __restrict__ 
     else if (j == ny - 1) ey = -(phi[(size_t)j * nx + i] - phi[(size_t)(j - 1) * nx + i]) / dy;
     else ey = -(phi[(size_t)(j + 1) * nx + i] - phi[(size_t)(j - 1) * nx + i]) / (2.0 * dy);
     E[(size_t)j * nx + i] = make_double2(ex, ey);
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void deposit_kernel(const T* __restrict__ x, const T* __restrict__ y, int n,
-                                                      int nx, int ny, unsigned long long* acc) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const T xp = x[p], yp = y[p];
-    const int ix = (int)floor(xp), iy = (int)floor(yp);
-    if (ix < 0 || ix >= nx - 1 || iy < 0 || iy >= ny - 1) return;
-    const T fx = xp - (T)ix, fy = yp - (T)iy;
-    const unsigned long long W = 1ull << PIC_QBITS;
-    const unsigned long long qx = (unsigned)(fx * (T)W), qy = (unsigned)(fy * (T)W);
-    unsigned long long* b = acc + (size_t)iy * nx + ix;
-    atomicAdd(b, (W - qx) * (W - qy));
-    atomicAdd(b + 1, qx * (W - qy));
-    atomicAdd(b + nx, (W - qx) * qy);
-    atomicAdd(b + nx + 1, qx * qy);
 }
 
 __global__ __launch_bounds__(256) void charge_batch_kernel(const ChargeArgs* __restrict__ args,
@@ -112,19 +92,6 @@ __global__ __launch_bounds__(256) void sumsq_kernel(const double* __restrict__ a
 int launch_sumsq(const double* a, size_t n, double* partial, hipStream_t s) {
     if (n == 0) return 0;
     hipLaunchKernelGGL(sumsq_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, n, partial);
-    PIC_HIP(hipGetLastError());
-    return 0;
-}
-
-int launch_deposit(int prec, const void* x, const void* y, int n, int nx, int ny, unsigned long long* acc,
-                   hipStream_t s) {
-    if (n <= 0) return 0;
-    if (prec == 4)
-        hipLaunchKernelGGL(deposit_kernel<float>, dim3(div_up(n, 256)), dim3(256), 0, s, (const float*)x,
-                           (const float*)y, n, nx, ny, acc);
-    else
-        hipLaunchKernelGGL(deposit_kernel<double>, dim3(div_up(n, 256)), dim3(256), 0, s, (const double*)x,
-                           (const double*)y, n, nx, ny, acc);
     PIC_HIP(hipGetLastError());
     return 0;
 }

@@ -38,7 +38,5 @@ int launch_efield(const double* phi, double2* E, int nx, int ny, double dx, doub
 int launch_zero_interior(double* f, int n, hipStream_t s);
 // partial[c] = sum of a[i]^2 over the 256-value chunk c, c < ceil(n / 256)
 int launch_sumsq(const double* a, size_t n, double* partial, hipStream_t s);
-int launch_deposit(int prec, const void* x, const void* y, int n, int nx, int ny, unsigned long long* acc,
-                   hipStream_t s);
 
 }  // namespace pic
