@@ -425,55 +425,65 @@ __device__ inline void lds_sweeps(T* e, const T* r, int n, T h, T omega, int ite
 //   W[p][q] = (2/(m+1))^2 / (-4 (sin^2(pi (p+1) / (2(m+1))) + sin^2(pi (q+1) / (2(m+1))))),
 // since S^2 = (m+1)/2 I and the columns of S are the eigenvectors of the 1D
 // second difference.  tab holds S then W, each 64 x 64 and zero outside
-// m x m.  The four 64 x 64 products run on the FP64 matrix cores
+// m x m.  The four products run on the FP64 matrix cores
 // (v_mfma_f64_16x16x4f64: A[i][k] in lane i + 16k, B[k][j] in lane j + 16k,
-// D[i][j] in lane j + 16 (i % 4), element i / 4).  Wavefront w = 4 rb + cb
-// keeps column block cb of S for all k in registers; S is symmetric, so the
-// same registers are the A operand of S's row block cb.  Products with S on
-// the right give tile (rb, cb), products with S on the left tile (cb, rb).
-// X and Y hold m x m FP64 values with row stride MG_DS in LDS.
+// D[i][j] in lane j + 16 (i % 4), element i / 4), halved by the mirror
+// symmetry S[a][m-1-b] = (-1)^a S[a][b] (and S symmetric): with S on the
+// right, P = X S has P[i][b] = E + O and P[i][m-1-b] = E - O, where E and O
+// sum over the even and the odd k only, for the (m+1)/2 columns b < (m+1)/2;
+// with S on the left the same holds for the rows.  Wavefront w < 8 computes
+// E and O of one 16 x 16 tile: rb = w / 2 the block of the LDS operand's free
+// index, cb = w % 2 the block of the halved index, S's entries at the even and
+// the odd k of column block cb in registers (S symmetric: the same registers
+// are the A operand when S is on the left).  X and Y hold m x m FP64 values
+// with row stride MG_DS in LDS.
 typedef double mg_v4d __attribute__((ext_vector_type(4)));
 
-// One 16 x 16 tile of a product over the full k range 0..63: the LDS operand
-// is the A side (LDS_LEFT: A[base + i][k]) or the B side (B[k][base + j]), the
-// register block F the other.  All 16 LDS values are loaded first (entries
-// beyond m read a clamped in-range address and are replaced by zero), then the
-// MFMAs of the k steps below m run on two accumulators.
+// E and O of one 16 x 16 tile: the LDS operand is the A side (LDS_LEFT:
+// A[base + i][k]) or the B side (B[k][base + j]), Fe / Fo the other at
+// k = 8 kb + 2 lk (+ 1).  All 16 LDS values are loaded first (entries beyond m
+// read a clamped in-range address and are replaced by zero), then the two
+// MFMA chains run side by side.
 template <bool LDS_LEFT>
-__device__ inline mg_v4d tile_product(const double* A, int base, const double (&F)[16], int m, int li, int lk) {
+__device__ inline void pair_product(const double* A, int base, const double (&Fe)[8], const double (&Fo)[8], int m,
+                                    int li, int lk, mg_v4d& E, mg_v4d& O) {
     double v[16];
 #pragma unroll
-    for (int kb = 0; kb < 16; ++kb) {
-        const int row = LDS_LEFT ? base + li : kb * 4 + lk;
-        const int col = LDS_LEFT ? kb * 4 + lk : base + li;
+    for (int q = 0; q < 16; ++q) {
+        const int k = 8 * (q >> 1) + 2 * lk + (q & 1);
+        const int row = LDS_LEFT ? base + li : k;
+        const int col = LDS_LEFT ? k : base + li;
         const double x = A[min(row, m - 1) * MG_DS + min(col, m - 1)];
-        v[kb] = (row < m && col < m) ? x : 0.0;
+        v[q] = (row < m && col < m) ? x : 0.0;
     }
-    mg_v4d a0 = {0.0, 0.0, 0.0, 0.0}, a1 = {0.0, 0.0, 0.0, 0.0};
+    E = mg_v4d{0.0, 0.0, 0.0, 0.0};
+    O = mg_v4d{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int kb = 0; kb < 16; kb += 2) {
-        if (kb * 4 >= m) break;   // the rest is zero padding
+    for (int kb = 0; kb < 8; ++kb) {
+        if (kb * 8 >= m) break;   // k >= 8 kb >= m: zero padding
         if (LDS_LEFT) {
-            a0 = __builtin_amdgcn_mfma_f64_16x16x4f64(v[kb], F[kb], a0, 0, 0, 0);
-            a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(v[kb + 1], F[kb + 1], a1, 0, 0, 0);
+            E = __builtin_amdgcn_mfma_f64_16x16x4f64(v[2 * kb], Fe[kb], E, 0, 0, 0);
+            O = __builtin_amdgcn_mfma_f64_16x16x4f64(v[2 * kb + 1], Fo[kb], O, 0, 0, 0);
         } else {
-            a0 = __builtin_amdgcn_mfma_f64_16x16x4f64(F[kb], v[kb], a0, 0, 0, 0);
-            a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(F[kb + 1], v[kb + 1], a1, 0, 0, 0);
+            E = __builtin_amdgcn_mfma_f64_16x16x4f64(Fe[kb], v[2 * kb], E, 0, 0, 0);
+            O = __builtin_amdgcn_mfma_f64_16x16x4f64(Fo[kb], v[2 * kb + 1], O, 0, 0, 0);
         }
     }
-    return a0 + a1;
 }
 
 template <typename T>
 __device__ void coarse_direct_solve(const MGCoarseTask<T>& tk, double* X, double* Y) {
-    const int n = tk.n_top, m = n - 2;
+    const int n = tk.n_top, m = n - 2, hm = (m + 1) / 2;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int rb = wave >> 2, cb = wave & 3;
+    const int rb = (wave >> 1) & 3, cb = wave & 1;
     const int li = lane & 15, lk = lane >> 4;
-    const bool live = rb * 16 < m && cb * 16 < m;   // tiles beyond m stay zero
+    const bool live = wave < 8 && rb * 16 < m && cb * 16 < hm;   // tiles beyond stay zero
     const gptr<const double> tab = G(tk.dtab);
     const gptr<const T> r = G(tk.rhs);
     const gptr<T> e = G(tk.e_out);
+    // the gate (the case converged: a no-op) is read with the data loads and
+    // tested once they are issued
+    const int stop = tk.gate ? *G(&tk.gate->converged) : 0;
     // R into X (m*m <= 3969 < 4 * 1024): all loads first, from clamped indices
     float rv[4];
 #pragma unroll
@@ -482,14 +492,24 @@ __device__ void coarse_direct_solve(const MGCoarseTask<T>& tk, double* X, double
         const int j = t / m, k = t - j * m;
         rv[u] = (float)r[(j + 1) * n + (k + 1)];
     }
-    double F[16];
-#pragma unroll
-    for (int kb = 0; kb < 16; ++kb) F[kb] = tab[(kb * 4 + lk) * 64 + cb * 16 + li];
-    // the scale of the transposed tile (cb, rb): h^2 W[cb*16 + i][rb*16 + j]
+    // S at the even / odd k of column block cb; h^2 W at the output rows
+    // a = cb*16 + 4q + lk and m-1-a (product 2), column rb*16 + li
+    double Fe[8], Fo[8], wsc[4], wsm[4];
     const double h2 = (double)tk.h_top * (double)tk.h_top;
-    double wsc[4];
+    if (wave < 8) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) wsc[q] = tab[4096 + (cb * 16 + 4 * q + lk) * 64 + rb * 16 + li] * h2;
+        for (int kb = 0; kb < 8; ++kb) {
+            Fe[kb] = tab[(8 * kb + 2 * lk) * 64 + cb * 16 + li];
+            Fo[kb] = tab[(8 * kb + 2 * lk + 1) * 64 + cb * 16 + li];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int aa = cb * 16 + 4 * q + lk, am = max(m - 1 - aa, 0);
+            wsc[q] = tab[4096 + aa * 64 + rb * 16 + li] * h2;
+            wsm[q] = tab[4096 + am * 64 + rb * 16 + li] * h2;
+        }
+    }
+    if (stop) return;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
         const int t = threadIdx.x + 1024 * u;
@@ -497,44 +517,56 @@ __device__ void coarse_direct_solve(const MGCoarseTask<T>& tk, double* X, double
         if (t < m * m) X[j * MG_DS + k] = (double)rv[u];
     }
     __syncthreads();
-    mg_v4d acc;
-    // T1 = R S, tile (rb, cb) -> Y
+    mg_v4d Ev, Ov;
+    // T1 = R S -> Y: rows i = rb*16 + 4q + lk, columns b = cb*16 + li and m-1-b
     if (live) {
-        acc = tile_product<true>(X, rb * 16, F, m, li, lk);
+        pair_product<true>(X, rb * 16, Fe, Fo, m, li, lk, Ev, Ov);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int i = rb * 16 + 4 * q + lk, j = cb * 16 + li;
-            if (i < m && j < m) Y[i * MG_DS + j] = acc[q];
+            const int i = rb * 16 + 4 * q + lk, b = cb * 16 + li;
+            if (i < m && b < hm) {
+                Y[i * MG_DS + b] = Ev[q] + Ov[q];
+                if (m - 1 - b != b) Y[i * MG_DS + (m - 1 - b)] = Ev[q] - Ov[q];
+            }
         }
     }
     __syncthreads();
-    // T2 = S T1, tile (cb, rb), times h^2 W -> X
+    // T2 = S T1, times h^2 W -> X: rows a = cb*16 + 4q + lk and m-1-a, columns j = rb*16 + li
     if (live) {
-        acc = tile_product<false>(Y, rb * 16, F, m, li, lk);
+        pair_product<false>(Y, rb * 16, Fe, Fo, m, li, lk, Ev, Ov);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int i = cb * 16 + 4 * q + lk, j = rb * 16 + li;
-            if (i < m && j < m) X[i * MG_DS + j] = acc[q] * wsc[q];
+            const int a = cb * 16 + 4 * q + lk, j = rb * 16 + li;
+            if (a < hm && j < m) {
+                X[a * MG_DS + j] = (Ev[q] + Ov[q]) * wsc[q];
+                if (m - 1 - a != a) X[(m - 1 - a) * MG_DS + j] = (Ev[q] - Ov[q]) * wsm[q];
+            }
         }
     }
     __syncthreads();
-    // T3 = T2 S, tile (rb, cb) -> Y
+    // T3 = T2 S -> Y, as T1
     if (live) {
-        acc = tile_product<true>(X, rb * 16, F, m, li, lk);
+        pair_product<true>(X, rb * 16, Fe, Fo, m, li, lk, Ev, Ov);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int i = rb * 16 + 4 * q + lk, j = cb * 16 + li;
-            if (i < m && j < m) Y[i * MG_DS + j] = acc[q];
+            const int i = rb * 16 + 4 * q + lk, b = cb * 16 + li;
+            if (i < m && b < hm) {
+                Y[i * MG_DS + b] = Ev[q] + Ov[q];
+                if (m - 1 - b != b) Y[i * MG_DS + (m - 1 - b)] = Ev[q] - Ov[q];
+            }
         }
     }
     __syncthreads();
-    // E = S T3, tile (cb, rb): E[i][j] is node (j + 1, i + 1)
+    // E = S T3, as T2 unscaled: E[a][j] is node (j + 1, a + 1)
     if (live) {
-        acc = tile_product<false>(Y, rb * 16, F, m, li, lk);
+        pair_product<false>(Y, rb * 16, Fe, Fo, m, li, lk, Ev, Ov);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int i = cb * 16 + 4 * q + lk, j = rb * 16 + li;
-            if (i < m && j < m) e[(i + 1) * n + (j + 1)] = (T)acc[q];
+            const int a = cb * 16 + 4 * q + lk, j = rb * 16 + li;
+            if (a < hm && j < m) {
+                e[(a + 1) * n + (j + 1)] = (T)(Ev[q] + Ov[q]);
+                if (m - 1 - a != a) e[(m - a) * n + (j + 1)] = (T)(Ev[q] - Ov[q]);
+            }
         }
     }
     // the boundary of the correction is zero
@@ -550,15 +582,15 @@ template <typename T, bool MASKED>
 __global__ __launch_bounds__(1024) void mg_coarse_stage(const StageRef<MGCoarseTask<T>> a) {
     MGCoarseTask<T> tk;
     stage_case(a, tk);
-    if (tk.gate && *G(&tk.gate->converged)) return;
     extern __shared__ __align__(16) unsigned char smem_mg[];
     if constexpr (sizeof(T) == 4) {
-        if (tk.dtab) {
+        if (tk.dtab) {   // tests the gate itself
             double* X = reinterpret_cast<double*>(smem_mg);
             coarse_direct_solve(tk, X, X + MG_DS * MG_DS);
             return;
         }
     }
+    if (tk.gate && *G(&tk.gate->converged)) return;
     const gptr<const T> e_in = G(tk.e_in);
     const gptr<const T> rhs_top = G(tk.rhs);
     const gptr<T> e_out = G(tk.e_out);
