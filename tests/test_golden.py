@@ -34,9 +34,13 @@ def test_oracle_reproduces_golden(orc, prec):
 def test_golden_fixture_sanity():
     ref = _load(4)
     c = ref["s0_counters"]
+    c1 = ref["s1_counters"]
     assert c[0] > 0 and c[2] > 0 and c[4] > 0       # absorbed, elastic, ionization
-    assert c[6] == c[7]                             # one born electron per born ion (none absorbed)
-    assert len(ref["s0_id"]) == 300 - c[0] + c[6] - (c[8] // 2 if c[8] else 0) or True
+    assert c[6] == c[7] == c[4]                     # one born electron and ion per ionization (none absorbed)
+    # particle balance: start - absorbed + born of the species that survived its birth pass
+    assert len(ref["s0_id"]) == 300 - c[0] + c[6]
+    assert len(ref["s1_id"]) == 300 - c1[0] + c[7]
+    assert len(np.unique(ref["s0_id"])) == len(ref["s0_id"])
 
 
 @pytest.mark.gpu

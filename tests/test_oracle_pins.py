@@ -642,3 +642,159 @@ def test_mg_embedded_electrode_matches_direct_solve(orc, n):
     fixed = np.where(disc, 1.0, np.where(block, -0.5, 0.0))
     ref = _masked_dense(rhs, mask, fixed, h)
     assert np.max(np.abs(phi - ref)) < 1e-9 * max(1.0, np.max(np.abs(ref)))
+
+
+# ------------------------------------------------- gather weights, selection --
+@pytest.mark.parametrize("prec", [4, 8])
+def test_push_gather_quadratic_potential_closed_form(orc, prec):
+    """CIC gather (kernels.cu:106-127) against a closed form that sees every
+    weight and node.  phi = a X^2 + b Y^2 + c X Y (X = i dx, Y = j dy) has
+    central differences equal to its gradient at every interior node, so the
+    node field is E = -(2a X + c Y, 2b Y + c X): linear in x and y, which
+    bilinear interpolation reproduces exactly anywhere inside a cell of
+    interior nodes.  Swapped weights or nodes (w10 <-> w01, a node one row
+    off), a missing term or a transposed grid give a different E wherever
+    fx != fy.  dx != dy and nx != ny catch swapped spacings and axes."""
+    nx, ny = 33, 25
+    dx, dy = 0.013, 0.007
+    a, b, c = 3.0e3, -1.7e3, 2.3e3
+    j, i = np.mgrid[0:ny, 0:nx]
+    X, Y = i * dx, j * dy
+    phi = a * X * X + b * Y * Y + c * X * Y
+    rng = np.random.default_rng(81)
+    n = 4000
+    # cells whose four nodes are interior: ix in [1, nx-3], iy in [1, ny-3]
+    x = rng.uniform(1.0, nx - 2.0, n)
+    y = rng.uniform(1.0, ny - 2.0, n)
+    vx0 = rng.normal(size=n)
+    vy0 = rng.normal(size=n)
+    kq = 1e-4
+    sp = orc.Species(prec, x, y, vx0, vy0, np.arange(n, dtype=np.uint32), kq)
+    xs, ys, vxs, vys = sp.x.astype(np.float64), sp.y.astype(np.float64), sp.vx.copy(), sp.vy.copy()
+    orc.push(sp, phi, nx, ny, dx, dy, 0.0, 0.0, dict(kick=True), 0, 1, 0, 0, box=False)
+    o = np.argsort(sp.id)
+    Ex = -(2 * a * xs * dx + c * ys * dy)
+    Ey = -(2 * b * ys * dy + c * xs * dx)
+    scale = kq * max(np.abs(Ex).max(), np.abs(Ey).max())
+    tol = 2e-6 if prec == 4 else 1e-11
+    assert np.max(np.abs(sp.vx[o] - (vxs + kq * Ex))) < tol * max(scale, 1.0)
+    assert np.max(np.abs(sp.vy[o] - (vys + kq * Ey))) < tol * max(scale, 1.0)
+    # the field varies by far more than the tolerance across a cell
+    assert kq * 2 * abs(a) * dx > 100 * tol * max(scale, 1.0)
+
+
+def _mcc_tables(K, emax, procs, below=0.0):
+    """procs: list of (type, threshold eV, n*sigma) with constant sigma above
+    threshold and `below` times that under it"""
+    return lambda Eg: np.stack([np.where(Eg >= thr, ns, below * ns) for _, thr, ns in procs])
+
+
+def test_mcc_process_selection_frequencies(orc):
+    """Null-collision selection among several processes (the build's
+    replacement of collision_kernel, mcc.cu:56-85, for a set of channels):
+    a candidate with R = u1 nu_max takes the first process whose cumulative
+    rate exceeds R, so process k is chosen with probability
+    p_null * nu_k(E) / nu_max, and a process above the particle's energy
+    never.  Checked within binomial bounds for electrons above both inelastic
+    thresholds and between them.  The tables hold the same sigma below the
+    thresholds too, so only the threshold test keeps a process out.  A
+    non-cumulative test (R < nu_k), reusing the candidacy uniform, or
+    ignoring thresholds each fails here."""
+    QE, ME = 1.602176634e-19, 9.1093837015e-31
+    K, emax, dt = 256, 100.0, 2e-9
+    procs = [(0, 0.0, 1e21 * 1e-19), (1, 11.5, 1e21 * 3e-20), (2, 15.8, 1e21 * 5e-20)]
+    m = _mcc_const(orc, 1e21, 0, ME, dt, K=K, emax=emax, types=[p[0] for p in procs],
+                   thr=[p[1] for p in procs], extra=_mcc_tables(K, emax, procs, below=1.0))
+    for e0, active in ((40.0, (0, 1, 2)), (13.0, (0, 1))):
+        n = 300000
+        v = math.sqrt(2 * e0 * QE / ME)
+        sp = orc.Species(8, np.full(n, 8.0), np.full(n, 8.0), np.full(n, v), np.zeros(n),
+                         np.arange(n, dtype=np.uint32), 0.0, mcc=m, other=1)
+        orc.push(sp, np.zeros((17, 17)), 17, 17, 1.0, 1.0, 0.0, 0.0, dict(mcc=True), 4, 21, 0, 0)
+        cand = int(sp.counters[orc.C_NULL_CAND])
+        assert abs(cand - n * m["p_null"]) < 5 * math.sqrt(n * m["p_null"])
+        for k, (_, thr, ns) in enumerate(procs):
+            hits = int(sp.counters[orc.C_PROC0 + k])
+            if k not in active:
+                assert hits == 0
+                continue
+            p = m["p_null"] * ns * v / m["nu_max"]
+            assert abs(hits - n * p) < 5 * math.sqrt(n * p * (1 - p)), (e0, k, hits, n * p)
+            assert hits > 1000
+
+
+def _gauss_moments_ok(vx, vy, sigma):
+    """mean 0, variance sigma^2, kurtosis 3 per component and uncorrelated
+    components, each within 5 standard errors"""
+    n = len(vx)
+    for a in (vx, vy):
+        assert abs(a.mean()) < 5 * sigma / math.sqrt(n)
+        var = a.var()
+        assert abs(var / sigma**2 - 1) < 5 * math.sqrt(2.0 / n)
+        kurt = np.mean((a - a.mean()) ** 4) / var**2
+        assert abs(kurt - 3.0) < 5 * math.sqrt(24.0 / n)
+    assert abs(np.corrcoef(vx, vy)[0, 1]) < 5 / math.sqrt(n)
+
+
+def test_mcc_charge_exchange_takes_gas_velocity(orc):
+    """Ar+ charge exchange: the ion takes the velocity of a gas atom, a 2D
+    Maxwellian of thermal speed gas_vth per component (Gaussian with mean 0,
+    variance gas_vth^2, kurtosis 3, independent components), whatever its
+    velocity before."""
+    QE, MAR = 1.602176634e-19, 6.6335209e-26
+    K, emax, dt = 64, 100.0, 1e-7
+    m = _mcc_const(orc, 1e21, 5e-19, MAR, dt, K=K, emax=emax, types=(3,), thr=(0.0,))
+    assert m["gas_vth"] == 300.0
+    n = 200000
+    v0 = 20000.0
+    sp = orc.Species(8, np.full(n, 8.0), np.full(n, 8.0), np.full(n, v0), np.zeros(n),
+                     np.arange(n, dtype=np.uint32), 0.0, mcc=m)
+    orc.push(sp, np.zeros((17, 17)), 17, 17, 1.0, 1.0, 0.0, 0.0, dict(mcc=True), 2, 17, 0, 1)
+    hit = sp.vx != v0
+    hits = int(sp.counters[orc.C_PROC0])
+    assert hit.sum() == hits and hits > 50000
+    _gauss_moments_ok(sp.vx[hit], sp.vy[hit], 300.0)
+    assert np.all(sp.vy[~hit] == 0.0)
+
+
+def test_mcc_ionization_products(orc):
+    """Ionization (the north-star channel beyond mcc.cu:88-112): of the energy
+    left after the threshold, E - thr, the primary keeps a uniform fraction
+    and the secondary electron the rest; both leave isotropically, and the
+    born ion takes a gas-atom velocity (Gaussian, gas_vth per component)."""
+    QE, ME = 1.602176634e-19, 9.1093837015e-31
+    K, emax, dt, thr = 256, 100.0, 3e-9, 15.8
+    procs = [(2, thr, 1e21 * 6e-20)]
+    m = _mcc_const(orc, 1e21, 0, ME, dt, K=K, emax=emax, types=(2,), thr=(thr,),
+                   extra=_mcc_tables(K, emax, procs))
+    n = 150000
+    e0 = 90.0
+    v = math.sqrt(2 * e0 * QE / ME)
+    sp = orc.Species(8, np.full(n, 8.0), np.full(n, 8.0), np.full(n, v), np.zeros(n),
+                     np.arange(n, dtype=np.uint32), 0.0, mcc=m, other=1)
+    born = orc.push(sp, np.zeros((17, 17)), 17, 17, 1.0, 1.0, 0.0, 0.0, dict(mcc=True), 1, 5, 0, 0)
+    n_ion = int(sp.counters[orc.C_PROC0])
+    assert n_ion > 40000 and len(born[0]) == n_ion and sp.n == n + n_ion
+    er = e0 - thr
+    prim = m["efac"] * (sp.vx[:n] ** 2 + sp.vy[:n] ** 2)
+    ionized = ~np.isclose(prim, e0, rtol=1e-12)
+    assert ionized.sum() == n_ion
+    frac = np.sort(prim[ionized] / er)
+    # Kolmogorov-Smirnov distance to U(0, 1); 1.95/sqrt(n) is the 0.1 % level
+    cdf = np.arange(1, n_ion + 1) / n_ion
+    d = max(np.max(cdf - frac), np.max(frac - (cdf - 1.0 / n_ion)))
+    assert d < 1.95 / math.sqrt(n_ion)
+    # the secondary takes the rest, in the order the primaries were ionized
+    sec = m["efac"] * (sp.vx[n:] ** 2 + sp.vy[n:] ** 2)
+    assert np.allclose(prim[ionized] + sec, er, rtol=1e-12, atol=1e-12 * er)
+    # directions of primary and secondary: uniform angles, independent of each other
+    for vx_, vy_ in ((sp.vx[:n][ionized], sp.vy[:n][ionized]), (sp.vx[n:], sp.vy[n:])):
+        ang = np.arctan2(vy_, vx_)
+        hist, _ = np.histogram(ang, bins=12, range=(-np.pi, np.pi))
+        assert np.all(np.abs(hist - n_ion / 12) < 5 * np.sqrt(n_ion / 12))
+    a1 = np.arctan2(sp.vy[:n][ionized], sp.vx[:n][ionized])
+    a2 = np.arctan2(sp.vy[n:], sp.vx[n:])
+    assert abs(np.corrcoef(np.cos(a1), np.cos(a2))[0, 1]) < 5 / math.sqrt(n_ion)
+    # born ions: gas-atom velocities
+    _gauss_moments_ok(born[2], born[3], 300.0)
+    assert np.all(born[0] == 8.0) and np.all(born[1] == 8.0)
