@@ -148,29 +148,6 @@ def poisson(n=2049, cpu=True):
     return out
 
 
-# cells per side with a multigrid ladder down to 3x3 or 4x4 nodes (32*2^k or
-# 32*3*2^k): the log-uniform draws are rounded to the nearest in log space
-GRID_SIZES = (64, 96, 128, 192, 256, 384, 512, 768, 1024)
-
-
-def round_cells(c, lo, hi):
-    import numpy as np
-    sizes = [s for s in GRID_SIZES if lo <= s <= hi]
-    return min(sizes, key=lambda s: abs(np.log(s) - np.log(c)))
-
-
-def ensemble_cases(n_cases=64, seed=4):
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    cases = []
-    for k in range(n_cases):
-        cells = round_cells(np.exp(rng.uniform(np.log(64), np.log(512))), 64, 512)
-        npart = int(min(4e6, max(1e4, 100 * cells * cells)))
-        te = float(rng.uniform(1.0, 5.0))
-        cases.append((cells, npart, te, 1000 + k))
-    return cases
-
-
 def ensemble(n_cases=64, steps=100, sequential=True):
     """Batched ensemble: 64 plasmas, cells log-uniform 64..512 (rounded to GRID_SIZES),
     clamp(100 cells^2, 1e4, 4e6) particles per species, Te ~ U(1,5) eV, MCC on;
@@ -178,7 +155,7 @@ def ensemble(n_cases=64, steps=100, sequential=True):
     another through the single-plasma path."""
     import pic_cuda_amd as pic
     from pic_cuda_amd import configs
-    specs = ensemble_cases(n_cases)
+    specs = configs.ensemble_cases(n_cases)
     res = {"cases": n_cases, "steps": steps,
            "particles": int(sum(2 * s[1] for s in specs))}
     if sequential:
@@ -357,14 +334,11 @@ def serve(n_jobs=24, rate=20.0, budget=200e9, seed=9, max_steps=2000):
     seed = job id, admission by an HBM budget, continuous batching."""
     import numpy as np
     import pic_cuda_amd as pic
-    rng = np.random.default_rng(seed)
-    t_arr = np.cumsum(rng.exponential(1.0 / rate, n_jobs))
+    from pic_cuda_amd import configs
+    jobs = configs.serve_jobs(n_jobs, rate, seed, max_steps=max_steps)
+    t_arr = [j["arrival_s"] for j in jobs]
     sv = pic.Server(budget, 256, pic.PREC_FP32)
-    ids = []
-    for k in range(n_jobs):
-        cells = round_cells(np.exp(rng.uniform(np.log(64), np.log(1024))), 64, 1024)
-        steps = int(rng.integers(200, max_steps + 1))
-        ids.append(sv.submit(cells, 100.0, steps, seed=k, arrival_s=float(t_arr[k])))
+    ids = [sv.submit(**j) for j in jobs]
     t0 = time.perf_counter()
     ticks = 0
     while True:

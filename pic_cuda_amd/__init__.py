@@ -828,6 +828,11 @@ class Ensemble:
     def step(self, n=1):
         _check(load().pic_ensemble_step(self._h, int(n)))
 
+    def size(self):
+        n = _i()
+        _check(load().pic_ensemble_size(self._h, ctypes.byref(n)))
+        return n.value
+
     def use_graph(self, on=True):
         _check(load().pic_ensemble_use_graph(self._h, 1 if on else 0))
 

@@ -4,6 +4,10 @@
 ``full_step``    1024^2 cells, 2e7 e- + 2e7 Ar+, null-collision MCC on an Ar
                  background of 1e21 m^-3, FP32 particles, 1000 steps.
 ``poisson``      2049^2 Dirichlet grid with 200 Gaussian charge blobs + noise.
+``ensemble_cases``  64 plasmas, cells log-uniform 64..512, 100 particles per
+                 cell and species (clamped to 1e4..4e6), Te ~ U(1, 5) eV, MCC on.
+``serve_jobs``   a Poisson-arrival stream of jobs, cells log-uniform 64..1024,
+                 100 particles per cell and species, steps U(200, 2000).
 
 Plasma: n = 1e15 m^-3, Te = 2 eV, Ti = 0.03 eV, dx = 0.5 Debye lengths,
 dt = 0.2 / omega_pe, grounded absorbing walls on the four box edges.
@@ -131,3 +135,42 @@ def poisson_rhs(n=2049, blobs=200, seed=1, noise=1e-3):
     rhs += rng.uniform(-noise, noise, (n, n))
     rhs[0, :] = rhs[-1, :] = rhs[:, 0] = rhs[:, -1] = 0.0
     return rhs
+
+
+# cells per side with a multigrid ladder down to 3x3 or 4x4 nodes (32*2^k or
+# 32*3*2^k): log-uniform draws are rounded to the nearest in log space
+GRID_SIZES = (64, 96, 128, 192, 256, 384, 512, 768, 1024)
+
+
+def round_cells(c, lo, hi):
+    sizes = [s for s in GRID_SIZES if lo <= s <= hi]
+    return min(sizes, key=lambda s: abs(np.log(s) - np.log(c)))
+
+
+def ensemble_cases(n_cases=64, seed=4):
+    """(cells, particles per species, Te eV, seed) of the batched ensemble:
+    cells log-uniform in [64, 512] (rounded to GRID_SIZES), clamp(100 cells^2,
+    1e4, 4e6) particles per species, Te ~ U(1, 5) eV, seed 1000 + case."""
+    rng = np.random.default_rng(seed)
+    cases = []
+    for k in range(n_cases):
+        cells = round_cells(np.exp(rng.uniform(np.log(64), np.log(512))), 64, 512)
+        npart = int(min(4e6, max(1e4, 100 * cells * cells)))
+        te = float(rng.uniform(1.0, 5.0))
+        cases.append((cells, npart, te, 1000 + k))
+    return cases
+
+
+def serve_jobs(n_jobs=24, rate=20.0, seed=9, min_steps=200, max_steps=2000):
+    """Job descriptions of the serving stream: Poisson arrivals at `rate` per
+    second, cells log-uniform in [64, 1024] (rounded to GRID_SIZES), 100
+    particles per cell and species, steps U(min_steps, max_steps), seed = job
+    index.  Dicts of pic_cuda_amd.Server.submit arguments."""
+    rng = np.random.default_rng(seed)
+    t_arr = np.cumsum(rng.exponential(1.0 / rate, n_jobs))
+    jobs = []
+    for k in range(n_jobs):
+        cells = round_cells(np.exp(rng.uniform(np.log(64), np.log(1024))), 64, 1024)
+        steps = int(rng.integers(min_steps, max_steps + 1))
+        jobs.append(dict(cells=cells, ppc=100.0, steps=steps, seed=k, arrival_s=float(t_arr[k])))
+    return jobs

@@ -1496,6 +1496,21 @@ extern "C" int pic_mg_set_level_sweeps(pic_mg* m, int n, const int* pre, const i
     return 0;
 }
 
+namespace pic {
+// The spill fill a step leaves (its slot counter, which keeps counting past the
+// capacity): above the capacity, particles of the species found no slot and
+// were dropped, so the step reports PIC_ERR_CAPACITY.
+int spill_status(const SpeciesStore& ss, int fill, int species) {
+    const int cap = ss.h_cap[ss.nb];
+    if (fill > cap) {
+        set_error("spill region full: " + std::to_string((long long)fill - cap) + " particles of species " +
+                  std::to_string(species) + " were lost in the last step (raise max_particles or re-bin more often)");
+        return PIC_ERR_CAPACITY;
+    }
+    return 0;
+}
+}  // namespace pic
+
 // One step as a graph launch.  The graph is captured again when the layout
 // (work items, LDS size), the pass flags or the cycle count change.
 template <typename T>
@@ -1570,6 +1585,7 @@ static int step_t(pic_sim* S, int n) {
             for (int s = 0; s < (int)p->sp.size(); ++s) {
                 const SpeciesStore& ss = p->sp[s];
                 if (S->h_status[s] > ss.h_cap[ss.nb] / 2) S->need_rebin = true;
+                if (int rc = spill_status(ss, S->h_status[s], s)) return rc;
             }
             continue;
         }
@@ -1600,6 +1616,7 @@ static int step_t(pic_sim* S, int n) {
         for (int s = 0; s < (int)p->sp.size(); ++s) {
             const SpeciesStore& ss = p->sp[s];
             if (S->h_status[s] > ss.h_cap[ss.nb] / 2) S->need_rebin = true;
+            if (int rc = spill_status(ss, S->h_status[s], s)) return rc;
         }
     }
     return 0;
@@ -1880,14 +1897,21 @@ static int elastic_t(pic_particles* p, int species, double density, double dt, i
                                     std::vector<double>(sig, sig + n)});
     MccBuild mb;
     if (int rc = build_mcc(&xs, p->sp, species, dt, mb)) return rc;
-    CaseDev<T>& C = hcase<T>(p);
-    const CaseDev<T> saved = C;
-    for (int s = 0; s < (int)p->sp.size(); ++s) C.sp[s].n_proc = 0;
+    // the table goes to the device before the descriptor changes, so a failure
+    // here leaves the store as it was
     std::vector<T> tt(mb.tab.size());
     for (size_t i = 0; i < tt.size(); ++i) tt[i] = (T)mb.tab[i];
     T* dtab = nullptr;
     PIC_HIP(hipMalloc(&dtab, tt.size() * sizeof(T)));
-    PIC_HIP(hipMemcpy(dtab, tt.data(), tt.size() * sizeof(T), hipMemcpyHostToDevice));
+    const hipError_t ec = hipMemcpy(dtab, tt.data(), tt.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (ec != hipSuccess) {
+        (void)hipFree(dtab);
+        set_error(std::string("elastic collisions: table upload: ") + hipGetErrorString(ec));
+        return PIC_ERR_HIP;
+    }
+    CaseDev<T>& C = hcase<T>(p);
+    const CaseDev<T> saved = C;
+    for (int s = 0; s < (int)p->sp.size(); ++s) C.sp[s].n_proc = 0;
     SpeciesDev<T>& D = C.sp[species];
     D.n_proc = 1;
     D.K = mb.info.K;
@@ -1915,9 +1939,15 @@ static int elastic_t(pic_particles* p, int species, double density, double dt, i
                             (long long)step, lds, lib_stream(), C.n_walls > 0);
     if (!rc) rc = launch_end_step<T>((CaseDev<T>*)p->d_case, p->d_state, 1, 0, lib_stream());
     if (!rc) p->h_cur ^= 1;
-    (void)hipStreamSynchronize(lib_stream());
+    const hipError_t es = hipStreamSynchronize(lib_stream());
+    if (!rc && es != hipSuccess) {
+        set_error(std::string("elastic collisions: ") + hipGetErrorString(es));
+        rc = PIC_ERR_HIP;
+    }
+    // the descriptor is restored on every path, then the table freed
     C = saved;
-    int rc2 = upload_case(p);
+    const int rc2 = upload_case(p);
+    (void)hipStreamSynchronize(lib_stream());
     (void)hipFree(dtab);
     return rc ? rc : rc2;
 }

@@ -278,6 +278,7 @@ static int ens_step_t(pic_ensemble* e) {
         if (int rc = enqueue_body<T>(e, 0)) return rc;
     }
     for (auto* S : e->m) S->p->h_cur ^= 1;
+    int err = 0;   // the first member that lost particles; every member's state is still advanced
     for (int k = 0; k < M; ++k) {
         pic_sim* S = e->m[k];
         const MGNorm& nm = e->mg->norm(k);
@@ -289,10 +290,11 @@ static int ens_step_t(pic_ensemble* e) {
         for (int s = 0; s < (int)S->p->sp.size(); ++s) {
             const SpeciesStore& ss = S->p->sp[s];
             if (e->h_status[k * PIC_MAX_SPECIES + s] > ss.h_cap[ss.nb] / 2) S->need_rebin = true;
+            if (!err) err = spill_status(ss, e->h_status[k * PIC_MAX_SPECIES + s], s);
         }
     }
     e->steps++;
-    return 0;
+    return err;
 }
 
 extern "C" int pic_ensemble_step(pic_ensemble* e, int n_steps) {
@@ -514,7 +516,13 @@ extern "C" int pic_server_tick(pic_server* sv, double now_s, int* n_active, int*
             it = sv->waiting.erase(it);
             return rc;
         }
-        if (int rc = pic_ensemble_add(sv->ens, j.s)) return rc;
+        if (int rc = pic_ensemble_add(sv->ens, j.s)) {
+            // the job's objects were built but are not counted in `used`: free them
+            destroy_job(j);
+            j.state = -1;
+            it = sv->waiting.erase(it);
+            return rc;
+        }
         j.state = 1;
         j.t_admit = now_s;
         sv->used += j.footprint;

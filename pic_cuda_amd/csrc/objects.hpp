@@ -155,6 +155,9 @@ namespace pic {
 void items_for_species(const pic_particles* p, int s, int chunk, std::vector<WorkItem>& out, int case_idx = 0);
 int upload_case(pic_particles* p);
 int upload_items(WorkItem** dptr, int* alloc, const std::vector<WorkItem>& items);
+// PIC_ERR_CAPACITY (with the message set) when a step dropped particles of a
+// species for want of spill slots; fill is the spill slot counter after the step
+int spill_status(const SpeciesStore& ss, int fill, int species);
 }  // namespace pic
 size_t cur_lds(pic_sim* S);
 int mcc_active(pic_sim* S);

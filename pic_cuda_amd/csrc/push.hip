@@ -481,7 +481,9 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
     // LDS: deposit window (u64), E window (T2), stage (4 T + id + key), counters,
     // energy-band constants of the collisions
     unsigned long long* accw = smem_u64;
-    T2* ew_lds = reinterpret_cast<T2*>(smem_u64 + (is_bin ? aw * ah : 0));
+    // the E window starts on a 16-byte boundary (an even count of u64 entries):
+    // double2 loads from LDS need it
+    T2* ew_lds = reinterpret_cast<T2*>(smem_u64 + (is_bin ? ((aw * ah + 1) & ~1) : 0));
     T* stage_base = reinterpret_cast<T*>(ew_lds + (is_bin ? ew * eh : 0));
     // the stage is split into one part per wavefront (PIC_MSTAGE / 4 entries)
     constexpr int WST = PIC_MSTAGE / 4;

@@ -54,7 +54,7 @@ int launch_clear_out(const CaseDev<T>* cases, const CaseState* states, int n_cas
 template <typename T>
 inline size_t push_lds_bytes(int bw, int bh, int halo) {
     const size_t aw = bw + 1 + 2 * halo, ah = bh + 1 + 2 * halo;
-    return aw * ah * 8 + (size_t)(bw + 1) * (bh + 1) * 2 * sizeof(T) + (size_t)PIC_MSTAGE * (4 * sizeof(T) + 8) +
+    return ((aw * ah + 1) & ~(size_t)1) * 8 + (size_t)(bw + 1) * (bh + 1) * 2 * sizeof(T) + (size_t)PIC_MSTAGE * (4 * sizeof(T) + 8) +
            PIC_NCOUNT * 4 + 32 * sizeof(T) + (size_t)4 * PIC_MQUEUE * (4 * sizeof(T) + 4);
 }
 
