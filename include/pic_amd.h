@@ -114,6 +114,17 @@ int pic_particles_download(pic_particles* p, int species, int64_t cap, double* x
    changed. Ordered on the library stream and finished before return. */
 int pic_particles_export_positions(pic_particles* p, int species, float* dst_device, int64_t cap, int units,
                                    int64_t* n);
+/* pic_particles_export_positions without any allocation: no counterpart in
+   the reference; for callers that cannot allocate device memory (code under
+   graph capture, a render or serving loop that forbids hipMalloc).  Same
+   arguments, results and ordering guarantees as pic_particles_export_positions;
+   its work-item table and counter live in a static device array of
+   256 B + 65536 work items (1 MiB per GPU, held until the process ends), so
+   a species with more than 65536 bins + spill chunks of 4096 slots is
+   rejected with PIC_ERR_CAPACITY.  Every call on a GPU shares that array:
+   calls that overlap on one GPU (from several host threads) race. */
+int pic_particles_export_positions_static(pic_particles* p, int species, float* dst_device, int64_t cap, int units,
+                                          int64_t* n);
 /* seeded initial state on the device (replaces initializeParticles, main.cu:17-32):
    uniform in the open box [0,nx-1]x[0,ny-1], Maxwellian sigma = vth per component */
 int pic_particles_init_maxwellian(pic_particles* p, int species, int64_t n, int nx, int ny, double vth,

@@ -179,6 +179,7 @@ SIGNATURES = {
     "pic_sim_set_wall_lists": (_i, [_vp, _i, _i]),
     "pic_sim_write_frame": (_i, [_vp, ctypes.c_char_p]),
     "pic_particles_export_positions": (_i, [_vp, _i, _vp, _i64, _i, _P(_i64)]),
+    "pic_particles_export_positions_static": (_i, [_vp, _i, _vp, _i64, _i, _P(_i64)]),
     "pic_sim_wall_list_stats": (_i, [_vp, _P(_i), _P(ctypes.c_longlong), _P(_i)]),
     "pic_sim_autotune": (_i, [_vp, _i, ctypes.POINTER(TuneReport)]),
     "pic_sim_set_level_sweeps": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),

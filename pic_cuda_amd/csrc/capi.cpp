@@ -442,6 +442,81 @@ extern "C" int pic_particles_export_positions(pic_particles* p, int species, flo
                         : export_t<double>(p, species, dst_device, cap, sx, sy, n);
 }
 
+// ---------------------------------------------------------------------------
+// Position export that allocates nothing: its work-item table and counter live
+// in `scratch`, a file-scope device array (one copy per GPU, held until the
+// process ends), instead of buffers the library grows with hipMalloc.
+// ---------------------------------------------------------------------------
+#define PIC_STATIC_EXPORT_ITEMS 65536
+#define PIC_STATIC_MAX_DEVICES 64
+static __device__ __align__(256) unsigned char scratch[256 + PIC_STATIC_EXPORT_ITEMS * sizeof(WorkItem)];
+
+// the device address of this GPU's copy of scratch, looked up once per device
+// with that device current (the array's name is not a device address in host code)
+static int scratch_ptr(unsigned char** out) {
+    static std::mutex mu;
+    static unsigned char* ptrs[PIC_STATIC_MAX_DEVICES] = {};
+    int dev = 0;
+    PIC_HIP(hipGetDevice(&dev));
+    PIC_REQUIRE(dev >= 0 && dev < PIC_STATIC_MAX_DEVICES, "device index beyond the static export's table");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptrs[dev]) {
+        void* p = nullptr;
+        PIC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        ptrs[dev] = (unsigned char*)p;
+    }
+    *out = ptrs[dev];
+    return 0;
+}
+
+template <typename T>
+static int export_static_t(pic_particles* p, int s, float* dst, int64_t cap, double sx, double sy, int64_t* nout) {
+    const SpeciesStore& st = p->sp[s];
+    const int chunk = 4096;
+    const int chunks = std::max(1, (st.h_cap[st.nb] + chunk - 1) / chunk);
+    const int n_items = st.nb + chunks;
+    if (n_items > PIC_STATIC_EXPORT_ITEMS) {
+        set_error("pic_particles_export_positions_static: the species needs " + std::to_string(n_items) +
+                  " work items (bins + spill chunks), more than the static maximum of " +
+                  std::to_string(PIC_STATIC_EXPORT_ITEMS) + "; use pic_particles_export_positions");
+        return PIC_ERR_CAPACITY;
+    }
+    unsigned char* base = nullptr;
+    if (int rc = scratch_ptr(&base)) return rc;
+    int32_t* counter = (int32_t*)base;
+    WorkItem* items = (WorkItem*)(base + 256);
+    hipStream_t sm = lib_stream();
+    // the items kernel clears the counter the previous call left behind
+    if (int rc = launch_species_items(items, counter, s, st.nb, chunks, sm)) return rc;
+    if (int rc = launch_export<T>((CaseDev<T>*)p->d_case, p->d_state, items, n_items, chunk, counter, (float2*)dst,
+                                  dst ? cap : 0, sx, sy, sm))
+        return rc;
+    int32_t n = 0;
+    PIC_HIP(hipMemcpyAsync(&n, counter, sizeof(n), hipMemcpyDeviceToHost, sm));
+    PIC_HIP(hipStreamSynchronize(sm));
+    *nout = n;
+    if (dst && n > cap) {
+        set_error("pic_particles_export_positions_static: output capacity too small");
+        return PIC_ERR_CAPACITY;
+    }
+    return 0;
+}
+
+extern "C" int pic_particles_export_positions_static(pic_particles* p, int species, float* dst_device, int64_t cap,
+                                                     int units, int64_t* n) {
+    PIC_REQUIRE(p && n && species >= 0 && species < (int)p->sp.size(), "bad species or NULL argument");
+    PIC_REQUIRE(units == 0 || units == 1, "units: 0 cells, 1 domain");
+    PIC_REQUIRE(cap >= 0, "negative capacity");
+    double sx = 1.0, sy = 1.0;
+    if (units == 1) {
+        PIC_REQUIRE(p->attached, "domain units need the particles attached to a simulation (its grid spacing)");
+        sx = p->attached->g->dx;
+        sy = p->attached->g->dy;
+    }
+    return p->prec == 4 ? export_static_t<float>(p, species, dst_device, cap, sx, sy, n)
+                        : export_static_t<double>(p, species, dst_device, cap, sx, sy, n);
+}
+
 template <typename T>
 static int init_t(pic_particles* p, int s, int64_t n, int nx, int ny, double vth, uint32_t seed, uint32_t case_id,
                   uint32_t id0) {

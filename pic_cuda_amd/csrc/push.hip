@@ -1144,6 +1144,28 @@ int launch_moments(const CaseDev<T>* cases, const CaseState* states, const WorkI
     return 0;
 }
 
+// the work items of one species of a store (case 0): its nb bins, then
+// `chunks` chunks of its spill region; thread 0 also clears the counter
+__global__ __launch_bounds__(256) void species_items_kernel(WorkItem* __restrict__ items, int32_t* counter,
+                                                            int species, int nb, int chunks) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k == 0) *G(counter) = 0;
+    if (k >= nb + chunks) return;
+    const gptr<int> w = G(&items[k].case_idx);
+    w[0] = 0;                       // case_idx
+    w[1] = species;
+    w[2] = k < nb ? k : nb;         // bin (nb: the spill region)
+    w[3] = k < nb ? 0 : k - nb;     // chunk
+}
+
+int launch_species_items(WorkItem* items, int32_t* counter, int species, int nb, int chunks, hipStream_t s) {
+    const int n = nb + chunks;
+    hipLaunchKernelGGL(species_items_kernel, dim3((n + 255) / 256), dim3(256), 0, s, items, counter, species, nb,
+                       chunks);
+    PIC_HIP(hipGetLastError());
+    return 0;
+}
+
 template <typename T>
 int launch_export(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items, int chunk,
                   int32_t* counter, float2* dst, int64_t cap, double sx, double sy, hipStream_t s) {

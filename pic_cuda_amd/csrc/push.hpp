@@ -35,6 +35,9 @@ int launch_moments(const CaseDev<T>* cases, const CaseState* states, const WorkI
 template <typename T>
 int launch_export(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items, int chunk,
                   int32_t* counter, float2* dst, int64_t cap, double sx, double sy, hipStream_t s);
+// the work items of a store's species (nb bins, then `chunks` spill chunks)
+// written on the device, and the counter at `counter` cleared
+int launch_species_items(WorkItem* items, int32_t* counter, int species, int nb, int chunks, hipStream_t s);
 // end of step: clear the next output's counts, flip the parity, advance the
 // step; status_out (PIC_MAX_SPECIES ints per case, may be null) receives the
 // spill fill of every species
