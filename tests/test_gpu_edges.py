@@ -17,11 +17,12 @@ def _sorted(x, y, vx, vy, ids):
     return x[o], y[o], vx[o], vy[o], ids[o]
 
 
-def _build(pic, cells, e_state, i_state, prec=4, bin_w=4, mcc=False, dt=1e-10, dx=1e-4, w=1e3, rebin_every=0):
+def _build(pic, cells, e_state, i_state, prec=4, bin_w=4, mcc=False, dt=1e-10, dx=1e-4, w=1e3, rebin_every=0,
+           cap=None):
     """A simulation of uploaded particles (cell units) and the oracle twin."""
     L = cells * dx
     grid = pic.Grid2D(cells + 1, cells + 1, L, L)
-    cap = 4 * max(len(e_state[0]), len(i_state[0]), 1) + 8192
+    cap = cap or 4 * max(len(e_state[0]), len(i_state[0]), 1) + 8192
     parts = pic.Particles2D(cap, prec)
     parts.change_species(0, ME, -QE)
     ion = parts.add_species(M_AR, QE)
@@ -173,3 +174,25 @@ def test_capacity_errors(pic):
     p.upload(0, np.ones(10), np.ones(10), np.zeros(10), np.zeros(10))
     assert p.count(0) == slots
     p.close()
+
+
+def test_spill_overflow_is_an_error(pic):
+    """2e6 electrons in the first column of 16-cell bins all jump 40 cells in
+    one step, into bins that were empty at the last re-binning: their back
+    parts hold 20 particles each, and the spill region (the store's capacity
+    beyond the bins: N/8 + 2^20 slots plus what the layout left) takes about
+    1.3e6, so the rest find no slot.  They are counted as lost and the step
+    reports PIC_ERR_CAPACITY instead of going on with less charge."""
+    cells, n = 64, 2000000
+    rng = np.random.default_rng(12)
+    dx, dt = 1e-4, 1e-10
+    e = (rng.uniform(1, 15, n), rng.uniform(1, 63, n), np.full(n, 40 * dx / dt), np.zeros(n),
+         np.arange(n, dtype=np.uint32))
+    i = (rng.uniform(1, 63, 1000), rng.uniform(1, 63, 1000), np.zeros(1000), np.zeros(1000),
+         np.arange(1000, dtype=np.uint32))
+    c = _build(pic, cells, e, i, bin_w=16, dt=dt, dx=dx, cap=n)
+    with pytest.raises(pic.PicError, match="spill region full"):
+        c["sim"].step(1)
+    lost = int(c["sim"].counters(0)[pic.CNT["lost"]])
+    assert lost > 100000
+    assert c["sim"].counters(1)[pic.CNT["lost"]] == 0
