@@ -59,7 +59,7 @@ extern "C" {
 #define PIC_MG_FP64 0
 #define PIC_MG_MIXED 1
 #define PIC_MG_PRESET_REFERENCE 0   /* Multigrid.cu defaults: SOR 1.25, 3/3/20, 1e-5, 50 cycles, FP64 */
-#define PIC_MG_PRESET_STEP 1        /* 2+2 Gauss-Seidel to 1e-8, direct 3x3 coarse, FP64 refinement */
+#define PIC_MG_PRESET_STEP 1        /* 2+2 Gauss-Seidel to 1e-8, FP64 refinement, exact coarse solve <= 257^2 */
 #define PIC_MG_PRESET_POISSON 2     /* 2+2 red-black SOR 1.2 to 1e-10, FP64 refinement */
 #define PIC_MG_PRESET_STEP_SOR 3    /* 2+2 red-black SOR 1.2 to 1e-8, FP64 refinement: the simulation
                                        default; over-relaxation cuts the V-cycle contraction from
@@ -175,8 +175,11 @@ typedef struct {
     int inner;           /* mixed mode: FP32 V-cycles per FP64 refinement step, 1..4 (0: 1);
                             the convergence test runs after each refinement step */
     int coarse_direct;   /* mixed mode, no electrodes: 1 solves the first level with at most
-                            65 x 65 nodes exactly (FP64 sine transform on the matrix cores)
-                            instead of recursing and sweeping below it; 0 keeps the sweeps.
+                            65 x 65 nodes exactly (FP64 sine transform on the matrix cores of
+                            one workgroup) instead of recursing and sweeping below it; 2 does
+                            the same on the first level with at most 257 x 257 nodes, as four
+                            FP64 matrix-core products over the whole GPU (grids whose levels
+                            skip 66..257 nodes fall back to 1); 0 keeps the sweeps.
                             FP64 mode ignores it (its cycle is the reference's,
                             Multigrid.cu:539-569) */
 } pic_mg_params;

@@ -748,9 +748,9 @@ extern "C" int pic_mg_default_params(int preset, pic_mg_params* out) {
     PIC_REQUIRE(out, "out is NULL");
     switch (preset) {
         case PIC_MG_PRESET_REFERENCE: *out = pic_mg_params{0, 3, 3, 20, 1.25, 1e-5, 50, PIC_MG_FP64, 1, 0}; break;
-        case PIC_MG_PRESET_STEP: *out = pic_mg_params{0, 2, 2, 1, 1.0, 1e-8, 100, PIC_MG_MIXED, 2, 1}; break;
+        case PIC_MG_PRESET_STEP: *out = pic_mg_params{0, 2, 2, 1, 1.0, 1e-8, 100, PIC_MG_MIXED, 2, 2}; break;
         case PIC_MG_PRESET_POISSON: *out = pic_mg_params{0, 2, 2, 1, 1.2, 1e-10, 100, PIC_MG_MIXED, 2, 1}; break;
-        case PIC_MG_PRESET_STEP_SOR: *out = pic_mg_params{0, 2, 2, 1, 1.2, 1e-8, 100, PIC_MG_MIXED, 2, 1}; break;
+        case PIC_MG_PRESET_STEP_SOR: *out = pic_mg_params{0, 2, 2, 1, 1.2, 1e-8, 100, PIC_MG_MIXED, 2, 2}; break;
         default: set_error("unknown multigrid preset"); return PIC_ERR_ARG;
     }
     return 0;
@@ -767,7 +767,7 @@ extern "C" int pic_mg_create(int n, const pic_mg_params* params, pic_mg** out) {
     PIC_REQUIRE(out && params, "NULL argument");
     PIC_REQUIRE(params->mode == PIC_MG_FP64 || params->mode == PIC_MG_MIXED, "bad multigrid mode");
     PIC_REQUIRE(params->inner >= 0 && params->inner <= 4, "inner cycles must be 0..4");
-    PIC_REQUIRE(params->coarse_direct == 0 || params->coarse_direct == 1, "coarse_direct must be 0 or 1");
+    PIC_REQUIRE(params->coarse_direct >= 0 && params->coarse_direct <= 2, "coarse_direct must be 0, 1 or 2");
     PIC_REQUIRE(lib_stream() != nullptr, "could not create a HIP stream");
     auto* m = new pic_mg();
     m->solver.reset(new MGSolver(n, to_mg(*params), lib_stream()));
@@ -1434,7 +1434,7 @@ extern "C" int pic_sim_create(pic_particles* p, pic_grid* g, pic_walls* w, pic_x
     PIC_REQUIRE(prm->halo >= 0 && prm->halo <= 8, "halo must be 0..8");
     PIC_REQUIRE(prm->mg.mode == PIC_MG_FP64 || prm->mg.mode == PIC_MG_MIXED, "bad multigrid mode");
     PIC_REQUIRE(prm->mg.inner >= 0 && prm->mg.inner <= 4, "inner cycles must be 0..4");
-    PIC_REQUIRE(prm->mg.coarse_direct == 0 || prm->mg.coarse_direct == 1, "coarse_direct must be 0 or 1");
+    PIC_REQUIRE(prm->mg.coarse_direct >= 0 && prm->mg.coarse_direct <= 2, "coarse_direct must be 0, 1 or 2");
     auto* S = new pic_sim();
     S->p = p;
     S->g = g;

@@ -32,7 +32,8 @@ struct MGParams {
     int max_cycles;
     int mode;
     int inner = 1;      // mixed mode: FP32 V-cycles per FP64 refinement step
-    int coarse_direct = 0;   // mixed mode: exact FP64 solve of the coarse kernel's level (n <= 65)
+    int coarse_direct = 0;   // mixed mode: exact FP64 solve of the coarse kernel's level (1: n <= 65;
+                             // 2: the first level of at most 257 nodes, several workgroups)
     int per_level = 0;  // 1: pre_l/post_l give each level's sweeps (index 0 = finest)
     int pre_l[16] = {}, post_l[16] = {};
     int pre_at(int l) const { return per_level ? pre_l[l] : pre; }
@@ -117,6 +118,20 @@ struct MGTile {
     int task, tx, ty, pad;
 };
 
+// one case of the multi-workgroup exact FP64 solve of a level of 66..257
+// nodes (mixed mode, coarse_direct 2): four P x P products, one launch each
+struct MGDstTask {
+    const float* rhs;     // the level's rhs, n x n
+    float* e_out;         // the level's correction, n x n (boundary written zero)
+    const double* S;      // P x P: S[a][b] = sin(pi (a+1)(b+1) / (m+1)), zero beyond m
+    const double* W;      // P x P: (2/(m+1))^2 / (-4 (sig_a + sig_b)), zero beyond m
+    double* T0;           // P x P FP64 intermediates
+    double* T1;
+    int n, m, P, gx;      // nodes, unknowns a side, padded size (multiple of 64), P / 16
+    double h2;
+    const MGNorm* gate;
+};
+
 class MGBatch;
 
 class MGSolver {
@@ -147,6 +162,14 @@ public:
     // sweeps on the coarsest level: a 4x4 coarsest level of an automatic ladder
     // (2x2 unknowns, Gauss-Seidel factor 1/4 per sweep) gets at least 32, which
     // solves it to rounding; a 3x3 level is solved exactly by one sweep
+    // the level the cycle ends on: the first level of at most 257 nodes when
+    // it is solved exactly by the multi-workgroup sine transform (mixed mode,
+    // coarse_direct 2, no electrodes, more than 65 nodes), else the coarse
+    // kernel's level
+    bool big_direct() const {
+        return p_.mode == PIC_MG_MIXED && p_.coarse_direct == 2 && mask_.empty() && lcd_ >= 0 && lcd_ < lc_;
+    }
+    int coarse_level() const { return big_direct() ? lcd_ : lc_; }
     int coarse_sweeps_eff() const {
         return (p_.levels <= 0 && !ns_.empty() && ns_.back() == 4) ? (p_.coarse_sweeps > 32 ? p_.coarse_sweeps : 32)
                                                                  : p_.coarse_sweeps;
@@ -155,7 +178,9 @@ public:
 private:
     friend class MGBatch;
     int n_, L_, lc_;
+    int lcd_ = -1;                 // first level of at most 257 nodes (mixed mode)
     int nc_ = 33;                  // largest level of the coarse kernel
+    double* dst_t_ = nullptr;      // big direct solve: two P x P FP64 intermediates
     MGParams p_;
     hipStream_t stream_;
     std::vector<int> ns_;
@@ -199,7 +224,8 @@ public:
 
 private:
     struct Stage {
-        int kind = 0;      // 0 smooth, 1 prolong, 2 coarse, 3 copy, 4 prolong + next cycle's smooth
+        int kind = 0;      // 0 smooth, 1 prolong, 2 coarse, 3 copy, 4 prolong + next cycle's smooth,
+                           // 5 a product of the big direct solve (phase in nu)
         int nu = 0;        // sweeps of a tiled stage
         bool restr = false;   // smooth stage also restricts the residual
         bool masked = false;  // some task has electrodes
@@ -215,9 +241,11 @@ private:
     template <typename T>
     int build_step(const std::vector<double>& h);
     int build_direct_tables();
+    int build_big_tables();
     template <typename T>
     int add_level_stage(int kind, const std::vector<MGLevelTask<T>>& tasks, int nu, bool restr,
                         std::vector<Stage>* dst = nullptr);
+    int add_dst_stages(const std::vector<MGDstTask>& tasks, std::vector<Stage>* dst);
     int launch_stage(const Stage& st);
     int launch_refinement();   // the inner V-cycles of one refinement step
     bool any_odd_steps() const;
@@ -248,6 +276,9 @@ private:
     // direct coarse solve: per member its tables (null: none)
     std::vector<const double*> dtab_;
     double* d_dtab_ = nullptr;
+    // big direct solve: per member its S and W tables (null: none)
+    std::vector<const double*> btab_;
+    double* d_btab_ = nullptr;
     int fixed_cycles_ = 0;
     int launches_ = 0;
 };

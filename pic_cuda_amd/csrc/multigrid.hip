@@ -50,6 +50,10 @@ namespace pic {
 #define MG_ND 65
 #define MG_DS 63
 #define MG_DLDS (2 * MG_DS * MG_DS * 8)
+// big direct solve (coarse_direct 2): levels up to 257 nodes, unknowns a
+// side padded to a multiple of 64
+#define MG_NDB 257
+static inline int dst_pad(int m) { return (m + 63) / 64 * 64; }
 // largest finest level (nodes a side) of a batch that fuses the finest
 // up/down stages between V-cycles
 #define MG_FUSE_MAX 1025
@@ -696,6 +700,86 @@ __global__ __launch_bounds__(1024) void mg_coarse_stage(const StageRef<MGCoarseT
         for (int i = tx; i < n_top; i += 32) e_out[j * n_top + i] = base[j * n_top + i];
 }
 
+// ---- mixed mode: exact FP64 solve of a level of 66..257 nodes -----------------
+// The same transform as coarse_direct_solve, e = S (h^2 W o (S R S)) S, for
+// m <= 255 unknowns a side, as four products of P x P matrices (P = m rounded
+// up to a multiple of 64, zero padded), one launch each:
+//   phase 0: T0 = R S          (R: the level's FP32 rhs, interior nodes)
+//   phase 1: T1 = (S T0) o h^2 W
+//   phase 2: T0 = T1 S
+//   phase 3: e  = S T0         (FP32, interior nodes; the boundary set to zero)
+// Workgroup = one 16 x 16 output tile; its four wavefronts take a quarter of
+// the k range each (P/4 = 16..64, 4..16 chained v_mfma_f64_16x16x4f64), then
+// wavefront 0 adds the four partial tiles in a fixed order, so the result does
+// not depend on scheduling.  S is symmetric, so its A-side entries S[i][k] are
+// read as S[k][i] (consecutive lanes, consecutive addresses).
+template <int PHASE>
+__global__ __launch_bounds__(256) void mg_dst_stage(const StageRef<MGDstTask> a) {
+    MGTile tile;
+    MGDstTask tk;
+    stage_item(a, tk, tile);
+    const int stop = tk.gate ? *G(&tk.gate->converged) : 0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int P = tk.P, m = tk.m, n = tk.n;
+    const int row0 = tile.ty * 16, col0 = tile.tx * 16;
+    const int nk = P >> 4;                 // MFMA steps of this wavefront (P/4 values of k, 4 per step)
+    const int k0 = wave * (P >> 2);
+    const gptr<const double> S = G(tk.S);
+    const gptr<const double> Bm = G(PHASE == 1 || PHASE == 3 ? (const double*)tk.T0 : tk.S);
+    const gptr<const double> Am = G(PHASE == 2 ? (const double*)tk.T1 : tk.S);
+    const gptr<const float> R = G(tk.rhs);
+    double av[16], bv[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        av[t] = 0.0;
+        bv[t] = 0.0;
+        if (t < nk) {
+            const int k = k0 + 4 * t + lk, i = row0 + li, j = col0 + li;
+            if (PHASE == 0) {
+                av[t] = (i < m && k < m) ? (double)R[(size_t)(i + 1) * n + (k + 1)] : 0.0;
+            } else if (PHASE == 2) {
+                av[t] = Am[(size_t)i * P + k];
+            } else {
+                av[t] = S[(size_t)k * P + i];   // S[i][k], S symmetric
+            }
+            bv[t] = Bm[(size_t)k * P + j];
+        }
+    }
+    if (stop) return;
+    mg_v4d acc = mg_v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+        if (t < nk) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t], bv[t], acc, 0, 0, 0);
+    __shared__ double red[4][4][64];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) red[wave][q][lane] = acc[q];
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double v = ((red[0][q][lane] + red[1][q][lane]) + red[2][q][lane]) + red[3][q][lane];
+            const int i = row0 + 4 * q + lk, j = col0 + li;   // D[i][j]: lane j + 16 (i % 4), element i / 4
+            if (PHASE == 0 || PHASE == 2) {
+                G(tk.T0)[(size_t)i * P + j] = v;
+            } else if (PHASE == 1) {
+                G(tk.T1)[(size_t)i * P + j] = v * (tk.h2 * G(tk.W)[(size_t)i * P + j]);
+            } else if (i < m && j < m) {
+                G(tk.e_out)[(size_t)(i + 1) * n + (j + 1)] = (float)v;
+            }
+        }
+    }
+    if (PHASE == 3 && tile.tx == 0 && tile.ty == 0) {
+        // the boundary of the correction is zero
+        for (int t = threadIdx.x; t < 4 * (n - 1); t += 256) {
+            const int side = t / (n - 1), p = t - side * (n - 1);
+            const int idx = side == 0 ? p : side == 1 ? p * n + (n - 1) : side == 2 ? (n - 1) * n + (n - 1 - p)
+                                                                              : (n - 1 - p) * n;
+            G(tk.e_out)[idx] = 0.0f;
+        }
+    }
+}
+
 // ---- FP64 refinement and norms --------------------------------------------------
 __device__ inline double block_sum256(double v, double* sh) {
     const int tid = threadIdx.x;
@@ -889,6 +973,7 @@ MGSolver::~MGSolver() {
     if (phi_tmp_) (void)hipFree(phi_tmp_);
     if (partials_) (void)hipFree(partials_);
     if (d_norm_) (void)hipFree(d_norm_);
+    if (dst_t_) (void)hipFree(dst_t_);
     for (auto* q : mask_) (void)hipFree(q);
     if (phi_bc_) (void)hipFree(phi_bc_);
 }
@@ -959,6 +1044,14 @@ int MGSolver::init() {
     nc_ = p_.mode == PIC_MG_MIXED ? MG_NC_F32 : MG_NC_F64;
     for (int l = 0; l < L; ++l)
         if (ns_[l] <= nc_) { lc_ = l; break; }
+    lcd_ = -1;
+    if (p_.mode == PIC_MG_MIXED)
+        for (int l = 0; l < L; ++l)
+            if (ns_[l] <= MG_NDB) { lcd_ = l; break; }
+    if (p_.mode == PIC_MG_MIXED && p_.coarse_direct == 2 && lcd_ >= 0 && lcd_ < lc_) {
+        const size_t P = dst_pad(ns_[lcd_] - 2);
+        PIC_HIP(hipMalloc(&dst_t_, 2 * P * P * sizeof(double)));
+    }
     for (int b = 0; b < 2; ++b) {
         ef_[b].assign(L, nullptr);
         ed_[b].assign(L, nullptr);
@@ -1086,6 +1179,8 @@ void MGBatch::free_all() {
     if (h_mirror_) (void)hipHostFree(h_mirror_);
     if (d_dtab_) (void)hipFree(d_dtab_);
     d_dtab_ = nullptr;
+    if (d_btab_) (void)hipFree(d_btab_);
+    d_btab_ = nullptr;
     d_fine_ = nullptr;
     d_chunk_tiles_ = d_res_tiles_ = nullptr;
     d_mirror_ = nullptr;
@@ -1133,6 +1228,29 @@ int MGBatch::add_level_stage(int kind, const std::vector<MGLevelTask<T>>& tasks,
     return 0;
 }
 
+// the four product launches of the big direct solve, over the (case, tile) items
+int MGBatch::add_dst_stages(const std::vector<MGDstTask>& tasks, std::vector<Stage>* dst) {
+    if (tasks.empty()) return 0;
+    std::vector<MGTile> tiles;
+    for (int k = 0; k < (int)tasks.size(); ++k)
+        for (int ty = 0; ty < tasks[k].gx; ++ty)
+            for (int tx = 0; tx < tasks[k].gx; ++tx) tiles.push_back(MGTile{k, tx, ty, 0});
+    for (int phase = 0; phase < 4; ++phase) {
+        Stage st;
+        st.kind = 5;
+        st.nu = phase;
+        st.n_items = (int)tiles.size();
+        st.n_tasks = (int)tasks.size();
+        st.gx = tasks[0].gx;
+        static_assert(sizeof(MGDstTask) <= sizeof(st.t0), "task does not fit the by-value slot");
+        std::memcpy(st.t0, &tasks[0], sizeof(MGDstTask));
+        if (int rc = upload_vec(&st.tasks, tasks)) return rc;
+        if (int rc = upload_vec((void**)&st.tiles, tiles)) return rc;
+        (dst ? dst : &cycle_)->push_back(st);
+    }
+    return 0;
+}
+
 // The stages of one refinement step (FP64 mode: one V-cycle): inner_ V-cycles,
 // the later ones starting from the correction so far.  Between two V-cycles
 // the finest up-leg stage of a member with tiled levels also runs the next
@@ -1154,7 +1272,7 @@ int MGBatch::build_step(const std::vector<double>& h) {
     auto rhs0 = [&](int k) -> const T* { return mixed_ ? (const T*)(void*)m_[k]->r32_ : (const T*)(void*)rhs_[k]; };
     auto hl = [&](int k, int l) { return h[k] * std::ldexp(1.0, l); };
     int D = 0;
-    for (auto* s : m_) D = std::max(D, s->lc_);
+    for (auto* s : m_) D = std::max(D, s->coarse_level());
     auto level_task = [&](int k, int d) {
         MGSolver* s = m_[k];
         const double hh = hl(k, d);
@@ -1190,7 +1308,7 @@ int MGBatch::build_step(const std::vector<double>& h) {
             std::vector<MGLevelTask<T>> tasks;
             for (int k = 0; k < M; ++k) {
                 MGSolver* s = m_[k];
-                if (s->lc_ <= d) continue;
+                if (s->coarse_level() <= d) continue;
                 MGLevelTask<T> t = level_task(k, d);
                 t.e_in = d == 0 ? in0(k) : nullptr;
                 t.e_out = d == 0 ? pre0[k] : E(s, 1, d);
@@ -1201,12 +1319,38 @@ int MGBatch::build_step(const std::vector<double>& h) {
             }
             if (int rc = add_level_stage<T>(0, tasks, p_.pre_at(d), true, dst)) return rc;
         }
-        // coarse part: the single-workgroup cycle, or tiled sweeps on a large coarsest level
+        // coarse part: the exact solve of a level of 66..257 nodes over several
+        // workgroups, the single-workgroup cycle, or tiled sweeps on a large coarsest level
+        if constexpr (sizeof(T) == 4) {
+            std::vector<MGDstTask> dt;
+            for (int k = 0; k < M; ++k) {
+                MGSolver* s = m_[k];
+                if (!s->big_direct() || !btab_[k]) continue;
+                const int lc = s->lcd_;
+                MGDstTask t{};
+                t.rhs = lc == 0 ? (const float*)rhs0(k) : (const float*)R(s, lc);
+                t.e_out = lc == 0 ? (float*)out0(k) : (float*)E(s, 0, lc);
+                t.n = s->ns_[lc];
+                t.m = t.n - 2;
+                t.P = dst_pad(t.m);
+                t.gx = t.P / 16;
+                t.S = btab_[k];
+                t.W = btab_[k] + (size_t)t.P * t.P;
+                t.T0 = s->dst_t_;
+                t.T1 = s->dst_t_ + (size_t)t.P * t.P;
+                const double hh = hl(k, lc);
+                t.h2 = hh * hh;
+                t.gate = s->d_norm_;
+                dt.push_back(t);
+            }
+            if (int rc = add_dst_stages(dt, dst)) return rc;
+        }
         {
             std::vector<MGCoarseTask<T>> ct;
             size_t lds = 0;
             for (int k = 0; k < M; ++k) {
                 MGSolver* s = m_[k];
+                if (s->big_direct()) continue;
                 const int lc = s->lc_;
                 if (s->ns_[lc] > s->nc_) continue;
                 MGCoarseTask<T> t{};
@@ -1247,7 +1391,7 @@ int MGBatch::build_step(const std::vector<double>& h) {
             // large coarsest levels: chunks of up to 3 sweeps, ping-pong, then a copy
             std::vector<int> big;
             for (int k = 0; k < M; ++k)
-                if (m_[k]->ns_[m_[k]->lc_] > m_[k]->nc_) big.push_back(k);
+                if (!m_[k]->big_direct() && m_[k]->ns_[m_[k]->lc_] > m_[k]->nc_) big.push_back(k);
             if (!big.empty()) {
                 std::vector<const T*> src(M, nullptr);
                 std::vector<int> which(M, 0);
@@ -1300,7 +1444,7 @@ int MGBatch::build_step(const std::vector<double>& h) {
             std::vector<MGLevelTask<T>> tasks;
             for (int k = 0; k < M; ++k) {
                 MGSolver* s = m_[k];
-                if (s->lc_ <= d) continue;
+                if (s->coarse_level() <= d) continue;
                 MGLevelTask<T> t = level_task(k, d);
                 t.e_in = d == 0 ? pre0[k] : E(s, 1, d);
                 t.coarse = E(s, 0, d + 1);
@@ -1319,19 +1463,69 @@ int MGBatch::build_step(const std::vector<double>& h) {
         }
         // after a fused stage the next cycle's finest up stage reads what it wrote
         for (int k = 0; k < M; ++k)
-            if (m_[k]->lc_ > 0 && !last && fuse_ok) std::swap(pre0[k], post0[k]);
+            if (m_[k]->coarse_level() > 0 && !last && fuse_ok) std::swap(pre0[k], post0[k]);
     }
-    for (int k = 0; k < M; ++k) final_e0_[k] = (m_[k]->lc_ > 0 && post0[k] != out0(k)) ? 1 : 0;
+    for (int k = 0; k < M; ++k) final_e0_[k] = (m_[k]->coarse_level() > 0 && post0[k] != out0(k)) ? 1 : 0;
     return 0;
 }
 
 // Direct coarse solve (mixed mode, members without electrodes whose coarse
 // kernel level has 3..65 nodes): the sine-transform tables of that size, one
 // set per size (S then W, 64 x 64 each, zero outside m x m).
+// Tables of the big direct solve, one set per size: S then W, each P x P,
+// zero outside m x m (m = n - 2 unknowns a side, P = m rounded up to 64).
+int MGBatch::build_big_tables() {
+    const int M = (int)m_.size();
+    std::vector<int> sizes;
+    std::vector<size_t> offs;
+    size_t total = 0;
+    for (int k = 0; k < M; ++k) {
+        const MGSolver* s = m_[k];
+        if (!s->big_direct()) continue;
+        const int n = s->ns_[s->lcd_];
+        if (std::find(sizes.begin(), sizes.end(), n) != sizes.end()) continue;
+        sizes.push_back(n);
+        offs.push_back(total);
+        const size_t P = dst_pad(n - 2);
+        total += 2 * P * P;
+    }
+    if (sizes.empty()) return 0;
+    std::vector<double> tab(total, 0.0);
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (size_t q = 0; q < sizes.size(); ++q) {
+        const int m = sizes[q] - 2, P = dst_pad(m);
+        double* S = &tab[offs[q]];
+        double* W = S + (size_t)P * P;
+        std::vector<long double> sig(m);
+        for (int p = 0; p < m; ++p) {
+            const long double sp = std::sin(pi * (p + 1) / (2.0L * (m + 1)));
+            sig[p] = sp * sp;
+        }
+        const long double norm = (2.0L / (m + 1)) * (2.0L / (m + 1));
+        for (int a = 0; a < m; ++a)
+            for (int b = 0; b < m; ++b) {
+                // (a+1)(b+1) reduced mod 2(m+1) keeps the sine argument in [0, 2 pi)
+                const long long t = (long long)(a + 1) * (b + 1) % (2LL * (m + 1));
+                S[(size_t)a * P + b] = (double)std::sin(pi * (long double)t / (long double)(m + 1));
+                W[(size_t)a * P + b] = (double)(norm / (-4.0L * (sig[a] + sig[b])));
+            }
+    }
+    if (int rc = upload_vec((void**)&d_btab_, tab)) return rc;
+    for (int k = 0; k < M; ++k) {
+        const MGSolver* s = m_[k];
+        if (!s->big_direct()) continue;
+        const size_t q = std::find(sizes.begin(), sizes.end(), s->ns_[s->lcd_]) - sizes.begin();
+        btab_[k] = d_btab_ + offs[q];
+    }
+    return 0;
+}
+
 int MGBatch::build_direct_tables() {
     const int M = (int)m_.size();
     dtab_.assign(M, nullptr);
+    btab_.assign(M, nullptr);
     if (!mixed_ || !p_.coarse_direct) return 0;
+    if (int rc = build_big_tables()) return rc;
     std::vector<int> sizes, want(M, 0);
     for (int k = 0; k < M; ++k) {
         const MGSolver* s = m_[k];
@@ -1519,6 +1713,15 @@ static void launch_stage_t(const MGBatchStageView& st, hipStream_t s) {
         if constexpr (sizeof(T) == 4) {
             const auto r = make_ref<MGLevelTask<float>>(st.t0, st.tasks, st.tiles, st.n_tasks, st.gx);
             launch_fused(st.nu, st.masked, g, st.lds, s, r);
+        }
+    } else if (st.kind == 5) {
+        const auto r = make_ref<MGDstTask>(st.t0, st.tasks, st.tiles, st.n_tasks, st.gx);
+        const dim3 b(256);
+        switch (st.nu) {
+        case 0: hipLaunchKernelGGL(mg_dst_stage<0>, g, b, 0, s, r); break;
+        case 1: hipLaunchKernelGGL(mg_dst_stage<1>, g, b, 0, s, r); break;
+        case 2: hipLaunchKernelGGL(mg_dst_stage<2>, g, b, 0, s, r); break;
+        default: hipLaunchKernelGGL(mg_dst_stage<3>, g, b, 0, s, r); break;
         }
     } else if (st.kind == 2) {
         const auto r = make_ref<MGCoarseTask<T>>(st.t0, st.tasks, nullptr, st.n_tasks, 0);

@@ -64,29 +64,40 @@ __device__ inline int back_cap(int cap) { return cap >> 2; }
 // front part of bin `key` (re-binning); back == true: the back part (movers and
 // born particles).  Overflow goes to the spill region; -1 when that is full too.
 // key == nb addresses the spill region directly.  Every lane of the wave must
-// call it.
+// call it.  The lanes are grouped by key first (a scalar loop over the distinct
+// keys, no memory access); then the lowest lane of every group adds the group's
+// size to its bin's counter, all groups in one atomic instruction, so a call
+// waits for one atomic round trip however many bins its lanes go to.
 template <typename T>
 __device__ inline int alloc_slot(const SpeciesDev<T>& S, int out, bool want, int key, bool back) {
     const int lane = __lane_id();
     const uint64_t lt = lanemask_lt();
     const int nb = S.nb[out];
-    int slot = -1;
-    uint64_t act = __ballot(want && key != nb);
+    const bool binned = want && key != nb;
+    // this lane's group: the lanes that want the same bin
+    uint64_t group = 0;
+    uint64_t act = __ballot(binned);
     while (act) {
-        const int leader = __ffsll((unsigned long long)act) - 1;
-        const int k = __shfl(key, leader);
-        const uint64_t m = __ballot(want && key == k) & act;
-        int base = 0;
-        if (lane == leader) base = gatomic_add(back ? &S.mcount[out][k] : &S.count[out][k], (int)__popcll(m));
-        base = __shfl(base, leader);
-        if ((m >> lane) & 1ull) slot = base + __popcll(m & lt);
+        const int first = __ffsll((unsigned long long)act) - 1;
+        const int k = __shfl(key, first);
+        const uint64_t m = __ballot(binned && key == k);
+        if (binned && key == k) group = m;
         act &= ~m;
     }
+    int cap = 0, start = 0;
+    if (binned) {
+        cap = G(S.cap[out])[key];
+        start = G(S.start[out])[key];
+    }
+    const int leader = binned ? __ffsll((unsigned long long)group) - 1 : lane;
+    int base = 0;
+    if (binned && leader == lane)
+        base = gatomic_add(back ? &S.mcount[out][key] : &S.count[out][key], (int)__popcll(group));
+    base = __shfl(base, leader);
     int dst = -1;
-    bool spill = want && key == nb;
-    if (want && key != nb) {
-        const int cap = G(S.cap[out])[key];
-        const int start = G(S.start[out])[key];
+    bool spill = want && !binned;
+    if (binned) {
+        const int slot = base + (int)__popcll(group & lt);
         if (back) {
             if (slot < back_cap(cap)) dst = start + cap - 1 - slot;
             else spill = true;
@@ -97,12 +108,12 @@ __device__ inline int alloc_slot(const SpeciesDev<T>& S, int out, bool want, int
     }
     const uint64_t om = __ballot(spill);
     if (om) {
-        const int leader = __ffsll((unsigned long long)om) - 1;
-        int base = 0;
-        if (lane == leader) base = gatomic_add(&S.count[out][nb], (int)__popcll(om));
-        base = __shfl(base, leader);
+        const int sl = __ffsll((unsigned long long)om) - 1;
+        int sbase = 0;
+        if (lane == sl) sbase = gatomic_add(&S.count[out][nb], (int)__popcll(om));
+        sbase = __shfl(sbase, sl);
         if (spill) {
-            const int s2 = base + __popcll(om & lt);
+            const int s2 = sbase + __popcll(om & lt);
             dst = (s2 < G(S.cap[out])[nb]) ? G(S.start[out])[nb] + s2 : -1;
         }
     }

@@ -105,14 +105,15 @@ def test_mg_reference_preset_bit_identical(pic, orc):
     assert np.array_equal(phi_g, phi_o)
 
 
-@pytest.mark.parametrize("direct", [1, 0])
+@pytest.mark.parametrize("direct", [2, 1, 0])
 @pytest.mark.parametrize("n", [129, 513])
 def test_mg_mixed_precision_accuracy(pic, orc, n, direct):
     rng = np.random.default_rng(2 * n)
     rhs = rng.normal(size=(n, n))
     h = 1.0 / (n - 1)
     # FP32 cycles in FP64 refinement to 1e-10; the coarsest part solved
-    # directly (FP64 sine transform at 33^2) or swept down to 3^2
+    # directly (FP64 sine transform at 257^2 or 129^2 over the GPU, or at 65^2
+    # in one workgroup) or swept down to 3^2
     params = pic.mg_params(pic.PRESET_POISSON, coarse_direct=direct)
     phi_g, st = _mg_gpu(pic, n, rhs, np.zeros((n, n)), h, params)
     assert st.rel_residual < 1e-10
@@ -206,8 +207,43 @@ def test_mg_direct_coarse_solve_is_exact(pic, n):
     assert np.linalg.norm(phi_c - exact) / np.linalg.norm(exact) < 1e-11
 
 
+def _dst_exact(rhs, h):
+    """the discrete Poisson solution (5-point Laplacian, zero Dirichlet
+    boundary) by scipy's type-I sine transform: the eigenvectors of the 1D
+    second difference, eigenvalues -(4/h^2)(sin^2(pi p/(2(m+1))) + sin^2(pi q/(2(m+1))))"""
+    from scipy.fft import dstn, idstn
+    m = rhs.shape[0] - 2
+    s = np.sin(np.pi * np.arange(1, m + 1) / (2 * (m + 1))) ** 2
+    lam = -(4.0 / h ** 2) * (s[:, None] + s[None, :])
+    out = np.zeros_like(rhs)
+    out[1:-1, 1:-1] = idstn(dstn(rhs[1:-1, 1:-1], type=1) / lam, type=1)
+    return out
+
+
+@pytest.mark.parametrize("n", [97, 129, 193, 257])
+def test_mg_big_direct_solve_is_exact(pic, n):
+    """coarse_direct 2 on a grid of 66..257 nodes: the whole correction
+    equation is solved by the multi-workgroup FP64 sine transform, so one
+    refinement step from zero is the discrete solution up to the FP32
+    rounding of its residual and correction, and two are at FP64 rounding
+    (against scipy's own sine transform).  The padded size P (m rounded up to
+    64) is 128, 128, 192 and 256 here."""
+    rng = np.random.default_rng(300 + n)
+    rhs = rng.normal(size=(n, n))
+    h = 0.61 / (n - 1)
+    exact = _dst_exact(rhs, h)
+    for cycles, tol in ((1, 1e-6), (2, 1e-12)):
+        prm = pic.mg_params(pic.PRESET_POISSON, coarse_direct=2, inner=1, tol=0.0, max_cycles=cycles)
+        phi, st = _mg_gpu(pic, n, rhs, np.zeros((n, n)), h, prm)
+        assert st.cycles == cycles
+        err = np.linalg.norm(phi - exact) / np.linalg.norm(exact)
+        assert err < tol, (cycles, err)
+        assert not np.any(phi[0, :]) and not np.any(phi[-1, :]) and not np.any(phi[:, 0]) and not np.any(phi[:, -1])
+
+
+@pytest.mark.parametrize("direct", [1, 2])
 @pytest.mark.parametrize("n", [65, 129, 513, 1025])
-def test_mg_mixed_inner_cycles(pic, orc, n):
+def test_mg_mixed_inner_cycles(pic, orc, n, direct):
     """1, 2 and 3 FP32 V-cycles per FP64 refinement step (65: the whole grid
     inside the single-workgroup coarse cycle; 1025: a large coarsest tiled
     level is not involved, four tiled levels are) all reach the tolerance and
@@ -220,7 +256,7 @@ def test_mg_mixed_inner_cycles(pic, orc, n):
     phi_o = phi0.copy()
     orc.mg_solve(phi_o, rhs, h, tol=1e-13, max_cycles=100)
     for inner in (1, 2, 3):
-        params = pic.mg_params(pic.PRESET_POISSON, inner=inner)
+        params = pic.mg_params(pic.PRESET_POISSON, inner=inner, coarse_direct=direct)
         phi_g, st = _mg_gpu(pic, n, rhs, phi0, h, params)
         assert st.rel_residual < 1e-10, (inner, st.rel_residual)
         assert st.cycles % inner == 0 and st.cycles <= 12, (inner, st.cycles)
