@@ -45,51 +45,50 @@ def read_traffic():
     return None
 
 
-def cpu_baseline(gpu_case, budget_s=12.0):
-    """The CPU oracle on a 256^2-cell, 1.25e6 + 1.25e6 particle sample of the
-    same workload (same particles per cell, physics and solver settings): the
-    OpenMP build on all the host threads OpenMP is given, and the serial build
-    on one core beside it."""
-    import numpy as np
+def cpu_baseline(gpu_case, omp_steps=4, serial_steps=2):
+    """The CPU oracle on the same workload (1024^2 cells, 2e7 + 2e7 particles
+    from the same seeded initial state, MCC, FP64 multigrid to 1e-8): one
+    untimed first pass, then a few timed full steps, with the OpenMP build on
+    all the host threads OpenMP is given and with the serial build on one
+    core."""
     from oracle import oracle as orc
-    from pic_cuda_amd import configs
     import pic_cuda_amd as pic
+    from pic_cuda_amd import configs
 
-    cells = 256
-    n = 1250000
-    pl = configs.Plasma(cells=cells)
+    pl = gpu_case["plasma"]
     nn = pl.nodes
+    n = gpu_case["n_per_species"]
     sim = gpu_case["sim"]
     mg = gpu_case["mg"]
     w = pl.density * pl.length * pl.length / n
     qs = 1.0 / float(1 << 40)
     cfac = [-(q * w * qs / (pl.dx * pl.dx)) / pic.EPS0 for q in (-pic.ELEMENTARY_CHARGE, pic.ELEMENTARY_CHARGE)]
+    infos = [sim.mcc_info(s) for s in range(2)]
 
-    def run(omp):
+    def run(omp, steps):
         sp = []
         for s in range(2):
-            info = sim.mcc_info(s)
             vth = pl.vte if s == 0 else pl.vti
             x, y, vx, vy, ids = orc.init_particles(4, n, nn, nn, vth, 1, 0, s, 0)
             kq = pic.ELEMENTARY_CHARGE * (-1 if s == 0 else 1) * pl.dt / (
                 pic.ELECTRON_MASS if s == 0 else configs.M_AR)
-            sp.append(orc.Species(4, x, y, vx, vy, ids, kq, mcc=info, other=info["product_species"]))
+            sp.append(orc.Species(4, x, y, vx, vy, ids, kq, mcc=infos[s], other=infos[s]["product_species"]))
         o = orc.OracleSim(sp, nn, nn, pl.dx, pl.dx, pl.dt, cfac,
                           dict(tol=mg.tol, max_cycles=mg.max_cycles, pre=mg.pre, post=mg.post,
                                coarse_sweeps=mg.coarse_sweeps, omega=mg.omega), omp=omp)
-        o.step(1)                      # first pass has no kick; time full steps after it
+        o.step(1)                      # the first pass has no kick; full steps follow
         t0 = time.perf_counter()
-        steps = 0
         psteps = 0
-        while steps < 40 and (time.perf_counter() - t0) < budget_s:
+        for _ in range(steps):
             psteps += sum(s.n for s in o.sp)
             o.step(1)
-            steps += 1
-        return psteps / (time.perf_counter() - t0), steps
+        rate = psteps / (time.perf_counter() - t0)
+        del o, sp
+        return rate
 
     threads = orc.lib(True).orc_num_threads()
-    v_omp, s_omp = run(True)
-    v_ser, s_ser = run(False)
+    v_omp = run(True, omp_steps)
+    v_ser = run(False, serial_steps)
     model = None
     try:
         for line in open("/proc/cpuinfo"):
@@ -99,9 +98,10 @@ def cpu_baseline(gpu_case, budget_s=12.0):
     except OSError:
         pass
     return {"value": v_omp, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": "oracle (OpenMP, %d threads), %d full steps of a 256^2-cell, 1.25e6 e- + 1.25e6 Ar+ sample "
-                      "(same ppc, MCC and 1e-8 FP64 multigrid)" % (threads, s_omp),
-            "serial_value": v_ser, "serial_steps": s_ser, "nproc": os.cpu_count(), "cpu_model": model}
+            "sample": "oracle (OpenMP, %d threads), %d full steps of the same 1024^2-cell, %.3g e- + %.3g Ar+ "
+                      "workload after one untimed first pass (MCC, FP64 multigrid to 1e-8)"
+                      % (threads, omp_steps, n, n),
+            "serial_value": v_ser, "serial_steps": serial_steps, "nproc": os.cpu_count(), "cpu_model": model}
 
 
 def poisson(n=2049, cpu=True):
@@ -129,9 +129,19 @@ def poisson(n=2049, cpu=True):
             best = st if best is None or st.ms < best.ms else best
         sols[name] = g.download(pic.FIELD_PHI)
         cyc = best.cycles
-        byts = n * n * (32 * cyc + 40 * (cyc + 1)) if name == "mixed" else n * n * 64 * cyc
-        out[name] = {"ms": round(best.ms, 3), "cycles": cyc, "rel_residual": best.rel_residual,
-                     "launches": best.launches, "algorithmic_GBps": round(byts / (best.ms * 1e-3) / 1e9, 1)}
+        if name == "mixed":
+            # V FP32 V-cycles at 32 B per fine node, K FP64 refinement steps
+            # (residual and correction) at 40 B per node, and the first
+            # residual of the solve (phi and rhs read, r32 written: 20 B)
+            steps = (cyc + prm.inner - 1) // max(1, prm.inner)
+            byts = n * n * (32 * cyc + 40 * steps + 20)
+        else:
+            steps = cyc
+            byts = n * n * 64 * cyc
+        gbps = byts / (best.ms * 1e-3) / 1e9
+        out[name] = {"ms": round(best.ms, 3), "cycles": cyc, "refinement_steps": steps,
+                     "rel_residual": best.rel_residual, "launches": best.launches,
+                     "algorithmic_GBps": round(gbps, 1), "roofline_frac": round(gbps / HBM_PEAK_GBS, 4)}
     ref = sols["fp64"]
     out["mixed_vs_fp64_rel_l2"] = float(np.linalg.norm(sols["mixed"] - ref) / np.linalg.norm(ref))
     out["speedup_mixed_over_fp64"] = round(out["fp64"]["ms"] / out["mixed"]["ms"], 3)
@@ -362,17 +372,20 @@ def serve(n_jobs=24, rate=20.0, budget=200e9, seed=9, max_steps=2000):
             "completion_s_p50": float(np.percentile(comp, 50)), "completion_s_p99": float(np.percentile(comp, 99))}
 
 
-def fp64_check(cells, scale, bw, steps=20):
+def fp64_check(cells, scale, bw, horizons=(1, 5, 20)):
     """Errors of the FP32 step against FP64 on the same workload.
     field_solve: the step's FP32-in-FP64 multigrid potential against an FP64
     multigrid solve of the same charge to 1e-13 (the solver's own error).
     trajectories: the FP32 step (FP32 particles, mixed solve) against the FP64
     step (FP64 particles, FP64 multigrid: the path that equals the FP64 oracle
-    bit for bit at test sizes) from the same initial state after `steps`
-    steps: the potential, energy and momentum diagnostics, live counts and
-    collision counts.  The particle runs drift apart as a few particles cross a
-    wall or a collision threshold in one precision and not the other; each
-    such particle moves the potential by about 1/sqrt(N) of its norm."""
+    bit for bit at test sizes) from the same initial state, at each horizon:
+    the potential, energy and momentum diagnostics, live counts and collision
+    counts.  Beside it, the FP64 run's own divergence: a second FP64 run from
+    the same state with every position moved by one FP32 ulp (the rounding the
+    FP32 run starts from).  A few particles cross a wall or a collision
+    threshold in one run and not the other, and each moves the potential by
+    about 1/sqrt(N) of its norm, so the two potentials drift apart even in
+    FP64; the FP32 error is judged against that floor."""
     import numpy as np
     import pic_cuda_amd as pic
     from pic_cuda_amd import configs
@@ -380,18 +393,36 @@ def fp64_check(cells, scale, bw, steps=20):
     a = configs.full_step(scale=scale, cells=cells, bin_w=bw, bin_h=bw)
     state = [a["particles"].download(s) for s in range(2)]
     b = configs.build(cells, n, pic.PREC_FP64, mcc=True, bin_w=bw, bin_h=bw, state=state)
-    del state
-    a["sim"].step(steps)
-    b["sim"].step(steps)
-    da, db = a["sim"].diagnostics(), b["sim"].diagnostics()
-    pa, pb = a["grid"].download(pic.FIELD_PHI), b["grid"].download(pic.FIELD_PHI)
+    up = np.float32(np.inf)
+    pert = [(np.nextafter(x.astype(np.float32), up).astype(np.float64),
+             np.nextafter(y.astype(np.float32), up).astype(np.float64), vx, vy, ids)
+            for (x, y, vx, vy, ids) in state]
+    c = configs.build(cells, n, pic.PREC_FP64, mcc=True, bin_w=bw, bin_h=bw, state=pert)
+    del state, pert
 
     def rel(x, y):
         return abs(x - y) / abs(y) if y else abs(x - y)
 
-    def collisions(c):
-        k = c["sim"].counters(0)
+    def collisions(case):
+        k = case["sim"].counters(0)
         return int(sum(k[pic.CNT["process%d" % q]] for q in range(4)))
+
+    def l2(p, q):
+        return float(np.linalg.norm(p - q) / np.linalg.norm(q))
+
+    per = []
+    done = 0
+    for hz in horizons:
+        for case in (a, b, c):
+            case["sim"].step(hz - done)
+        done = hz
+        pa, pb, pc = (case["grid"].download(pic.FIELD_PHI) for case in (a, b, c))
+        per.append({"steps": hz, "phi_rel_l2_fp32_vs_fp64": l2(pa, pb), "phi_rel_l2_fp64_ulp_vs_fp64": l2(pc, pb),
+                    "alive": [case["particles"].count(-1) for case in (a, b, c)],
+                    "electron_collisions": [collisions(case) for case in (a, b, c)]})
+    steps = horizons[-1]
+    da, db = a["sim"].diagnostics(), b["sim"].diagnostics()
+    pa, pb = a["grid"].download(pic.FIELD_PHI), b["grid"].download(pic.FIELD_PHI)
     # momentum scale of a species: sum of w m |v| ~ sqrt(2 KE M), M = n w m its mass
     pl = configs.Plasma(te=2.0, cells=cells)
     w = pl.density * pl.length * pl.length / n
@@ -406,21 +437,45 @@ def fp64_check(cells, scale, bw, steps=20):
     mg64 = pic.Multigrid(g.m, pic.mg_params(pic.PRESET_POISSON, mode=pic.MG_FP64, tol=1e-13, max_cycles=100))
     st64 = mg64.solve(phi_p, rho_p, g.dx)
     p64 = ref.download(pic.FIELD_PHI)
-    solve = {"phi_rel_l2": float(np.linalg.norm(pa - p64) / np.linalg.norm(p64)), "fp64_cycles": st64.cycles,
-             "fp64_rel_residual": st64.rel_residual, "step_tol": 1e-8}
+    solve = {"phi_rel_l2": l2(pa, p64), "fp64_cycles": st64.cycles, "fp64_rel_residual": st64.rel_residual,
+             "step_tol": 1e-8}
     traj = {"steps": steps, "reference": "FP64 particles and FP64 multigrid from the same initial state",
-           "phi_rel_l2": float(np.linalg.norm(pa - pb) / np.linalg.norm(pb)),
-           "kinetic_rel": [rel(da["kinetic"][s], db["kinetic"][s]) for s in range(2)],
-           "momentum_diff_rel_to_scale": [
-               float(np.hypot(da["px"][s] - db["px"][s], da["py"][s] - db["py"][s]) /
-                     np.sqrt(2.0 * db["kinetic"][s] * db["count"][s] * w * masses[s])) for s in range(2)],
-           "field_energy_rel": rel(da["field_energy"], db["field_energy"]),
-           "alive_fp32": [int(c) for c in da["count"]], "alive_fp64": [int(c) for c in db["count"]],
-           "electron_collisions_fp32": collisions(a), "electron_collisions_fp64": collisions(b)}
+            "phi_rel_l2": l2(pa, pb),
+            "kinetic_rel": [rel(da["kinetic"][s], db["kinetic"][s]) for s in range(2)],
+            "momentum_diff_rel_to_scale": [
+                float(np.hypot(da["px"][s] - db["px"][s], da["py"][s] - db["py"][s]) /
+                      np.sqrt(2.0 * db["kinetic"][s] * db["count"][s] * w * masses[s])) for s in range(2)],
+            "field_energy_rel": rel(da["field_energy"], db["field_energy"]),
+            "alive_fp32": [int(k) for k in da["count"]], "alive_fp64": [int(k) for k in db["count"]],
+            "electron_collisions_fp32": collisions(a), "electron_collisions_fp64": collisions(b),
+            "horizons": per}
     out = {"field_solve": solve, "trajectories": traj}
-    for c in (a, b):
-        c["sim"].close()
-        c["particles"].close()
+    for case in (a, b, c):
+        case["sim"].close()
+        case["particles"].close()
+    return out
+
+
+def fp64_rate(cells, bw, steps=40):
+    """particle-steps/s of the same workload with FP64 particles and the FP64
+    multigrid (the reference's precision), for context beside the FP32 line"""
+    import pic_cuda_amd as pic
+    from pic_cuda_amd import configs
+    c = configs.build(cells, 20000000, pic.PREC_FP64, mcc=True, bin_w=bw, bin_h=bw)
+    sim = c["sim"]
+    sim.step(5)
+    pic.synchronize()
+    st0 = sim.stats()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    pic.synchronize()
+    el = time.perf_counter() - t0
+    st1 = sim.stats()
+    out = {"value": steps * 0.5 * (st0.alive + st1.alive) / el, "ms_per_step": round(1e3 * el / steps, 4),
+           "push_ms": round((st1.ms_push - st0.ms_push) / steps, 4),
+           "field_ms": round((st1.ms_field - st0.ms_field) / steps, 4), "steps": steps}
+    sim.close()
+    c["particles"].close()
     return out
 
 
@@ -482,9 +537,13 @@ def main():
               flush=True)
         cpu = cpu_baseline(case)
     acc = None
+    rate64 = None
     if not a.no_fp64_check:
         print("bench: comparing 20 FP32 steps with the FP64 path", file=sys.stderr, flush=True)
         acc = fp64_check(a.cells, a.scale, a.bin)
+        if a.scale == 1.0:
+            print("bench: the FP64 rate", file=sys.stderr, flush=True)
+            rate64 = fp64_rate(a.cells, a.bin)
     cycles = (st1.total_mg_cycles - st0.total_mg_cycles) / a.steps
     line = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps,
@@ -497,7 +556,7 @@ def main():
         "roofline": roof, "cpu_baseline": cpu,
         "field_ms_per_step": round(field_ms, 4), "mg_cycles_per_step": cycles,
         "alive_start": int(st0.alive), "alive_end": int(st1.alive),
-        "fp64_error": acc,
+        "fp64_error": acc, "fp64_particles": rate64,
     }
     print(json.dumps(line))
 

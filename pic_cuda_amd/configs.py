@@ -87,7 +87,7 @@ def build(cells, n_per_species, precision, mcc, seed=1, case_id=0, mg=None, bin_
         mg = mg_params(PRESET_STEP_SOR, mode=MG_MIXED if precision == PREC_FP32 else MG_FP64)
     sim = Simulation(parts, grid, bnd, xs, sim_params(dt=pl.dt, seed=seed, case_id=case_id, bin_w=bin_w,
                                                         bin_h=bin_h, halo=halo, mcc=1 if mcc else 0, mg=mg))
-    return dict(plasma=pl, grid=grid, particles=parts, walls=bnd, xs=xs, sim=sim, mg=mg)
+    return dict(plasma=pl, grid=grid, particles=parts, walls=bnd, xs=xs, sim=sim, mg=mg, n_per_species=n_per_species)
 
 
 def plain_small(scale=1.0, **kw):
