@@ -202,10 +202,13 @@ __device__ inline void deposit(T x, T y, int nx, int ny, bool use_lds, int wx0, 
     const int ix = (int)dfloor(x), iy = (int)dfloor(y);
     if (ix < 0 || ix >= nx - 1 || iy < 0 || iy >= ny - 1) return;
     const T fx = x - (T)ix, fy = y - (T)iy;
-    const unsigned long long W = 1ull << PIC_QBITS;
-    const unsigned long long qx = (unsigned)(fx * (T)W), qy = (unsigned)(fy * (T)W);
-    const unsigned long long w00 = (W - qx) * (W - qy), w10 = qx * (W - qy);
-    const unsigned long long w01 = (W - qx) * qy, w11 = qx * qy;
+    const unsigned W = 1u << PIC_QBITS;
+    // fx, fy in [0, 1): qx, qy and W - qx, W - qy fit in 21 bits (the masks
+    // change no value and let the products use the 24-bit multiplier)
+    const unsigned qx = (unsigned)(fx * (T)W) & 0x1FFFFFu, qy = (unsigned)(fy * (T)W) & 0x1FFFFFu;
+    const unsigned px = (W - qx) & 0x1FFFFFu, py = (W - qy) & 0x1FFFFFu;
+    const unsigned long long w00 = (unsigned long long)px * py, w10 = (unsigned long long)qx * py;
+    const unsigned long long w01 = (unsigned long long)px * qy, w11 = (unsigned long long)qx * qy;
     const int lx = ix - wx0, ly = iy - wy0;
     if (use_lds && lx >= 0 && lx < aw - 1 && ly >= 0 && ly < ah - 1) {
         unsigned long long* b = lds + ly * aw + lx;
@@ -671,18 +674,32 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
         nq = 0;
     };
 
+    // The particles of the next pass are loaded before this pass's stores:
+    // the memory counter completes in order, so waiting for them never waits
+    // for the stores, and no default value or branch precedes the loads (every
+    // lane loads; lanes past the end load the last particle, n > 0).
+    T px, py, pvx, pvy;
+    uint32_t pid;
+    {
+        const int src = rg.at(min((int)threadIdx.x, n - 1));
+        px = G(S.x[in])[src];
+        py = G(S.y[in])[src];
+        pvx = G(S.vx[in])[src];
+        pvy = G(S.vy[in])[src];
+        pid = G(S.id[in])[src];
+    }
     for (int base = 0; base < n; base += blockDim.x) {
         const int t = base + threadIdx.x;
         const bool valid = t < n;
-        T x = (T)0, y = (T)0, vx = (T)0, vy = (T)0;
-        uint32_t id = 0;
-        if (valid) {
-            const int src = rg.at(t);
-            x = G(S.x[in])[src];
-            y = G(S.y[in])[src];
-            vx = G(S.vx[in])[src];
-            vy = G(S.vy[in])[src];
-            id = G(S.id[in])[src];
+        T x = px, y = py, vx = pvx, vy = pvy;
+        uint32_t id = pid;
+        if (base + (int)blockDim.x < n) {
+            const int src = rg.at(min(t + (int)blockDim.x, n - 1));
+            px = G(S.x[in])[src];
+            py = G(S.y[in])[src];
+            pvx = G(S.vx[in])[src];
+            pvy = G(S.vy[in])[src];
+            pid = G(S.id[in])[src];
         }
         if (valid && kick) {
             const int ix = (int)dfloor(x), iy = (int)dfloor(y);
