@@ -40,10 +40,11 @@
 namespace pic {
 
 #define MG_TILE 32
-// largest level handled by the single-workgroup coarse kernel (its LDS stays
-// below 64 KB: FP32 65^2, FP64 33^2)
+// largest level handled by the single-workgroup coarse kernel: 65^2 in both
+// precisions (FP64: the levels' e and rhs and the residual scratch take
+// 125 KB of LDS, within gfx950's 160 KB per workgroup)
 #define MG_NC_F32 65
-#define MG_NC_F64 33
+#define MG_NC_F64 65
 // direct coarse solve: levels up to 65 nodes (63 unknowns a side); the two
 // FP64 LDS buffers have row stride MG_DS (63 x 63 each: 63.5 KB in all, and
 // 16 consecutive rows fall in distinct bank pairs)
