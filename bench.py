@@ -338,14 +338,10 @@ def export(reps=20):
             "frame_ms": round(1e3 * frame_s, 2), "frame_MB": round(size / 1e6, 1)}
 
 
-def serve(n_jobs=24, rate=20.0, budget=200e9, seed=9, max_steps=2000):
-    """Serving: Poisson arrivals (rate jobs/s), cells log-uniform 64..1024
-    (rounded to GRID_SIZES), 100 particles per cell and species, steps U(200, max),
-    seed = job id, admission by an HBM budget, continuous batching."""
+def _serve_stream(jobs, budget):
+    """run a job stream through the server on the wall clock; per-job results"""
     import numpy as np
     import pic_cuda_amd as pic
-    from pic_cuda_amd import configs
-    jobs = configs.serve_jobs(n_jobs, rate, seed, max_steps=max_steps)
     t_arr = [j["arrival_s"] for j in jobs]
     sv = pic.Server(budget, 256, pic.PREC_FP32)
     ids = [sv.submit(**j) for j in jobs]
@@ -358,18 +354,55 @@ def serve(n_jobs=24, rate=20.0, budget=200e9, seed=9, max_steps=2000):
         if run == 0 and wait == 0:
             break
         if run == 0:
-            time.sleep(max(0.0, min(0.05, float(min(t_arr[i] for i in range(n_jobs)
+            time.sleep(max(0.0, min(0.05, float(min(t_arr[i] for i in range(len(jobs))
                                                          if sv.result(ids[i]).state == 0)) - now)))
     wall = time.perf_counter() - t0
     results = [sv.result(j) for j in ids]
     step_ms = np.concatenate([sv.step_ms(j) for j in ids])
     comp = np.array([r.done_s - r.arrival_s for r in results])
+    wait_s = np.array([r.admit_s - r.arrival_s for r in results])
     psteps = sum(r.particles * r.steps_done for r in results)
     sv.close()
-    return {"jobs": n_jobs, "rate_per_s": rate, "wall_s": round(wall, 3), "jobs_per_s": n_jobs / wall,
-            "ticks": ticks, "particle_steps_per_s": psteps / wall,
+    return {"jobs": len(jobs), "wall_s": round(wall, 3), "jobs_per_s": len(jobs) / wall, "ticks": ticks,
+            "particle_steps_per_s": psteps / wall,
             "step_ms_p50": float(np.percentile(step_ms, 50)), "step_ms_p99": float(np.percentile(step_ms, 99)),
-            "completion_s_p50": float(np.percentile(comp, 50)), "completion_s_p99": float(np.percentile(comp, 99))}
+            "completion_s_p50": float(np.percentile(comp, 50)), "completion_s_p99": float(np.percentile(comp, 99)),
+            "queue_wait_s_p50": float(np.percentile(wait_s, 50)), "queue_wait_s_p99": float(np.percentile(wait_s, 99))}
+
+
+def serve(n_jobs=24, rate=20.0, budget=200e9, seed=9, max_steps=2000):
+    """Serving: Poisson arrivals (rate jobs/s), cells log-uniform 64..1024
+    (rounded to GRID_SIZES), 100 particles per cell and species, steps U(200, max),
+    seed = job id, admission by an HBM budget, continuous batching.
+    stream: the stated arrival rate (20/s, far above what one GPU serves, so the
+    queue fills and the run measures saturated throughput); steady: the same
+    jobs arriving at 60 % of that throughput (the queue stays short: the
+    latencies of a sustained load); sequential: the same jobs one after
+    another through the single-plasma path (no batching), the baseline."""
+    import pic_cuda_amd as pic
+    from pic_cuda_amd import configs
+    jobs = configs.serve_jobs(n_jobs, rate, seed, max_steps=max_steps)
+    out = {"jobs": n_jobs, "rate_per_s": rate, "budget_GB": budget / 1e9}
+    out["stream"] = _serve_stream(jobs, budget)
+    t0 = time.perf_counter()
+    for j in jobs:
+        c = configs.build(j["cells"], int(round(j["ppc"] * j["cells"] * j["cells"])), pic.PREC_FP32, mcc=True,
+                          seed=j["seed"])
+        c["sim"].step(j["steps"])
+        pic.synchronize()
+        c["sim"].close()
+        c["particles"].close()
+        del c
+    seq = time.perf_counter() - t0
+    out["sequential"] = {"wall_s": round(seq, 3), "jobs_per_s": n_jobs / seq}
+    slow = 0.6 * out["stream"]["jobs_per_s"]
+    steady = [dict(j, arrival_s=j["arrival_s"] * rate / slow) for j in jobs]
+    out["steady"] = dict(_serve_stream(steady, budget), rate_per_s=round(slow, 3))
+    out["speedup_stream_over_sequential"] = round(out["stream"]["jobs_per_s"] / out["sequential"]["jobs_per_s"], 2)
+    # flat keys of the saturated stream (the stated workload)
+    for k in ("jobs_per_s", "step_ms_p50", "step_ms_p99", "completion_s_p50", "completion_s_p99"):
+        out[k] = out["stream"][k]
+    return out
 
 
 def fp64_check(cells, scale, bw, horizons=(1, 5, 20)):
