@@ -86,7 +86,10 @@ int pic_device_to_host(void* host, const void* dev, int64_t bytes);
 /* ---- Particles2D: particles.cuh:3-39, particles.cu:112-261 --------------- */
 /* Particles2D(int num_particles) particles.cu:112-150.  Species 0 exists with
    m = 1, q = 0.1 like the reference constructor (particles.cu:146-149).
-   max_particles is the capacity PER SPECIES.  precision: PIC_PREC_FP32/64. */
+   max_particles is the capacity PER SPECIES.  precision: PIC_PREC_FP32/64.
+   A species holds about 1.6 max_particles + 2^20 slots in each of its two
+   buffers, at most 2^30 (FP32) or 2^29 (FP64): the particle kernels address
+   each array with 32-bit byte offsets. */
 int pic_particles_create(int64_t max_particles, int precision, pic_particles** out);
 /* ~Particles2D() particles.cu:153-171 */
 int pic_particles_destroy(pic_particles* p);

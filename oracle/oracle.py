@@ -71,6 +71,7 @@ def _load(omp=False):
     d, i, u32, i64 = ctypes.c_double, ctypes.c_int, ctypes.c_uint32, ctypes.c_int64
     vp = ctypes.c_void_p
     lib.orc_philox4x32_10.argtypes = [vp, vp, vp]
+    lib.orc_philox2x32_10.argtypes = [vp, u32, vp]
     lib.orc_det_log.restype = d
     lib.orc_det_log.argtypes = [d]
     lib.orc_efield.argtypes = [vp, i, i, d, d, vp, vp]
@@ -115,6 +116,13 @@ def philox(ctr, key):
     k = np.asarray(key, dtype=np.uint32)
     o = np.zeros(4, dtype=np.uint32)
     lib().orc_philox4x32_10(_p(c), _p(k), _p(o))
+    return o
+
+
+def philox2(ctr, key):
+    c = np.asarray(ctr, dtype=np.uint32)
+    o = np.zeros(2, dtype=np.uint32)
+    lib().orc_philox2x32_10(_p(c), int(key) & 0xFFFFFFFF, _p(o))
     return o
 
 

@@ -47,6 +47,36 @@ extern "C" void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], 
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
+/* Philox2x32-10 (same paper).  The collision-candidacy draw of every       */
+/* particle and step: first word over counter {id, step} with key            */
+/* cand_key = first word over {seed, case} with key species.                 */
+extern "C" void orc_philox2x32_10(const uint32_t ctr[2], uint32_t key, uint32_t out[2]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], k = key;
+    for (int r = 0; r < 10; ++r) {
+        if (r) k += 0x9E3779B9u;
+        uint64_t p = (uint64_t)0xD256D193u * c0;
+        uint32_t hi = (uint32_t)(p >> 32), lo = (uint32_t)p;
+        c0 = hi ^ k ^ c1;
+        c1 = lo;
+    }
+    out[0] = c0; out[1] = c1;
+}
+
+static uint32_t cand_key(uint32_t seed, uint32_t case_id, uint32_t species) {
+    const uint32_t c[2] = {seed, case_id};
+    uint32_t o[2];
+    orc_philox2x32_10(c, species, o);
+    return o[0];
+}
+
+template <typename T>
+static T cand_uni(uint32_t id, uint32_t step, uint32_t key) {
+    const uint32_t c[2] = {id, step};
+    uint32_t o[2];
+    orc_philox2x32_10(c, key, o);
+    return (T)(o[0] >> 8) * (T)0x1p-24;
+}
+
 /* Deterministic natural log: x = m 2^e with m in [sqrt(1/2), sqrt(2)),
    log m = 2 atanh(f), f = (m-1)/(m+1), odd series to f^23, Horner with fma. */
 extern "C" double orc_det_log(double x) {
@@ -476,8 +506,9 @@ static inline void push_one(const orc_push_params* p, const double* phi, const T
     bool have_born = false;
     PP<T> be{}, bi{};
     if (p->do_mcc && p->n_proc > 0) {
+        T u0 = cand_uni<T>(id, p->step, cand_key(p->seed, p->case_id, p->species));
         Stream rs(id, p->step, p->case_id, p->seed, p->species);
-        T u0 = rs.uni<T>(), u1 = rs.uni<T>();
+        T u1 = rs.uni<T>();
         T v2 = std::fma(vx, vx, vy * vy);
         T E = efac * v2;
         const int band = mcc_band(E, (T)p->inv_emax);

@@ -43,6 +43,7 @@ enum { ORC_ELASTIC = 0, ORC_EXCITATION = 1, ORC_IONIZATION = 2, ORC_CHARGE_EXCHA
 
 /* ---- random numbers -------------------------------------------------- */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void orc_philox2x32_10(const uint32_t ctr[2], uint32_t key, uint32_t out[2]);
 double orc_det_log(double x);
 
 /* ---- fields ------------------------------------------------------------ */

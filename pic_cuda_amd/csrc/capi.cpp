@@ -58,8 +58,11 @@ static void fill_species_ptrs(pic_particles* p, int s) {
         S.cap[par] = st.d_cap[par];
         S.count[par] = st.d_count[par];
         S.mcount[par] = st.d_mcount[par];
+        S.blk[par] = (char*)st.x[par];
+        S.lay[par] = st.d_count[par];
         S.nb[par] = st.nb;
     }
+    S.pcap = (uint32_t)st.capacity;
     S.counters = st.d_counters;
     S.key = (uint32_t)s;
     S.other = -1;
@@ -99,17 +102,16 @@ void items_for_species(const pic_particles* p, int s, int chunk, std::vector<Wor
 }
 
 static int alloc_layout_parity(SpeciesStore& st, int par, int entries) {
-    if (st.d_start[par]) PIC_HIP(hipFree(st.d_start[par]));
-    if (st.d_cap[par]) PIC_HIP(hipFree(st.d_cap[par]));
+    // one block: count, mcount, start, cap (entries each)
     if (st.d_count[par]) PIC_HIP(hipFree(st.d_count[par]));
-    if (st.d_mcount[par]) PIC_HIP(hipFree(st.d_mcount[par]));
     st.d_start[par] = st.d_cap[par] = st.d_count[par] = st.d_mcount[par] = nullptr;
-    PIC_HIP(hipMalloc(&st.d_start[par], (size_t)entries * sizeof(int32_t)));
-    PIC_HIP(hipMalloc(&st.d_cap[par], (size_t)entries * sizeof(int32_t)));
-    PIC_HIP(hipMalloc(&st.d_count[par], (size_t)entries * sizeof(int32_t)));
-    PIC_HIP(hipMalloc(&st.d_mcount[par], (size_t)entries * sizeof(int32_t)));
-    PIC_HIP(hipMemset(st.d_count[par], 0, (size_t)entries * sizeof(int32_t)));
-    PIC_HIP(hipMemset(st.d_mcount[par], 0, (size_t)entries * sizeof(int32_t)));
+    int32_t* b = nullptr;
+    PIC_HIP(hipMalloc(&b, 4 * (size_t)entries * sizeof(int32_t)));
+    st.d_count[par] = b;
+    st.d_mcount[par] = b + entries;
+    st.d_start[par] = b + 2 * (size_t)entries;
+    st.d_cap[par] = b + 3 * (size_t)entries;
+    PIC_HIP(hipMemset(b, 0, 2 * (size_t)entries * sizeof(int32_t)));
     return 0;
 }
 
@@ -123,14 +125,18 @@ static int create_species(pic_particles* p, double m, double q) {
     // bins get 1.5x their population plus 64 slots at every re-binning
     const int64_t cap = p->max_per_species + p->max_per_species / 2 + p->max_per_species / 8 + (1 << 20);
     PIC_REQUIRE(cap < (int64_t)INT32_MAX, "capacity per species must stay below 2^31 slots");
+    PIC_REQUIRE(cap * (int64_t)p->prec < ((int64_t)1 << 32), "capacity per species must stay below 2^32 bytes per array");
     st.capacity = cap;
     const size_t eb = (size_t)p->prec;
     for (int par = 0; par < 2; ++par) {
-        PIC_HIP(hipMalloc(&st.x[par], cap * eb));
-        PIC_HIP(hipMalloc(&st.y[par], cap * eb));
-        PIC_HIP(hipMalloc(&st.vx[par], cap * eb));
-        PIC_HIP(hipMalloc(&st.vy[par], cap * eb));
-        PIC_HIP(hipMalloc(&st.id[par], cap * sizeof(uint32_t)));
+        // one block: x, y, vx, vy, id (cap slots each)
+        char* b = nullptr;
+        PIC_HIP(hipMalloc(&b, cap * (4 * eb + sizeof(uint32_t))));
+        st.x[par] = b;
+        st.y[par] = b + cap * eb;
+        st.vx[par] = b + 2 * cap * eb;
+        st.vy[par] = b + 3 * cap * eb;
+        st.id[par] = (uint32_t*)(b + 4 * cap * eb);
         if (int rc = alloc_layout_parity(st, par, 1)) return rc;
     }
     st.layout_alloc = 1;
@@ -242,15 +248,8 @@ extern "C" int pic_particles_destroy(pic_particles* p) {
     (void)hipStreamSynchronize(lib_stream());
     for (auto& st : p->sp) {
         for (int par = 0; par < 2; ++par) {
-            (void)hipFree(st.x[par]);
-            (void)hipFree(st.y[par]);
-            (void)hipFree(st.vx[par]);
-            (void)hipFree(st.vy[par]);
-            (void)hipFree(st.id[par]);
-            (void)hipFree(st.d_start[par]);
-            (void)hipFree(st.d_cap[par]);
-            (void)hipFree(st.d_count[par]);
-            (void)hipFree(st.d_mcount[par]);
+            (void)hipFree(st.x[par]);         // the particle block
+            (void)hipFree(st.d_count[par]);   // the layout block
         }
         (void)hipFree(st.d_counters);
     }
@@ -1255,6 +1254,7 @@ static int rebin_t(pic_sim* S, int bw, int bh) {
         D.cap[out] = st.d_cap[out];
         D.count[out] = st.d_count[out];
         D.mcount[out] = st.d_mcount[out];
+        D.lay[out] = st.d_count[out];
         D.nb[out] = nb_new;
     }
     if (int rc = upload_case(p)) return rc;
@@ -1281,6 +1281,7 @@ static int rebin_t(pic_sim* S, int bw, int bh) {
         D.cap[in] = st.d_cap[in];
         D.count[in] = st.d_count[in];
         D.mcount[in] = st.d_mcount[in];
+        D.lay[in] = st.d_count[in];
         D.nb[in] = nb_new;
     }
     C.bw = bw;

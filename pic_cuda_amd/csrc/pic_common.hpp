@@ -69,6 +69,37 @@ __host__ __device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
+// Philox2x32-10 (same paper): counter {c0, c1}, key k; one 32-bit product per
+// round against two for the 4x32 form.
+__host__ __device__ inline void philox2x32_10(uint32_t c0, uint32_t c1, uint32_t k, uint32_t out[2]) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) k += 0x9E3779B9u;
+        const uint64_t p = (uint64_t)0xD256D193u * c0;
+        const uint32_t hi = (uint32_t)(p >> 32), lo = (uint32_t)p;
+        c0 = hi ^ k ^ c1;
+        c1 = lo;
+    }
+    out[0] = c0;
+    out[1] = c1;
+}
+
+// The collision-candidacy draw, made for every particle at every step: the
+// first word of Philox2x32-10 over counter {id, step}, keyed by cand_key of
+// (seed, case, species) (itself the first word of Philox2x32-10 over {seed,
+// case} with key species).  The collision itself draws from Stream.
+__host__ __device__ inline uint32_t cand_key(uint32_t seed, uint32_t case_id, uint32_t species) {
+    uint32_t o[2];
+    philox2x32_10(seed, case_id, species, o);
+    return o[0];
+}
+template <typename T>
+__host__ __device__ inline T cand_uni(uint32_t id, uint32_t step, uint32_t key) {
+    uint32_t o[2];
+    philox2x32_10(id, step, key, o);
+    return (T)(o[0] >> 8) * (T)0x1p-24;
+}
+
 // Natural log with a fixed evaluation order: x = m 2^e, m in [sqrt(1/2), sqrt(2)),
 // log m = 2 atanh((m-1)/(m+1)) by the odd series to degree 23 (Horner, fma).
 __host__ __device__ inline double det_log(double x) {

@@ -21,6 +21,12 @@ struct SpeciesDev {
     int32_t* count[2];         // front part of each bin (survivors that stayed); spill fill at nb
     int32_t* mcount[2];        // back part of each bin (particles that moved in)
     int nb[2];
+    // the same arrays packed: x, y, vx, vy, id of parity p in one block of
+    // pcap slots per array (blk[p]); count, mcount, start, cap of parity p in
+    // one block of 4 x (nb[p] + 1) entries (lay[p])
+    char* blk[2];
+    int32_t* lay[2];
+    uint32_t pcap;
     // physics
     T kq;                      // q dt / m
     T efac;                    // E[eV] = efac * v^2

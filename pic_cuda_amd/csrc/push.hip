@@ -52,6 +52,40 @@ __device__ inline double ey_node(const double* __restrict__ phi_, int nx, int ny
     return -(phi[(size_t)(j + 1) * nx + i] - phi[(size_t)(j - 1) * nx + i]) / (2.0 * dy);
 }
 
+// a species' arrays of one parity (pointers into its packed blocks)
+template <typename T>
+struct PartsOf {
+    T* x;
+    T* y;
+    T* vx;
+    T* vy;
+    uint32_t* id;
+};
+template <typename T>
+__device__ inline PartsOf<T> parts_of(const SpeciesDev<T>& S, int par) {
+    T* b = (T*)S.blk[par];
+    const size_t c = S.pcap;
+    return {b, b + c, b + 2 * c, b + 3 * c, (uint32_t*)(b + 4 * c)};
+}
+struct LayOf {
+    int32_t* count;
+    int32_t* mcount;
+    const int32_t* start;
+    const int32_t* cap;
+};
+template <typename T>
+__device__ inline LayOf lay_of(const SpeciesDev<T>& S, int par) {
+    int32_t* b = S.lay[par];
+    const int e = S.nb[par] + 1;
+    return {b, b + e, b + 2 * e, b + 3 * e};
+}
+// element i of an array: a 32-bit byte offset from the array's base (which is
+// uniform and stays in scalar registers); every array holds under 2^32 bytes
+template <typename T>
+__device__ inline gptr<T> el(T* a, uint32_t i) {
+    return (gptr<T>)((gptr<char>)G((char*)a) + i * (uint32_t)sizeof(T));
+}
+
 __device__ inline uint64_t lanemask_lt() {
     const int lane = __lane_id();
     return (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -73,6 +107,7 @@ __device__ inline int alloc_slot(const SpeciesDev<T>& S, int out, bool want, int
     const int lane = __lane_id();
     const uint64_t lt = lanemask_lt();
     const int nb = S.nb[out];
+    const LayOf L = lay_of(S, out);
     const bool binned = want && key != nb;
     // this lane's group: the lanes that want the same bin
     uint64_t group = 0;
@@ -86,13 +121,13 @@ __device__ inline int alloc_slot(const SpeciesDev<T>& S, int out, bool want, int
     }
     int cap = 0, start = 0;
     if (binned) {
-        cap = G(S.cap[out])[key];
-        start = G(S.start[out])[key];
+        cap = *el(L.cap, key);
+        start = *el(L.start, key);
     }
     const int leader = binned ? __ffsll((unsigned long long)group) - 1 : lane;
     int base = 0;
     if (binned && leader == lane)
-        base = gatomic_add(back ? &S.mcount[out][key] : &S.count[out][key], (int)__popcll(group));
+        base = gatomic_add(back ? L.mcount + key : L.count + key, (int)__popcll(group));
     base = __shfl(base, leader);
     int dst = -1;
     bool spill = want && !binned;
@@ -110,11 +145,11 @@ __device__ inline int alloc_slot(const SpeciesDev<T>& S, int out, bool want, int
     if (om) {
         const int sl = __ffsll((unsigned long long)om) - 1;
         int sbase = 0;
-        if (lane == sl) sbase = gatomic_add(&S.count[out][nb], (int)__popcll(om));
+        if (lane == sl) sbase = gatomic_add(L.count + nb, (int)__popcll(om));
         sbase = __shfl(sbase, sl);
         if (spill) {
             const int s2 = sbase + __popcll(om & lt);
-            dst = (s2 < G(S.cap[out])[nb]) ? G(S.start[out])[nb] + s2 : -1;
+            dst = (s2 < *el(L.cap, nb)) ? *el(L.start, nb) + s2 : -1;
         }
     }
     return dst;
@@ -282,13 +317,24 @@ __device__ inline int mcc_band(T E, T inv_emax) {
     return min(14 + e, 15);
 }
 
-// The null-collision candidacy test of mcc_collide alone: the first uniform
-// of the particle's draw against the candidate probability of its energy band.
+// The candidacy uniform on the device.  The key passes through an empty asm
+// statement inside the particle loop, so its ten round keys are formed there
+// (one scalar add each) instead of being hoisted into ten scalar registers
+// for the whole loop (they were spilled to vector lanes and read back one by
+// one).
+template <typename T>
+__device__ inline T cand_uni_dev(uint32_t id, uint32_t step, uint32_t key) {
+    asm volatile("" : "+s"(key));
+    return cand_uni<T>(id, step, key);
+}
+
+// The null-collision candidacy test of mcc_collide alone: the candidacy draw
+// against the candidate probability of the particle's energy band.  ckey:
+// cand_key(seed, case, species).
 template <typename T>
 __device__ inline bool mcc_candidate(const SpeciesDev<T>& S, const T* band_lds, uint32_t id, uint32_t step,
-                                     uint32_t case_id, uint32_t seed, T vx, T vy) {
-    Stream rs(id, step, case_id, seed, S.key);
-    const T u0 = rs.uni<T>();
+                                     uint32_t ckey, T vx, T vy) {
+    const T u0 = cand_uni_dev<T>(id, step, ckey);
     const T v2 = dfma(vx, vx, vy * vy);
     const T E = S.efac * v2;
     return u0 < band_lds[mcc_band(E, S.inv_emax)];
@@ -299,10 +345,11 @@ __device__ inline bool mcc_candidate(const SpeciesDev<T>& S, const T* band_lds, 
 // majorant is that of the particle's energy band (band values in LDS).
 template <typename T>
 __device__ inline bool mcc_collide(const SpeciesDev<T>& S, const T* band_lds, uint32_t id, uint32_t step,
-                                   uint32_t case_id, uint32_t seed, T x, T y, T& vx, T& vy, unsigned* lcnt,
-                                   Born<T>& b) {
+                                   uint32_t case_id, uint32_t seed, uint32_t ckey, T x, T y, T& vx, T& vy,
+                                   unsigned* lcnt, Born<T>& b) {
+    const T u0 = cand_uni_dev<T>(id, step, ckey);
     Stream rs(id, step, case_id, seed, S.key);
-    const T u0 = rs.uni<T>(), u1 = rs.uni<T>();
+    const T u1 = rs.uni<T>();
     const T v2 = dfma(vx, vx, vy * vy);
     const T E = S.efac * v2;
     const int band = mcc_band(E, S.inv_emax);
@@ -379,16 +426,17 @@ struct Range {
 template <typename T>
 __device__ inline Range item_range(const SpeciesDev<T>& S, int in, const WorkItem& it, int chunk) {
     const int nb = S.nb[in];
+    const LayOf L = lay_of(S, in);
     Range r;
     if (it.bin < nb) {
-        const int start = G(S.start[in])[it.bin], cap = G(S.cap[in])[it.bin];
-        r.n1 = min(G(S.count[in])[it.bin], front_cap(cap));
-        r.n2 = min(G(S.mcount[in])[it.bin], back_cap(cap));
+        const int start = *el(L.start, it.bin), cap = *el(L.cap, it.bin);
+        r.n1 = min(*el(L.count, it.bin), front_cap(cap));
+        r.n2 = min(*el(L.mcount, it.bin), back_cap(cap));
         r.b1 = start;
         r.b2 = start + cap - r.n2;
     } else {
-        const int tot = min(G(S.count[in])[nb], G(S.cap[in])[nb]);
-        r.b1 = G(S.start[in])[nb] + it.chunk * chunk;
+        const int tot = min(*el(L.count, nb), *el(L.cap, nb));
+        r.b1 = *el(L.start, nb) + it.chunk * chunk;
         r.n1 = max(0, min(chunk, tot - it.chunk * chunk));
         r.b2 = 0;
         r.n2 = 0;
@@ -430,12 +478,13 @@ __device__ inline void flush_stage(const SpeciesDev<T>& S, int out, const Stage<
         const int dst = alloc_slot(S, out, want, key, true);
         if (want) {
             if (dst >= 0) {
-                G(S.x[out])[dst] = sg.x[e];
-                G(S.y[out])[dst] = sg.y[e];
-                G(S.vx[out])[dst] = sg.vx[e];
-                G(S.vy[out])[dst] = sg.vy[e];
-                G(S.id[out])[dst] = sg.id[e];
-                if (dst >= G(S.start[out])[S.nb[out]]) atomicAdd(lcnt + PIC_C_SPILLED, 1u);
+                const PartsOf<T> P = parts_of(S, out);
+                *el(P.x, dst) = sg.x[e];
+                *el(P.y, dst) = sg.y[e];
+                *el(P.vx, dst) = sg.vx[e];
+                *el(P.vy, dst) = sg.vy[e];
+                *el(P.id, dst) = sg.id[e];
+                if (dst >= *el(lay_of(S, out).start, S.nb[out])) atomicAdd(lcnt + PIC_C_SPILLED, 1u);
             } else {
                 atomicAdd(lcnt + PIC_C_LOST, 1u);
             }
@@ -528,6 +577,7 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
     // the kick and collisions of a pass complete step st.step - 1 (the drift
     // starts step st.step), so collisions are keyed on st.step - 1
     const uint32_t mcc_step = mcc_key >= 0 ? (uint32_t)mcc_key : st.step - 1u;
+    const uint32_t ckey = cand_key(C.seed, C.case_id, S.key);
 
     for (int k = threadIdx.x; k < PIC_NCOUNT; k += blockDim.x) lcnt[k] = 0u;
     if (threadIdx.x < 32) band_lds[threadIdx.x] = threadIdx.x < 16 ? S.band_pnull[threadIdx.x] : S.band_numax[threadIdx.x - 16];
@@ -554,8 +604,9 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
     const T kq = S.kq, dtdx = C.dtdx, dtdy = C.dtdy;
     const SpeciesDev<T>* O = (S.other >= 0) ? &C.sp[S.other] : nullptr;
     const int wall_bin = is_bin ? it.bin : -1;
-    const int home_start = is_bin ? G(S.start[out])[it.bin] : 0;
-    const int home_front = is_bin ? front_cap(G(S.cap[out])[it.bin]) : 0;
+    const LayOf Lo = lay_of(S, out);
+    const int home_start = is_bin ? *el(Lo.start, it.bin) : 0;
+    const int home_front = is_bin ? front_cap(*el(Lo.cap, it.bin)) : 0;
     const int lane = __lane_id();
 
     // Drift, walls, deposit and compaction of one particle per lane after the
@@ -568,7 +619,7 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
         int key = 0, key_e = 0, key_i = 0;
         Born<T> b;
         if (valid) {
-            if (FULL) born = mcc_collide(S, band_lds, id, mcc_step, C.case_id, C.seed, x, y, vx, vy, lcnt, b);
+            if (FULL) born = mcc_collide(S, band_lds, id, mcc_step, C.case_id, C.seed, ckey, x, y, vx, vy, lcnt, b);
             const T x0 = x, y0 = y;
             if (drift) {
                 x = dfma(vx, dtdx, x);
@@ -619,11 +670,12 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
                 const int slot = sbase + __popcll(sm & lanemask_lt());
                 if (slot < home_front) {
                     const int dst = home_start + slot;
-                    G(S.x[out])[dst] = x;
-                    G(S.y[out])[dst] = y;
-                    G(S.vx[out])[dst] = vx;
-                    G(S.vy[out])[dst] = vy;
-                    G(S.id[out])[dst] = id;
+                    const PartsOf<T> P = parts_of(S, out);
+                    *el(P.x, dst) = x;
+                    *el(P.y, dst) = y;
+                    *el(P.vx, dst) = vx;
+                    *el(P.vy, dst) = vy;
+                    *el(P.id, dst) = id;
                 } else {
                     stay = false;   // front part full: goes with the movers
                 }
@@ -638,11 +690,12 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
                 const int di = alloc_slot(*O, out, bi_alive, key_i, true);
                 if (bi_alive) {
                     if (di >= 0) {
-                        G(O->x[out])[di] = b.ix;
-                        G(O->y[out])[di] = b.iy;
-                        G(O->vx[out])[di] = b.ivx;
-                        G(O->vy[out])[di] = b.ivy;
-                        G(O->id[out])[di] = b.iid;
+                        const PartsOf<T> P = parts_of(*O, out);
+                        *el(P.x, di) = b.ix;
+                        *el(P.y, di) = b.iy;
+                        *el(P.vx, di) = b.ivx;
+                        *el(P.vy, di) = b.ivy;
+                        *el(P.id, di) = b.iid;
                     } else {
                         atomicAdd(lcnt + PIC_C_LOST, 1u);
                     }
@@ -682,11 +735,12 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
     uint32_t pid;
     {
         const int src = rg.at(min((int)threadIdx.x, n - 1));
-        px = G(S.x[in])[src];
-        py = G(S.y[in])[src];
-        pvx = G(S.vx[in])[src];
-        pvy = G(S.vy[in])[src];
-        pid = G(S.id[in])[src];
+        const PartsOf<T> P = parts_of(S, in);
+        px = *el(P.x, src);
+        py = *el(P.y, src);
+        pvx = *el(P.vx, src);
+        pvy = *el(P.vy, src);
+        pid = *el(P.id, src);
     }
     for (int base = 0; base < n; base += blockDim.x) {
         const int t = base + threadIdx.x;
@@ -695,11 +749,12 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
         uint32_t id = pid;
         if (base + (int)blockDim.x < n) {
             const int src = rg.at(min(t + (int)blockDim.x, n - 1));
-            px = G(S.x[in])[src];
-            py = G(S.y[in])[src];
-            pvx = G(S.vx[in])[src];
-            pvy = G(S.vy[in])[src];
-            pid = G(S.id[in])[src];
+            const PartsOf<T> P = parts_of(S, in);
+            px = *el(P.x, src);
+            py = *el(P.y, src);
+            pvx = *el(P.vx, src);
+            pvy = *el(P.vy, src);
+            pid = *el(P.id, src);
         }
         if (valid && kick) {
             const int ix = (int)dfloor(x), iy = (int)dfloor(y);
@@ -735,7 +790,7 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
         // which is processed (one candidate per lane) whenever it would overflow
         bool cand = false;
         if (mcc) {
-            cand = valid && mcc_candidate(S, band_lds, id, mcc_step, C.case_id, C.seed, vx, vy);
+            cand = valid && mcc_candidate(S, band_lds, id, mcc_step, ckey, vx, vy);
             const uint64_t cmask = __ballot(cand);
             if (cmask) {
                 if (nq + __popcll(cmask) > PIC_MQUEUE) flush_queue();
@@ -756,7 +811,7 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
     if (nmw > 0) flush_stage(S, out, sg, nmw, lcnt);
     // every wavefront's stay count, deposits and counters are in LDS now
     __syncthreads();
-    if (is_bin && threadIdx.x == 0) G(S.count[out])[it.bin] = min(sh_stay, home_front);
+    if (is_bin && threadIdx.x == 0) *el(Lo.count, it.bin) = min(sh_stay, home_front);
     if (is_bin) {
         for (int k = threadIdx.x; k < aw * ah; k += blockDim.x) {
             const unsigned long long v = accw[k];

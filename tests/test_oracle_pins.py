@@ -47,6 +47,40 @@ def test_philox_random_counters(orc):
         assert list(orc.philox(ctr, key)) == _philox_py(ctr, key)
 
 
+def test_philox2_known_answers(orc):
+    """The collision-candidacy generator, Philox2x32-10, against Random123's
+    published vectors."""
+    kat = json.load(open(os.path.join(GOLDEN, "philox2x32_10_kat.json")))
+    for v in kat["vectors"]:
+        ctr = [int(h, 16) for h in v["ctr"]]
+        out = ["%08x" % w for w in orc.philox2(ctr, int(v["key"], 16))]
+        assert out == v["out"]
+
+
+def test_candidacy_draw_is_uniform_and_keyed(orc):
+    """The candidacy uniform of the oracle's MCC (Philox2x32-10 over {id, step},
+    key from (seed, case, species)): a chi-square test over 64 cells, no
+    correlation between consecutive steps of one particle, and distinct
+    streams for neighbouring cases and species."""
+    def u(ids, step, key):
+        return np.array([orc.philox2([int(i), step], key)[0] >> 8 for i in ids], dtype=np.float64) * 2.0 ** -24
+
+    def key(seed, case, species):
+        return int(orc.philox2([seed, case], species)[0])
+
+    ids = np.arange(20000)
+    k = key(7, 0, 0)
+    a, b = u(ids, 11, k), u(ids, 12, k)
+    hist = np.bincount((a * 64).astype(int), minlength=64)
+    chi2 = float(((hist - len(a) / 64) ** 2 / (len(a) / 64)).sum())
+    assert chi2 < 63 + 5 * math.sqrt(2 * 63)
+    assert abs(np.corrcoef(a, b)[0, 1]) < 5 / math.sqrt(len(a))
+    keys = {key(7, c, s) for c in range(64) for s in range(4)}
+    assert len(keys) == 256
+    c = u(ids, 11, key(7, 1, 0))
+    assert abs(np.corrcoef(a, c)[0, 1]) < 5 / math.sqrt(len(a))
+
+
 # ------------------------------------------------------------ det_log -------
 def test_det_log_against_libm(orc):
     rng = np.random.default_rng(1)
