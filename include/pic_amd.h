@@ -60,7 +60,7 @@ extern "C" {
 #define PIC_MG_MIXED 1
 #define PIC_MG_PRESET_REFERENCE 0   /* Multigrid.cu defaults: SOR 1.25, 3/3/20, 1e-5, 50 cycles, FP64 */
 #define PIC_MG_PRESET_STEP 1        /* 2+2 Gauss-Seidel to 1e-8, FP64 refinement, exact coarse solve <= 257^2 */
-#define PIC_MG_PRESET_POISSON 2     /* 2+2 red-black SOR 1.2 to 1e-10, FP64 refinement */
+#define PIC_MG_PRESET_POISSON 2     /* 2+2 red-black SOR 1.2 to 1e-10, FP64 refinement, exact coarse solve <= 257^2 */
 #define PIC_MG_PRESET_STEP_SOR 3    /* 2+2 red-black SOR 1.2 to 1e-8, FP64 refinement: the simulation
                                        default; over-relaxation cuts the V-cycle contraction from
                                        about 0.06 (Gauss-Seidel) to about 0.01 per cycle */

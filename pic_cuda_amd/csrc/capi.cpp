@@ -748,7 +748,7 @@ extern "C" int pic_mg_default_params(int preset, pic_mg_params* out) {
     switch (preset) {
         case PIC_MG_PRESET_REFERENCE: *out = pic_mg_params{0, 3, 3, 20, 1.25, 1e-5, 50, PIC_MG_FP64, 1, 0}; break;
         case PIC_MG_PRESET_STEP: *out = pic_mg_params{0, 2, 2, 1, 1.0, 1e-8, 100, PIC_MG_MIXED, 2, 2}; break;
-        case PIC_MG_PRESET_POISSON: *out = pic_mg_params{0, 2, 2, 1, 1.2, 1e-10, 100, PIC_MG_MIXED, 2, 1}; break;
+        case PIC_MG_PRESET_POISSON: *out = pic_mg_params{0, 2, 2, 1, 1.2, 1e-10, 100, PIC_MG_MIXED, 2, 2}; break;
         case PIC_MG_PRESET_STEP_SOR: *out = pic_mg_params{0, 2, 2, 1, 1.2, 1e-8, 100, PIC_MG_MIXED, 2, 2}; break;
         default: set_error("unknown multigrid preset"); return PIC_ERR_ARG;
     }

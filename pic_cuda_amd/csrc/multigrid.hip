@@ -36,6 +36,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 namespace pic {
 
@@ -119,13 +120,36 @@ __device__ inline void stage_case(const StageRef<Task>& a, Task& tk) {
     else tk = a.t0;
 }
 
-// the SOR update of one node (the reference's expression)
+// The 5-point Laplacian times h^2 at node idx.  FP64 (the reference's V-cycle,
+// bitwise with it): the sum of the neighbours minus 4 c.  FP32 (the inner
+// cycles of the mixed solve): the sum of the four differences to the centre.
+// Neighbours of a smooth correction lie within a factor of two of the centre,
+// so each difference is exact in FP32 and only the small sum rounds; in the
+// reference form the sum of the neighbours rounds at the size of the
+// correction itself, and h^-2 amplifies that rounding into the residual,
+// which limited what two FP32 V-cycles gain per FP64 refinement step.
+template <typename T>
+__device__ inline T lap_h2(const T* e, int idx, int stride) {
+    const T c = e[idx];
+    if constexpr (std::is_same<T, float>::value) {
+        return ((e[idx + stride] - c) + (e[idx - stride] - c)) + ((e[idx + 1] - c) + (e[idx - 1] - c));
+    } else {
+        return e[idx - stride] + e[idx + stride] + e[idx - 1] + e[idx + 1] - (T)4 * c;
+    }
+}
+
+// the SOR update of one node: FP64 the reference's expression; FP32 the same
+// update as c + (omega/4) (lap_h2 - h^2 r)
 template <typename T>
 __device__ inline T sor_node(const T* e, const T* r, int idx, int stride, T h2, T omega) {
     const T c = e[idx];
-    const T s = e[idx + stride] + e[idx - stride] + e[idx + 1] + e[idx - 1];
-    const T nv = (T)0.25 * (s - h2 * r[idx]);
-    return tfma(omega, nv, ((T)1 - omega) * c);
+    if constexpr (std::is_same<T, float>::value) {
+        return tfma(omega * (T)0.25, tfma(-h2, r[idx], lap_h2(e, idx, stride)), c);
+    } else {
+        const T s = e[idx + stride] + e[idx - stride] + e[idx + 1] + e[idx - 1];
+        const T nv = (T)0.25 * (s - h2 * r[idx]);
+        return tfma(omega, nv, ((T)1 - omega) * c);
+    }
 }
 
 // One colour of red-black SOR on a W x W LDS tile whose element (0,0) is the
@@ -167,8 +191,7 @@ __device__ inline void restrict_tile(const T* e, const T* r, T* res, const uint8
         T v = (T)0;
         const int idx = (gj - gj0) * W + (gi - gi0);
         if (gi > 0 && gi < n - 1 && gj > 0 && gj < n - 1 && !(mk && mk[idx])) {
-            const T c = e[idx];
-            const T lap = (e[idx - W] + e[idx + W] + e[idx - 1] + e[idx + 1] - (T)4 * c) * inv_h2;
+            const T lap = lap_h2(e, idx, W) * inv_h2;
             v = r[idx] - lap;
         }
         res[k] = v;
@@ -652,7 +675,7 @@ __global__ __launch_bounds__(1024) void mg_coarse_stage(const StageRef<MGCoarseT
                 const int k = j * nf + i;
                 T v = (T)0;
                 if (!(i == 0 || i == nf - 1 || j == 0 || j == nf - 1) && !cm.fixed(l, i, j)) {
-                    const T lap = (e[k - nf] + e[k + nf] + e[k - 1] + e[k + 1] - (T)4 * e[k]) * inv_h2;
+                    const T lap = lap_h2(e, k, nf) * inv_h2;
                     v = rl[k] - lap;
                 }
                 res[k] = v;

@@ -73,7 +73,7 @@ def test_mg_presets(pic):
     poi = pic.mg_params(pic.PRESET_POISSON)
     assert (poi.omega, poi.inner) == (1.2, 2)
     assert ref.inner == 1 and st.inner == 2
-    assert ref.coarse_direct == 0 and st.coarse_direct == sor.coarse_direct == 2 and poi.coarse_direct == 1
+    assert ref.coarse_direct == 0 and st.coarse_direct == sor.coarse_direct == poi.coarse_direct == 2
     with pytest.raises(pic.PicError):
         pic.mg_params(7)
 
