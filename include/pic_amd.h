@@ -11,7 +11,9 @@
  *     domain is [0, m-1] x [0, n-1]), velocities in m/s.  The C++ and Python
  *     mirrors of Particles2D convert from domain units.
  *   - grid fields are FP64, row-major, m (x, fast) by n (y) nodes.
- *   - one HIP stream per object; calls on one object are not thread-safe.
+ *   - the library's device work runs in order on one HIP stream of its own
+ *     (default priority); device pointers handed in are used on that stream.
+ *     Calls are not thread-safe: call the library from one host thread.
  */
 #ifndef PIC_AMD_H
 #define PIC_AMD_H
