@@ -36,6 +36,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <tuple>
+#include <mutex>
+#include <map>
 #include <type_traits>
 
 namespace pic {
@@ -1493,6 +1496,43 @@ int MGBatch::build_step(const std::vector<double>& h) {
     return 0;
 }
 
+// The tables of the sine-transform solve for m unknowns a side laid out P x P
+// (S then W, zero outside m x m), computed in long double once per (m, P,
+// form) and kept for the process: a batch rebuild, which every admission to a
+// serving ensemble causes, then only copies them (computing a 255^2 set took
+// 5 ms of host time per size and rebuild).  reduced: the big solve's form,
+// the sine argument (a+1)(b+1) taken mod 2(m+1).
+static const std::vector<double>& dst_tables(int m, int P, bool reduced) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, bool>, std::vector<double>> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    const auto key = std::make_tuple(m, P, reduced);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    std::vector<double> tab(2 * (size_t)P * P, 0.0);
+    double* S = tab.data();
+    double* W = S + (size_t)P * P;
+    const long double pi = 3.141592653589793238462643383279502884L;
+    std::vector<long double> sig(m);
+    for (int p = 0; p < m; ++p) {
+        const long double sp = std::sin(pi * (p + 1) / (2.0L * (m + 1)));
+        sig[p] = sp * sp;
+    }
+    const long double norm = (2.0L / (m + 1)) * (2.0L / (m + 1));
+    for (int a = 0; a < m; ++a)
+        for (int b = 0; b < m; ++b) {
+            if (reduced) {
+                // (a+1)(b+1) reduced mod 2(m+1) keeps the sine argument in [0, 2 pi)
+                const long long t = (long long)(a + 1) * (b + 1) % (2LL * (m + 1));
+                S[(size_t)a * P + b] = (double)std::sin(pi * (long double)t / (long double)(m + 1));
+            } else {
+                S[(size_t)a * P + b] = (double)std::sin(pi * (long double)((a + 1) * (b + 1)) / (long double)(m + 1));
+            }
+            W[(size_t)a * P + b] = (double)(norm / (-4.0L * (sig[a] + sig[b])));
+        }
+    return cache.emplace(key, std::move(tab)).first->second;
+}
+
 // Direct coarse solve (mixed mode, members without electrodes whose coarse
 // kernel level has 3..65 nodes): the sine-transform tables of that size, one
 // set per size (S then W, 64 x 64 each, zero outside m x m).
@@ -1515,24 +1555,10 @@ int MGBatch::build_big_tables() {
     }
     if (sizes.empty()) return 0;
     std::vector<double> tab(total, 0.0);
-    const long double pi = 3.141592653589793238462643383279502884L;
     for (size_t q = 0; q < sizes.size(); ++q) {
         const int m = sizes[q] - 2, P = dst_pad(m);
-        double* S = &tab[offs[q]];
-        double* W = S + (size_t)P * P;
-        std::vector<long double> sig(m);
-        for (int p = 0; p < m; ++p) {
-            const long double sp = std::sin(pi * (p + 1) / (2.0L * (m + 1)));
-            sig[p] = sp * sp;
-        }
-        const long double norm = (2.0L / (m + 1)) * (2.0L / (m + 1));
-        for (int a = 0; a < m; ++a)
-            for (int b = 0; b < m; ++b) {
-                // (a+1)(b+1) reduced mod 2(m+1) keeps the sine argument in [0, 2 pi)
-                const long long t = (long long)(a + 1) * (b + 1) % (2LL * (m + 1));
-                S[(size_t)a * P + b] = (double)std::sin(pi * (long double)t / (long double)(m + 1));
-                W[(size_t)a * P + b] = (double)(norm / (-4.0L * (sig[a] + sig[b])));
-            }
+        const std::vector<double>& t = dst_tables(m, P, true);
+        std::copy(t.begin(), t.end(), tab.begin() + offs[q]);
     }
     if (int rc = upload_vec((void**)&d_btab_, tab)) return rc;
     for (int k = 0; k < M; ++k) {
@@ -1562,21 +1588,8 @@ int MGBatch::build_direct_tables() {
     const size_t per = 2 * 64 * 64;
     std::vector<double> tab(sizes.size() * per, 0.0);
     for (size_t q = 0; q < sizes.size(); ++q) {
-        const int m = sizes[q] - 2;
-        double* S = &tab[q * per];
-        double* W = S + 64 * 64;
-        const long double pi = 3.141592653589793238462643383279502884L;
-        std::vector<long double> sig(m);
-        for (int p = 0; p < m; ++p) {
-            const long double sp = std::sin(pi * (p + 1) / (2.0L * (m + 1)));
-            sig[p] = sp * sp;
-        }
-        const long double norm = (2.0L / (m + 1)) * (2.0L / (m + 1));
-        for (int a = 0; a < m; ++a)
-            for (int b = 0; b < m; ++b) {
-                S[a * 64 + b] = (double)std::sin(pi * (long double)((a + 1) * (b + 1)) / (long double)(m + 1));
-                W[a * 64 + b] = (double)(norm / (-4.0L * (sig[a] + sig[b])));
-            }
+        const std::vector<double>& t = dst_tables(sizes[q] - 2, 64, false);
+        std::copy(t.begin(), t.end(), tab.begin() + q * per);
     }
     if (int rc = upload_vec((void**)&d_dtab_, tab)) return rc;
     for (int k = 0; k < M; ++k)
