@@ -18,6 +18,7 @@ sample of the same workload).
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -578,6 +579,17 @@ def main():
             print("bench: the FP64 rate", file=sys.stderr, flush=True)
             rate64 = fp64_rate(a.cells, a.bin)
     cycles = (st1.total_mg_cycles - st0.total_mg_cycles) / a.steps
+    # the whole step by the per-step grid byte rule: 16 B/node charge, 32 B/node
+    # per FP32 V-cycle, 40 B/node per FP64 refinement step, 16 B/node E field
+    nodes = (a.cells + 1) ** 2
+    inner = max(1, case["mg"].inner)
+    grid_bytes = nodes * (16 + 32 * cycles + 40 * math.ceil(cycles / inner) + 16)
+    step_bytes = BYTES_PER_PARTICLE * alive_mean + grid_bytes
+    step_gbps = step_bytes / (el / a.steps) / 1e9
+    field_gbps = grid_bytes / (field_ms * 1e-3) / 1e9
+    step_roof = {"bytes_per_step": int(step_bytes), "achieved": round(step_gbps, 1), "frac": round(step_gbps / HBM_PEAK_GBS, 4),
+                 "field_bytes_per_step": int(grid_bytes), "field_achieved": round(field_gbps, 1),
+                 "field_frac": round(field_gbps / HBM_PEAK_GBS, 4), "unit": "GB/s"}
     line = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": 1e3 * el / a.steps, "higher_is_better": True, "scaling": "strong",
@@ -586,7 +598,7 @@ def main():
                                "absorbing box walls, FP32 particles, FP32-in-FP64 multigrid (2+2 red-black SOR 1.2) to 1e-8"
                                % (a.cells, alive_mean / 2, alive_mean / 2),
                    "cells": a.cells, "particles": int(st0.alive), "bin": a.bin},
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "step_roofline": step_roof, "cpu_baseline": cpu,
         "field_ms_per_step": round(field_ms, 4), "mg_cycles_per_step": cycles,
         "alive_start": int(st0.alive), "alive_end": int(st1.alive),
         "fp64_error": acc, "fp64_particles": rate64,
