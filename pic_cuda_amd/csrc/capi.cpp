@@ -1187,9 +1187,12 @@ static int rebin_t(pic_sim* S, int bw, int bh) {
     const int ns = (int)p->sp.size();
     std::vector<WorkItem> items;
     std::vector<int> off(ns + 1, 0);
+    // spill-region chunks of the sort are larger than the push's: a fresh store
+    // is all spill, and the counting kernels pay per workgroup for the bins
+    const int sort_chunk = std::max(S->spill_chunk, 32768);
     for (int s = 0; s < ns; ++s) {
         off[s] = (int)items.size();
-        items_for_species(p, s, S->spill_chunk, items);
+        items_for_species(p, s, sort_chunk, items);
     }
     off[ns] = (int)items.size();
     if (int rc = upload_items(&p->d_items, &p->items_alloc, items)) return rc;
@@ -1199,7 +1202,7 @@ static int rebin_t(pic_sim* S, int bw, int bh) {
         SpeciesStore& st = p->sp[s];
         PIC_HIP(hipMemsetAsync(p->d_scratch, 0, (size_t)(nb_new + 1) * sizeof(int32_t), sm));
         if (int rc = launch_hist<T>((CaseDev<T>*)p->d_case, p->d_state, p->d_items + off[s], off[s + 1] - off[s],
-                                    S->spill_chunk, bw, bh, nbx, nb_new, p->d_scratch, sm))
+                                    sort_chunk, bw, bh, nbx, nb_new, p->d_scratch, sm))
             return rc;
         std::vector<int32_t> hist(nb_new + 1);
         PIC_HIP(hipMemcpyAsync(hist.data(), p->d_scratch, (size_t)(nb_new + 1) * sizeof(int32_t),
@@ -1259,8 +1262,8 @@ static int rebin_t(pic_sim* S, int bw, int bh) {
     }
     if (int rc = upload_case(p)) return rc;
     PIC_HIP(hipMemsetAsync(p->d_ll, 0, sizeof(long long), sm));
-    if (int rc = launch_scatter<T>((CaseDev<T>*)p->d_case, p->d_state, p->d_items, (int)items.size(), S->spill_chunk,
-                                   bw, bh, nbx, (unsigned long long*)p->d_ll, sm))
+    if (int rc = launch_scatter<T>((CaseDev<T>*)p->d_case, p->d_state, p->d_items, (int)items.size(), sort_chunk,
+                                   bw, bh, nbx, nb_new, (unsigned long long*)p->d_ll, sm))
         return rc;
     if (int rc = launch_end_step<T>((CaseDev<T>*)p->d_case, p->d_state, 1, 0, sm)) return rc;
     long long lost = 0;

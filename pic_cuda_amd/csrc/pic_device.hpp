@@ -88,6 +88,9 @@ enum { PIC_F_KICK = 1, PIC_F_MCC = 2, PIC_F_DRIFT = 4, PIC_F_DEPOSIT = 8 };
 #define PIC_MSTAGE 768
 // entries of each wavefront's queue of deferred collision candidates
 #define PIC_MQUEUE 64
+// layouts of up to this many bins are sorted in LDS when re-binning (two int
+// per bin: 64 KB)
+#define PIC_SORT_LDS_BINS 8192
 
 // work item of the particle kernels: a bin of a species of a case, or a chunk
 // of its spill region (bin == nb)

@@ -958,6 +958,119 @@ __global__ __launch_bounds__(256) void scatter_kernel(const CaseDev<T>* __restri
     }
 }
 
+// Counting sort of the re-binning in LDS (layouts of up to PIC_SORT_LDS_BINS
+// bins): a workgroup counts its particles per destination bin in LDS and adds
+// the nonzero counts to the global histogram, one atomic per (workgroup, bin);
+// the scatter counts the same way, reserves each bin's range with one global
+// atomic per (workgroup, bin), then takes each particle's slot from an LDS
+// atomic on that range.  The wavefront-aggregated kernels above remain for
+// larger layouts.
+template <typename T>
+__global__ __launch_bounds__(256) void hist_lds_kernel(const CaseDev<T>* __restrict__ cases,
+                                                       const CaseState* __restrict__ states,
+                                                       const WorkItem* __restrict__ items, int chunk, int bw, int bh,
+                                                       int nbx, int nb_new, int32_t* hist) {
+    extern __shared__ int lh[];
+    const WorkItem it = items[blockIdx.x];
+    const CaseDev<T>& C = cases[it.case_idx];
+    const SpeciesDev<T>& S = C.sp[it.species];
+    const int in = states[it.case_idx].cur;
+    const Range rg = item_range(S, in, it, chunk);
+    const int n = rg.n();
+    if (n <= 0) return;
+    const PartsOf<T> P = parts_of(S, in);
+    for (int k = threadIdx.x; k < nb_new; k += blockDim.x) lh[k] = 0;
+    __syncthreads();
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        const int src = rg.at(t);
+        atomicAdd(&lh[key_for(C.nx - 2, C.ny - 2, bw, bh, nbx, nb_new, *el(P.x, src), *el(P.y, src))], 1);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < nb_new; k += blockDim.x) {
+        const int v = lh[k];
+        if (v) gatomic_add(hist + k, v);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void scatter_lds_kernel(const CaseDev<T>* __restrict__ cases,
+                                                          const CaseState* __restrict__ states,
+                                                          const WorkItem* __restrict__ items, int chunk, int bw,
+                                                          int bh, int nbx, unsigned long long* lost) {
+    extern __shared__ int lh[];   // per bin: next output position, then the end of the bin's front part
+    const WorkItem it = items[blockIdx.x];
+    const CaseDev<T>& C = cases[it.case_idx];
+    const SpeciesDev<T>& S = C.sp[it.species];
+    const int in = states[it.case_idx].cur, out = in ^ 1;
+    const Range rg = item_range(S, in, it, chunk);
+    const int n = rg.n();
+    if (n <= 0) return;
+    const int nb = S.nb[out];
+    int* pos = lh;
+    int* lim = lh + nb;
+    const PartsOf<T> Pi = parts_of(S, in), Po = parts_of(S, out);
+    const LayOf Lo = lay_of(S, out);
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) pos[k] = 0;
+    __syncthreads();
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        const int src = rg.at(t);
+        atomicAdd(&pos[key_for(C.nx - 2, C.ny - 2, bw, bh, nbx, nb, *el(Pi.x, src), *el(Pi.y, src))], 1);
+    }
+    __syncthreads();
+    // this workgroup's range in the front part of every bin it fills, as
+    // absolute positions in the output buffer
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+        const int v = pos[k];
+        if (v) {
+            const int start = *el(Lo.start, k);
+            pos[k] = start + gatomic_add(Lo.count + k, v);
+            lim[k] = start + front_cap(*el(Lo.cap, k));
+        }
+    }
+    __syncthreads();
+    // the next pass's particles are loaded before this pass's stores
+    T px, py, pvx, pvy;
+    uint32_t pid;
+    {
+        const int src = rg.at(min((int)threadIdx.x, n - 1));
+        px = *el(Pi.x, src);
+        py = *el(Pi.y, src);
+        pvx = *el(Pi.vx, src);
+        pvy = *el(Pi.vy, src);
+        pid = *el(Pi.id, src);
+    }
+    for (int base = 0; base < n; base += blockDim.x) {
+        const int t = base + threadIdx.x;
+        const T x = px, y = py, vx = pvx, vy = pvy;
+        const uint32_t id = pid;
+        if (base + (int)blockDim.x < n) {
+            const int src = rg.at(min(t + (int)blockDim.x, n - 1));
+            px = *el(Pi.x, src);
+            py = *el(Pi.y, src);
+            pvx = *el(Pi.vx, src);
+            pvy = *el(Pi.vy, src);
+            pid = *el(Pi.id, src);
+        }
+        if (t >= n) continue;
+        const int key = key_for(C.nx - 2, C.ny - 2, bw, bh, nbx, nb, x, y);
+        int dst = atomicAdd(&pos[key], 1);
+        if (dst >= lim[key]) {
+            // front part full (a layout smaller than the population): the spill region
+            const int s2 = gatomic_add(Lo.count + nb, 1);
+            dst = s2 < *el(Lo.cap, nb) ? *el(Lo.start, nb) + s2 : -1;
+        }
+        if (dst >= 0) {
+            *el(Po.x, dst) = x;
+            *el(Po.y, dst) = y;
+            *el(Po.vx, dst) = vx;
+            *el(Po.vy, dst) = vy;
+            *el(Po.id, dst) = id;
+        } else {
+            atomicAdd(lost, 1ull);
+        }
+    }
+}
+
 // flatten the input layout of one species into buffer `out` from index 0
 template <typename T>
 __global__ __launch_bounds__(256) void gather_kernel(const CaseDev<T>* __restrict__ cases,
@@ -1129,18 +1242,26 @@ template <typename T>
 int launch_hist(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items, int chunk,
                 int bw, int bh, int nbx, int nb_new, int32_t* hist, hipStream_t s) {
     if (n_items <= 0) return 0;
-    hipLaunchKernelGGL(hist_kernel<T>, dim3(n_items), dim3(256), 0, s, cases, states, items, chunk, bw, bh, nbx,
-                       nb_new, hist);
+    if (nb_new <= PIC_SORT_LDS_BINS)
+        hipLaunchKernelGGL(hist_lds_kernel<T>, dim3(n_items), dim3(256), (size_t)nb_new * sizeof(int), s, cases, states,
+                           items, chunk, bw, bh, nbx, nb_new, hist);
+    else
+        hipLaunchKernelGGL(hist_kernel<T>, dim3(n_items), dim3(256), 0, s, cases, states, items, chunk, bw, bh, nbx,
+                           nb_new, hist);
     PIC_HIP(hipGetLastError());
     return 0;
 }
 
 template <typename T>
 int launch_scatter(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items,
-                   int chunk, int bw, int bh, int nbx, unsigned long long* lost, hipStream_t s) {
+                   int chunk, int bw, int bh, int nbx, int nb_new, unsigned long long* lost, hipStream_t s) {
     if (n_items <= 0) return 0;
-    hipLaunchKernelGGL(scatter_kernel<T>, dim3(n_items), dim3(256), 0, s, cases, states, items, chunk, bw, bh, nbx,
-                       lost);
+    if (nb_new <= PIC_SORT_LDS_BINS)
+        hipLaunchKernelGGL(scatter_lds_kernel<T>, dim3(n_items), dim3(256), 2 * (size_t)nb_new * sizeof(int), s, cases,
+                           states, items, chunk, bw, bh, nbx, lost);
+    else
+        hipLaunchKernelGGL(scatter_kernel<T>, dim3(n_items), dim3(256), 0, s, cases, states, items, chunk, bw, bh, nbx,
+                           lost);
     PIC_HIP(hipGetLastError());
     return 0;
 }
@@ -1312,7 +1433,7 @@ int launch_counts(const CaseDev<T>* cases, const CaseState* states, int c, int s
     template int launch_hist<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int, int, int, int, int,   \
                                 int, int32_t*, hipStream_t);                                                     \
     template int launch_scatter<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int, int, int, int, int, \
-                                   unsigned long long*, hipStream_t);                                            \
+                                   int, unsigned long long*, hipStream_t);                                       \
     template int launch_gather<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int, int, int32_t*,       \
                                   hipStream_t);                                                                  \
     template int launch_moments<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int, int, double*,       \

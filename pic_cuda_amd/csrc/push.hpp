@@ -24,7 +24,7 @@ int launch_hist(const CaseDev<T>* cases, const CaseState* states, const WorkItem
                 int bw, int bh, int nbx, int nb_new, int32_t* hist, hipStream_t s);
 template <typename T>
 int launch_scatter(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items,
-                   int chunk, int bw, int bh, int nbx, unsigned long long* lost, hipStream_t s);
+                   int chunk, int bw, int bh, int nbx, int nb_new, unsigned long long* lost, hipStream_t s);
 template <typename T>
 int launch_gather(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items, int chunk,
                   int32_t* counter, hipStream_t s);
