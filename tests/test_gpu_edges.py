@@ -157,6 +157,24 @@ def test_dense_clump_spills_and_rebins(pic, orc):
     assert c["sim"].counters(0)[pic.CNT["lost"]] == 0
 
 
+@pytest.mark.parametrize("cells,bin_w", [(256, 3), (384, 4)])
+def test_rebin_with_many_bins(pic, orc, cells, bin_w):
+    """Re-binning counts bins in LDS up to 8192 bins and falls back to
+    wavefront-aggregated kernels above: 86^2 = 7396 bins of 3 x 3 cells and
+    96^2 = 9216 bins of 4 x 4 cells, a clump that spills and is re-binned
+    every two steps, every pass equal to the oracle."""
+    rng = np.random.default_rng(cells)
+    n = 30000
+    c0 = cells / 2
+    e = _state(rng.uniform(c0, c0 + 2.0, n), rng.uniform(c0, c0 + 2.0, n), rng.normal(0, 2e5, n),
+               rng.normal(0, 2e5, n))
+    i = _state(rng.uniform(0, cells, 5000), rng.uniform(0, cells, 5000), np.zeros(5000), np.zeros(5000))
+    c = _build(pic, cells, e, i, prec=4, bin_w=bin_w, w=1.0, rebin_every=2)
+    _run_against_oracle(pic, orc, c, 4, passes=6)
+    assert c["sim"].stats().rebins >= 2
+    assert c["sim"].counters(0)[pic.CNT["lost"]] == 0
+
+
 def test_capacity_errors(pic):
     """A store holds max + max/2 + max/8 + 2^20 slots per species; beyond them
     init and upload fail with a capacity error and leave the store as it was."""
