@@ -55,10 +55,13 @@ namespace pic {
 #define MG_ND 65
 #define MG_DS 63
 #define MG_DLDS (2 * MG_DS * MG_DS * 8)
-// big direct solve (coarse_direct 2): levels up to 257 nodes, unknowns a
+// big direct solve (coarse_direct 2): levels up to 513 nodes, unknowns a
 // side padded to a multiple of 64
-#define MG_NDB 257
+#define MG_NDB 513
 static inline int dst_pad(int m) { return (m + 63) / 64 * 64; }
+// tiles a side of a tiled stage on a level of n nodes (the kernels' edge_nodes
+// write the boundary row and column this leaves out when n = 32 g + 1)
+static inline int mg_tiles(int n) { return n <= 1 ? 1 : div_up(n - 1, MG_TILE); }
 // largest finest level (nodes a side) of a batch that fuses the finest
 // up/down stages between V-cycles
 #define MG_FUSE_MAX 1025
@@ -222,6 +225,52 @@ __device__ inline void restrict_tile(const T* e, const T* r, T* res, const uint8
     rhs_c[(size_t)jc * nc + ic] = s * (T)0.0625;
 }
 
+// bilinear prolongation of the coarse value at fine node (gi, gj), with the
+// coarse patch in LDS: element (0,0) of the CW-wide patch is coarse (ic0, jc0)
+template <typename T>
+__device__ inline T prolong_lds(const T* C, int CW, int ic0, int jc0, int nc, int gi, int gj) {
+    const int ic = gi >> 1, jc = gj >> 1;
+    const int c = (jc - jc0) * CW + (ic - ic0);
+    const bool io = gi & 1, jo = gj & 1;
+    if (!io && !jo) return C[c];
+    if (io && !jo) return (ic + 1 < nc) ? (T)0.5 * (C[c] + C[c + 1]) : C[c];
+    if (!io && jo) return (jc + 1 < nc) ? (T)0.5 * (C[c] + C[c + CW]) : C[c];
+    return (ic + 1 < nc && jc + 1 < nc) ? (T)0.25 * (C[c] + C[c + 1] + C[c + CW] + C[c + CW + 1]) : C[c];
+}
+
+// The tiles of a level cover nodes [0, 32 g) a side with g = ceil((n - 1) / 32)
+// (mg_tiles), which leaves out the last boundary row and column when
+// n = 32 g + 1 -- every level of a 2^k + 1 ladder from 33 nodes up, where a
+// tile of them would hold nothing else (2049^2: 4225 tiles -> 4096).  A
+// boundary node is never smoothed, so its output is its input (plus the
+// coarse correction in an up stage): the tiles next to that row and column
+// write it, from global memory (a stage with no halo has no copy of it in
+// LDS), and set the coarse rhs of the coarse boundary nodes they would have
+// owned to zero (a boundary rhs is never read).
+template <typename T>
+__device__ inline void edge_nodes(int i0, int j0, int n, const gptr<const T> e_in, const T* C, int CW, int ic0,
+                                  int jc0, const gptr<const uint8_t> gmask, const gptr<T> e_out, const gptr<T> rhs_c) {
+    const bool right = i0 + MG_TILE == n - 1, bottom = j0 + MG_TILE == n - 1;
+    if (!right && !bottom) return;
+    const int tid = threadIdx.x;
+    int gi = -1, gj = -1;
+    if (tid < MG_TILE) {
+        if (right) { gi = n - 1; gj = j0 + tid; }
+    } else if (tid >= 64 && tid < 64 + MG_TILE) {
+        if (bottom) { gi = i0 + tid - 64; gj = n - 1; }
+    } else if (tid == 128) {
+        if (right && bottom) { gi = n - 1; gj = n - 1; }
+    }
+    const int nc = (n + 1) / 2;
+    if (gi >= 0 && gi < n && gj < n) {
+        const size_t g = (size_t)gj * n + gi;
+        T v = e_in ? e_in[g] : (T)0;
+        if (C && !(gmask && gmask[g])) v = v + prolong_lds(C, CW, ic0, jc0, nc, gi, gj);
+        e_out[g] = v;
+        if (rhs_c && !(gi & 1) && !(gj & 1)) rhs_c[(size_t)(gj >> 1) * nc + (gi >> 1)] = (T)0;
+    }
+}
+
 // Down-leg stage: load tile + halo, NU sweeps, store the tile; with RESTRICT
 // also the residual on tile + 1 ring and the restriction to the coarse nodes
 // (2ic, 2jc) inside the tile.  Without RESTRICT (tiled sweeps of a large
@@ -294,22 +343,10 @@ __global__ __launch_bounds__(256) void mg_smooth_stage(const StageRef<MGLevelTas
         const int gi = i0 + k % MG_TILE, gj = j0 + k / MG_TILE;
         if (gi < n && gj < n) e_out[(size_t)gj * n + gi] = e[(gj - gj0) * W + (gi - gi0)];
     }
+    edge_nodes<T>(i0, j0, n, e_in, nullptr, 0, 0, 0, gmask, e_out, RESTRICT ? rhs_c : G((T*)nullptr));
     if constexpr (RESTRICT) {
         if (rhs_c) restrict_tile<T, W>(e, r, r + WW, mk, i0, j0, gi0, gj0, n, inv_h2, rhs_c);
     }
-}
-
-// bilinear prolongation of the coarse value at fine node (gi, gj), with the
-// coarse patch in LDS: element (0,0) of the CW-wide patch is coarse (ic0, jc0)
-template <typename T>
-__device__ inline T prolong_lds(const T* C, int CW, int ic0, int jc0, int nc, int gi, int gj) {
-    const int ic = gi >> 1, jc = gj >> 1;
-    const int c = (jc - jc0) * CW + (ic - ic0);
-    const bool io = gi & 1, jo = gj & 1;
-    if (!io && !jo) return C[c];
-    if (io && !jo) return (ic + 1 < nc) ? (T)0.5 * (C[c] + C[c + 1]) : C[c];
-    if (!io && jo) return (jc + 1 < nc) ? (T)0.5 * (C[c] + C[c + CW]) : C[c];
-    return (ic + 1 < nc && jc + 1 < nc) ? (T)0.25 * (C[c] + C[c + 1] + C[c + CW] + C[c + CW + 1]) : C[c];
 }
 
 // Up-leg stage: e = e_in + P(coarse) on tile + halo 2*NU, NU sweeps, store the
@@ -409,6 +446,7 @@ __global__ __launch_bounds__(256) void mg_prolong_stage(const StageRef<MGLevelTa
         const int gi = i0 + k % MG_TILE, gj = j0 + k / MG_TILE;
         if (gi < n && gj < n) e_out[(size_t)gj * n + gi] = e[(gj - gj0) * W + (gi - gi0)];
     }
+    edge_nodes<T>(i0, j0, n, e_in, C, CW, ic0, jc0, gmask, e_out, RESTRICT ? rhs_c : G((T*)nullptr));
     if constexpr (RESTRICT) {
         if (rhs_c) restrict_tile<T, W>(e, r, res, mk, i0, j0, gi0, gj0, n, inv_h2, rhs_c);
     }
@@ -727,73 +765,139 @@ __global__ __launch_bounds__(1024) void mg_coarse_stage(const StageRef<MGCoarseT
         for (int i = tx; i < n_top; i += 32) e_out[j * n_top + i] = base[j * n_top + i];
 }
 
-// ---- mixed mode: exact FP64 solve of a level of 66..257 nodes -----------------
+// ---- mixed mode: exact FP64 solve of a level of 66..513 nodes -----------------
 // The same transform as coarse_direct_solve, e = S (h^2 W o (S R S)) S, for
-// m <= 255 unknowns a side, as four products of P x P matrices (P = m rounded
-// up to a multiple of 64, zero padded), one launch each:
-//   phase 0: T0 = R S          (R: the level's FP32 rhs, interior nodes)
-//   phase 1: T1 = (S T0) o h^2 W
-//   phase 2: T0 = T1 S
-//   phase 3: e  = S T0         (FP32, interior nodes; the boundary set to zero)
-// Workgroup = one 16 x 16 output tile; its four wavefronts take a quarter of
-// the k range each (P/4 = 16..64, 4..16 chained v_mfma_f64_16x16x4f64), then
-// wavefront 0 adds the four partial tiles in a fixed order, so the result does
-// not depend on scheduling.  S is symmetric, so its A-side entries S[i][k] are
-// read as S[k][i] (consecutive lanes, consecutive addresses).
-template <int PHASE>
+// m <= 511 unknowns a side, as four products of P x P matrices (P = m rounded
+// up to a multiple of 64, zero padded), one launch each, S alternately on the
+// left and on the right:
+//   phase 0: Y = S R                (R: the level's FP32 rhs, interior nodes)
+//   phase 1: T1 = (Y S) o h^2 W
+//   phase 2: V = S T1
+//   phase 3: e  = V S               (FP32, interior nodes; the boundary set to zero)
+// The operand layout of v_mfma_f64_16x16x4f64 (A[i][k] in lane i + 16 k,
+// B[k][j] in lane j + 16 k) reads a row-major B and a column-major A with
+// consecutive lanes at consecutive addresses, so phases 0 and 2 store their
+// product transposed (T0 = Y^T, then V^T), which phases 1 and 3 read as their
+// A operand; S is symmetric, so its A-side entries S[i][k] are read as S[k][i].
+// Each product is halved by the mirror symmetry of S (S[k][m-1-b] =
+// (-1)^k S[k][b]): with S on the left, rows a and m-1-a of S X are E + O and
+// E - O, where E and O sum over the even and the odd k only, so only the
+// nb = (m+1)/2 rows a < nb are computed; with S on the right the same holds for
+// the columns.  A workgroup owns a 16 x 16 tile of E and O: free block tile.tx
+// (columns with S on the left, rows with S on the right) and halved block
+// tile.ty.  Its four wavefronts take a quarter of the k range each (E and O
+// chains of P/32 steps; every operand of a wavefront is loaded before its first
+// product), and the four partial tiles are added in LDS in a fixed order, so
+// the result does not depend on scheduling.  Only entries inside m x m are
+// written; T0 and T1 are zeroed once when allocated, so their padding stays
+// zero and the padded products add exact zeros.
+template <int PHASE, int NS>
 __global__ __launch_bounds__(256) void mg_dst_stage(const StageRef<MGDstTask> a) {
     MGTile tile;
     MGDstTask tk;
     stage_item(a, tk, tile);
     const int stop = tk.gate ? *G(&tk.gate->converged) : 0;
+    constexpr bool LEFT = PHASE == 0 || PHASE == 2;   // S is the left operand
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int P = tk.P, m = tk.m, n = tk.n;
-    const int row0 = tile.ty * 16, col0 = tile.tx * 16;
-    const int nk = P >> 4;                 // MFMA steps of this wavefront (P/4 values of k, 4 per step)
+    const int nb = (m + 1) >> 1;           // computed rows (left) / columns (right)
+    const int f0 = tile.tx * 16, h0 = tile.ty * 16;
+    // steps of each chain: P/4 values of k, half of them even.  The launch's
+    // NS is the largest of its cases; a smaller case loads its last step's
+    // operands again (in-range addresses, no branch in front of the loads) and
+    // zeroes them, so its extra products add exact zeros.
+    const int ns = P >> 5;
     const int k0 = wave * (P >> 2);
     const gptr<const double> S = G(tk.S);
-    const gptr<const double> Bm = G(PHASE == 1 || PHASE == 3 ? (const double*)tk.T0 : tk.S);
-    const gptr<const double> Am = G(PHASE == 2 ? (const double*)tk.T1 : tk.S);
+    const gptr<const double> T0 = G((const double*)tk.T0);
+    const gptr<const double> T1 = G((const double*)tk.T1);
     const gptr<const float> R = G(tk.rhs);
-    double av[16], bv[16];
+    double ae[NS], be[NS], ao[NS], bo[NS];
+    float re[NS], ro[NS];   // phase 0: R as loaded, converted when used (no wait behind each load)
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
-        av[t] = 0.0;
-        bv[t] = 0.0;
-        if (t < nk) {
-            const int k = k0 + 4 * t + lk, i = row0 + li, j = col0 + li;
+    for (int t = 0; t < NS; ++t) {
+        const bool live = t < ns;
+        const int ke = k0 + 8 * min(t, ns - 1) + 2 * lk, ko = ke + 1;
+        if (LEFT) {
+            // A[i][k] = S[h0 + li][k] = S[k][h0 + li]; B[k][j] = R or T1 at column f0 + li
+            const int i = h0 + li, j = f0 + li;
+            ae[t] = S[(size_t)ke * P + i];
+            ao[t] = S[(size_t)ko * P + i];
             if (PHASE == 0) {
-                av[t] = (i < m && k < m) ? (double)R[(size_t)(i + 1) * n + (k + 1)] : 0.0;
-            } else if (PHASE == 2) {
-                av[t] = Am[(size_t)i * P + k];
+                // R: interior node (k + 1, j + 1), clamped into the grid (zeroed beyond m when used)
+                const int jc = min(j, m - 1), kec = min(ke, m - 1), koc = min(ko, m - 1);
+                re[t] = R[(size_t)(kec + 1) * n + (jc + 1)];
+                ro[t] = R[(size_t)(koc + 1) * n + (jc + 1)];
             } else {
-                av[t] = S[(size_t)k * P + i];   // S[i][k], S symmetric
+                be[t] = T1[(size_t)ke * P + j];
+                bo[t] = T1[(size_t)ko * P + j];
             }
-            bv[t] = Bm[(size_t)k * P + j];
+        } else {
+            // A[i][k] = X[f0 + li][k] = T0[k][f0 + li] (X^T stored); B[k][j] = S[k][h0 + li]
+            const int i = f0 + li, j = h0 + li;
+            ae[t] = T0[(size_t)ke * P + i];
+            ao[t] = T0[(size_t)ko * P + i];
+            be[t] = S[(size_t)ke * P + j];
+            bo[t] = S[(size_t)ko * P + j];
         }
+        if (!live) ae[t] = ao[t] = 0.0;
     }
     if (stop) return;
-    mg_v4d acc = mg_v4d{0.0, 0.0, 0.0, 0.0};
+    mg_v4d accE = mg_v4d{0.0, 0.0, 0.0, 0.0}, accO = mg_v4d{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int t = 0; t < 16; ++t)
-        if (t < nk) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t], bv[t], acc, 0, 0, 0);
-    __shared__ double red[4][4][64];
+    for (int t = 0; t < NS; ++t) {
+        if (PHASE == 0) {
+            const int ke = k0 + 8 * min(t, ns - 1) + 2 * lk, j = f0 + li;
+            be[t] = (j < m && ke < m) ? (double)re[t] : 0.0;
+            bo[t] = (j < m && ke + 1 < m) ? (double)ro[t] : 0.0;
+        }
+        accE = __builtin_amdgcn_mfma_f64_16x16x4f64(ae[t], be[t], accE, 0, 0, 0);
+        accO = __builtin_amdgcn_mfma_f64_16x16x4f64(ao[t], bo[t], accO, 0, 0, 0);
+    }
+    // partial tiles: D[r][c] sits in lane c + 16 (r % 4), element r / 4
+    __shared__ double red[2][4][4][64];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) red[wave][q][lane] = acc[q];
+    for (int q = 0; q < 4; ++q) {
+        red[0][wave][q][lane] = accE[q];
+        red[1][wave][q][lane] = accO[q];
+    }
     __syncthreads();
-    if (wave == 0) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const double v = ((red[0][q][lane] + red[1][q][lane]) + red[2][q][lane]) + red[3][q][lane];
-            const int i = row0 + 4 * q + lk, j = col0 + li;   // D[i][j]: lane j + 16 (i % 4), element i / 4
-            if (PHASE == 0 || PHASE == 2) {
-                G(tk.T0)[(size_t)i * P + j] = v;
-            } else if (PHASE == 1) {
-                G(tk.T1)[(size_t)i * P + j] = v * (tk.h2 * G(tk.W)[(size_t)i * P + j]);
-            } else if (i < m && j < m) {
-                G(tk.e_out)[(size_t)(i + 1) * n + (j + 1)] = (float)v;
-            }
+    // thread -> tile entry (r, c), ordered so that consecutive lanes store to
+    // consecutive addresses: r fastest for the transposed stores of phases 0
+    // and 2, c fastest for the row-major stores of phases 1 and 3
+    const int u = threadIdx.x & 15, v = threadIdx.x >> 4;
+    const int r = LEFT ? u : v, c = LEFT ? v : u;
+    const int sl = c + 16 * (r & 3), sq = r >> 2;
+    const double vE = ((red[0][0][sq][sl] + red[0][1][sq][sl]) + red[0][2][sq][sl]) + red[0][3][sq][sl];
+    const double vO = ((red[1][0][sq][sl] + red[1][1][sq][sl]) + red[1][2][sq][sl]) + red[1][3][sq][sl];
+    // the computed entry (row, col) of the product and its mirror (row2, col2)
+    int row, col, row2, col2;
+    if (LEFT) {
+        row = h0 + r;
+        row2 = m - 1 - row;
+        col = col2 = f0 + c;
+    } else {
+        row = row2 = f0 + r;
+        col = h0 + c;
+        col2 = m - 1 - col;
+    }
+    const int hidx = LEFT ? row : col;     // index along the halved side
+    const int fidx = LEFT ? col : row;     // index along the free side
+    if (hidx < nb && fidx < m) {
+        const double v1 = vE + vO, v2 = vE - vO;
+        const bool mirror = m - 1 - hidx != hidx;
+        if (LEFT) {
+            // stored transposed: T0[col][row]
+            G(tk.T0)[(size_t)col * P + row] = v1;
+            if (mirror) G(tk.T0)[(size_t)col2 * P + row2] = v2;
+        } else if (PHASE == 1) {
+            const gptr<const double> W = G(tk.W);
+            G(tk.T1)[(size_t)row * P + col] = v1 * (tk.h2 * W[(size_t)row * P + col]);
+            if (mirror) G(tk.T1)[(size_t)row2 * P + col2] = v2 * (tk.h2 * W[(size_t)row2 * P + col2]);
+        } else {
+            G(tk.e_out)[(size_t)(row + 1) * n + (col + 1)] = (float)v1;
+            if (mirror) G(tk.e_out)[(size_t)(row2 + 1) * n + (col2 + 1)] = (float)v2;
         }
     }
     if (PHASE == 3 && tile.tx == 0 && tile.ty == 0) {
@@ -1078,6 +1182,8 @@ int MGSolver::init() {
     if (p_.mode == PIC_MG_MIXED && p_.coarse_direct == 2 && lcd_ >= 0 && lcd_ < lc_) {
         const size_t P = dst_pad(ns_[lcd_] - 2);
         PIC_HIP(hipMalloc(&dst_t_, 2 * P * P * sizeof(double)));
+        // the products write only inside m x m: the padding stays zero
+        PIC_HIP(hipMemsetAsync(dst_t_, 0, 2 * P * P * sizeof(double), stream_));
     }
     for (int b = 0; b < 2; ++b) {
         ef_[b].assign(L, nullptr);
@@ -1232,7 +1338,7 @@ int MGBatch::add_level_stage(int kind, const std::vector<MGLevelTask<T>>& tasks,
             const int chunks = (int)(((size_t)n * n + 255) / 256);
             for (int c = 0; c < chunks; ++c) tiles.push_back(MGTile{k, c, 0, 0});
         } else {
-            const int g = div_up(n, MG_TILE);
+            const int g = mg_tiles(n);
             for (int ty = 0; ty < g; ++ty)
                 for (int tx = 0; tx < g; ++tx) tiles.push_back(MGTile{k, tx, ty, 0});
         }
@@ -1246,7 +1352,7 @@ int MGBatch::add_level_stage(int kind, const std::vector<MGLevelTask<T>>& tasks,
     st.n_tasks = (int)tasks.size();
     for (const auto& t : tasks) st.masked = st.masked || t.mask != nullptr;
     st.lds = kind == 3 ? 0 : tiled_lds(kind, nu, restr, sizeof(T), st.masked);
-    st.gx = kind == 3 ? 0 : div_up(tasks[0].n, MG_TILE);
+    st.gx = kind == 3 ? 0 : mg_tiles(tasks[0].n);
     static_assert(sizeof(MGLevelTask<T>) <= sizeof(st.t0), "task does not fit the by-value slot");
     std::memcpy(st.t0, &tasks[0], sizeof(MGLevelTask<T>));
     if (int rc = upload_vec(&st.tasks, tasks)) return rc;
@@ -1260,12 +1366,15 @@ int MGBatch::add_dst_stages(const std::vector<MGDstTask>& tasks, std::vector<Sta
     if (tasks.empty()) return 0;
     std::vector<MGTile> tiles;
     for (int k = 0; k < (int)tasks.size(); ++k)
-        for (int ty = 0; ty < tasks[k].gx; ++ty)
+        for (int ty = 0; ty < tasks[k].nh; ++ty)
             for (int tx = 0; tx < tasks[k].gx; ++tx) tiles.push_back(MGTile{k, tx, ty, 0});
+    int ns = 2;   // chain steps of the largest case: P / 32
+    for (const auto& t : tasks) ns = std::max(ns, t.P / 32);
     for (int phase = 0; phase < 4; ++phase) {
         Stage st;
         st.kind = 5;
         st.nu = phase;
+        st.ns = ns;
         st.n_items = (int)tiles.size();
         st.n_tasks = (int)tasks.size();
         st.gx = tasks[0].gx;
@@ -1346,7 +1455,7 @@ int MGBatch::build_step(const std::vector<double>& h) {
             }
             if (int rc = add_level_stage<T>(0, tasks, p_.pre_at(d), true, dst)) return rc;
         }
-        // coarse part: the exact solve of a level of 66..257 nodes over several
+        // coarse part: the exact solve of a level of 66..513 nodes over several
         // workgroups, the single-workgroup cycle, or tiled sweeps on a large coarsest level
         if constexpr (sizeof(T) == 4) {
             std::vector<MGDstTask> dt;
@@ -1361,6 +1470,7 @@ int MGBatch::build_step(const std::vector<double>& h) {
                 t.m = t.n - 2;
                 t.P = dst_pad(t.m);
                 t.gx = t.P / 16;
+                t.nh = div_up((t.m + 1) / 2, 16);
                 t.S = btab_[k];
                 t.W = btab_[k] + (size_t)t.P * t.P;
                 t.T0 = s->dst_t_;
@@ -1523,8 +1633,11 @@ static const std::vector<double>& dst_tables(int m, int P, bool reduced) {
         for (int b = 0; b < m; ++b) {
             if (reduced) {
                 // (a+1)(b+1) reduced mod 2(m+1) keeps the sine argument in [0, 2 pi)
+                // (sin of a multiple of pi is exactly zero, so the middle
+                // column's odd-k sum in the halved products is zero as well)
                 const long long t = (long long)(a + 1) * (b + 1) % (2LL * (m + 1));
-                S[(size_t)a * P + b] = (double)std::sin(pi * (long double)t / (long double)(m + 1));
+                S[(size_t)a * P + b] = t % (m + 1) == 0 ? 0.0
+                                                         : (double)std::sin(pi * (long double)t / (long double)(m + 1));
             } else {
                 S[(size_t)a * P + b] = (double)std::sin(pi * (long double)((a + 1) * (b + 1)) / (long double)(m + 1));
             }
@@ -1673,7 +1786,7 @@ int MGBatch::build(const std::vector<MGSolver*>& members, const std::vector<doub
 }
 
 struct MGBatchStageView {
-    int kind, nu;
+    int kind, nu, ns;
     bool restr, masked;
     const void* tasks;
     const MGTile* tiles;
@@ -1740,6 +1853,18 @@ static void launch_tiled(int kind, int nu, bool restr, bool masked, dim3 g, size
     }
 }
 
+// the big direct solve's product of one phase, chains of NS steps
+template <int NS>
+static void launch_dst(int phase, dim3 g, hipStream_t s, const StageRef<MGDstTask>& r) {
+    const dim3 b(256);
+    switch (phase) {
+    case 0: hipLaunchKernelGGL((mg_dst_stage<0, NS>), g, b, 0, s, r); break;
+    case 1: hipLaunchKernelGGL((mg_dst_stage<1, NS>), g, b, 0, s, r); break;
+    case 2: hipLaunchKernelGGL((mg_dst_stage<2, NS>), g, b, 0, s, r); break;
+    default: hipLaunchKernelGGL((mg_dst_stage<3, NS>), g, b, 0, s, r); break;
+    }
+}
+
 template <typename T>
 static void launch_stage_t(const MGBatchStageView& st, hipStream_t s) {
     const dim3 g(st.n_items);
@@ -1753,12 +1878,15 @@ static void launch_stage_t(const MGBatchStageView& st, hipStream_t s) {
         }
     } else if (st.kind == 5) {
         const auto r = make_ref<MGDstTask>(st.t0, st.tasks, st.tiles, st.n_tasks, st.gx);
-        const dim3 b(256);
-        switch (st.nu) {
-        case 0: hipLaunchKernelGGL(mg_dst_stage<0>, g, b, 0, s, r); break;
-        case 1: hipLaunchKernelGGL(mg_dst_stage<1>, g, b, 0, s, r); break;
-        case 2: hipLaunchKernelGGL(mg_dst_stage<2>, g, b, 0, s, r); break;
-        default: hipLaunchKernelGGL(mg_dst_stage<3>, g, b, 0, s, r); break;
+        switch (st.ns) {
+        case 2: launch_dst<2>(st.nu, g, s, r); break;
+        case 4: launch_dst<4>(st.nu, g, s, r); break;
+        case 6: launch_dst<6>(st.nu, g, s, r); break;
+        case 8: launch_dst<8>(st.nu, g, s, r); break;
+        case 10: launch_dst<10>(st.nu, g, s, r); break;
+        case 12: launch_dst<12>(st.nu, g, s, r); break;
+        case 14: launch_dst<14>(st.nu, g, s, r); break;
+        default: launch_dst<16>(st.nu, g, s, r); break;
         }
     } else if (st.kind == 2) {
         const auto r = make_ref<MGCoarseTask<T>>(st.t0, st.tasks, nullptr, st.n_tasks, 0);
@@ -1771,7 +1899,7 @@ static void launch_stage_t(const MGBatchStageView& st, hipStream_t s) {
 }
 
 int MGBatch::launch_stage(const Stage& st) {
-    const MGBatchStageView v{st.kind, st.nu, st.restr, st.masked, st.tasks, st.tiles, st.n_items, st.lds,
+    const MGBatchStageView v{st.kind, st.nu, st.ns, st.restr, st.masked, st.tasks, st.tiles, st.n_items, st.lds,
                              st.n_tasks, st.gx, st.t0};
     if (st.fp64) launch_stage_t<double>(v, stream_);
     else launch_stage_t<float>(v, stream_);
