@@ -33,7 +33,7 @@ struct MGParams {
     int mode;
     int inner = 1;      // mixed mode: FP32 V-cycles per FP64 refinement step
     int coarse_direct = 0;   // mixed mode: exact FP64 solve of the coarse kernel's level (1: n <= 65;
-                             // 2: the first level of at most 513 nodes, several workgroups)
+                             // 2: the first level of at most 257 nodes, several workgroups)
     int per_level = 0;  // 1: pre_l/post_l give each level's sweeps (index 0 = finest)
     int pre_l[16] = {}, post_l[16] = {};
     int pre_at(int l) const { return per_level ? pre_l[l] : pre; }
@@ -118,7 +118,7 @@ struct MGTile {
     int task, tx, ty, pad;
 };
 
-// one case of the multi-workgroup exact FP64 solve of a level of 66..513
+// one case of the multi-workgroup exact FP64 solve of a level of 66..257
 // nodes (mixed mode, coarse_direct 2): four P x P products, one launch each
 struct MGDstTask {
     const float* rhs;     // the level's rhs, n x n
@@ -163,7 +163,7 @@ public:
     // sweeps on the coarsest level: a 4x4 coarsest level of an automatic ladder
     // (2x2 unknowns, Gauss-Seidel factor 1/4 per sweep) gets at least 32, which
     // solves it to rounding; a 3x3 level is solved exactly by one sweep
-    // the level the cycle ends on: the first level of at most 513 nodes when
+    // the level the cycle ends on: the first level of at most 257 nodes when
     // it is solved exactly by the multi-workgroup sine transform (mixed mode,
     // coarse_direct 2, no electrodes, more than 65 nodes), else the coarse
     // kernel's level
@@ -179,7 +179,7 @@ public:
 private:
     friend class MGBatch;
     int n_, L_, lc_;
-    int lcd_ = -1;                 // first level of at most 513 nodes (mixed mode)
+    int lcd_ = -1;                 // first level of at most 257 nodes (mixed mode)
     int nc_ = 33;                  // largest level of the coarse kernel
     double* dst_t_ = nullptr;      // big direct solve: two P x P FP64 intermediates
     MGParams p_;

@@ -55,9 +55,13 @@ namespace pic {
 #define MG_ND 65
 #define MG_DS 63
 #define MG_DLDS (2 * MG_DS * MG_DS * 8)
-// big direct solve (coarse_direct 2): levels up to 513 nodes, unknowns a
-// side padded to a multiple of 64
-#define MG_NDB 513
+// big direct solve (coarse_direct 2): levels up to 257 nodes, unknowns a
+// side padded to a multiple of 64.  An exact solve at 513^2 (P = 512, chains
+// of 16 steps) measured the same as a tiled 513^2 level over the exact 257^2
+// solve on one grid (1025^2 steps, the 2049^2 Poisson solve) and slower in a
+// batch, where the tiled stages of many cases share launches and the 8x
+// larger transform does not.
+#define MG_NDB 257
 static inline int dst_pad(int m) { return (m + 63) / 64 * 64; }
 // tiles a side of a tiled stage on a level of n nodes (the kernels' edge_nodes
 // write the boundary row and column this leaves out when n = 32 g + 1)
@@ -765,9 +769,9 @@ __global__ __launch_bounds__(1024) void mg_coarse_stage(const StageRef<MGCoarseT
         for (int i = tx; i < n_top; i += 32) e_out[j * n_top + i] = base[j * n_top + i];
 }
 
-// ---- mixed mode: exact FP64 solve of a level of 66..513 nodes -----------------
+// ---- mixed mode: exact FP64 solve of a level of 66..257 nodes -----------------
 // The same transform as coarse_direct_solve, e = S (h^2 W o (S R S)) S, for
-// m <= 511 unknowns a side, as four products of P x P matrices (P = m rounded
+// m <= 255 unknowns a side, as four products of P x P matrices (P = m rounded
 // up to a multiple of 64, zero padded), one launch each, S alternately on the
 // left and on the right:
 //   phase 0: Y = S R                (R: the level's FP32 rhs, interior nodes)
@@ -805,7 +809,7 @@ __global__ __launch_bounds__(256) void mg_dst_stage(const StageRef<MGDstTask> a)
     const int f0 = tile.tx * 16, h0 = tile.ty * 16;
     // steps of each chain: P/4 values of k, half of them even.  The launch's
     // NS is the largest of its cases; a smaller case loads its last step's
-    // operands again (in-range addresses, no branch in front of the loads) and
+    // operands again (in-range addresses, no branch anywhere in the chain) and
     // zeroes them, so its extra products add exact zeros.
     const int ns = P >> 5;
     const int k0 = wave * (P >> 2);
@@ -817,7 +821,6 @@ __global__ __launch_bounds__(256) void mg_dst_stage(const StageRef<MGDstTask> a)
     float re[NS], ro[NS];   // phase 0: R as loaded, converted when used (no wait behind each load)
 #pragma unroll
     for (int t = 0; t < NS; ++t) {
-        const bool live = t < ns;
         const int ke = k0 + 8 * min(t, ns - 1) + 2 * lk, ko = ke + 1;
         if (LEFT) {
             // A[i][k] = S[h0 + li][k] = S[k][h0 + li]; B[k][j] = R or T1 at column f0 + li
@@ -841,7 +844,6 @@ __global__ __launch_bounds__(256) void mg_dst_stage(const StageRef<MGDstTask> a)
             be[t] = S[(size_t)ke * P + j];
             bo[t] = S[(size_t)ko * P + j];
         }
-        if (!live) ae[t] = ao[t] = 0.0;
     }
     if (stop) return;
     mg_v4d accE = mg_v4d{0.0, 0.0, 0.0, 0.0}, accO = mg_v4d{0.0, 0.0, 0.0, 0.0};
@@ -852,6 +854,7 @@ __global__ __launch_bounds__(256) void mg_dst_stage(const StageRef<MGDstTask> a)
             be[t] = (j < m && ke < m) ? (double)re[t] : 0.0;
             bo[t] = (j < m && ke + 1 < m) ? (double)ro[t] : 0.0;
         }
+        if (t >= ns) ae[t] = ao[t] = 0.0;
         accE = __builtin_amdgcn_mfma_f64_16x16x4f64(ae[t], be[t], accE, 0, 0, 0);
         accO = __builtin_amdgcn_mfma_f64_16x16x4f64(ao[t], bo[t], accO, 0, 0, 0);
     }
@@ -1455,7 +1458,7 @@ int MGBatch::build_step(const std::vector<double>& h) {
             }
             if (int rc = add_level_stage<T>(0, tasks, p_.pre_at(d), true, dst)) return rc;
         }
-        // coarse part: the exact solve of a level of 66..513 nodes over several
+        // coarse part: the exact solve of a level of 66..257 nodes over several
         // workgroups, the single-workgroup cycle, or tiled sweeps on a large coarsest level
         if constexpr (sizeof(T) == 4) {
             std::vector<MGDstTask> dt;
@@ -1878,15 +1881,12 @@ static void launch_stage_t(const MGBatchStageView& st, hipStream_t s) {
         }
     } else if (st.kind == 5) {
         const auto r = make_ref<MGDstTask>(st.t0, st.tasks, st.tiles, st.n_tasks, st.gx);
+        static_assert((MG_NDB - 1) / 32 == 8, "chain lengths instantiated up to P = 256");
         switch (st.ns) {
         case 2: launch_dst<2>(st.nu, g, s, r); break;
         case 4: launch_dst<4>(st.nu, g, s, r); break;
         case 6: launch_dst<6>(st.nu, g, s, r); break;
-        case 8: launch_dst<8>(st.nu, g, s, r); break;
-        case 10: launch_dst<10>(st.nu, g, s, r); break;
-        case 12: launch_dst<12>(st.nu, g, s, r); break;
-        case 14: launch_dst<14>(st.nu, g, s, r); break;
-        default: launch_dst<16>(st.nu, g, s, r); break;
+        default: launch_dst<8>(st.nu, g, s, r); break;
         }
     } else if (st.kind == 2) {
         const auto r = make_ref<MGCoarseTask<T>>(st.t0, st.tasks, nullptr, st.n_tasks, 0);
