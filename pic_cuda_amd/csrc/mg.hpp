@@ -229,6 +229,7 @@ private:
                            // 5 a product of the big direct solve (phase in nu)
         int nu = 0;        // sweeps of a tiled stage
         int ns = 0;        // kind 5: steps of each product chain (P / 32 of the largest case)
+        int bs = 256;      // kinds 0 and 1: threads per tile
         bool restr = false;   // smooth stage also restricts the residual
         bool masked = false;  // some task has electrodes
         bool fp64 = false;

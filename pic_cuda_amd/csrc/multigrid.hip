@@ -63,6 +63,9 @@ namespace pic {
 // larger transform does not.
 #define MG_NDB 257
 static inline int dst_pad(int m) { return (m + 63) / 64 * 64; }
+// tiled launches of at most this many tiles (one per CU of an MI355X) run
+// 1024 threads per tile
+#define MG_SMALL_LAUNCH 256
 // tiles a side of a tiled stage on a level of n nodes (the kernels' edge_nodes
 // write the boundary row and column this leaves out when n = 32 g + 1)
 static inline int mg_tiles(int n) { return n <= 1 ? 1 : div_up(n - 1, MG_TILE); }
@@ -166,13 +169,13 @@ __device__ inline T sor_node(const T* e, const T* r, int idx, int stride, T h2, 
 // global node (gi0, gj0).  Only rows and columns [lo, W-1-lo] are updated: after
 // s half-sweeps the outer s rings are stale, so half-sweep s uses lo = s + 1.
 // Colour c updates the nodes with (gi + gj) odd for c = 1, even for c = 0.
-template <typename T, int W>
+template <typename T, int W, int BS>
 __device__ inline void tile_colour(T* e, const T* r, const uint8_t* mk, int gi0, int gj0, int n, T h2, T omega,
                                    int colour, int lo) {
     const int span = W - 2 * lo;
     const int halfc = (span + 1) / 2;
     const int items = span * halfc;
-    for (int k = threadIdx.x; k < items; k += 256) {
+    for (int k = threadIdx.x; k < items; k += BS) {
         const int q = k / halfc;
         const int lj = lo + q, m = k - q * halfc;
         const int gj = gj0 + lj;
@@ -190,12 +193,12 @@ __device__ inline void tile_colour(T* e, const T* r, const uint8_t* mk, int gi0,
 // full-weighting restriction to the coarse nodes (2ic, 2jc) inside the tile;
 // e and r are the W-wide LDS tile whose element (0,0) is node (gi0, gj0).
 // Fixed (electrode) nodes have zero residual.
-template <typename T, int W>
+template <typename T, int W, int BS>
 __device__ inline void restrict_tile(const T* e, const T* r, T* res, const uint8_t* mk, int i0, int j0, int gi0,
                                      int gj0, int n, T inv_h2, const gptr<T> rhs_c) {
     const int tid = threadIdx.x;
     constexpr int RW = MG_TILE + 2;
-    for (int k = tid; k < RW * RW; k += 256) {
+    for (int k = tid; k < RW * RW; k += BS) {
         const int q = k / RW;
         const int gi = i0 - 1 + (k - q * RW), gj = j0 - 1 + q;
         T v = (T)0;
@@ -209,7 +212,8 @@ __device__ inline void restrict_tile(const T* e, const T* r, T* res, const uint8
     __syncthreads();
     const int nc = (n + 1) / 2;
     constexpr int TC = MG_TILE / 2;
-    const int ic = i0 / 2 + tid % TC, jc = j0 / 2 + tid / TC;   // TC*TC == 256
+    if (tid >= TC * TC) return;   // one thread per coarse node of the tile (256)
+    const int ic = i0 / 2 + tid % TC, jc = j0 / 2 + tid / TC;
     if (ic >= nc || jc >= nc) return;
     const int i = 2 * ic, j = 2 * jc;
     if (i >= n || j >= n) {
@@ -280,12 +284,12 @@ __device__ inline void edge_nodes(int i0, int j0, int n, const gptr<const T> e_i
 // (2ic, 2jc) inside the tile.  Without RESTRICT (tiled sweeps of a large
 // coarsest level) the halo is 2*NU.  With electrodes the mask tile is staged
 // in LDS too: fixed nodes are not smoothed and have zero residual.
-template <typename T, int NU, bool RESTRICT, bool MASKED>
-__global__ __launch_bounds__(256) void mg_smooth_stage(const StageRef<MGLevelTask<T>> a) {
+template <typename T, int NU, bool RESTRICT, bool MASKED, int BS>
+__global__ __launch_bounds__(BS) void mg_smooth_stage(const StageRef<MGLevelTask<T>> a) {
     constexpr int H = RESTRICT ? 2 * NU + 2 : 2 * NU;
     constexpr int W = MG_TILE + 2 * H;
     constexpr int WW = W * W;
-    constexpr int NL = (WW + 255) / 256;
+    constexpr int NL = (WW + BS - 1) / BS;
     MGTile tile;
     MGLevelTask<T> tk;
     stage_item(a, tk, tile);
@@ -312,7 +316,7 @@ __global__ __launch_bounds__(256) void mg_smooth_stage(const StageRef<MGLevelTas
     uint8_t mv[NL];
 #pragma unroll
     for (int q = 0; q < NL; ++q) {
-        const int k = tid + 256 * q;
+        const int k = tid + BS * q;
         const int lj = k / W, li = k - lj * W;
         const int gi = gi0 + li, gj = gj0 + lj;
         ev[q] = (T)0;
@@ -328,7 +332,7 @@ __global__ __launch_bounds__(256) void mg_smooth_stage(const StageRef<MGLevelTas
     if (stop) return;
 #pragma unroll
     for (int q = 0; q < NL; ++q) {
-        const int k = tid + 256 * q;
+        const int k = tid + BS * q;
         if (k < WW) {
             e[k] = ev[q];
             r[k] = rv[q];
@@ -338,18 +342,18 @@ __global__ __launch_bounds__(256) void mg_smooth_stage(const StageRef<MGLevelTas
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < 2 * NU; ++s) {
-        tile_colour<T, W>(e, r, mk, gi0, gj0, n, h2, omega, 1 - (s & 1), s + 1);
+        tile_colour<T, W, BS>(e, r, mk, gi0, gj0, n, h2, omega, 1 - (s & 1), s + 1);
         __syncthreads();
     }
 #pragma unroll
-    for (int q = 0; q < (MG_TILE * MG_TILE) / 256; ++q) {
-        const int k = tid + 256 * q;
+    for (int q = 0; q < (MG_TILE * MG_TILE) / BS; ++q) {
+        const int k = tid + BS * q;
         const int gi = i0 + k % MG_TILE, gj = j0 + k / MG_TILE;
         if (gi < n && gj < n) e_out[(size_t)gj * n + gi] = e[(gj - gj0) * W + (gi - gi0)];
     }
     edge_nodes<T>(i0, j0, n, e_in, nullptr, 0, 0, 0, gmask, e_out, RESTRICT ? rhs_c : G((T*)nullptr));
     if constexpr (RESTRICT) {
-        if (rhs_c) restrict_tile<T, W>(e, r, r + WW, mk, i0, j0, gi0, gj0, n, inv_h2, rhs_c);
+        if (rhs_c) restrict_tile<T, W, BS>(e, r, r + WW, mk, i0, j0, gi0, gj0, n, inv_h2, rhs_c);
     }
 }
 
@@ -358,15 +362,15 @@ __global__ __launch_bounds__(256) void mg_smooth_stage(const StageRef<MGLevelTas
 // refinement step) NU is the post-sweeps of this cycle plus the pre-sweeps of
 // the next, the halo is 2*NU+2, and the residual is restricted into rhs_c as
 // the next cycle's down leg would: the same arithmetic in one launch fewer.
-template <typename T, int NU, bool RESTRICT, bool MASKED>
-__global__ __launch_bounds__(256) void mg_prolong_stage(const StageRef<MGLevelTask<T>> a) {
+template <typename T, int NU, bool RESTRICT, bool MASKED, int BS>
+__global__ __launch_bounds__(BS) void mg_prolong_stage(const StageRef<MGLevelTask<T>> a) {
     constexpr int H = RESTRICT ? 2 * NU + 2 : 2 * NU;
     constexpr int W = MG_TILE + 2 * H;
     constexpr int WW = W * W;
-    constexpr int NL = (WW + 255) / 256;
+    constexpr int NL = (WW + BS - 1) / BS;
     constexpr int CW = W / 2 + 2;
     constexpr int CC = CW * CW;
-    constexpr int NCL = (CC + 255) / 256;
+    constexpr int NCL = (CC + BS - 1) / BS;
     MGTile tile;
     MGLevelTask<T> tk;
     stage_item(a, tk, tile);
@@ -395,7 +399,7 @@ __global__ __launch_bounds__(256) void mg_prolong_stage(const StageRef<MGLevelTa
     uint8_t mv[NL];
 #pragma unroll
     for (int q = 0; q < NCL; ++q) {
-        const int k = tid + 256 * q;
+        const int k = tid + BS * q;
         const int cj = k / CW, ci = k - cj * CW;
         const int ic = ic0 + ci, jc = jc0 + cj;
         cv[q] = (T)0;
@@ -403,7 +407,7 @@ __global__ __launch_bounds__(256) void mg_prolong_stage(const StageRef<MGLevelTa
     }
 #pragma unroll
     for (int q = 0; q < NL; ++q) {
-        const int k = tid + 256 * q;
+        const int k = tid + BS * q;
         const int lj = k / W, li = k - lj * W;
         const int gi = gi0 + li, gj = gj0 + lj;
         ev[q] = (T)0;
@@ -419,13 +423,13 @@ __global__ __launch_bounds__(256) void mg_prolong_stage(const StageRef<MGLevelTa
     if (stop) return;
 #pragma unroll
     for (int q = 0; q < NCL; ++q) {
-        const int k = tid + 256 * q;
+        const int k = tid + BS * q;
         if (k < CC) C[k] = cv[q];
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < NL; ++q) {
-        const int k = tid + 256 * q;
+        const int k = tid + BS * q;
         if (k < WW) {
             const int lj = k / W, li = k - lj * W;
             const int gi = gi0 + li, gj = gj0 + lj;
@@ -441,18 +445,18 @@ __global__ __launch_bounds__(256) void mg_prolong_stage(const StageRef<MGLevelTa
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < 2 * NU; ++s) {
-        tile_colour<T, W>(e, r, mk, gi0, gj0, n, h2, omega, 1 - (s & 1), s + 1);
+        tile_colour<T, W, BS>(e, r, mk, gi0, gj0, n, h2, omega, 1 - (s & 1), s + 1);
         __syncthreads();
     }
 #pragma unroll
-    for (int q = 0; q < (MG_TILE * MG_TILE) / 256; ++q) {
-        const int k = tid + 256 * q;
+    for (int q = 0; q < (MG_TILE * MG_TILE) / BS; ++q) {
+        const int k = tid + BS * q;
         const int gi = i0 + k % MG_TILE, gj = j0 + k / MG_TILE;
         if (gi < n && gj < n) e_out[(size_t)gj * n + gi] = e[(gj - gj0) * W + (gi - gi0)];
     }
     edge_nodes<T>(i0, j0, n, e_in, C, CW, ic0, jc0, gmask, e_out, RESTRICT ? rhs_c : G((T*)nullptr));
     if constexpr (RESTRICT) {
-        if (rhs_c) restrict_tile<T, W>(e, r, res, mk, i0, j0, gi0, gj0, n, inv_h2, rhs_c);
+        if (rhs_c) restrict_tile<T, W, BS>(e, r, res, mk, i0, j0, gi0, gj0, n, inv_h2, rhs_c);
     }
 }
 
@@ -1039,6 +1043,22 @@ __global__ __launch_bounds__(256) void dirichlet_stage(const StageRef<MGFineTask
 // once the flag is set, so cycles enqueued after convergence change nothing.
 // Relative residual ||r||/||rhs|| (the reference divides by sum rhs^2,
 // Multigrid.cu:466).
+// Thread t's share of a fixed-order sum: p[t], p[t + 256], p[t + 512], ... added
+// in that order.  The loads of eight terms are issued before their adds (in
+// range: clamped), so their latencies overlap instead of adding up.
+__device__ inline double strided_sum(const gptr<const double> p, int np) {
+    double acc = 0.0;
+    for (int base = threadIdx.x; base < np; base += 8 * 256) {
+        double v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = p[min(base + 256 * q, np - 1)];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (base + 256 * q < np) acc += v[q];
+    }
+    return acc;
+}
+
 __global__ __launch_bounds__(256) void norm_final_stage(const StageRef<MGFineTask> a, int which, double tol,
                                                         int max_cycles, int inner) {
     MGFineTask tk;
@@ -1047,15 +1067,11 @@ __global__ __launch_bounds__(256) void norm_final_stage(const StageRef<MGFineTas
     if (which == 2 && out->converged) return;
     __shared__ double sh[256];
     const int np = tk.nblocks;
-    double acc = 0.0;
-    for (int k = threadIdx.x; k < np; k += 256) acc += G(tk.partial)[k];
-    const double tot = block_sum256(acc, sh);
+    const double tot = block_sum256(strided_sum(G((const double*)tk.partial), np), sh);
     double tot_rhs = 0.0;
     if (which == 1) {   // a new solve: the rhs norm from the same pass, counters reset
         __syncthreads();
-        acc = 0.0;
-        for (int k = threadIdx.x; k < np; k += 256) acc += G(tk.partial)[np + k];
-        tot_rhs = block_sum256(acc, sh);
+        tot_rhs = block_sum256(strided_sum(G((const double*)tk.partial) + np, np), sh);
     }
     if (threadIdx.x == 0) {
         if (which == 1) {
@@ -1355,6 +1371,9 @@ int MGBatch::add_level_stage(int kind, const std::vector<MGLevelTask<T>>& tasks,
     st.n_tasks = (int)tasks.size();
     for (const auto& t : tasks) st.masked = st.masked || t.mask != nullptr;
     st.lds = kind == 3 ? 0 : tiled_lds(kind, nu, restr, sizeof(T), st.masked);
+    // a launch of at most one tile per CU (the small levels) is bound by the
+    // latency of one workgroup's sweeps: 1024 threads per tile instead of 256
+    st.bs = (kind == 0 || kind == 1) && st.n_items <= MG_SMALL_LAUNCH ? 1024 : 256;
     st.gx = kind == 3 ? 0 : mg_tiles(tasks[0].n);
     static_assert(sizeof(MGLevelTask<T>) <= sizeof(st.t0), "task does not fit the by-value slot");
     std::memcpy(st.t0, &tasks[0], sizeof(MGLevelTask<T>));
@@ -1789,7 +1808,7 @@ int MGBatch::build(const std::vector<MGSolver*>& members, const std::vector<doub
 }
 
 struct MGBatchStageView {
-    int kind, nu, ns;
+    int kind, nu, ns, bs;
     bool restr, masked;
     const void* tasks;
     const MGTile* tiles;
@@ -1810,26 +1829,34 @@ static StageRef<Task> make_ref(const void* t0, const void* tasks, const MGTile* 
     return r;
 }
 
+template <typename T, int NU, bool M, int BS>
+static void launch_tiled_bs(int kind, bool restr, dim3 g, size_t lds, hipStream_t s, const StageRef<MGLevelTask<T>>& r) {
+    const dim3 b(BS);
+    if (kind == 1) hipLaunchKernelGGL((mg_prolong_stage<T, NU, false, M, BS>), g, b, lds, s, r);
+    else if (restr) hipLaunchKernelGGL((mg_smooth_stage<T, NU, true, M, BS>), g, b, lds, s, r);
+    else hipLaunchKernelGGL((mg_smooth_stage<T, NU, false, M, BS>), g, b, lds, s, r);
+}
+
 template <typename T, int NU, bool M>
-static void launch_tiled_nu(int kind, bool restr, dim3 g, size_t lds, hipStream_t s, const StageRef<MGLevelTask<T>>& r) {
-    if (kind == 1) hipLaunchKernelGGL((mg_prolong_stage<T, NU, false, M>), g, dim3(256), lds, s, r);
-    else if (restr) hipLaunchKernelGGL((mg_smooth_stage<T, NU, true, M>), g, dim3(256), lds, s, r);
-    else hipLaunchKernelGGL((mg_smooth_stage<T, NU, false, M>), g, dim3(256), lds, s, r);
+static void launch_tiled_nu(int kind, bool restr, int bs, dim3 g, size_t lds, hipStream_t s,
+                            const StageRef<MGLevelTask<T>>& r) {
+    if (bs == 1024) launch_tiled_bs<T, NU, M, 1024>(kind, restr, g, lds, s, r);
+    else launch_tiled_bs<T, NU, M, 256>(kind, restr, g, lds, s, r);
 }
 
 template <typename T, int NU>
-static void launch_tiled_m(int kind, bool restr, bool masked, dim3 g, size_t lds, hipStream_t s,
+static void launch_tiled_m(int kind, bool restr, bool masked, int bs, dim3 g, size_t lds, hipStream_t s,
                            const StageRef<MGLevelTask<T>>& r) {
-    if (masked) launch_tiled_nu<T, NU, true>(kind, restr, g, lds, s, r);
-    else launch_tiled_nu<T, NU, false>(kind, restr, g, lds, s, r);
+    if (masked) launch_tiled_nu<T, NU, true>(kind, restr, bs, g, lds, s, r);
+    else launch_tiled_nu<T, NU, false>(kind, restr, bs, g, lds, s, r);
 }
 
 // the finest up-leg stage fused with the next V-cycle's down-leg stage (mixed
 // mode only, so FP32): NU = post + pre sweeps, 0..6
 template <int NU>
 static void launch_fused_m(bool masked, dim3 g, size_t lds, hipStream_t s, const StageRef<MGLevelTask<float>>& r) {
-    if (masked) hipLaunchKernelGGL((mg_prolong_stage<float, NU, true, true>), g, dim3(256), lds, s, r);
-    else hipLaunchKernelGGL((mg_prolong_stage<float, NU, true, false>), g, dim3(256), lds, s, r);
+    if (masked) hipLaunchKernelGGL((mg_prolong_stage<float, NU, true, true, 256>), g, dim3(256), lds, s, r);
+    else hipLaunchKernelGGL((mg_prolong_stage<float, NU, true, false, 256>), g, dim3(256), lds, s, r);
 }
 
 static void launch_fused(int nu, bool masked, dim3 g, size_t lds, hipStream_t s,
@@ -1846,13 +1873,13 @@ static void launch_fused(int nu, bool masked, dim3 g, size_t lds, hipStream_t s,
 }
 
 template <typename T>
-static void launch_tiled(int kind, int nu, bool restr, bool masked, dim3 g, size_t lds, hipStream_t s,
+static void launch_tiled(int kind, int nu, bool restr, bool masked, int bs, dim3 g, size_t lds, hipStream_t s,
                          const StageRef<MGLevelTask<T>>& r) {
     switch (nu) {
-    case 0: launch_tiled_m<T, 0>(kind, restr, masked, g, lds, s, r); break;
-    case 1: launch_tiled_m<T, 1>(kind, restr, masked, g, lds, s, r); break;
-    case 2: launch_tiled_m<T, 2>(kind, restr, masked, g, lds, s, r); break;
-    default: launch_tiled_m<T, 3>(kind, restr, masked, g, lds, s, r); break;
+    case 0: launch_tiled_m<T, 0>(kind, restr, masked, bs, g, lds, s, r); break;
+    case 1: launch_tiled_m<T, 1>(kind, restr, masked, bs, g, lds, s, r); break;
+    case 2: launch_tiled_m<T, 2>(kind, restr, masked, bs, g, lds, s, r); break;
+    default: launch_tiled_m<T, 3>(kind, restr, masked, bs, g, lds, s, r); break;
     }
 }
 
@@ -1873,7 +1900,7 @@ static void launch_stage_t(const MGBatchStageView& st, hipStream_t s) {
     const dim3 g(st.n_items);
     if (st.kind == 0 || st.kind == 1) {
         const auto r = make_ref<MGLevelTask<T>>(st.t0, st.tasks, st.tiles, st.n_tasks, st.gx);
-        launch_tiled<T>(st.kind, st.nu, st.restr, st.masked, g, st.lds, s, r);
+        launch_tiled<T>(st.kind, st.nu, st.restr, st.masked, st.bs, g, st.lds, s, r);
     } else if (st.kind == 4) {
         if constexpr (sizeof(T) == 4) {
             const auto r = make_ref<MGLevelTask<float>>(st.t0, st.tasks, st.tiles, st.n_tasks, st.gx);
@@ -1899,7 +1926,7 @@ static void launch_stage_t(const MGBatchStageView& st, hipStream_t s) {
 }
 
 int MGBatch::launch_stage(const Stage& st) {
-    const MGBatchStageView v{st.kind, st.nu, st.ns, st.restr, st.masked, st.tasks, st.tiles, st.n_items, st.lds,
+    const MGBatchStageView v{st.kind, st.nu, st.ns, st.bs, st.restr, st.masked, st.tasks, st.tiles, st.n_items, st.lds,
                              st.n_tasks, st.gx, st.t0};
     if (st.fp64) launch_stage_t<double>(v, stream_);
     else launch_stage_t<float>(v, stream_);
