@@ -64,7 +64,7 @@ namespace pic {
 #define MG_NDB 257
 static inline int dst_pad(int m) { return (m + 63) / 64 * 64; }
 // tiled launches of at most this many tiles (one per CU of an MI355X) run
-// 1024 threads per tile
+// 1024 threads per tile, up to four times as many 512
 #define MG_SMALL_LAUNCH 256
 // tiles a side of a tiled stage on a level of n nodes (the kernels' edge_nodes
 // write the boundary row and column this leaves out when n = 32 g + 1)
@@ -1372,8 +1372,10 @@ int MGBatch::add_level_stage(int kind, const std::vector<MGLevelTask<T>>& tasks,
     for (const auto& t : tasks) st.masked = st.masked || t.mask != nullptr;
     st.lds = kind == 3 ? 0 : tiled_lds(kind, nu, restr, sizeof(T), st.masked);
     // a launch of at most one tile per CU (the small levels) is bound by the
-    // latency of one workgroup's sweeps: 1024 threads per tile instead of 256
-    st.bs = (kind == 0 || kind == 1) && st.n_items <= MG_SMALL_LAUNCH ? 1024 : 256;
+    // latency of one workgroup's sweeps: 1024 threads per tile; up to four
+    // tiles per CU, 512 (the CU's 32 wavefronts either way); 256 beyond
+    st.bs = 256;
+    if (kind == 0 || kind == 1) st.bs = st.n_items <= MG_SMALL_LAUNCH ? 1024 : st.n_items <= 4 * MG_SMALL_LAUNCH ? 512 : 256;
     st.gx = kind == 3 ? 0 : mg_tiles(tasks[0].n);
     static_assert(sizeof(MGLevelTask<T>) <= sizeof(st.t0), "task does not fit the by-value slot");
     std::memcpy(st.t0, &tasks[0], sizeof(MGLevelTask<T>));
@@ -1841,6 +1843,7 @@ template <typename T, int NU, bool M>
 static void launch_tiled_nu(int kind, bool restr, int bs, dim3 g, size_t lds, hipStream_t s,
                             const StageRef<MGLevelTask<T>>& r) {
     if (bs == 1024) launch_tiled_bs<T, NU, M, 1024>(kind, restr, g, lds, s, r);
+    else if (bs == 512) launch_tiled_bs<T, NU, M, 512>(kind, restr, g, lds, s, r);
     else launch_tiled_bs<T, NU, M, 256>(kind, restr, g, lds, s, r);
 }
 
