@@ -1376,6 +1376,7 @@ int MGBatch::add_level_stage(int kind, const std::vector<MGLevelTask<T>>& tasks,
     // tiles per CU, 512 (the CU's 32 wavefronts either way); 256 beyond
     st.bs = 256;
     if (kind == 0 || kind == 1) st.bs = st.n_items <= MG_SMALL_LAUNCH ? 1024 : st.n_items <= 4 * MG_SMALL_LAUNCH ? 512 : 256;
+    if (kind == 4 && st.n_items <= 4 * MG_SMALL_LAUNCH) st.bs = 512;
     st.gx = kind == 3 ? 0 : mg_tiles(tasks[0].n);
     static_assert(sizeof(MGLevelTask<T>) <= sizeof(st.t0), "task does not fit the by-value slot");
     std::memcpy(st.t0, &tasks[0], sizeof(MGLevelTask<T>));
@@ -1856,22 +1857,29 @@ static void launch_tiled_m(int kind, bool restr, bool masked, int bs, dim3 g, si
 
 // the finest up-leg stage fused with the next V-cycle's down-leg stage (mixed
 // mode only, so FP32): NU = post + pre sweeps, 0..6
-template <int NU>
-static void launch_fused_m(bool masked, dim3 g, size_t lds, hipStream_t s, const StageRef<MGLevelTask<float>>& r) {
-    if (masked) hipLaunchKernelGGL((mg_prolong_stage<float, NU, true, true, 256>), g, dim3(256), lds, s, r);
-    else hipLaunchKernelGGL((mg_prolong_stage<float, NU, true, false, 256>), g, dim3(256), lds, s, r);
+template <int NU, int BS>
+static void launch_fused_bs(bool masked, dim3 g, size_t lds, hipStream_t s, const StageRef<MGLevelTask<float>>& r) {
+    if (masked) hipLaunchKernelGGL((mg_prolong_stage<float, NU, true, true, BS>), g, dim3(BS), lds, s, r);
+    else hipLaunchKernelGGL((mg_prolong_stage<float, NU, true, false, BS>), g, dim3(BS), lds, s, r);
 }
 
-static void launch_fused(int nu, bool masked, dim3 g, size_t lds, hipStream_t s,
+template <int NU>
+static void launch_fused_m(bool masked, int bs, dim3 g, size_t lds, hipStream_t s,
+                           const StageRef<MGLevelTask<float>>& r) {
+    if (bs == 512) launch_fused_bs<NU, 512>(masked, g, lds, s, r);
+    else launch_fused_bs<NU, 256>(masked, g, lds, s, r);
+}
+
+static void launch_fused(int nu, bool masked, int bs, dim3 g, size_t lds, hipStream_t s,
                          const StageRef<MGLevelTask<float>>& r) {
     switch (nu) {
-    case 0: launch_fused_m<0>(masked, g, lds, s, r); break;
-    case 1: launch_fused_m<1>(masked, g, lds, s, r); break;
-    case 2: launch_fused_m<2>(masked, g, lds, s, r); break;
-    case 3: launch_fused_m<3>(masked, g, lds, s, r); break;
-    case 4: launch_fused_m<4>(masked, g, lds, s, r); break;
-    case 5: launch_fused_m<5>(masked, g, lds, s, r); break;
-    default: launch_fused_m<6>(masked, g, lds, s, r); break;
+    case 0: launch_fused_m<0>(masked, bs, g, lds, s, r); break;
+    case 1: launch_fused_m<1>(masked, bs, g, lds, s, r); break;
+    case 2: launch_fused_m<2>(masked, bs, g, lds, s, r); break;
+    case 3: launch_fused_m<3>(masked, bs, g, lds, s, r); break;
+    case 4: launch_fused_m<4>(masked, bs, g, lds, s, r); break;
+    case 5: launch_fused_m<5>(masked, bs, g, lds, s, r); break;
+    default: launch_fused_m<6>(masked, bs, g, lds, s, r); break;
     }
 }
 
@@ -1907,7 +1915,7 @@ static void launch_stage_t(const MGBatchStageView& st, hipStream_t s) {
     } else if (st.kind == 4) {
         if constexpr (sizeof(T) == 4) {
             const auto r = make_ref<MGLevelTask<float>>(st.t0, st.tasks, st.tiles, st.n_tasks, st.gx);
-            launch_fused(st.nu, st.masked, g, st.lds, s, r);
+            launch_fused(st.nu, st.masked, st.bs, g, st.lds, s, r);
         }
     } else if (st.kind == 5) {
         const auto r = make_ref<MGDstTask>(st.t0, st.tasks, st.tiles, st.n_tasks, st.gx);
