@@ -84,6 +84,9 @@ int pic_device_synchronize(void);
 int pic_device_alloc(int64_t bytes, void** out);
 int pic_device_free(void* ptr);
 int pic_device_to_host(void* host, const void* dev, int64_t bytes);
+/* host -> device copy (after the library's work), for filling the staging
+   arrays of pic_particles_staging without a HIP runtime of one's own */
+int pic_host_to_device(void* dev, const void* host, int64_t bytes);
 
 /* ---- Particles2D: particles.cuh:3-39, particles.cu:112-261 --------------- */
 /* Particles2D(int num_particles) particles.cu:112-150.  Species 0 exists with
@@ -110,6 +113,24 @@ int pic_particles_upload(pic_particles* p, int species, int64_t n, const double*
 /* device -> host copy of a species (order unspecified; sort by id to compare) */
 int pic_particles_download(pic_particles* p, int species, int64_t cap, double* x, double* y, double* vx,
                            double* vy, uint32_t* id, int64_t* n);
+/* The public device arrays of Particles2D (particles.cuh:7-20: r, r_old, v,
+   type) for callers that fill them with cudaMemcpy, as main.cu:232-238 does:
+   a staging area of n_max entries in the reference's layout -- r, r_old, v as
+   (x, y) double pairs in domain units, type as the species index of each
+   entry (zeroed: species 0).  r_old is allocated only when asked for and is
+   never read (the step does not use old positions).  A larger n_max
+   reallocates (the pointers change); any output pointer may be NULL. */
+int pic_particles_staging(pic_particles* p, int64_t n_max, double** r, double** r_old, double** v,
+                          int32_t** type);
+/* particles.last_alive = n (main.cu:238): the first n staged entries become
+   particles of their species, with ids continuing from the last staged batch.
+   Without a simulation the hand-over waits for pic_sim_create, whose grid gives
+   the cell size; in a running simulation it happens at once and the next step
+   re-bins.  The type array is zeroed again.  PIC_ERR_ARG for a type that is
+   not a species, PIC_ERR_CAPACITY when a species has no room. */
+int pic_particles_set_staged(pic_particles* p, int64_t n);
+/* the same hand-over now, with the cell size given (x / dx, y / dy) */
+int pic_particles_commit_staging(pic_particles* p, int64_t n, double dx, double dy);
 /* convertPositionsToFloat (kernels.cu:268-273) as used by updateVBO
    (render.h:384-405): the positions of every live particle of `species` as
    float (x, y) pairs packed into the device buffer dst_device (room for cap

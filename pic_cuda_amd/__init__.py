@@ -124,6 +124,10 @@ SIGNATURES = {
     "pic_device_alloc": (_i, [_i64, _P(_vp)]),
     "pic_device_free": (_i, [_vp]),
     "pic_device_to_host": (_i, [_vp, _vp, _i64]),
+    "pic_host_to_device": (_i, [_vp, _vp, _i64]),
+    "pic_particles_staging": (_i, [_vp, _i64, _P(_vp), _P(_vp), _P(_vp), _P(_vp)]),
+    "pic_particles_set_staged": (_i, [_vp, _i64]),
+    "pic_particles_commit_staging": (_i, [_vp, _i64, _d, _d]),
     "pic_particles_create": (_i, [_i64, _i, _P(_vp)]),
     "pic_particles_destroy": (_i, [_vp]),
     "pic_particles_add_species": (_i, [_vp, _d, _d, _P(_i)]),
@@ -251,6 +255,19 @@ def synchronize():
     _check(load().pic_device_synchronize())
 
 
+def host_to_device(dev_ptr, array):
+    """Copy a contiguous numpy array to a device address (after the library's work)."""
+    a = np.ascontiguousarray(array)
+    _check(load().pic_host_to_device(ctypes.c_void_p(dev_ptr), a.ctypes.data_as(ctypes.c_void_p), a.nbytes))
+
+
+def device_to_host(dev_ptr, count, dtype):
+    """Copy `count` elements of `dtype` from a device address into a new numpy array."""
+    out = np.empty(int(count), dtype=dtype)
+    _check(load().pic_device_to_host(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(dev_ptr), out.nbytes))
+    return out
+
+
 class DeviceBuffer:
     """Device memory from the library (no torch needed): `ptr` for the entry
     points that take device pointers, to_numpy() to read it back."""
@@ -324,6 +341,24 @@ class Particles2D:
                 _lib.pic_particles_destroy(self._h)
         except Exception:
             pass
+
+    def staging(self, n_max, r_old=False):
+        """The reference's public device arrays r, r_old, v, type
+        (particles.cuh:7-20) as a staging area of n_max entries: a dict of
+        device addresses (r, v: (x, y) double pairs in domain units; type:
+        int32 species index, zeroed).  Fill them with `host_to_device`."""
+        r, ro, v, t = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        _check(load().pic_particles_staging(self._h, int(n_max), ctypes.byref(r),
+                                            ctypes.byref(ro) if r_old else None, ctypes.byref(v), ctypes.byref(t)))
+        return {"r": r.value, "r_old": ro.value if r_old else None, "v": v.value, "type": t.value}
+
+    def set_staged(self, n):
+        """particles.last_alive = n (main.cu:238): the first n staged entries
+        become particles (when a simulation takes them, or at once in one)."""
+        _check(load().pic_particles_set_staged(self._h, int(n)))
+
+    def commit_staging(self, n, dx, dy):
+        _check(load().pic_particles_commit_staging(self._h, int(n), float(dx), float(dy)))
 
     def add_species(self, m, q):
         """add_species(double m, double q) -- argument order (m, q), particles.cu:174."""

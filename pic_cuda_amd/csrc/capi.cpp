@@ -209,6 +209,13 @@ extern "C" int pic_device_to_host(void* host, const void* dev, int64_t bytes) {
     return 0;
 }
 
+extern "C" int pic_host_to_device(void* dev, const void* host, int64_t bytes) {
+    PIC_REQUIRE(host && dev && bytes >= 0, "NULL argument or negative size");
+    PIC_HIP(hipStreamSynchronize(lib_stream()));
+    if (bytes) PIC_HIP(hipMemcpy(dev, host, (size_t)bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+
 // ============================================================================
 // Particles2D
 // ============================================================================
@@ -258,6 +265,10 @@ extern "C" int pic_particles_destroy(pic_particles* p) {
     (void)hipFree(p->d_items);
     (void)hipFree(p->d_scratch);
     (void)hipFree(p->d_ll);
+    (void)hipFree(p->stage_r);
+    (void)hipFree(p->stage_r_old);
+    (void)hipFree(p->stage_v);
+    (void)hipFree(p->stage_type);
     delete p;
     return 0;
 }
@@ -357,6 +368,117 @@ extern "C" int pic_particles_upload(pic_particles* p, int species, int64_t n, co
     }
     return p->prec == 4 ? upload_t<float>(p, species, n, x, y, vx, vy, id)
                         : upload_t<double>(p, species, n, x, y, vx, vy, id);
+}
+
+// ---- the reference's public device arrays (particles.cuh:7-20) ------------
+extern "C" int pic_particles_staging(pic_particles* p, int64_t n_max, double** r, double** r_old, double** v,
+                                     int32_t** type) {
+    PIC_REQUIRE(p, "particles is NULL");
+    PIC_REQUIRE(n_max >= 0 && n_max < INT32_MAX, "bad staging size");
+    PIC_REQUIRE(p->stage_pending == 0, "staged particles are waiting for a simulation");
+    if (n_max > p->stage_cap) {
+        PIC_HIP(hipStreamSynchronize(lib_stream()));
+        (void)hipFree(p->stage_r);
+        (void)hipFree(p->stage_r_old);
+        (void)hipFree(p->stage_v);
+        (void)hipFree(p->stage_type);
+        p->stage_r = p->stage_r_old = p->stage_v = nullptr;
+        p->stage_type = nullptr;
+        p->stage_cap = 0;
+        PIC_HIP(hipMalloc(&p->stage_r, (size_t)n_max * 2 * sizeof(double)));
+        PIC_HIP(hipMalloc(&p->stage_v, (size_t)n_max * 2 * sizeof(double)));
+        PIC_HIP(hipMalloc(&p->stage_type, (size_t)n_max * sizeof(int32_t)));
+        // species 0 until a caller writes the types
+        PIC_HIP(hipMemset(p->stage_type, 0, (size_t)n_max * sizeof(int32_t)));
+        p->stage_cap = n_max;
+    }
+    if (r_old && !p->stage_r_old && p->stage_cap > 0)
+        PIC_HIP(hipMalloc(&p->stage_r_old, (size_t)p->stage_cap * 2 * sizeof(double)));
+    if (r) *r = p->stage_r;
+    if (r_old) *r_old = p->stage_r_old;
+    if (v) *v = p->stage_v;
+    if (type) *type = p->stage_type;
+    return 0;
+}
+
+template <typename T>
+static int commit_staging_t(pic_particles* p, int64_t n, double dx, double dy) {
+    const int ns = (int)p->sp.size(), par = p->h_cur;
+    constexpr int NC = PIC_MAX_SPECIES + 1;
+    if (int rc = ensure_scratch(p, 2 * NC)) return rc;
+    hipStream_t sm = lib_stream();
+    int32_t* counts = p->d_scratch;
+    int32_t* fill = p->d_scratch + NC;
+    PIC_HIP(hipMemsetAsync(p->d_scratch, 0, 2 * NC * sizeof(int32_t), sm));
+    if (int rc = launch_staged_count(p->stage_type, n, ns, counts, sm)) return rc;
+    int32_t hc[NC] = {};
+    PIC_HIP(hipMemcpyAsync(hc, counts, sizeof(hc), hipMemcpyDeviceToHost, sm));
+    PIC_HIP(hipStreamSynchronize(sm));
+    if (hc[PIC_MAX_SPECIES]) {
+        set_error("staged particles: " + std::to_string(hc[PIC_MAX_SPECIES]) + " entries of type[] are not a species");
+        return PIC_ERR_ARG;
+    }
+    StagedDst<T> d{};
+    int32_t nc[PIC_MAX_SPECIES] = {};
+    for (int s = 0; s < ns; ++s) {
+        SpeciesStore& st = p->sp[s];
+        const int b = st.nb;
+        int32_t cnt = 0;
+        if (int rc = read_count(p, s, par, b, &cnt)) return rc;
+        if (cnt < 0 || (int64_t)cnt + hc[s] > st.h_cap[b]) {
+            set_error("staged particles: not enough free slots for species " + std::to_string(s) +
+                      " (capacity per species exceeded)");
+            return PIC_ERR_CAPACITY;
+        }
+        const size_t off = (size_t)st.h_start[b] + cnt;
+        d.x[s] = (T*)st.x[par] + off;
+        d.y[s] = (T*)st.y[par] + off;
+        d.vx[s] = (T*)st.vx[par] + off;
+        d.vy[s] = (T*)st.vy[par] + off;
+        d.id[s] = st.id[par] + off;
+        nc[s] = cnt + hc[s];
+    }
+    d.fill = fill;
+    if (int rc = launch_staged_append<T>(p->stage_r, p->stage_v, p->stage_type, n, dx, dy, ns, d, p->stage_next_id, sm))
+        return rc;
+    for (int s = 0; s < ns; ++s)
+        PIC_HIP(hipMemcpyAsync(p->sp[s].d_count[par] + p->sp[s].nb, &nc[s], sizeof(int32_t), hipMemcpyHostToDevice, sm));
+    // the next fill starts from species 0 again
+    PIC_HIP(hipMemsetAsync(p->stage_type, 0, (size_t)n * sizeof(int32_t), sm));
+    PIC_HIP(hipStreamSynchronize(sm));
+    p->stage_next_id += (uint32_t)n;
+    p->stage_pending = 0;
+    return 0;
+}
+
+static int commit_staging(pic_particles* p, int64_t n, double dx, double dy) {
+    PIC_REQUIRE(n >= 0 && n <= p->stage_cap, "more staged particles than the staging area holds");
+    PIC_REQUIRE(dx > 0 && dy > 0, "cell size must be positive");
+    if (n == 0) {
+        p->stage_pending = 0;
+        return 0;
+    }
+    return p->prec == 4 ? commit_staging_t<float>(p, n, dx, dy) : commit_staging_t<double>(p, n, dx, dy);
+}
+
+extern "C" int pic_particles_commit_staging(pic_particles* p, int64_t n, double dx, double dy) {
+    PIC_REQUIRE(p, "particles is NULL");
+    if (p->attached) {
+        if (int rc = sim_sync_impl(p->attached)) return rc;
+    }
+    if (int rc = commit_staging(p, n, dx, dy)) return rc;
+    if (p->attached) p->attached->need_rebin = true;   // bin them before the next step
+    return 0;
+}
+
+extern "C" int pic_particles_set_staged(pic_particles* p, int64_t n) {
+    PIC_REQUIRE(p, "particles is NULL");
+    PIC_REQUIRE(n >= 0 && n <= p->stage_cap, "more staged particles than the staging area holds");
+    if (!p->attached) {
+        p->stage_pending = n;   // handed over when a simulation takes the particles
+        return 0;
+    }
+    return pic_particles_commit_staging(p, n, p->attached->g->dx, p->attached->g->dy);
 }
 
 template <typename T>
@@ -1478,6 +1600,8 @@ extern "C" int pic_sim_create(pic_particles* p, pic_grid* g, pic_walls* w, pic_x
     std::memset(S->h_status, 0, PIC_MAX_SPECIES * sizeof(int));
     for (int k = 0; k < 4; ++k)
         if (hipEventCreate(&S->ev[k]) != hipSuccess) return fail(PIC_ERR_HIP);
+    // particles staged in the reference's layout before the simulation existed
+    if (p->stage_pending > 0 && (rc = commit_staging(p, p->stage_pending, g->dx, g->dy))) return fail(rc);
     p->attached = S;
     if ((rc = sim_rebin(S, S->bin_w, S->bin_h))) {
         p->attached = nullptr;

@@ -58,6 +58,16 @@ struct pic_particles {
     int scratch_alloc = 0;
     long long* d_ll = nullptr;
     pic_sim* attached = nullptr;
+    // the reference's public device arrays (Particles2D::r, r_old, v, type):
+    // a staging area of stage_cap entries; stage_pending entries are handed to
+    // their species when a simulation takes the particles
+    double* stage_r = nullptr;
+    double* stage_r_old = nullptr;
+    double* stage_v = nullptr;
+    int32_t* stage_type = nullptr;
+    int64_t stage_cap = 0;
+    int64_t stage_pending = 0;
+    uint32_t stage_next_id = 0;    // ids of staged particles continue from here
 };
 
 struct pic_grid {

@@ -27,6 +27,7 @@
 // region (index nb).
 #include "pic_device.hpp"
 #include "field.hpp"
+#include "push.hpp"
 #include <type_traits>
 
 namespace pic {
@@ -873,6 +874,55 @@ __global__ void add_count_kernel(int32_t* c, int n) {
 }
 
 // ---------------------------------------------------------------------------
+// particles staged in the reference's layout (Particles2D::r, v, type of
+// particles.cuh:7-20: (x, y) double pairs in domain units, an int species
+// index per particle) handed to their species' spill regions
+// ---------------------------------------------------------------------------
+// counts[s]: staged entries of species s among the first n; counts[PIC_MAX_SPECIES]:
+// entries whose type is not a species
+__global__ void staged_count_kernel(const int32_t* __restrict__ type, long long n, int n_species,
+                                    int32_t* counts) {
+    __shared__ int c[PIC_MAX_SPECIES + 1];
+    if (threadIdx.x <= PIC_MAX_SPECIES) c[threadIdx.x] = 0;
+    __syncthreads();
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int t = type[i];
+        atomicAdd(&c[(t >= 0 && t < n_species) ? t : PIC_MAX_SPECIES], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x <= PIC_MAX_SPECIES && c[threadIdx.x]) atomicAdd(&counts[threadIdx.x], c[threadIdx.x]);
+}
+
+// entry i goes to the next free spill slot of its species (one atomic per
+// species present in the wavefront), with id id0 + i; positions divided by the
+// cell size into cell units
+template <typename T>
+__global__ void staged_append_kernel(const double* __restrict__ r, const double* __restrict__ v,
+                                     const int32_t* __restrict__ type, long long n, double dx, double dy,
+                                     int n_species, StagedDst<T> d, uint32_t id0) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < n;
+    const int s = live ? type[i] : -1;   // every type was checked by the count pass
+    const int lane = __lane_id();
+    int slot = -1;
+    for (int q = 0; q < n_species; ++q) {
+        const uint64_t m = __ballot(s == q);
+        if (!m) continue;
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(d.fill + q, __popcll(m));
+        base = __shfl(base, leader);
+        if (s == q) slot = base + __popcll(m & lanemask_lt());
+    }
+    if (!live) return;
+    d.x[s][slot] = (T)(r[2 * i] / dx);
+    d.y[s][slot] = (T)(r[2 * i + 1] / dy);
+    d.vx[s][slot] = (T)v[2 * i];
+    d.vy[s][slot] = (T)v[2 * i + 1];
+    d.id[s][slot] = id0 + (uint32_t)i;
+}
+
+// ---------------------------------------------------------------------------
 // re-binning: histogram of new bin keys, then scatter into the new layout
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -1226,6 +1276,24 @@ int launch_init(T* x, T* y, T* vx, T* vy, uint32_t* id, int32_t* count_ptr, int 
     return 0;
 }
 
+int launch_staged_count(const int32_t* type, long long n, int n_species, int32_t* counts, hipStream_t s) {
+    if (n <= 0) return 0;
+    const long long blocks = std::min<long long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(staged_count_kernel, dim3((unsigned)blocks), dim3(256), 0, s, type, n, n_species, counts);
+    PIC_HIP(hipGetLastError());
+    return 0;
+}
+
+template <typename T>
+int launch_staged_append(const double* r, const double* v, const int32_t* type, long long n, double dx, double dy,
+                         int n_species, const StagedDst<T>& d, uint32_t id0, hipStream_t s) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(staged_append_kernel<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, r, v, type, n,
+                       dx, dy, n_species, d, id0);
+    PIC_HIP(hipGetLastError());
+    return 0;
+}
+
 template <typename T>
 int launch_init_uniform(T* x, T* y, T* vx, T* vy, uint32_t* id, int32_t* count_ptr, int n, double x0, double x1,
                         double y0, double y1, double v0, double v1, uint32_t seed, uint32_t case_id,
@@ -1426,6 +1494,8 @@ int launch_counts(const CaseDev<T>* cases, const CaseState* states, int c, int s
     template int launch_push<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int, int, int, long long,  \
                                 size_t, hipStream_t, bool);                                                      \
     template int launch_counts<T>(const CaseDev<T>*, const CaseState*, int, int, long long*, hipStream_t);       \
+    template int launch_staged_append<T>(const double*, const double*, const int32_t*, long long, double, double, \
+                                         int, const StagedDst<T>&, uint32_t, hipStream_t);                         \
     template int launch_init<T>(T*, T*, T*, T*, uint32_t*, int32_t*, int, int, int, double, uint32_t, uint32_t,  \
                                 uint32_t, uint32_t, hipStream_t);                                                \
     template int launch_init_uniform<T>(T*, T*, T*, T*, uint32_t*, int32_t*, int, double, double, double, double, \

@@ -19,6 +19,24 @@ template <typename T>
 int launch_init_uniform(T* x, T* y, T* vx, T* vy, uint32_t* id, int32_t* count_ptr, int n, double x0, double x1,
                         double y0, double y1, double v0, double v1, uint32_t seed, uint32_t case_id,
                         uint32_t species, uint32_t id0, hipStream_t s);
+// destinations of staged particles: per species, the five arrays at its first
+// free spill slot, and the slots taken so far (zeroed before the launch)
+template <typename T>
+struct StagedDst {
+    T* x[PIC_MAX_SPECIES];
+    T* y[PIC_MAX_SPECIES];
+    T* vx[PIC_MAX_SPECIES];
+    T* vy[PIC_MAX_SPECIES];
+    uint32_t* id[PIC_MAX_SPECIES];
+    int32_t* fill;
+};
+// per-species counts of type[0..n) into counts[0..PIC_MAX_SPECIES) and, at
+// counts[PIC_MAX_SPECIES], the entries that are not a species (counts zeroed before)
+int launch_staged_count(const int32_t* type, long long n, int n_species, int32_t* counts, hipStream_t s);
+// entry i (r, v: (x, y) double pairs in domain units) to species type[i], id id0 + i
+template <typename T>
+int launch_staged_append(const double* r, const double* v, const int32_t* type, long long n, double dx, double dy,
+                         int n_species, const StagedDst<T>& d, uint32_t id0, hipStream_t s);
 template <typename T>
 int launch_hist(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items, int chunk,
                 int bw, int bh, int nbx, int nb_new, int32_t* hist, hipStream_t s);

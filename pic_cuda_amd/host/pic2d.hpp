@@ -69,10 +69,38 @@ private:
 
 class Particles2D {
 public:
-    explicit Particles2D(int64_t num_particles, int precision = PIC_PREC_FP32) {
+    // staging: allocate the reference's public device arrays r, r_old, v and
+    // type for num_particles entries (needed by callers that cudaMemcpy into
+    // them, main.cu:232-238; 52 B per entry)
+    explicit Particles2D(int64_t num_particles, int precision = PIC_PREC_FP32, bool staging = true) {
         check(pic_particles_create(num_particles, precision, &h_), "Particles2D");
         max_particles = num_particles;
+        if (staging)
+            check(pic_particles_staging(h_, num_particles, &r, &r_old, &v, &type), "Particles2D staging");
     }
+    // particles.cuh:7-20: (x, y) double pairs in domain units and the species
+    // index of each entry, filled by the caller; r_old is accepted and unused
+    double* r = nullptr;
+    double* r_old = nullptr;
+    double* v = nullptr;
+    int32_t* type = nullptr;
+    // particles.cuh:20: `particles.last_alive = n` hands the first n entries
+    // of r, v and type to their species (when a Simulation takes the
+    // particles, or at once in a running one); reading it counts the live
+    // particles of all species
+    class LastAlive {
+    public:
+        explicit LastAlive(Particles2D* o) : o_(o) {}
+        LastAlive& operator=(int64_t n) {
+            check(pic_particles_set_staged(o_->h_, n), "last_alive");
+            return *this;
+        }
+        operator int64_t() const { return o_->count(); }
+
+    private:
+        Particles2D* o_;
+    };
+    LastAlive last_alive{this};
     ~Particles2D() { pic_particles_destroy(h_); }
     Particles2D(const Particles2D&) = delete;
     Particles2D& operator=(const Particles2D&) = delete;

@@ -49,7 +49,7 @@ int main(int argc, char** argv) {
     const double dt = 0.2 / std::sqrt(dens * qe * qe / (eps0 * me));
     try {
         pic2d::Grid2D grid(cells + 1, cells + 1, L, L);
-        pic2d::Particles2D particles((int64_t)(n * 1.02) + 4096, prec);
+        pic2d::Particles2D particles((int64_t)(n * 1.02) + 4096, prec, /*staging=*/false);
         particles.change_species(0, me, -qe);
         particles.add_species(mar, qe);
         particles.set_weight(0, dens * L * L / n);
