@@ -189,13 +189,14 @@ __device__ inline void tile_colour(T* e, const T* r, const uint8_t* mk, int gi0,
     }
 }
 
-// Residual on the tile + 1 ring (into the (MG_TILE+2)^2 patch res) and its
-// full-weighting restriction to the coarse nodes (2ic, 2jc) inside the tile;
-// e and r are the W-wide LDS tile whose element (0,0) is node (gi0, gj0).
-// Fixed (electrode) nodes have zero residual.
+// Residual on the tile + 1 ring and its full-weighting restriction to the
+// coarse nodes (2ic, 2jc) inside the tile; e and r are the W-wide LDS tile
+// whose element (0,0) is node (gi0, gj0).  The residual replaces r in place
+// (each node's update reads only its own r and the unchanged e), so the stage
+// needs no LDS of its own for it.  Fixed (electrode) nodes have zero residual.
 template <typename T, int W, int BS>
-__device__ inline void restrict_tile(const T* e, const T* r, T* res, const uint8_t* mk, int i0, int j0, int gi0,
-                                     int gj0, int n, T inv_h2, const gptr<T> rhs_c) {
+__device__ inline void restrict_tile(const T* e, T* r, const uint8_t* mk, int i0, int j0, int gi0, int gj0, int n,
+                                     T inv_h2, const gptr<T> rhs_c) {
     const int tid = threadIdx.x;
     constexpr int RW = MG_TILE + 2;
     for (int k = tid; k < RW * RW; k += BS) {
@@ -207,7 +208,7 @@ __device__ inline void restrict_tile(const T* e, const T* r, T* res, const uint8
             const T lap = lap_h2(e, idx, W) * inv_h2;
             v = r[idx] - lap;
         }
-        res[k] = v;
+        r[idx] = v;
     }
     __syncthreads();
     const int nc = (n + 1) / 2;
@@ -220,16 +221,16 @@ __device__ inline void restrict_tile(const T* e, const T* r, T* res, const uint8
         rhs_c[(size_t)jc * nc + ic] = (T)0;
         return;
     }
-    const int f = (j - j0 + 1) * RW + (i - i0 + 1);
-    T s = (T)4 * res[f];
-    if (i > 0) s += (T)2 * res[f - 1];
-    if (i < n - 1) s += (T)2 * res[f + 1];
-    if (j > 0) s += (T)2 * res[f - RW];
-    if (j < n - 1) s += (T)2 * res[f + RW];
-    if (i > 0 && j > 0) s += res[f - RW - 1];
-    if (i < n - 1 && j > 0) s += res[f - RW + 1];
-    if (i > 0 && j < n - 1) s += res[f + RW - 1];
-    if (i < n - 1 && j < n - 1) s += res[f + RW + 1];
+    const int f = (j - gj0) * W + (i - gi0);
+    T s = (T)4 * r[f];
+    if (i > 0) s += (T)2 * r[f - 1];
+    if (i < n - 1) s += (T)2 * r[f + 1];
+    if (j > 0) s += (T)2 * r[f - W];
+    if (j < n - 1) s += (T)2 * r[f + W];
+    if (i > 0 && j > 0) s += r[f - W - 1];
+    if (i < n - 1 && j > 0) s += r[f - W + 1];
+    if (i > 0 && j < n - 1) s += r[f + W - 1];
+    if (i < n - 1 && j < n - 1) s += r[f + W + 1];
     rhs_c[(size_t)jc * nc + ic] = s * (T)0.0625;
 }
 
@@ -307,8 +308,7 @@ __global__ __launch_bounds__(BS) void mg_smooth_stage(const StageRef<MGLevelTask
     T* r = e + WW;
     // electrodes: only the MASKED instantiation carries the mask code
     const gptr<const uint8_t> gmask = MASKED ? G(tk.mask) : nullptr;
-    uint8_t* mk = (MASKED && gmask) ? reinterpret_cast<uint8_t*>(r + WW + (RESTRICT ? (MG_TILE + 2) * (MG_TILE + 2) : 0))
-                                    : nullptr;
+    uint8_t* mk = (MASKED && gmask) ? reinterpret_cast<uint8_t*>(r + WW) : nullptr;
     const int tid = threadIdx.x;
     const int i0 = tile.tx * MG_TILE, j0 = tile.ty * MG_TILE;
     const int gi0 = i0 - H, gj0 = j0 - H;
@@ -353,7 +353,7 @@ __global__ __launch_bounds__(BS) void mg_smooth_stage(const StageRef<MGLevelTask
     }
     edge_nodes<T>(i0, j0, n, e_in, nullptr, 0, 0, 0, gmask, e_out, RESTRICT ? rhs_c : G((T*)nullptr));
     if constexpr (RESTRICT) {
-        if (rhs_c) restrict_tile<T, W, BS>(e, r, r + WW, mk, i0, j0, gi0, gj0, n, inv_h2, rhs_c);
+        if (rhs_c) restrict_tile<T, W, BS>(e, r, mk, i0, j0, gi0, gj0, n, inv_h2, rhs_c);
     }
 }
 
@@ -386,10 +386,9 @@ __global__ __launch_bounds__(BS) void mg_prolong_stage(const StageRef<MGLevelTas
     T* e = reinterpret_cast<T*>(smem_mg);
     T* r = e + WW;
     T* C = r + WW;
-    T* res = C + CC;   // RESTRICT: the residual patch
+
     const gptr<const uint8_t> gmask = MASKED ? G(tk.mask) : nullptr;
-    uint8_t* mk = (MASKED && gmask) ? reinterpret_cast<uint8_t*>(res + (RESTRICT ? (MG_TILE + 2) * (MG_TILE + 2) : 0))
-                                    : nullptr;
+    uint8_t* mk = (MASKED && gmask) ? reinterpret_cast<uint8_t*>(C + CC) : nullptr;
     const int tid = threadIdx.x;
     const int i0 = tile.tx * MG_TILE, j0 = tile.ty * MG_TILE;
     const int gi0 = i0 - H, gj0 = j0 - H;   // even: H and i0 are
@@ -456,7 +455,7 @@ __global__ __launch_bounds__(BS) void mg_prolong_stage(const StageRef<MGLevelTas
     }
     edge_nodes<T>(i0, j0, n, e_in, C, CW, ic0, jc0, gmask, e_out, RESTRICT ? rhs_c : G((T*)nullptr));
     if constexpr (RESTRICT) {
-        if (rhs_c) restrict_tile<T, W, BS>(e, r, res, mk, i0, j0, gi0, gj0, n, inv_h2, rhs_c);
+        if (rhs_c) restrict_tile<T, W, BS>(e, r, mk, i0, j0, gi0, gj0, n, inv_h2, rhs_c);
     }
 }
 
@@ -1303,15 +1302,14 @@ static size_t coarse_lds_bytes(int n_top, int Lc, bool direct) {
     return tot * sizeof(T);
 }
 
-// LDS of a tiled stage: e and rhs over tile + halo, plus the residual patch
-// (smooth with restriction) or the coarse patch (prolongation), or both (the
-// fused up/down stage, kind 4)
+// LDS of a tiled stage: e and rhs over tile + halo (the residual before a
+// restriction replaces rhs in place), plus the coarse patch of a prolongation
+// (kinds 1 and 4)
 static size_t tiled_lds(int kind, int nu, bool restr, size_t es, bool masked) {
     const bool res = (kind == 0 && restr) || kind == 4;
     const size_t H = res ? 2 * nu + 2 : 2 * nu;
     const size_t W = MG_TILE + 2 * H;
     size_t tot = 2 * W * W;
-    if (res) tot += (MG_TILE + 2) * (MG_TILE + 2);
     if (kind == 1 || kind == 4) tot += (W / 2 + 2) * (W / 2 + 2);
     return tot * es + (masked ? W * W : 0);   // + the electrode mask tile (bytes)
 }
