@@ -169,23 +169,45 @@ __device__ inline T sor_node(const T* e, const T* r, int idx, int stride, T h2, 
 // global node (gi0, gj0).  Only rows and columns [lo, W-1-lo] are updated: after
 // s half-sweeps the outer s rings are stale, so half-sweep s uses lo = s + 1.
 // Colour c updates the nodes with (gi + gj) odd for c = 1, even for c = 0.
-template <typename T, int W, int BS>
+// W is even, so every row of the span holds span/2 nodes of each colour:
+// thread t takes colour slot t % (span/2) of rows t / (span/2), then every
+// rows-per-pass row after it (index arithmetic once per half-sweep, not per
+// node).  EDGE: the tile reaches the domain boundary, whose nodes are skipped;
+// an interior tile tests no bounds.
+template <typename T, int W, int BS, bool EDGE>
 __device__ inline void tile_colour(T* e, const T* r, const uint8_t* mk, int gi0, int gj0, int n, T h2, T omega,
                                    int colour, int lo) {
+    static_assert(W % 2 == 0, "tile width must be even");
     const int span = W - 2 * lo;
-    const int halfc = (span + 1) / 2;
-    const int items = span * halfc;
-    for (int k = threadIdx.x; k < items; k += BS) {
-        const int q = k / halfc;
-        const int lj = lo + q, m = k - q * halfc;
-        const int gj = gj0 + lj;
-        const int li = lo + ((colour + gi0 + gj + lo) & 1) + 2 * m;
-        if (li > W - 1 - lo) continue;
-        const int gi = gi0 + li;
-        if (gi <= 0 || gi >= n - 1 || gj <= 0 || gj >= n - 1) continue;
+    const int halfc = span / 2;
+    const int rp = BS / halfc;             // rows per pass
+    const int t = threadIdx.x;
+    if (t >= rp * halfc) return;
+    const int m = t % halfc, q0 = t / halfc;
+    const int c0 = (colour + gi0 + gj0) & 1;
+    for (int q = q0; q < span; q += rp) {
+        const int lj = lo + q;
+        const int li = lo + ((c0 + q) & 1) + 2 * m;
+        if (EDGE) {
+            const int gi = gi0 + li, gj = gj0 + lj;
+            if (gi <= 0 || gi >= n - 1 || gj <= 0 || gj >= n - 1) continue;
+        }
         const int idx = lj * W + li;
         if (mk && mk[idx]) continue;   // electrode node: held fixed
         e[idx] = sor_node(e, r, idx, W, h2, omega);
+    }
+}
+
+// the tile's half-sweeps, with the bound tests only where the tile's LDS
+// window reaches the domain boundary
+template <typename T, int W, int BS, int NSWEEP>
+__device__ inline void tile_sweeps(T* e, const T* r, const uint8_t* mk, int gi0, int gj0, int n, T h2, T omega) {
+    const bool edge = gi0 <= 0 || gj0 <= 0 || gi0 + W - 1 >= n - 1 || gj0 + W - 1 >= n - 1;
+#pragma unroll
+    for (int s = 0; s < 2 * NSWEEP; ++s) {
+        if (edge) tile_colour<T, W, BS, true>(e, r, mk, gi0, gj0, n, h2, omega, 1 - (s & 1), s + 1);
+        else tile_colour<T, W, BS, false>(e, r, mk, gi0, gj0, n, h2, omega, 1 - (s & 1), s + 1);
+        __syncthreads();
     }
 }
 
@@ -199,16 +221,21 @@ __device__ inline void restrict_tile(const T* e, T* r, const uint8_t* mk, int i0
                                      T inv_h2, const gptr<T> rhs_c) {
     const int tid = threadIdx.x;
     constexpr int RW = MG_TILE + 2;
-    for (int k = tid; k < RW * RW; k += BS) {
-        const int q = k / RW;
-        const int gi = i0 - 1 + (k - q * RW), gj = j0 - 1 + q;
-        T v = (T)0;
-        const int idx = (gj - gj0) * W + (gi - gi0);
-        if (gi > 0 && gi < n - 1 && gj > 0 && gj < n - 1 && !(mk && mk[idx])) {
-            const T lap = lap_h2(e, idx, W) * inv_h2;
-            v = r[idx] - lap;
+    constexpr int RP = BS / RW;            // rows per pass; thread t takes column t % RW
+    if (tid < RP * RW) {
+        const int c = tid % RW;
+        const int gi = i0 - 1 + c;
+        const bool col_in = gi > 0 && gi < n - 1;
+        for (int q = tid / RW; q < RW; q += RP) {
+            const int gj = j0 - 1 + q;
+            T v = (T)0;
+            const int idx = (gj - gj0) * W + (gi - gi0);
+            if (col_in && gj > 0 && gj < n - 1 && !(mk && mk[idx])) {
+                const T lap = lap_h2(e, idx, W) * inv_h2;
+                v = r[idx] - lap;
+            }
+            r[idx] = v;
         }
-        r[idx] = v;
     }
     __syncthreads();
     const int nc = (n + 1) / 2;
@@ -340,11 +367,7 @@ __global__ __launch_bounds__(BS) void mg_smooth_stage(const StageRef<MGLevelTask
         }
     }
     __syncthreads();
-#pragma unroll
-    for (int s = 0; s < 2 * NU; ++s) {
-        tile_colour<T, W, BS>(e, r, mk, gi0, gj0, n, h2, omega, 1 - (s & 1), s + 1);
-        __syncthreads();
-    }
+    tile_sweeps<T, W, BS, NU>(e, r, mk, gi0, gj0, n, h2, omega);
 #pragma unroll
     for (int q = 0; q < (MG_TILE * MG_TILE) / BS; ++q) {
         const int k = tid + BS * q;
@@ -442,11 +465,7 @@ __global__ __launch_bounds__(BS) void mg_prolong_stage(const StageRef<MGLevelTas
         }
     }
     __syncthreads();
-#pragma unroll
-    for (int s = 0; s < 2 * NU; ++s) {
-        tile_colour<T, W, BS>(e, r, mk, gi0, gj0, n, h2, omega, 1 - (s & 1), s + 1);
-        __syncthreads();
-    }
+    tile_sweeps<T, W, BS, NU>(e, r, mk, gi0, gj0, n, h2, omega);
 #pragma unroll
     for (int q = 0; q < (MG_TILE * MG_TILE) / BS; ++q) {
         const int k = tid + BS * q;
