@@ -298,6 +298,9 @@ typedef struct {
     int64_t graph_steps;     /* steps run as a graph launch */
     int64_t graph_captures;  /* graph (re)captures */
     double ms_graph;         /* device time of the graph steps */
+    int64_t ahead_steps;     /* eager steps whose push ran ahead, behind the previous step's solve */
+    int64_t ahead_held;      /* pushes enqueued ahead and held back by their gate (solve not stopped
+                                after the predicted cycles, or re-binning due) and enqueued again */
 } pic_sim_report;
 
 /* null-collision constants of a species (inputs of the oracle's MCC).  The

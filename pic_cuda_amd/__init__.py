@@ -72,7 +72,8 @@ class SimReport(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int64), ("alive", ctypes.c_int64), ("last_mg_cycles", ctypes.c_int),
                 ("last_mg_rel", ctypes.c_double), ("total_mg_cycles", ctypes.c_int64), ("rebins", ctypes.c_int),
                 ("ms_push", ctypes.c_double), ("ms_field", ctypes.c_double), ("ms_total", ctypes.c_double),
-                ("graph_steps", ctypes.c_int64), ("graph_captures", ctypes.c_int64), ("ms_graph", ctypes.c_double)]
+                ("graph_steps", ctypes.c_int64), ("graph_captures", ctypes.c_int64), ("ms_graph", ctypes.c_double),
+                ("ahead_steps", ctypes.c_int64), ("ahead_held", ctypes.c_int64)]
 
 
 class Diagnostics(ctypes.Structure):

@@ -51,7 +51,8 @@ def add_argon_processes(xs, electron=0, ion=1):
 
 
 def build(cells, n_per_species, precision, mcc, seed=1, case_id=0, mg=None, bin_w=16, bin_h=16, halo=2,
-          gas_density=1e21, gas_t=0.03, te=2.0, capacity=None, walls=None, processes=None, state=None):
+          gas_density=1e21, gas_t=0.03, te=2.0, capacity=None, walls=None, processes=None, state=None,
+          rebin_every=0):
     """Create particles, grid, walls, cross sections and the step driver.
     `walls`: internal wall segments (x1, y1, x2, y2) in cell units, added to
     the absorbing box. `processes(xs, electron, ion)` fills the cross-section
@@ -86,7 +87,8 @@ def build(cells, n_per_species, precision, mcc, seed=1, case_id=0, mg=None, bin_
     if mg is None:
         mg = mg_params(PRESET_STEP_SOR, mode=MG_MIXED if precision == PREC_FP32 else MG_FP64)
     sim = Simulation(parts, grid, bnd, xs, sim_params(dt=pl.dt, seed=seed, case_id=case_id, bin_w=bin_w,
-                                                        bin_h=bin_h, halo=halo, mcc=1 if mcc else 0, mg=mg))
+                                                        bin_h=bin_h, halo=halo, mcc=1 if mcc else 0, mg=mg,
+                                                        rebin_every=rebin_every))
     return dict(plasma=pl, grid=grid, particles=parts, walls=bnd, xs=xs, sim=sim, mg=mg, n_per_species=n_per_species)
 
 

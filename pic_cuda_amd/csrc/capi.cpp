@@ -1590,6 +1590,7 @@ extern "C" int pic_sim_create(pic_particles* p, pic_grid* g, pic_walls* w, pic_x
     // the spill status is written by end_step_kernel straight into pinned,
     // coherent host memory (read after the step's synchronization)
     if (hipMalloc(&S->phi_prev, (size_t)g->m * g->n * sizeof(double)) != hipSuccess ||
+        hipMalloc(&S->d_rebin, sizeof(int)) != hipSuccess || hipMemset(S->d_rebin, 0, sizeof(int)) != hipSuccess ||
         hipHostMalloc(&S->h_status, PIC_MAX_SPECIES * sizeof(int), hipHostMallocMapped | hipHostMallocCoherent) !=
             hipSuccess ||
         hipHostGetDevicePointer((void**)&S->d_status, S->h_status, 0) != hipSuccess) {
@@ -1598,7 +1599,7 @@ extern "C" int pic_sim_create(pic_particles* p, pic_grid* g, pic_walls* w, pic_x
     }
     S->mg->set_phi_save(S->phi_prev);
     std::memset(S->h_status, 0, PIC_MAX_SPECIES * sizeof(int));
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < 5; ++k)
         if (hipEventCreate(&S->ev[k]) != hipSuccess) return fail(PIC_ERR_HIP);
     // particles staged in the reference's layout before the simulation existed
     if (p->stage_pending > 0 && (rc = commit_staging(p, p->stage_pending, g->dx, g->dy))) return fail(rc);
@@ -1761,21 +1762,32 @@ static int graph_step(pic_sim* S, int flags, size_t lds, MGStats* st) {
     return S->mg->complete_fixed(st);
 }
 
+// Eager steps.  The push of step i + 1 is enqueued before the host has read
+// step i's solve and spill status, gated on the device (the solve's stop flag
+// set after the cycles enqueued, no species in need of re-binning), so the GPU
+// runs it while the host reads the status; the host tests the same two facts
+// and enqueues the push again only if the gate held it back.  A step that
+// re-bins first (periodic re-binning) or ends the call is not run ahead.
 template <typename T>
 static int step_t(pic_sim* S, int n) {
     pic_particles* p = S->p;
     hipStream_t sm = lib_stream();
     const size_t nn = (size_t)S->g->m * S->g->n;
-    const size_t lds = cur_lds(S);
     const int mcc = mcc_active(S);
+    const bool walls = S->d_walls_cell != nullptr;
+    const int full = PIC_F_DRIFT | PIC_F_DEPOSIT | PIC_F_KICK | (mcc ? PIC_F_MCC : 0);
+    bool ahead = false;   // this step's push ran behind the previous step's solve
+    int pe = 0;           // this step's push events: ev[pe], ev[pe + 1]
     for (int i = 0; i < n; ++i) {
-        if (S->need_rebin || (S->rebin_every > 0 && S->steps > 0 && S->steps % S->rebin_every == 0)) {
+        if (!ahead &&
+            (S->need_rebin || (S->rebin_every > 0 && S->steps > 0 && S->steps % S->rebin_every == 0))) {
             if (int rc = sim_rebin(S, S->bin_w, S->bin_h)) return rc;
         }
-        const int flags = PIC_F_DRIFT | PIC_F_DEPOSIT | (S->pending ? (PIC_F_KICK | (mcc ? PIC_F_MCC : 0)) : 0);
+        const size_t lds = cur_lds(S);
+        const int flags = S->pending ? full : (PIC_F_DRIFT | PIC_F_DEPOSIT);
         if (S->use_graph && S->pending && S->last_cycles > 0) {
             MGStats st{};
-            if (int rc = graph_step<T>(S, flags, cur_lds(S), &st)) return rc;
+            if (int rc = graph_step<T>(S, flags, lds, &st)) return rc;
             S->have_prev = true;
             p->h_cur ^= 1;
             float a = 0;
@@ -1792,23 +1804,44 @@ static int step_t(pic_sim* S, int n) {
             }
             continue;
         }
-        PIC_HIP(hipEventRecord(S->ev[0], sm));
-        if (int rc = launch_push<T>((CaseDev<T>*)p->d_case, p->d_state, S->d_items, S->n_items, S->spill_chunk, flags,
-                                    -1, lds, sm, S->d_walls_cell != nullptr))
-            return rc;
-        PIC_HIP(hipEventRecord(S->ev[1], sm));
+        if (!ahead) {
+            PIC_HIP(hipEventRecord(S->ev[pe], sm));
+            if (int rc = launch_push<T>((CaseDev<T>*)p->d_case, p->d_state, S->d_items, S->n_items, S->spill_chunk,
+                                        flags, -1, lds, sm, walls))
+                return rc;
+            PIC_HIP(hipEventRecord(S->ev[pe + 1], sm));
+        }
         if (int rc = launch_charge_end_step<T>(S->charge, nn, (CaseDev<T>*)p->d_case, p->d_state, 1, sm,
-                                               S->d_status))
+                                               S->d_status, S->d_rebin))
             return rc;
         p->h_cur ^= 1;
         S->have_prev = true;   // the solve's first residual pass copies phi to phi_prev
-        MGStats st{};
-        if (int rc = S->mg->solve(S->g->phi, S->g->rhs, S->g->dx, &st)) return rc;
+        if (int rc = S->mg->prepare(S->g->phi, S->g->rhs, S->g->dx)) return rc;
+        if (int rc = S->mg->enqueue_fixed(S->mg->last_cycles())) return rc;
         PIC_HIP(hipEventRecord(S->ev[2], sm));
+        // the next step runs ahead unless it re-bins by the period, ends the
+        // call or is a graph step
+        const int pn = 3 - pe;
+        const bool next = i + 1 < n && !S->use_graph &&
+                          !(S->rebin_every > 0 && (S->steps + 1) % S->rebin_every == 0);
+        if (next) {
+            PIC_HIP(hipEventRecord(S->ev[pn], sm));
+            if (int rc = launch_push<T>((CaseDev<T>*)p->d_case, p->d_state, S->d_items, S->n_items, S->spill_chunk,
+                                        full, -1, lds, sm, walls, S->mg->stop_flag(), S->d_rebin))
+                return rc;
+            PIC_HIP(hipEventRecord(S->ev[pn + 1], sm));
+        }
         PIC_HIP(hipEventSynchronize(S->ev[2]));
+        const bool stopped = S->mg->fixed_stopped();
+        MGStats st{};
+        if (int rc = S->mg->complete_fixed(&st)) return rc;
+        if (!stopped) {   // cycles beyond the enqueued ones: time them too
+            PIC_HIP(hipEventRecord(S->ev[2], sm));
+            PIC_HIP(hipEventSynchronize(S->ev[2]));
+        }
         float a = 0, b = 0;
-        (void)hipEventElapsedTime(&a, S->ev[0], S->ev[1]);
-        (void)hipEventElapsedTime(&b, S->ev[1], S->ev[2]);
+        (void)hipEventElapsedTime(&a, S->ev[pe], S->ev[pe + 1]);
+        (void)hipEventElapsedTime(&b, S->ev[pe + 1], S->ev[2]);
         S->ms_push += a;
         S->ms_field += b;
         S->pending = true;
@@ -1816,10 +1849,20 @@ static int step_t(pic_sim* S, int n) {
         S->last_cycles = st.cycles;
         S->last_rel = st.rel_residual;
         S->total_cycles += st.cycles;
+        bool rebin = false;
         for (int s = 0; s < (int)p->sp.size(); ++s) {
             const SpeciesStore& ss = p->sp[s];
-            if (S->h_status[s] > ss.h_cap[ss.nb] / 2) S->need_rebin = true;
+            if (S->h_status[s] > ss.h_cap[ss.nb] / 2) rebin = true;
             if (int rc = spill_status(ss, S->h_status[s], s)) return rc;
+        }
+        if (rebin) S->need_rebin = true;
+        // the gate let the pushed-ahead step run exactly when both hold
+        ahead = next && stopped && !rebin;
+        if (ahead) {
+            pe = pn;
+            S->ahead_steps++;
+        } else if (next) {
+            S->ahead_held++;
         }
     }
     return 0;
@@ -1945,6 +1988,8 @@ extern "C" int pic_sim_stats(pic_sim* S, pic_sim_report* out) {
     out->graph_steps = S->graph_steps;
     out->graph_captures = S->graph_captures;
     out->ms_graph = S->ms_graph;
+    out->ahead_steps = S->ahead_steps;
+    out->ahead_held = S->ahead_held;
     return 0;
 }
 
@@ -2064,6 +2109,7 @@ extern "C" int pic_sim_destroy(pic_sim* S) {
     (void)hipFree(S->d_frame);
     (void)hipFree(S->d_items);
     (void)hipFree(S->phi_prev);
+    (void)hipFree(S->d_rebin);
     if (S->h_status) (void)hipHostFree(S->h_status);
     for (auto& e : S->ev)
         if (e) (void)hipEventDestroy(e);

@@ -148,6 +148,12 @@ public:
     int enqueue_fixed(int cycles);
     int complete_fixed(MGStats* st);
     int last_cycles() const;
+    // the device stop flag of the solve (set once the tolerance is met or the
+    // cycle limit reached; a kernel enqueued after the solve can gate on it),
+    // and its value after the cycles of enqueue_fixed, read once the stream has
+    // run them and before complete_fixed
+    const int* stop_flag() const;
+    bool fixed_stopped() const;
     // embedded electrodes (host arrays of n*n): mask 0 solved, 1 Dirichlet at
     // phi_bc, 2 solid (held at its value); null mask removes them
     int set_mask(const uint8_t* mask, const double* phi_bc);

@@ -1298,6 +1298,10 @@ int MGSolver::complete_fixed(MGStats* st) {
 
 int MGSolver::last_cycles() const { return solo_ ? solo_->last_cycles() : 1; }
 
+const int* MGSolver::stop_flag() const { return d_norm_ ? &d_norm_->converged : nullptr; }
+
+bool MGSolver::fixed_stopped() const { return solo_ && solo_->norm(0).converged != 0; }
+
 void MGSolver::set_phi_save(double* dst) {
     if (dst != phi_save_) {   // the launch tables of the batch of one hold the pointer
         delete solo_;

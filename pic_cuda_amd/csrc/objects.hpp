@@ -142,8 +142,12 @@ struct pic_sim {
     double ms_push = 0, ms_field = 0, ms_total = 0, ms_graph = 0;
     int* h_status = nullptr;   // pinned, coherent host memory
     int* d_status = nullptr;   // its device address
+    // device flag of the end of a step: some species' spill region is more
+    // than half full (the re-binning rule), read by a push enqueued ahead
+    int* d_rebin = nullptr;
     pic::ChargeArgs charge{};
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // push timing of alternate steps (0-1, 3-4) and the end of a step's solve (2)
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     struct pic_ensemble* ens = nullptr;   // batch this simulation belongs to
     // potential before the last step's solve: with the charge, the last
     // warm-started solve, which the tuner times
@@ -158,6 +162,7 @@ struct pic_sim {
     int g_shrink = 0;   // consecutive graph steps that needed fewer cycles than captured
     size_t g_lds = 0;
     int64_t graph_steps = 0, graph_captures = 0;
+    int64_t ahead_steps = 0, ahead_held = 0;
 };
 
 namespace pic {

@@ -516,14 +516,22 @@ __device__ inline void stage_put(const Stage<T>& sg, int& nm, bool want, T x, T 
 // ---------------------------------------------------------------------------
 // the fused particle kernel
 // ---------------------------------------------------------------------------
+// gate_conv / gate_rebin (null: no gate): a push enqueued ahead of the host's
+// check of the previous step does nothing unless *gate_conv (the solve's
+// convergence flag) is set and *gate_rebin (no species needs re-binning) is
+// clear; both are read with the item's set-up loads and tested before the
+// first global write.
 template <typename T, bool WALLS>
 __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict__ cases,
                                                    const CaseState* __restrict__ states,
                                                    const WorkItem* __restrict__ items, int spill_chunk,
-                                                   int flags_arg, long long mcc_key) {
+                                                   int flags_arg, long long mcc_key,
+                                                   const int* __restrict__ gate_conv,
+                                                   const int* __restrict__ gate_rebin) {
     using T2 = typename Vec2<T>::type;
     extern __shared__ unsigned long long smem_u64[];
     __shared__ int sh_stay;
+    const int go = (gate_conv ? *gate_conv : 1) && !(gate_rebin ? *gate_rebin : 0);
     const WorkItem it = items[blockIdx.x];
     const CaseDev<T>& C = cases[it.case_idx];
     const CaseState st = states[it.case_idx];
@@ -534,6 +542,7 @@ __global__ __launch_bounds__(256) void push_kernel(const CaseDev<T>* __restrict_
     const bool is_bin = it.bin < S.nb[in];
     // an empty bin still publishes its (zero) stay count; nothing else to do
     if (n <= 0) return;
+    if (!go) return;   // uniform over the block: a gated push whose step is not ready writes nothing
 
     const int nx = C.nx, ny = C.ny;
     const int H = C.halo;
@@ -1180,7 +1189,7 @@ __global__ __launch_bounds__(256) void count_kernel(const CaseDev<T>* __restrict
 // after the Poisson solve to decide on re-binning)
 template <typename T>
 __device__ inline void end_step_block(const CaseDev<T>* __restrict__ cases, CaseState* states, int c,
-                                      int advance_step, int* __restrict__ status_out) {
+                                      int advance_step, int* __restrict__ status_out, int* __restrict__ rebin_out) {
     const CaseDev<T>& C = cases[c];
     const int in = states[c].cur;
     __syncthreads();
@@ -1205,13 +1214,24 @@ __device__ inline void end_step_block(const CaseDev<T>* __restrict__ cases, Case
                 }
                 status_out[c * PIC_MAX_SPECIES + s] = v;
             }
+        // the host's re-binning rule (a spill region more than half full), on
+        // the device for a push enqueued before the host has read the status
+        if (rebin_out) {
+            int need = 0;
+            for (int s = 0; s < C.ns; ++s) {
+                const SpeciesDev<T>& S = C.sp[s];
+                const int nb = S.nb[in ^ 1];
+                if (S.count[in ^ 1][nb] > S.cap[in ^ 1][nb] / 2) need = 1;
+            }
+            rebin_out[c] = need;
+        }
     }
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void end_step_kernel(const CaseDev<T>* __restrict__ cases, CaseState* states,
                                                        int advance_step, int* __restrict__ status_out) {
-    end_step_block(cases, states, (int)blockIdx.x, advance_step, status_out);
+    end_step_block(cases, states, (int)blockIdx.x, advance_step, status_out, nullptr);
 }
 
 // The end of a single-case step and the charge conversion in one launch (they
@@ -1221,9 +1241,10 @@ template <typename T>
 __global__ __launch_bounds__(256) void charge_end_step_kernel(ChargeArgs a, size_t nn,
                                                               const CaseDev<T>* __restrict__ cases,
                                                               CaseState* states, int advance_step,
-                                                              int* __restrict__ status_out) {
+                                                              int* __restrict__ status_out,
+                                                              int* __restrict__ rebin_out) {
     if (blockIdx.x == 0) {
-        end_step_block(cases, states, 0, advance_step, status_out);
+        end_step_block(cases, states, 0, advance_step, status_out, rebin_out);
     } else {
         const size_t i = (size_t)(blockIdx.x - 1) * blockDim.x + threadIdx.x;
         if (i < nn) charge_node(a, i);
@@ -1253,14 +1274,15 @@ __global__ __launch_bounds__(256) void clear_out_kernel(const CaseDev<T>* __rest
 // ---------------------------------------------------------------------------
 template <typename T>
 int launch_push(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items,
-                int spill_chunk, int flags, long long mcc_key, size_t lds, hipStream_t s, bool walls) {
+                int spill_chunk, int flags, long long mcc_key, size_t lds, hipStream_t s, bool walls,
+                const int* gate_conv, const int* gate_rebin) {
     if (n_items <= 0) return 0;
     if (walls)
         hipLaunchKernelGGL((push_kernel<T, true>), dim3(n_items), dim3(256), lds, s, cases, states, items,
-                           spill_chunk, flags, mcc_key);
+                           spill_chunk, flags, mcc_key, gate_conv, gate_rebin);
     else
         hipLaunchKernelGGL((push_kernel<T, false>), dim3(n_items), dim3(256), lds, s, cases, states, items,
-                           spill_chunk, flags, mcc_key);
+                           spill_chunk, flags, mcc_key, gate_conv, gate_rebin);
     PIC_HIP(hipGetLastError());
     return 0;
 }
@@ -1459,10 +1481,10 @@ int launch_gather(const CaseDev<T>* cases, const CaseState* states, const WorkIt
 
 template <typename T>
 int launch_charge_end_step(const ChargeArgs& a, size_t nn, const CaseDev<T>* cases, CaseState* states, int advance,
-                           hipStream_t s, int* status_out) {
+                           hipStream_t s, int* status_out, int* rebin_out) {
     const int n_charge = (int)((nn + 255) / 256);
     hipLaunchKernelGGL(charge_end_step_kernel<T>, dim3(n_charge + 1), dim3(256), 0, s, a, nn, cases, states,
-                       advance, status_out);
+                       advance, status_out, rebin_out);
     PIC_HIP(hipGetLastError());
     return 0;
 }
@@ -1492,7 +1514,7 @@ int launch_counts(const CaseDev<T>* cases, const CaseState* states, int c, int s
 
 #define PIC_LAUNCH_INST(T)                                                                                       \
     template int launch_push<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int, int, int, long long,  \
-                                size_t, hipStream_t, bool);                                                      \
+                                size_t, hipStream_t, bool, const int*, const int*);                              \
     template int launch_counts<T>(const CaseDev<T>*, const CaseState*, int, int, long long*, hipStream_t);       \
     template int launch_staged_append<T>(const double*, const double*, const int32_t*, long long, double, double, \
                                          int, const StagedDst<T>&, uint32_t, hipStream_t);                         \
@@ -1512,7 +1534,7 @@ int launch_counts(const CaseDev<T>* cases, const CaseState* states, int c, int s
                                   float2*, int64_t, double, double, hipStream_t);                                \
     template int launch_end_step<T>(const CaseDev<T>*, CaseState*, int, int, hipStream_t, int*);                 \
     template int launch_charge_end_step<T>(const ChargeArgs&, size_t, const CaseDev<T>*, CaseState*, int,         \
-                                           hipStream_t, int*);                                                    \
+                                           hipStream_t, int*, int*);                                              \
     template int launch_clear_out<T>(const CaseDev<T>*, const CaseState*, int, hipStream_t);
 PIC_LAUNCH_INST(float)
 PIC_LAUNCH_INST(double)

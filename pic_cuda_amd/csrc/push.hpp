@@ -6,10 +6,13 @@
 namespace pic {
 
 // walls: some case of the launch has internal wall segments (false launches
-// the instantiation without the segment test)
+// the instantiation without the segment test); gate_conv / gate_rebin (device
+// ints, null: no gate): the push runs only if *gate_conv != 0 and
+// *gate_rebin == 0 when it starts, and otherwise writes nothing
 template <typename T>
 int launch_push(const CaseDev<T>* cases, const CaseState* states, const WorkItem* items, int n_items,
-                int spill_chunk, int flags, long long mcc_key, size_t lds, hipStream_t s, bool walls);
+                int spill_chunk, int flags, long long mcc_key, size_t lds, hipStream_t s, bool walls,
+                const int* gate_conv = nullptr, const int* gate_rebin = nullptr);
 template <typename T>
 int launch_counts(const CaseDev<T>* cases, const CaseState* states, int c, int sp, long long* out, hipStream_t s);
 template <typename T>
@@ -63,10 +66,11 @@ template <typename T>
 int launch_end_step(const CaseDev<T>* cases, CaseState* states, int n_cases, int advance, hipStream_t s,
                     int* status_out = nullptr);
 // the end of step of a single case and the charge conversion of nn nodes in
-// one launch (disjoint data)
+// one launch (disjoint data); rebin_out (may be null) receives 1 if some
+// species' spill region is more than half full, else 0
 template <typename T>
 int launch_charge_end_step(const ChargeArgs& a, size_t nn, const CaseDev<T>* cases, CaseState* states, int advance,
-                           hipStream_t s, int* status_out);
+                           hipStream_t s, int* status_out, int* rebin_out = nullptr);
 // clear the bin counts of the output parity (after a trial push)
 template <typename T>
 int launch_clear_out(const CaseDev<T>* cases, const CaseState* states, int n_cases, hipStream_t s);

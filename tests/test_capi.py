@@ -33,6 +33,37 @@ def test_python_signatures_cover_the_header(pic):
     assert sorted(pic.SIGNATURES) == declared_functions()
 
 
+STRUCTS = [("pic_mg_params", "MGParams"), ("pic_mg_stats", "MGStats"), ("pic_xs_process", "XsProcess"),
+           ("pic_sim_params", "SimParams"), ("pic_sim_report", "SimReport"), ("pic_mcc_info", "MccInfo"),
+           ("pic_diagnostics", "Diagnostics"), ("pic_tune_report", "TuneReport"), ("pic_job_desc", "JobDesc"),
+           ("pic_job_result", "JobResult")]
+
+
+def test_struct_layouts_match_header(pic, tmp_path):
+    """Every structure of the ABI has the same size and field offsets in the
+    header (compiled by gcc) as in its Python mirror."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not found")
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "pic_amd.h"', "int main(void) {"]
+    want = []
+    for cname, pyname in STRUCTS:
+        cls = getattr(pic, pyname)
+        lines.append('printf("%%zu\\n", sizeof(%s));' % cname)
+        want.append(ctypes.sizeof(cls))
+        for f in cls._fields_:
+            lines.append('printf("%%zu\\n", offsetof(%s, %s));' % (cname, f[0]))
+            want.append(getattr(cls, f[0]).offset)
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert got == want
+
+
 def test_read_cross_section_matches_file(pic):
     path = os.path.join(GOLDEN, "csf_Ar.txt")
     e, s = pic.read_cross_section(path)

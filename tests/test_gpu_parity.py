@@ -284,9 +284,10 @@ def test_solve_multigrid_reference_signature(pic, orc):
 
 
 # ------------------------------------------------------------ full passes ----
-def _small_case(pic, prec, mcc, cells=32, n=20000, gas=1e22, seed=7, bin_w=16, te=8.0):
+def _small_case(pic, prec, mcc, cells=32, n=20000, gas=1e22, seed=7, bin_w=16, te=8.0, rebin_every=0):
     from pic_cuda_amd import configs
-    return configs.build(cells, n, prec, mcc=mcc, seed=seed, gas_density=gas, bin_w=bin_w, bin_h=bin_w, te=te)
+    return configs.build(cells, n, prec, mcc=mcc, seed=seed, gas_density=gas, bin_w=bin_w, bin_h=bin_w, te=te,
+                         rebin_every=rebin_every)
 
 
 def _oracle_from(pic, orc, case, prec, seed=7):
@@ -437,6 +438,38 @@ def test_rebinning_does_not_change_results(pic, orc):
         for u, v in zip(ga, gb):
             assert np.array_equal(u, v)
         assert np.array_equal(a["sim"].counters(s)[:11], b["sim"].counters(s)[:11])
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_steps_run_ahead_equal_single_steps(pic, orc, prec):
+    """Within one call of several steps, each push is enqueued before the host
+    has read the previous step's status, gated on the device; the gate holds
+    it back while the solve still needs more cycles than predicted (a cold
+    start), and a step that re-bins first (every 4 steps here) is not run
+    ahead.  The steps run one call at a time never run ahead; both give the
+    same bits.  (The spill-fill half of the gate, the host's re-binning rule
+    evaluated on the device, needs a spill region more than half full, which
+    the 2^20 spare slots of every species keep out of reach at this size.)"""
+    a = _small_case(pic, prec, True, bin_w=4, te=8.0, rebin_every=4)
+    b = _small_case(pic, prec, True, bin_w=4, te=8.0, rebin_every=4)
+    steps = 12
+    for _ in range(steps):
+        a["sim"].step(1)
+    b["sim"].step(steps)
+    ra, rb = a["sim"].stats(), b["sim"].stats()
+    assert ra.ahead_steps == 0 and ra.ahead_held == 0
+    queued = sum(1 for i in range(steps - 1) if (i + 1) % 4 != 0)   # not before a periodic re-binning
+    assert rb.ahead_steps > 0 and rb.ahead_held > 0 and rb.ahead_steps + rb.ahead_held == queued, (
+        rb.ahead_steps, rb.ahead_held)
+    assert ra.rebins == rb.rebins == 2, (ra.rebins, rb.rebins)
+    assert ra.total_mg_cycles == rb.total_mg_cycles
+    assert np.array_equal(a["grid"].download(pic.FIELD_PHI), b["grid"].download(pic.FIELD_PHI))
+    for s in range(2):
+        assert np.array_equal(a["sim"].counters(s)[:11], b["sim"].counters(s)[:11])
+        ga = _sorted(*a["particles"].download(s))
+        gb = _sorted(*b["particles"].download(s))
+        for u, v in zip(ga, gb):
+            assert np.array_equal(u, v)
 
 
 def test_plain_small_fp64_end_to_end(pic, orc):
