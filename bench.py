@@ -523,6 +523,9 @@ def main():
     ap.add_argument("--bin", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-check", action="store_true")
+    ap.add_argument("--tune", action="store_true",
+                    help="run the autotuner (bin tile, SOR factor, cycles per refinement step, sweeps per "
+                         "level) after the warm-up steps, before the timed ones")
     ap.add_argument("--workload", default="all",
                     choices=["all", "step", "poisson", "ensemble", "serve", "graph", "tune", "walls", "export"])
     ap.add_argument("--ens-steps", type=int, default=500)
@@ -550,6 +553,12 @@ def main():
     print("bench: setup %.1f s, %d particles" % (time.perf_counter() - t_setup, case["particles"].last_alive),
           file=sys.stderr, flush=True)
     sim.step(a.warmup)
+    tuned = None
+    if a.tune:
+        t_tune = time.perf_counter()
+        tuned = sim.autotune()
+        tuned["tune_s"] = round(time.perf_counter() - t_tune, 3)
+        sim.step(3)   # the tuned layout's first steps settle the cycle count
     pic.synchronize()
     st0 = sim.stats()
     t0 = time.perf_counter()
@@ -600,6 +609,7 @@ def main():
                    "cells": a.cells, "particles": int(st0.alive), "bin": a.bin},
         "roofline": roof, "step_roofline": step_roof, "cpu_baseline": cpu,
         "field_ms_per_step": round(field_ms, 4), "mg_cycles_per_step": cycles,
+        "steps_run_ahead": int(st1.ahead_steps - st0.ahead_steps), "tuned": tuned,
         "alive_start": int(st0.alive), "alive_end": int(st1.alive),
         "fp64_error": acc, "fp64_particles": rate64,
     }
