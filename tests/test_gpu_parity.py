@@ -447,9 +447,9 @@ def test_steps_run_ahead_equal_single_steps(pic, orc, prec):
     it back while the solve still needs more cycles than predicted (a cold
     start), and a step that re-bins first (every 4 steps here) is not run
     ahead.  The steps run one call at a time never run ahead; both give the
-    same bits.  (The spill-fill half of the gate, the host's re-binning rule
-    evaluated on the device, needs a spill region more than half full, which
-    the 2^20 spare slots of every species keep out of reach at this size.)"""
+    same bits.  (The spill-fill half of the gate needs a spill region more
+    than half full, out of reach at this size: see
+    test_run_ahead_held_back_for_rebinning.)"""
     a = _small_case(pic, prec, True, bin_w=4, te=8.0, rebin_every=4)
     b = _small_case(pic, prec, True, bin_w=4, te=8.0, rebin_every=4)
     steps = 12
