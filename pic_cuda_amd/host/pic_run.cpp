@@ -4,7 +4,13 @@
 //
 //   pic_run [--cells 128] [--particles 100000] [--steps 100] [--mcc 0|1]
 //           [--fp64] [--seed 1] [--te 2.0] [--gas 1e21]
-//           [--frames PATH --every K]
+//           [--frames PATH --every K] [--staged]
+//
+// --staged fills the particles the way main.cu:232-238 does: host arrays of
+// positions and velocities copied into the public device arrays r and v,
+// species indices into type, then `last_alive = n` (the first n entries are
+// electrons, the next n ions; positions and velocities from Weyl sequences,
+// x_i = L (0.02 + 0.96 frac(i a)), v_i = vth sqrt(3) (2 frac(i c) - 1)).
 //
 // With --frames, every K steps the positions are written to the frame file
 // PATH (replaced in place), where main.cu hands them to updateVBO every
@@ -17,6 +23,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 
 int main(int argc, char** argv) {
     int cells = 128, steps = 100, mcc = 0, prec = PIC_PREC_FP32;
@@ -25,6 +32,7 @@ int main(int argc, char** argv) {
     double te = 2.0, gas = 1e21;
     const char* frames = nullptr;
     int every = 0;
+    bool staged = false;
     for (int i = 1; i < argc; ++i) {
         auto arg = [&](const char* k) { return std::strcmp(argv[i], k) == 0 && i + 1 < argc; };
         if (arg("--cells")) cells = std::atoi(argv[++i]);
@@ -37,6 +45,7 @@ int main(int argc, char** argv) {
         else if (arg("--frames")) frames = argv[++i];
         else if (arg("--every")) every = std::atoi(argv[++i]);
         else if (std::strcmp(argv[i], "--fp64") == 0) prec = PIC_PREC_FP64;
+        else if (std::strcmp(argv[i], "--staged") == 0) staged = true;
         else {
             std::fprintf(stderr, "unknown argument %s\n", argv[i]);
             return 2;
@@ -49,13 +58,33 @@ int main(int argc, char** argv) {
     const double dt = 0.2 / std::sqrt(dens * qe * qe / (eps0 * me));
     try {
         pic2d::Grid2D grid(cells + 1, cells + 1, L, L);
-        pic2d::Particles2D particles((int64_t)(n * 1.02) + 4096, prec, /*staging=*/false);
+        // staged: the reference's MAX_PARTICLES holds both species
+        pic2d::Particles2D particles(staged ? 2 * n + 4096 : (int64_t)(n * 1.02) + 4096, prec, staged);
         particles.change_species(0, me, -qe);
         particles.add_species(mar, qe);
         particles.set_weight(0, dens * L * L / n);
         particles.set_weight(1, dens * L * L / n);
-        particles.init_maxwellian(0, n, grid, std::sqrt(te * qe / me), seed);
-        particles.init_maxwellian(1, n, grid, std::sqrt(0.03 * qe / mar), seed);
+        const double vte = std::sqrt(te * qe / me), vti = std::sqrt(0.03 * qe / mar);
+        if (staged) {
+            std::vector<double> r(4 * n), v(4 * n);
+            std::vector<int32_t> type(2 * n);
+            for (long long i = 0; i < 2 * n; ++i) {
+                const double vth = i < n ? vte : vti;
+                r[2 * i] = L * (0.02 + 0.96 * std::fmod((double)i * 0.6180339887498949, 1.0));
+                r[2 * i + 1] = L * (0.02 + 0.96 * std::fmod((double)i * 0.7548776662466927, 1.0));
+                v[2 * i] = vth * 1.7320508075688772 * (2.0 * std::fmod((double)i * 0.5698402909980532, 1.0) - 1.0);
+                v[2 * i + 1] = vth * 1.7320508075688772 * (2.0 * std::fmod((double)i * 0.8191725133961645, 1.0) - 1.0);
+                type[i] = i < n ? 0 : 1;
+            }
+            pic2d::check(pic_host_to_device(particles.r, r.data(), (int64_t)(r.size() * sizeof(double))), "copy r");
+            pic2d::check(pic_host_to_device(particles.v, v.data(), (int64_t)(v.size() * sizeof(double))), "copy v");
+            pic2d::check(pic_host_to_device(particles.type, type.data(), (int64_t)(type.size() * sizeof(int32_t))),
+                         "copy type");
+            particles.last_alive = 2 * n;   // handed over when the Simulation takes the particles
+        } else {
+            particles.init_maxwellian(0, n, grid, vte, seed);
+            particles.init_maxwellian(1, n, grid, vti, seed);
+        }
         pic2d::Boundaries2D walls(L, L, cells + 1, cells + 1);
         walls.add_box();
         walls.compile_walls();
