@@ -529,12 +529,14 @@ def main():
     ap.add_argument("--workload", default="all",
                     choices=["all", "step", "poisson", "ensemble", "serve", "graph", "tune", "walls", "export"])
     ap.add_argument("--ens-steps", type=int, default=500)
+    ap.add_argument("--ens-no-seq", action="store_true",
+                    help="ensemble workload: skip the one-after-another runs (for profiling the batched steps)")
     ap.add_argument("--serve-jobs", type=int, default=24)
     a = ap.parse_args()
     if a.workload in ("poisson", "ensemble", "serve", "graph", "tune", "walls", "export"):
         import pic_cuda_amd as pic
         pic.load()
-        fn = {"poisson": lambda: poisson(cpu=not a.no_cpu), "ensemble": lambda: ensemble(steps=a.ens_steps),
+        fn = {"poisson": lambda: poisson(cpu=not a.no_cpu), "ensemble": lambda: ensemble(steps=a.ens_steps, sequential=not a.ens_no_seq),
               "serve": lambda: serve(n_jobs=a.serve_jobs), "graph": lambda: graph(), "tune": lambda: tune(),
               "walls": lambda: walls(), "export": lambda: export()}[a.workload]
         print(json.dumps({"workload": a.workload, **fn()}))
