@@ -120,3 +120,14 @@ def test_last_error_reports_message(pic):
     rc = lib.pic_xs_set_table(None, 1, 1.0)
     assert rc == -1
     assert b"bad table" in lib.pic_last_error()
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """The benchmark runs on one GPU; asked for more it exits with status 2 and
+    says why (before touching a device)."""
+    import subprocess
+    import sys
+    out = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 2
+    assert "one GPU" in out.stderr and not out.stdout.strip()
