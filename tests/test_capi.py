@@ -131,3 +131,9 @@ def test_bench_refuses_more_than_one_gpu():
                          text=True, timeout=120)
     assert out.returncode == 2
     assert "one GPU" in out.stderr and not out.stdout.strip()
+
+
+def test_missing_library_fails_loudly(pic, tmp_path):
+    """Without the built HIP library the package raises; it has no CPU path."""
+    with pytest.raises(RuntimeError, match="libpic_amd.so not found"):
+        pic.load(str(tmp_path / "libpic_amd.so"))
