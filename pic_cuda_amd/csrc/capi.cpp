@@ -1620,21 +1620,33 @@ int mcc_active(pic_sim* S) {
     return 0;
 }
 
-// The work of one step after the host decisions: push, end of step, spill
-// status to the host, charge, and a multigrid solve of `cycles` cycles with no
-// host synchronization.  Graph mode captures exactly this.
+// The charge conversion and end of step that the first residual pass of the
+// step's solve runs (the spill status goes to the host's pinned buffer)
+static MGStepFuse step_fuse(pic_sim* S) {
+    MGStepFuse f{};
+    f.charge = S->charge;
+    f.cases = S->p->d_case;
+    f.states = S->p->d_state;
+    f.status = S->d_status;
+    f.rebin = S->d_rebin;
+    f.fp32 = S->p->prec == 4;
+    return f;
+}
+
+// The work of one step after the host decisions: push, then the multigrid
+// solve of `cycles` cycles whose first residual pass also converts the charge
+// and ends the step (spill status to the host), with no host synchronization.
+// Graph mode captures exactly this.
 template <typename T>
 static int enqueue_step_body(pic_sim* S, int flags, size_t lds, int cycles) {
     pic_particles* p = S->p;
     hipStream_t sm = lib_stream();
-    const size_t nn = (size_t)S->g->m * S->g->n;
     if (int rc = launch_push<T>((CaseDev<T>*)p->d_case, p->d_state, S->d_items, S->n_items, S->spill_chunk, flags, -1,
                                 lds, sm, S->d_walls_cell != nullptr))
         return rc;
-    if (int rc = launch_charge_end_step<T>(S->charge, nn, (CaseDev<T>*)p->d_case, p->d_state, 1, sm, S->d_status))
-        return rc;
     // the solve's first residual pass also copies phi (its warm start) to phi_prev
-    return S->mg->enqueue_fixed(cycles);
+    const MGStepFuse f = step_fuse(S);
+    return S->mg->enqueue_fixed(cycles, &f);
 }
 
 static void drop_graph(pic_sim* S) {
@@ -1772,7 +1784,6 @@ template <typename T>
 static int step_t(pic_sim* S, int n) {
     pic_particles* p = S->p;
     hipStream_t sm = lib_stream();
-    const size_t nn = (size_t)S->g->m * S->g->n;
     const int mcc = mcc_active(S);
     const bool walls = S->d_walls_cell != nullptr;
     const int full = PIC_F_DRIFT | PIC_F_DEPOSIT | PIC_F_KICK | (mcc ? PIC_F_MCC : 0);
@@ -1811,13 +1822,12 @@ static int step_t(pic_sim* S, int n) {
                 return rc;
             PIC_HIP(hipEventRecord(S->ev[pe + 1], sm));
         }
-        if (int rc = launch_charge_end_step<T>(S->charge, nn, (CaseDev<T>*)p->d_case, p->d_state, 1, sm,
-                                               S->d_status, S->d_rebin))
-            return rc;
         p->h_cur ^= 1;
         S->have_prev = true;   // the solve's first residual pass copies phi to phi_prev
         if (int rc = S->mg->prepare(S->g->phi, S->g->rhs, S->g->dx)) return rc;
-        if (int rc = S->mg->enqueue_fixed(S->mg->last_cycles())) return rc;
+        // its first residual pass also converts the charge and ends the step
+        const MGStepFuse fuse = step_fuse(S);
+        if (int rc = S->mg->enqueue_fixed(S->mg->last_cycles(), &fuse)) return rc;
         PIC_HIP(hipEventRecord(S->ev[2], sm));
         // the next step runs ahead unless it re-bins by the period, ends the
         // call or is a graph step

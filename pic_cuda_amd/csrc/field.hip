@@ -1,6 +1,7 @@
 // field.hip -- grid kernels of the step outside the Poisson solve.
 //
-//  charge_node (field.hpp), in charge_end_step_kernel and charge_batch_kernel:
+//  charge_node (field.hpp), in charge_batch_kernel and in the first residual
+//  pass of a single simulation's solve (step_residual_stage, multigrid.hip):
 //                  Q20 accumulators -> charge density rho and Poisson rhs
 //                  (-rho/eps0); clears the accumulators for the next step.
 //                  Replaces cudaMemset(rho) + updateDensity's FP64 result

@@ -15,8 +15,8 @@ struct ChargeArgs {
 };
 
 // node i of the charge conversion: rhs (and rho) from the Q20 accumulators,
-// which are cleared when a.zero is set
-__device__ inline void charge_node(const ChargeArgs& a, size_t i) {
+// which are cleared when a.zero is set; returns the rhs value written
+__device__ inline double charge_node(const ChargeArgs& a, size_t i) {
     double r = 0.0, q = 0.0;
     for (int s = 0; s < a.ns; ++s) {
         const double v = (double)a.acc[s][i];
@@ -26,6 +26,7 @@ __device__ inline void charge_node(const ChargeArgs& a, size_t i) {
     }
     if (a.rho) a.rho[i] = q;
     a.rhs[i] = r;
+    return r;
 }
 
 

@@ -16,6 +16,8 @@
 // device flag.  A single solve is a batch of one.
 #pragma once
 #include "pic_common.hpp"
+#include "pic_device.hpp"
+#include "field.hpp"
 #include <algorithm>
 #include <vector>
 
@@ -135,6 +137,20 @@ struct MGDstTask {
 
 class MGBatch;
 
+// A step's charge conversion and end of step, run by the first residual pass
+// of the step's solve (a solve of one case): the conversion's arguments, the
+// case's descriptor (CaseDev<float> when fp32, else CaseDev<double>) and
+// state, and where the end of step writes the spill fill and the re-binning
+// flag (either may be null).
+struct MGStepFuse {
+    ChargeArgs charge;
+    const void* cases;
+    CaseState* states;
+    int* status;
+    int* rebin;
+    bool fp32;
+};
+
 class MGSolver {
 public:
     MGSolver(int n, const MGParams& p, hipStream_t s);
@@ -145,7 +161,9 @@ public:
     // capture, enqueue_fixed (no host synchronization) inside it, and
     // complete_fixed once the stream has run it
     int prepare(double* phi, const double* rhs, double h);
-    int enqueue_fixed(int cycles);
+    // fuse (may be null): the first residual pass also converts the step's
+    // charge into rhs and ends the step
+    int enqueue_fixed(int cycles, const MGStepFuse* fuse = nullptr);
     int complete_fixed(MGStats* st);
     int last_cycles() const;
     // the device stop flag of the solve (set once the tolerance is met or the
@@ -222,7 +240,7 @@ public:
     // a solve of a fixed number of cycles with no host synchronization (it can
     // be captured in a graph), and the completion after the stream has run it:
     // members that have not stopped get further cycles until they do
-    int enqueue_fixed(int cycles);
+    int enqueue_fixed(int cycles, const MGStepFuse* fuse = nullptr);
     int complete_fixed();
     int last_cycles() const { return last_cycles_; }
     int last_launches() const { return launches_; }
@@ -259,6 +277,7 @@ private:
     int launch_refinement();   // the inner V-cycles of one refinement step
     bool any_odd_steps() const;
     int enqueue_fine(int what, int which, int parity);
+    int enqueue_step_residual(const MGStepFuse& f);
     void free_all();
 
     hipStream_t stream_;

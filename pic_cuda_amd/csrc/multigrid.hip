@@ -33,6 +33,7 @@
 //                     ||r||/||rhs||; the reference divides by sum rhs^2,
 //                     Multigrid.cu:466).
 #include "mg.hpp"
+#include "step_device.hpp"
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -103,9 +104,9 @@ __device__ inline int xcd_block() {
     return x * q + min(x, r) + (b >> 3);
 }
 
+// the item of block number b (after the XCD renumbering)
 template <typename Task>
-__device__ inline void stage_item(const StageRef<Task>& a, Task& tk, MGTile& tile) {
-    const int b = xcd_block();
+__device__ inline void stage_item_at(const StageRef<Task>& a, Task& tk, MGTile& tile, int b) {
     if (a.tasks) {
         tile = a.tiles[b];
         tk = a.tasks[tile.task];
@@ -115,6 +116,11 @@ __device__ inline void stage_item(const StageRef<Task>& a, Task& tk, MGTile& til
         tile.tx = a.gx ? b % a.gx : b;
         tile.ty = a.gx ? b / a.gx : 0;
     }
+}
+
+template <typename Task>
+__device__ inline void stage_item(const StageRef<Task>& a, Task& tk, MGTile& tile) {
+    stage_item_at(a, tk, tile, xcd_block());
 }
 
 // Stop flag of a task without one: always zero.  Reading the flag through an
@@ -959,13 +965,13 @@ __device__ inline double block_sum256(double v, double* sh) {
 // which == 1: the residual before the first cycle; it also writes the tile's
 // sum of rhs^2 (every node, the boundary included) after the r^2 sums and,
 // with phi_save, copies phi.
-// which == 2: a residual at the end of a cycle, a no-op once the case stopped.
-template <int MODE>
-__global__ __launch_bounds__(256) void ir_residual_stage(const StageRef<MGFineTask> a, int which, int parity) {
-    MGTile tile;
-    MGFineTask tk;
-    stage_item(a, tk, tile);
-    if (which == 2 && *G(&tk.norm->converged)) return;
+// CHARGE (with which == 1): the rhs of the tile's nodes comes from the step's
+// charge conversion, done here node by node (rhs and rho written, the
+// accumulators cleared) instead of in a launch of its own, with the values
+// that launch would write.
+template <int MODE, bool CHARGE>
+__device__ inline void residual_tile(const MGFineTask& tk, const MGTile& tile, int which, int parity,
+                                     const ChargeArgs& chg) {
     __shared__ double sh[256];
     const int n = tk.n;
     const gptr<const double> src = G((MODE == 2 && parity) ? tk.phi_tmp : tk.phi);
@@ -986,7 +992,12 @@ __global__ __launch_bounds__(256) void ir_residual_stage(const StageRef<MGFineTa
     double acc = 0.0, acc_rhs = 0.0;
     const gptr<double> save = G(which == 1 ? tk.phi_save : nullptr);
     if (i < n) {
-        double col[6], lf[4], rt[4], rh[4];
+        double col[6], lf[4], rt[4], rh[4], fv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int j = jb + q;
+            fv[q] = (CHARGE && j < n) ? charge_node(chg, (size_t)j * n + i) : 0.0;
+        }
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
             const int j = jb - 1 + q;
@@ -998,7 +1009,7 @@ __global__ __launch_bounds__(256) void ir_residual_stage(const StageRef<MGFineTa
             const bool in = j > 0 && j < n - 1 && i > 0 && i < n - 1;
             lf[q] = in ? node(i - 1, j) : 0.0;
             rt[q] = in ? node(i + 1, j) : 0.0;
-            rh[q] = in ? rhs[(size_t)j * n + i] : 0.0;
+            rh[q] = in ? (CHARGE ? fv[q] : rhs[(size_t)j * n + i]) : 0.0;
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -1015,7 +1026,7 @@ __global__ __launch_bounds__(256) void ir_residual_stage(const StageRef<MGFineTa
             if (MODE >= 1) r32[g] = (float)r;
             acc += r * r;
             if (which == 1) {
-                const double f = rhs[g];
+                const double f = CHARGE ? fv[q] : rhs[g];
                 acc_rhs += f * f;
                 if (save) save[g] = c;
             }
@@ -1029,6 +1040,37 @@ __global__ __launch_bounds__(256) void ir_residual_stage(const StageRef<MGFineTa
         const double s2 = block_sum256(acc_rhs, sh);
         if (threadIdx.x == 0) G(tk.partial)[tk.nblocks + tix] = s2;
     }
+}
+
+// which == 2: a residual at the end of a cycle, a no-op once the case stopped.
+template <int MODE>
+__global__ __launch_bounds__(256) void ir_residual_stage(const StageRef<MGFineTask> a, int which, int parity) {
+    MGTile tile;
+    MGFineTask tk;
+    stage_item(a, tk, tile);
+    if (which == 2 && *G(&tk.norm->converged)) return;
+    residual_tile<MODE, false>(tk, tile, which, parity, ChargeArgs{});
+}
+
+// The first residual of a step's solve (which == 1) with the step's charge
+// conversion and end of step in the same launch, for a solve of one case: one
+// block more than the residual tiles; the block renumbered first ends the step
+// (clears, flips, spill status and re-binning flag, as end_step_kernel), the
+// others take the residual tiles in their usual order.
+template <int MODE, typename T>
+__global__ __launch_bounds__(256) void step_residual_stage(const StageRef<MGFineTask> a, const ChargeArgs chg,
+                                                           const CaseDev<T>* __restrict__ cases, CaseState* states,
+                                                           int* __restrict__ status_out,
+                                                           int* __restrict__ rebin_out) {
+    const int b = xcd_block();
+    if (b == 0) {
+        end_step_block(cases, states, 0, 1, status_out, rebin_out);
+        return;
+    }
+    MGTile tile;
+    MGFineTask tk;
+    stage_item_at(a, tk, tile, b - 1);
+    residual_tile<MODE, true>(tk, tile, 1, 0, chg);
 }
 
 // End of a mixed-mode solve: after an odd number of executed cycles the
@@ -1283,9 +1325,9 @@ int MGSolver::solve(double* phi, const double* rhs, double h, MGStats* st) {
     return 0;
 }
 
-int MGSolver::enqueue_fixed(int cycles) {
+int MGSolver::enqueue_fixed(int cycles, const MGStepFuse* fuse) {
     PIC_REQUIRE(solo_, "multigrid: prepare() first");
-    return solo_->enqueue_fixed(cycles);
+    return solo_->enqueue_fixed(cycles, fuse);
 }
 
 int MGSolver::complete_fixed(MGStats* st) {
@@ -1996,6 +2038,32 @@ int MGBatch::enqueue_fine(int what, int which, int parity) {
     return 0;
 }
 
+// The first residual (which 1) of a one-case solve as step_residual_stage,
+// which also converts the step's charge and ends the step, then its norm.
+int MGBatch::enqueue_step_residual(const MGStepFuse& f) {
+    const auto res = make_ref<MGFineTask>(&h_fine_[0], d_fine_, d_res_tiles_, 1, h_fine_[0].gx);
+    const auto cases = make_ref<MGFineTask>(&h_fine_[0], d_fine_, nullptr, 1, 0);
+    const dim3 g(n_res_ + 1), b(256);
+    if (f.fp32) {
+        const auto* c = (const CaseDev<float>*)f.cases;
+        if (mixed_) hipLaunchKernelGGL((step_residual_stage<1, float>), g, b, 0, stream_, res, f.charge, c, f.states,
+                                       f.status, f.rebin);
+        else hipLaunchKernelGGL((step_residual_stage<0, float>), g, b, 0, stream_, res, f.charge, c, f.states,
+                                f.status, f.rebin);
+    } else {
+        const auto* c = (const CaseDev<double>*)f.cases;
+        if (mixed_) hipLaunchKernelGGL((step_residual_stage<1, double>), g, b, 0, stream_, res, f.charge, c, f.states,
+                                       f.status, f.rebin);
+        else hipLaunchKernelGGL((step_residual_stage<0, double>), g, b, 0, stream_, res, f.charge, c, f.states,
+                                f.status, f.rebin);
+    }
+    hipLaunchKernelGGL(norm_final_stage, dim3(1), dim3(256), 0, stream_, cases, 1, p_.tol, max_outer_ * inner_,
+                       inner_);
+    launches_ += 2;
+    PIC_HIP(hipGetLastError());
+    return 0;
+}
+
 // Some member ended after an odd number of refinement steps, so its solution
 // is in phi_tmp and the gated copy back is needed (read after a synchronization).
 bool MGBatch::any_odd_steps() const {
@@ -2046,13 +2114,18 @@ int MGBatch::solve() {
     return 0;
 }
 
-int MGBatch::enqueue_fixed(int cycles) {
+int MGBatch::enqueue_fixed(int cycles, const MGStepFuse* fuse) {
     PIC_REQUIRE(!m_.empty(), "multigrid batch: not built");
+    PIC_REQUIRE(!fuse || m_.size() == 1, "multigrid batch: a fused step residual needs a batch of one");
     launches_ = 0;
     // cycles: V-cycles, run as whole refinement steps
     const int K = std::max(1, std::min((cycles + inner_ - 1) / inner_, max_outer_));
     enqueue_fine(0, 0, 0);
-    enqueue_fine(1, 1, 0);
+    if (fuse) {
+        if (int rc = enqueue_step_residual(*fuse)) return rc;
+    } else {
+        enqueue_fine(1, 1, 0);
+    }
     for (int b = 0; b < K; ++b) {
         if (int rc = launch_refinement()) return rc;
         enqueue_fine(1, 2, b & 1);

@@ -27,6 +27,7 @@
 // region (index nb).
 #include "pic_device.hpp"
 #include "field.hpp"
+#include "step_device.hpp"
 #include "push.hpp"
 #include <type_traits>
 
@@ -1183,72 +1184,10 @@ __global__ __launch_bounds__(256) void count_kernel(const CaseDev<T>* __restrict
     if (threadIdx.x == 0) *out = sh[0];
 }
 
-// one block per case: clear the counts of the buffer that becomes the next
-// output, flip the parity and advance the step counter; with status_out, also
-// the spill fill of every species of the new state (read back by the host
-// after the Poisson solve to decide on re-binning)
-template <typename T>
-__device__ inline void end_step_block(const CaseDev<T>* __restrict__ cases, CaseState* states, int c,
-                                      int advance_step, int* __restrict__ status_out, int* __restrict__ rebin_out) {
-    const CaseDev<T>& C = cases[c];
-    const int in = states[c].cur;
-    __syncthreads();
-    for (int s = 0; s < C.ns; ++s) {
-        const SpeciesDev<T>& S = C.sp[s];
-        const int nb = S.nb[in];
-        for (int k = threadIdx.x; k <= nb; k += blockDim.x) {
-            S.count[in][k] = 0;
-            S.mcount[in][k] = 0;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        states[c].cur = in ^ 1;
-        if (advance_step) states[c].step += 1;
-        if (status_out)
-            for (int s = 0; s < PIC_MAX_SPECIES; ++s) {
-                int v = 0;
-                if (s < C.ns) {
-                    const SpeciesDev<T>& S = C.sp[s];
-                    v = S.count[in ^ 1][S.nb[in ^ 1]];
-                }
-                status_out[c * PIC_MAX_SPECIES + s] = v;
-            }
-        // the host's re-binning rule (a spill region more than half full), on
-        // the device for a push enqueued before the host has read the status
-        if (rebin_out) {
-            int need = 0;
-            for (int s = 0; s < C.ns; ++s) {
-                const SpeciesDev<T>& S = C.sp[s];
-                const int nb = S.nb[in ^ 1];
-                if (S.count[in ^ 1][nb] > S.cap[in ^ 1][nb] / 2) need = 1;
-            }
-            rebin_out[c] = need;
-        }
-    }
-}
-
 template <typename T>
 __global__ __launch_bounds__(256) void end_step_kernel(const CaseDev<T>* __restrict__ cases, CaseState* states,
                                                        int advance_step, int* __restrict__ status_out) {
     end_step_block(cases, states, (int)blockIdx.x, advance_step, status_out, nullptr);
-}
-
-// The end of a single-case step and the charge conversion in one launch (they
-// touch disjoint data): block 0 ends the step of case 0 (first, so that its
-// latency overlaps the conversion), the other blocks convert 256 nodes each.
-template <typename T>
-__global__ __launch_bounds__(256) void charge_end_step_kernel(ChargeArgs a, size_t nn,
-                                                              const CaseDev<T>* __restrict__ cases,
-                                                              CaseState* states, int advance_step,
-                                                              int* __restrict__ status_out,
-                                                              int* __restrict__ rebin_out) {
-    if (blockIdx.x == 0) {
-        end_step_block(cases, states, 0, advance_step, status_out, rebin_out);
-    } else {
-        const size_t i = (size_t)(blockIdx.x - 1) * blockDim.x + threadIdx.x;
-        if (i < nn) charge_node(a, i);
-    }
 }
 
 // Undo the bin counts a push left in the output parity (trial pushes of the
@@ -1479,15 +1418,6 @@ int launch_gather(const CaseDev<T>* cases, const CaseState* states, const WorkIt
     return 0;
 }
 
-template <typename T>
-int launch_charge_end_step(const ChargeArgs& a, size_t nn, const CaseDev<T>* cases, CaseState* states, int advance,
-                           hipStream_t s, int* status_out, int* rebin_out) {
-    const int n_charge = (int)((nn + 255) / 256);
-    hipLaunchKernelGGL(charge_end_step_kernel<T>, dim3(n_charge + 1), dim3(256), 0, s, a, nn, cases, states,
-                       advance, status_out, rebin_out);
-    PIC_HIP(hipGetLastError());
-    return 0;
-}
 
 template <typename T>
 int launch_end_step(const CaseDev<T>* cases, CaseState* states, int n_cases, int advance, hipStream_t s,
@@ -1533,8 +1463,6 @@ int launch_counts(const CaseDev<T>* cases, const CaseState* states, int c, int s
     template int launch_export<T>(const CaseDev<T>*, const CaseState*, const WorkItem*, int, int, int32_t*,       \
                                   float2*, int64_t, double, double, hipStream_t);                                \
     template int launch_end_step<T>(const CaseDev<T>*, CaseState*, int, int, hipStream_t, int*);                 \
-    template int launch_charge_end_step<T>(const ChargeArgs&, size_t, const CaseDev<T>*, CaseState*, int,         \
-                                           hipStream_t, int*, int*);                                              \
     template int launch_clear_out<T>(const CaseDev<T>*, const CaseState*, int, hipStream_t);
 PIC_LAUNCH_INST(float)
 PIC_LAUNCH_INST(double)

@@ -65,12 +65,6 @@ int launch_species_items(WorkItem* items, int32_t* counter, int species, int nb,
 template <typename T>
 int launch_end_step(const CaseDev<T>* cases, CaseState* states, int n_cases, int advance, hipStream_t s,
                     int* status_out = nullptr);
-// the end of step of a single case and the charge conversion of nn nodes in
-// one launch (disjoint data); rebin_out (may be null) receives 1 if some
-// species' spill region is more than half full, else 0
-template <typename T>
-int launch_charge_end_step(const ChargeArgs& a, size_t nn, const CaseDev<T>* cases, CaseState* states, int advance,
-                           hipStream_t s, int* status_out, int* rebin_out = nullptr);
 // clear the bin counts of the output parity (after a trial push)
 template <typename T>
 int launch_clear_out(const CaseDev<T>* cases, const CaseState* states, int n_cases, hipStream_t s);
