@@ -1042,7 +1042,8 @@ __device__ inline void residual_tile(const MGFineTask& tk, const MGTile& tile, i
     }
 }
 
-// which == 2: a residual at the end of a cycle, a no-op once the case stopped.
+// One residual tile per workgroup (residual_tile); with which == 2, the
+// residual at the end of a cycle, a no-op once the case stopped.
 template <int MODE>
 __global__ __launch_bounds__(256) void ir_residual_stage(const StageRef<MGFineTask> a, int which, int parity) {
     MGTile tile;
@@ -1056,7 +1057,8 @@ __global__ __launch_bounds__(256) void ir_residual_stage(const StageRef<MGFineTa
 // conversion and end of step in the same launch, for a solve of one case: one
 // block more than the residual tiles; the block renumbered first ends the step
 // (clears, flips, spill status and re-binning flag, as end_step_kernel), the
-// others take the residual tiles in their usual order.
+// others take the residual tiles (tile numbers and partial sums as in
+// ir_residual_stage, so the norms are the same bits).
 template <int MODE, typename T>
 __global__ __launch_bounds__(256) void step_residual_stage(const StageRef<MGFineTask> a, const ChargeArgs chg,
                                                            const CaseDev<T>* __restrict__ cases, CaseState* states,
